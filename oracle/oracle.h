@@ -1,0 +1,76 @@
+/* ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke(), bench.py CPU baseline).
+ *
+ * C API of the FP64 CPU restatement of the 3d-eulerian-amr hydro right-hand side.  Every array
+ * is a "padded SoA block": per block, per field, a GNX^3 cube (GNX = n + 2g), x fastest, i.e.
+ * element (f, i, j, k) of block b sits at ((b*nf + f)*GNX + k)*GNX*GNX + j*GNX + i.  This is the
+ * reference's Array3d<State,GNX,GNX,GNX> (array3d.h:11, x fastest) with fields split out. */
+#ifndef AMRH_ORACLE_H
+#define AMRH_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int n;          /* interior cells per edge (euler/euler_defs.h:27: GNX = 8 + 2*BW) */
+  int g;          /* ghost width BW (euler/euler_defs.h:25) */
+  int nf;         /* fields (euler/state.h:16 STATE_NF = 6) */
+  int recon;      /* 0 = minmod theta (reconstruct.cpp:60-71), 1 = PPM (reconstruct.cpp:22-58,94) */
+  int z_reflect;  /* euler/euler_defs.h:21 Z_REFLECT */
+  double gamma, rho_floor, ei_floor, theta, cfl, grid_dim;
+} orc_cfg;
+
+/* --- per pencil (for pinning against the reference's own State/Reconstruct) ---------------- */
+/* q: len x nf conserved states (AoS).  Writes face fluxes F[i] (face between cells i-1 and i) and
+ * the two face eigenvalues eig[2*i], eig[2*i+1] for faces g..len-g; other entries untouched. */
+int orc_pencil_flux(const orc_cfg* c, int dir, int len, const double* q, double* F, double* eig);
+/* reconstructed face states (conserved, after to_con) ql[i], qr[i] for faces g..len-g */
+int orc_pencil_faces(const orc_cfg* c, int len, const double* q, double* ql, double* qr);
+void orc_floor(const orc_cfg* c, double* u);
+void orc_outflow(const orc_cfg* c, int face, double* u);
+
+/* --- flat batch of independent blocks (each with its own cell width h[b]) ------------------ */
+/* One RK substep over every block: clear_difs, x/y/z flux + divergence, add_difs (a9-a11).
+ * U: padded blocks (ghosts must be filled); U0: padded blocks (interior read).  Updates the
+ * interior of U in place.  If D_out != NULL it receives the divergence (padded layout). */
+int orc_batch_substep(const orc_cfg* c, int nb, double* U, const double* U0, const double* h,
+                      double dt, double beta, double* D_out);
+/* Per block: max over all interior faces of the three directions of |v|+cs (grid/grid.cpp:422-473,
+ * before the division by h). */
+int orc_batch_max_eig(const orc_cfg* c, int nb, const double* U, double* eig);
+/* Grid::max_dt of one block with width h: 0.4*(1/(max_eig/h)) (grid/grid.cpp:474-481) */
+double orc_block_dt(const orc_cfg* c, double max_eig, double h);
+/* Ghost fill of isolated blocks whose six neighbours are all outflow boundaries
+ * (grid/grid_phys_bound.cpp:27-54 with euler/state.cpp:45-84); z lower face reflects if z_reflect. */
+int orc_batch_outflow_ghosts(const orc_cfg* c, int nb, double* U);
+/* n RK steps of order rk (euler/hydro_driver.cpp:231-249) on a flat batch with outflow ghost
+ * refresh before every substep; dt fixed.  U0 scratch of the same size as U. */
+int orc_batch_rk_steps(const orc_cfg* c, int nb, double* U, double* U0, const double* h, double dt,
+                       int rk, int nsteps);
+
+/* --- octree (GridNode recursion, grid/grid_node.cpp) -------------------------------------- */
+typedef struct orc_tree orc_tree;
+orc_tree* orc_tree_create(const orc_cfg* c);
+void orc_tree_destroy(orc_tree* t);
+/* add the child `child` (bit0 x, bit1 y, bit2 z; oct_node/child_index.cpp:66-76) of node `parent`;
+ * returns the new node's index.  Node 0 is the root. */
+int orc_tree_add_child(orc_tree* t, int parent, int child);
+int orc_tree_nnodes(const orc_tree* t);
+/* level, location[3], offset[3] (grid/grid_node.cpp:276), h */
+int orc_tree_node_info(const orc_tree* t, int node, int* level, int* loc, int* off, double* h);
+double* orc_tree_node_U(orc_tree* t, int node); /* padded nf x GNX^3, writable */
+int orc_tree_enforce_boundaries(orc_tree* t);   /* grid/grid_node.cpp:59-112 */
+int orc_tree_inject_from_children(orc_tree* t); /* grid/grid_node.cpp:1035-1074 */
+int orc_tree_store(orc_tree* t);                /* grid/grid_node.cpp:495-515 */
+double orc_tree_max_dt(const orc_tree* t);      /* grid/grid_node.cpp:163-174 */
+/* HydroDriver::sub_step (euler/hydro_driver.cpp:260-274) followed by inject_from_children */
+int orc_tree_substep(orc_tree* t, double dt, double beta);
+/* sum over uncovered cells of U*h^3 (grid/grid_node.cpp:1489-1529, a0 only) and the root ledger */
+int orc_tree_state_sum(const orc_tree* t, double* sum);
+int orc_tree_flow_off(const orc_tree* t, double* fo);
+/* face flux array of a node for the last computed direction (debug) */
+const double* orc_tree_node_F(const orc_tree* t, int node);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,205 @@
+"""ctypes access to the FP64 CPU oracle (oracle/liboracle.so) and, in this container only, to the
+reference-built harness (oracle/_ref/*.so).  Test infrastructure: the product never imports this."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+
+class OrcCfg(C.Structure):
+    _fields_ = [("n", C.c_int), ("g", C.c_int), ("nf", C.c_int), ("recon", C.c_int),
+                ("z_reflect", C.c_int), ("gamma", C.c_double), ("rho_floor", C.c_double),
+                ("ei_floor", C.c_double), ("theta", C.c_double), ("cfl", C.c_double),
+                ("grid_dim", C.c_double)]
+
+
+def euler_cfg(n=8, g=2, recon=0, z_reflect=0, grid_dim=1.0):
+    """Ideal-gas constants of euler/state.cpp:9-11 and euler/euler_defs.h:20,46."""
+    return OrcCfg(n=n, g=g, nf=6, recon=recon, z_reflect=z_reflect, gamma=5.0 / 3.0,
+                  rho_floor=1.0e-20, ei_floor=1.0e-20, theta=1.3, cfl=0.4, grid_dim=grid_dim)
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(DP)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = C.CDLL(ORACLE_SO)
+        cfgp = C.POINTER(OrcCfg)
+        L.orc_pencil_flux.argtypes = [cfgp, C.c_int, C.c_int, DP, DP, DP]
+        L.orc_pencil_faces.argtypes = [cfgp, C.c_int, DP, DP, DP]
+        L.orc_floor.argtypes = [cfgp, DP]
+        L.orc_floor.restype = None
+        L.orc_outflow.argtypes = [cfgp, C.c_int, DP]
+        L.orc_outflow.restype = None
+        L.orc_batch_substep.argtypes = [cfgp, C.c_int, DP, DP, DP, C.c_double, C.c_double, DP]
+        L.orc_batch_max_eig.argtypes = [cfgp, C.c_int, DP, DP]
+        L.orc_block_dt.argtypes = [cfgp, C.c_double, C.c_double]
+        L.orc_block_dt.restype = C.c_double
+        L.orc_batch_outflow_ghosts.argtypes = [cfgp, C.c_int, DP]
+        L.orc_batch_rk_steps.argtypes = [cfgp, C.c_int, DP, DP, DP, C.c_double, C.c_int, C.c_int]
+        L.orc_tree_create.argtypes = [cfgp]
+        L.orc_tree_create.restype = C.c_void_p
+        L.orc_tree_destroy.argtypes = [C.c_void_p]
+        L.orc_tree_destroy.restype = None
+        L.orc_tree_add_child.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_tree_nnodes.argtypes = [C.c_void_p]
+        L.orc_tree_node_info.argtypes = [C.c_void_p, C.c_int, IP, IP, IP, DP]
+        L.orc_tree_node_U.argtypes = [C.c_void_p, C.c_int]
+        L.orc_tree_node_U.restype = DP
+        L.orc_tree_node_F.argtypes = [C.c_void_p, C.c_int]
+        L.orc_tree_node_F.restype = DP
+        for fn in ("orc_tree_enforce_boundaries", "orc_tree_inject_from_children", "orc_tree_store"):
+            getattr(L, fn).argtypes = [C.c_void_p]
+        L.orc_tree_max_dt.argtypes = [C.c_void_p]
+        L.orc_tree_max_dt.restype = C.c_double
+        L.orc_tree_substep.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_tree_state_sum.argtypes = [C.c_void_p, DP]
+        L.orc_tree_flow_off.argtypes = [C.c_void_p, DP]
+        _lib = L
+    return _lib
+
+
+def ref_lib(kind="euler"):
+    """The reference-built harness; only in this container (needs /root/reference to build)."""
+    path = os.path.join(REF_DIR, "libref_%s.so" % kind)
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.ref_pencil_flux.argtypes = [C.c_int, DP, DP, DP]
+    L.ref_pencil_flux.restype = None
+    L.ref_pencil_faces.argtypes = [DP, DP, DP]
+    L.ref_pencil_faces.restype = None
+    L.ref_floor.argtypes = [DP]
+    L.ref_floor.restype = None
+    L.ref_outflow.argtypes = [C.c_int, DP]
+    L.ref_outflow.restype = None
+    L.ref_pg.argtypes = [DP]
+    L.ref_pg.restype = C.c_double
+    return L
+
+
+# ---- convenience wrappers -------------------------------------------------------------------
+
+def pencil_flux(cfg, d, q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q.shape[0]
+    F = np.full((n, cfg.nf), np.nan)
+    eig = np.full(2 * n, np.nan)
+    rc = lib().orc_pencil_flux(C.byref(cfg), d, n, dptr(q), dptr(F), dptr(eig))
+    assert rc == 0
+    return F, eig.reshape(n, 2)
+
+
+def batch_substep(cfg, U, U0, h, dt, beta, want_D=False):
+    nb = U.shape[0]
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    D = np.zeros_like(U) if want_D else None
+    rc = lib().orc_batch_substep(C.byref(cfg), nb, dptr(U), dptr(U0), dptr(h), dt, beta,
+                                 dptr(D) if want_D else None)
+    assert rc == 0
+    return D
+
+
+def batch_max_eig(cfg, U):
+    e = np.zeros(U.shape[0])
+    assert lib().orc_batch_max_eig(C.byref(cfg), U.shape[0], dptr(U), dptr(e)) == 0
+    return e
+
+
+def block_dt(cfg, m, h):
+    return lib().orc_block_dt(C.byref(cfg), m, h)
+
+
+def outflow_ghosts(cfg, U):
+    assert lib().orc_batch_outflow_ghosts(C.byref(cfg), U.shape[0], dptr(U)) == 0
+
+
+def rk_steps(cfg, U, h, dt, rk, nsteps):
+    U0 = np.empty_like(U)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    assert lib().orc_batch_rk_steps(C.byref(cfg), U.shape[0], dptr(U), dptr(U0), dptr(h), dt, rk,
+                                    nsteps) == 0
+
+
+class Tree:
+    """Octree of the oracle (GridNode recursion restated)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.L = lib()
+        self.t = self.L.orc_tree_create(C.byref(cfg))
+        assert self.t
+        self.G = cfg.n + 2 * cfg.g
+
+    def __del__(self):
+        if getattr(self, "t", None):
+            self.L.orc_tree_destroy(self.t)
+            self.t = None
+
+    def add_child(self, parent, c):
+        r = self.L.orc_tree_add_child(self.t, parent, c)
+        assert r >= 0
+        return r
+
+    @property
+    def nnodes(self):
+        return self.L.orc_tree_nnodes(self.t)
+
+    def info(self, i):
+        lev = C.c_int()
+        loc = (C.c_int * 3)()
+        off = (C.c_int * 3)()
+        h = C.c_double()
+        assert self.L.orc_tree_node_info(self.t, i, C.byref(lev), loc, off, C.byref(h)) == 0
+        return lev.value, tuple(loc), tuple(off), h.value
+
+    def U(self, i):
+        p = self.L.orc_tree_node_U(self.t, i)
+        G = self.G
+        return np.ctypeslib.as_array(p, shape=(self.cfg.nf, G, G, G))
+
+    def F(self, i):
+        p = self.L.orc_tree_node_F(self.t, i)
+        G = self.G
+        return np.ctypeslib.as_array(p, shape=(self.cfg.nf, G, G, G))
+
+    def enforce_boundaries(self):
+        self.L.orc_tree_enforce_boundaries(self.t)
+
+    def inject_from_children(self):
+        self.L.orc_tree_inject_from_children(self.t)
+
+    def store(self):
+        self.L.orc_tree_store(self.t)
+
+    def max_dt(self):
+        return self.L.orc_tree_max_dt(self.t)
+
+    def substep(self, dt, beta):
+        self.L.orc_tree_substep(self.t, dt, beta)
+
+    def state_sum(self):
+        s = np.zeros(self.cfg.nf)
+        self.L.orc_tree_state_sum(self.t, dptr(s))
+        return s
+
+    def flow_off(self):
+        s = np.zeros(self.cfg.nf)
+        self.L.orc_tree_flow_off(self.t, dptr(s))
+        return s
