@@ -1,0 +1,299 @@
+// C ABI of amrhydro (include/amrhydro.h): context, buffers, launches.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "amrh_internal.h"
+#include "amrh_layout.h"
+#include "amrhydro.h"
+
+using namespace amrh;
+
+struct amrh_ctx {
+  amrh_config cfg;
+  int nb = 0;
+  Layout L{8, 2, 6};
+  double* buf[2] = {nullptr, nullptr};
+  int cur = 0;     // buffer holding the current stage
+  int u0buf = -1;  // buffer holding the RK base, -1 = none stored
+  double* h_dev = nullptr;
+  double* eig_dev = nullptr;
+  unsigned long long* gmax_dev = nullptr;
+  int* err_dev = nullptr;
+  double* pad_stage = nullptr;  // padded SoA staging for upload/download
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  bool have_h = false, have_u = false;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(AMRH_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+long padded_elems(const amrh_ctx* c) {
+  const long P = c->cfg.n + 2 * c->cfg.g;
+  return (long)c->nb * c->cfg.nf * P * P * P;
+}
+long record_elems(const amrh_ctx* c) { return (long)c->nb * c->L.block(); }
+
+int check_cfg(const amrh_config* c) {
+  if (!c) return fail(AMRH_EINVAL, "null config");
+  if (c->physics != AMRH_PHYS_EULER) return fail(AMRH_EINVAL, "physics %d not supported", c->physics);
+  if (c->nf != 6) return fail(AMRH_EINVAL, "EULER physics has 6 fields, got %d", c->nf);
+  if (c->n != 8 && c->n != 16) return fail(AMRH_EINVAL, "n must be 8 or 16, got %d", c->n);
+  if (c->g != 2) return fail(AMRH_EINVAL, "minmod reconstruction needs g = 2, got %d", c->g);
+  if (c->recon != AMRH_RECON_MINMOD) return fail(AMRH_EINVAL, "recon %d not supported", c->recon);
+  if (c->precision != AMRH_FP64 && c->precision != AMRH_MIXED)
+    return fail(AMRH_EINVAL, "precision %d unknown", c->precision);
+  if (!(c->gamma > 1.0)) return fail(AMRH_EINVAL, "gamma must exceed 1");
+  return AMRH_OK;
+}
+
+PhysParams params(const amrh_config& c) {
+  PhysParams P;
+  P.gamma = c.gamma;
+  P.rho_floor = c.rho_floor;
+  P.ei_floor = c.ei_floor;
+  P.theta = c.theta;
+  P.cfl = c.cfl;
+  return P;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* amrh_last_error(void) { return g_err.c_str(); }
+
+int amrh_create(const amrh_config* cfg, int nblocks, amrh_ctx** out) {
+  if (!out) return fail(AMRH_EINVAL, "null out");
+  *out = nullptr;
+  if (int rc = check_cfg(cfg)) return rc;
+  if (nblocks <= 0) return fail(AMRH_EINVAL, "nblocks must be positive");
+  amrh_ctx* c = new amrh_ctx;
+  c->cfg = *cfg;
+  c->nb = nblocks;
+  c->L = Layout{cfg->n, cfg->g, cfg->nf};
+  const size_t rec_bytes = sizeof(double) * (size_t)record_elems(c);
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < 2 && e == hipSuccess; q++) e = hipMalloc(&c->buf[q], rec_bytes);
+  if (e == hipSuccess) e = hipMalloc(&c->h_dev, sizeof(double) * nblocks);
+  if (e == hipSuccess) e = hipMalloc(&c->eig_dev, sizeof(double) * nblocks);
+  if (e == hipSuccess) e = hipMalloc(&c->gmax_dev, sizeof(unsigned long long) * 2);
+  if (e == hipSuccess) e = hipMalloc(&c->err_dev, sizeof(int));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemset(c->err_dev, 0, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->buf[0], 0, rec_bytes);
+  if (e == hipSuccess) e = hipMemset(c->buf[1], 0, rec_bytes);
+  if (e != hipSuccess) {
+    amrh_destroy(c);
+    return fail(AMRH_EHIP, "allocation failed: %s", hipGetErrorString(e));
+  }
+  c->stream = c->own_stream;
+  *out = c;
+  return AMRH_OK;
+}
+
+void amrh_destroy(amrh_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int q = 0; q < 2; q++)
+    if (c->buf[q]) (void)hipFree(c->buf[q]);
+  if (c->h_dev) (void)hipFree(c->h_dev);
+  if (c->eig_dev) (void)hipFree(c->eig_dev);
+  if (c->gmax_dev) (void)hipFree(c->gmax_dev);
+  if (c->err_dev) (void)hipFree(c->err_dev);
+  if (c->pad_stage) (void)hipFree(c->pad_stage);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+int amrh_set_stream(amrh_ctx* c, void* stream) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return AMRH_OK;
+}
+
+int amrh_sync(amrh_ctx* c) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRH_OK;
+}
+
+int amrh_set_block_h(amrh_ctx* c, const double* h) {
+  if (!c || !h) return fail(AMRH_EINVAL, "null argument");
+  for (int b = 0; b < c->nb; b++)
+    if (!(h[b] > 0.0)) return fail(AMRH_EINVAL, "h[%d] = %g is not positive", b, h[b]);
+  HIPCHK(hipMemcpyAsync(c->h_dev, h, sizeof(double) * c->nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_h = true;
+  return AMRH_OK;
+}
+
+static int ensure_stage(amrh_ctx* c) {
+  if (!c->pad_stage) HIPCHK(hipMalloc(&c->pad_stage, sizeof(double) * (size_t)padded_elems(c)));
+  return AMRH_OK;
+}
+
+int amrh_upload_device(amrh_ctx* c, const void* pad) {
+  if (!c || !pad) return fail(AMRH_EINVAL, "null argument");
+  HIPCHK(launch_pack(c->cfg.n, (const double*)pad, c->buf[c->cur], record_elems(c), true, c->stream));
+  c->have_u = true;
+  c->u0buf = -1;
+  return AMRH_OK;
+}
+
+int amrh_upload(amrh_ctx* c, const double* U) {
+  if (!c || !U) return fail(AMRH_EINVAL, "null argument");
+  if (int rc = ensure_stage(c)) return rc;
+  HIPCHK(hipMemcpyAsync(c->pad_stage, U, sizeof(double) * (size_t)padded_elems(c), hipMemcpyHostToDevice,
+                        c->stream));
+  int rc = amrh_upload_device(c, c->pad_stage);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRH_OK;
+}
+
+int amrh_download_device(amrh_ctx* c, void* pad) {
+  if (!c || !pad) return fail(AMRH_EINVAL, "null argument");
+  if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
+  HIPCHK(hipMemsetAsync(pad, 0, sizeof(double) * (size_t)padded_elems(c), c->stream));
+  HIPCHK(launch_pack(c->cfg.n, (const double*)pad, c->buf[c->cur], record_elems(c), false, c->stream));
+  return AMRH_OK;
+}
+
+int amrh_download(amrh_ctx* c, double* U) {
+  if (!c || !U) return fail(AMRH_EINVAL, "null argument");
+  if (int rc = ensure_stage(c)) return rc;
+  if (int rc = amrh_download_device(c, c->pad_stage)) return rc;
+  HIPCHK(hipMemcpyAsync(U, c->pad_stage, sizeof(double) * (size_t)padded_elems(c), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRH_OK;
+}
+
+int amrh_store(amrh_ctx* c) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
+  c->u0buf = c->cur;
+  return AMRH_OK;
+}
+
+int amrh_fill_outflow_ghosts(amrh_ctx* c) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
+  HIPCHK(launch_outflow_ghosts(c->cfg.n, c->buf[c->cur], c->nb, c->cfg.z_reflect, c->stream));
+  return AMRH_OK;
+}
+
+static int substep_impl(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
+  const bool need_u0 = beta != 1.0;
+  if (need_u0 && c->u0buf < 0) return fail(AMRH_ESTATE, "beta != 1 needs amrh_store() first");
+  SubstepArgs a{};
+  a.Uin = c->buf[c->cur];
+  int out = c->cur;
+  if (c->u0buf == c->cur) out = 1 - c->cur;  // never overwrite the RK base
+  a.Uout = c->buf[out];
+  a.U0 = c->u0buf >= 0 ? c->buf[c->u0buf] : c->buf[c->cur];
+  a.h = c->h_dev;
+  a.dt_dev = dt_dev;
+  a.dt = dt;
+  a.beta = beta;
+  a.nblocks = c->nb;
+  a.block0 = 0;
+  a.P = params(c->cfg);
+  a.err = c->err_dev;
+  HIPCHK(launch_substep(c->cfg.n, a, c->cfg.precision == AMRH_MIXED, c->stream));
+  c->cur = out;
+  return AMRH_OK;
+}
+
+int amrh_substep(amrh_ctx* c, double dt, double beta) { return substep_impl(c, dt, beta, nullptr); }
+
+int amrh_substep_dev(amrh_ctx* c, const double* dt_dev, double beta) {
+  if (!dt_dev) return fail(AMRH_EINVAL, "null dt_dev");
+  return substep_impl(c, 0.0, beta, dt_dev);
+}
+
+static int maxeig_impl(amrh_ctx* c, bool per_block, bool global) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
+  MaxEigArgs m{};
+  m.U = c->buf[c->cur];
+  m.h = c->h_dev;
+  m.block_eig = per_block ? c->eig_dev : nullptr;
+  m.gmax = global ? c->gmax_dev : nullptr;
+  m.nblocks = c->nb;
+  m.P = params(c->cfg);
+  if (global) HIPCHK(hipMemsetAsync(c->gmax_dev, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(launch_maxeig(c->cfg.n, m, c->cfg.precision == AMRH_MIXED, c->stream));
+  return AMRH_OK;
+}
+
+int amrh_block_max_eig(amrh_ctx* c, double* out) {
+  if (!out) return fail(AMRH_EINVAL, "null out");
+  if (int rc = maxeig_impl(c, true, false)) return rc;
+  HIPCHK(hipMemcpyAsync(out, c->eig_dev, sizeof(double) * c->nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRH_OK;
+}
+
+int amrh_max_speed(amrh_ctx* c, double* out) {
+  if (!out) return fail(AMRH_EINVAL, "null out");
+  if (int rc = maxeig_impl(c, false, true)) return rc;
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, c->gmax_dev, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::memcpy(out, &bits, sizeof(double));
+  return AMRH_OK;
+}
+
+int amrh_dt_device(amrh_ctx* c, double scale, double* dt_dev) {
+  if (!dt_dev) return fail(AMRH_EINVAL, "null dt_dev");
+  if (int rc = maxeig_impl(c, false, true)) return rc;
+  HIPCHK(launch_dt_from_gmax(c->gmax_dev, c->cfg.cfl, scale, dt_dev, c->stream));
+  return AMRH_OK;
+}
+
+int amrh_error(amrh_ctx* c, int* flags) {
+  if (!c || !flags) return fail(AMRH_EINVAL, "null argument");
+  int v = 0;
+  HIPCHK(hipMemcpyAsync(&v, c->err_dev, sizeof v, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemsetAsync(c->err_dev, 0, sizeof(int), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *flags = v;
+  return AMRH_OK;
+}
+
+double amrh_substep_bytes(const amrh_ctx* c, int beta_is_one) {
+  if (!c) return 0.0;
+  const double n = c->cfg.n, g = c->cfg.g, nf = c->cfg.nf;
+  const double n3 = n * n * n;
+  const double read_u = n3 + 6.0 * g * n * n;
+  const double per_block = 8.0 * nf * (read_u + n3 + (beta_is_one ? 0.0 : n3));
+  return per_block * c->nb;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+// Ideal-gas ("EULER") physics on the device, two forms.
+//
+// Exact64: FP64, every operation in the reference's order (euler/state.cpp:148-258,
+//   reconstruct.cpp:60-71, real.h:20-70, grid/grid.cpp:238-239), for a build without FP
+//   contraction: matches the FP64 CPU path bit for bit except where libm/OCML pow differ by an ulp.
+// Mixed: reconstruction, fluxes and wavespeeds in FP32, starting from primitives that were formed
+//   in FP64.  Pressure is taken from the reconstructed internal energy directly (the reference
+//   forms E = eps + ek and then E - ek, which is the same number in exact arithmetic but cancels
+//   catastrophically in FP32 at high Mach number); the dual-energy test is the same inequality.
+//   Divergence accumulation, the RK combination and the floor stay FP64 in the kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace amrh {
+
+enum { RHO = 0, SX = 1, SY = 2, SZ = 3, ET = 4, TAU = 5, NFE = 6 };
+
+struct PhysParams {
+  double gamma, rho_floor, ei_floor, theta, cfl;
+};
+
+// ------------------------------------------------------------------------------------------
+// Exact FP64
+// ------------------------------------------------------------------------------------------
+namespace x64 {
+
+__device__ __forceinline__ double rmax(double a, double b) { return a > b ? a : b; }   // real.h:20
+__device__ __forceinline__ double rmin(double a, double b) { return a > b ? b : a; }   // real.h:38
+__device__ __forceinline__ double sgn(double a) { return a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : 0.0); }
+__device__ __forceinline__ double minmod2(double a, double b) {                         // real.h:60
+  return 0.5 * (sgn(a) + sgn(b)) * rmin(fabs(a), fabs(b));
+}
+__device__ __forceinline__ double minmod_theta(double a, double b, double th) {         // real.h:64
+  return minmod2(th * a, minmod2(th * b, 0.5 * (a + b)));
+}
+__device__ __forceinline__ double ek(const double* u) {                                // state.cpp:148
+  return 0.5 * (u[SX] * u[SX] + u[SY] * u[SY] + u[SZ] * u[SZ]) / u[RHO];
+}
+__device__ __forceinline__ double pg(const PhysParams& P, const double* u) {           // state.cpp:156
+  const double ei0 = u[ET] - ek(u);
+  if (ei0 < 0.001 * u[ET]) return (P.gamma - 1.0) * pow(u[TAU], P.gamma);
+  return (P.gamma - 1.0) * ei0;
+}
+__device__ __forceinline__ void to_prim(double* u) {                                   // state.cpp:246
+  u[ET] -= ek(u);
+  u[SX] /= u[RHO];
+  u[SY] /= u[RHO];
+  u[SZ] /= u[RHO];
+}
+__device__ __forceinline__ void to_con(double* u) {                                    // state.cpp:253
+  u[SX] *= u[RHO];
+  u[SY] *= u[RHO];
+  u[SZ] *= u[RHO];
+  u[ET] += ek(u);
+}
+__device__ __forceinline__ void floor_state(const PhysParams& P, double* u) {          // state.cpp:86
+  u[RHO] = rmax(u[RHO], P.rho_floor);
+  const double ei0 = u[ET] - ek(u);
+  if (ei0 > 0.1 * u[ET]) u[TAU] = pow(rmax(ei0, P.ei_floor), 1.0 / P.gamma);
+}
+// slope of the middle cell b from neighbours a, c (reconstruct.cpp:81-85)
+__device__ __forceinline__ void slope(const double* a, const double* b, const double* c, double* s,
+                                      double th) {
+#pragma unroll
+  for (int l = 0; l < NFE; l++) s[l] = minmod_theta(c[l] - b[l], b[l] - a[l], th);
+}
+// KT face flux in direction D from the primitive states and slopes of the two adjacent cells
+// (reconstruct.cpp:86-89, grid/grid.cpp:233-239, state.cpp:171-220).  eig = max(eL, eR).
+template <int D>
+__device__ __forceinline__ void kt_face(const PhysParams& P, const double* pl, const double* sl,
+                                        const double* pr, const double* sr, double* F, double& eig) {
+  double ql[NFE], qr[NFE];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) {
+    ql[l] = pl[l] + sl[l] * 0.5;
+    qr[l] = pr[l] - sr[l] * 0.5;
+  }
+  to_con(ql);
+  to_con(qr);
+  const double vl = ql[SX + D] / ql[RHO], vr = qr[SX + D] / qr[RHO];
+  const double pgl = pg(P, ql), pgr = pg(P, qr);
+  const double el = fabs(vl) + sqrt(P.gamma * pgl / ql[RHO]);
+  const double er = fabs(vr) + sqrt(P.gamma * pgr / qr[RHO]);
+  const double a = rmax(el, er);
+  eig = a;
+#pragma unroll
+  for (int l = 0; l < NFE; l++) {
+    double fl = ql[l] * vl, fr = qr[l] * vr;
+    if (l == SX + D) {
+      fl += pgl;
+      fr += pgr;
+    }
+    if (l == ET) {
+      fl += vl * pgl;
+      fr += vr * pgr;
+    }
+    F[l] = ((fl + fr) - (qr[l] - ql[l]) * a) * 0.5;
+  }
+}
+template <int D>
+__device__ __forceinline__ double face_eig(const PhysParams& P, const double* pl, const double* sl,
+                                           const double* pr, const double* sr) {
+  double ql[NFE], qr[NFE];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) {
+    ql[l] = pl[l] + sl[l] * 0.5;
+    qr[l] = pr[l] - sr[l] * 0.5;
+  }
+  to_con(ql);
+  to_con(qr);
+  const double el = fabs(ql[SX + D] / ql[RHO]) + sqrt(P.gamma * pg(P, ql) / ql[RHO]);
+  const double er = fabs(qr[SX + D] / qr[RHO]) + sqrt(P.gamma * pg(P, qr) / qr[RHO]);
+  return rmax(el, er);
+}
+
+}  // namespace x64
+
+// ------------------------------------------------------------------------------------------
+// Mixed: FP32 reconstruction / flux / wavespeed
+// ------------------------------------------------------------------------------------------
+namespace f32 {
+
+struct PhysF {
+  float gamma, gm1, theta;
+};
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// x^y for x > 0 via the hardware log2/exp2 (about 1-2 ulp in FP32)
+__device__ __forceinline__ float fast_pow(float x, float y) {
+  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+
+// minmod(theta*a, theta*b, (a+b)/2): nonzero only when all three agree in sign
+__device__ __forceinline__ float minmod_theta(float a, float b, float th) {
+  const float c = 0.5f * (a + b);
+  const float m = fminf(fminf(fabsf(th * a), fabsf(th * b)), fabsf(c));
+  return (a * b > 0.0f) ? copysignf(m, a) : 0.0f;
+}
+__device__ __forceinline__ void slope(const float* a, const float* b, const float* c, float* s, float th) {
+#pragma unroll
+  for (int l = 0; l < NFE; l++) s[l] = minmod_theta(c[l] - b[l], b[l] - a[l], th);
+}
+
+// one reconstructed face state (primitive q: rho, vx, vy, vz, eps, tau) -> conserved U, flux, eig
+template <int D>
+__device__ __forceinline__ void state_flux(const PhysF& P, const float* q, float* U, float* f, float& e) {
+  const float rho = q[RHO];
+  const float v2 = q[SX] * q[SX] + q[SY] * q[SY] + q[SZ] * q[SZ];
+  const float ekin = 0.5f * rho * v2;
+  const float E = q[ET] + ekin;
+  const float p = (q[ET] < 0.001f * E) ? P.gm1 * fast_pow(q[TAU], P.gamma) : P.gm1 * q[ET];
+  const float vd = q[SX + D];
+  const float cs = fast_sqrt(P.gamma * p * fast_rcp(rho));
+  e = fabsf(vd) + cs;
+  U[RHO] = rho;
+  U[SX] = rho * q[SX];
+  U[SY] = rho * q[SY];
+  U[SZ] = rho * q[SZ];
+  U[ET] = E;
+  U[TAU] = q[TAU];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) f[l] = U[l] * vd;
+  f[SX + D] += p;
+  f[ET] += vd * p;
+}
+
+template <int D>
+__device__ __forceinline__ void kt_face(const PhysF& P, const float* pl, const float* sl, const float* pr,
+                                        const float* sr, float* F, float& eig) {
+  float ql[NFE], qr[NFE], UL[NFE], UR[NFE], fl[NFE], fr[NFE], el, er;
+#pragma unroll
+  for (int l = 0; l < NFE; l++) {
+    ql[l] = pl[l] + 0.5f * sl[l];
+    qr[l] = pr[l] - 0.5f * sr[l];
+  }
+  state_flux<D>(P, ql, UL, fl, el);
+  state_flux<D>(P, qr, UR, fr, er);
+  const float a = fmaxf(el, er);
+  eig = a;
+#pragma unroll
+  for (int l = 0; l < NFE; l++) F[l] = 0.5f * ((fl[l] + fr[l]) - a * (UR[l] - UL[l]));
+}
+
+template <int D>
+__device__ __forceinline__ float face_eig(const PhysF& P, const float* pl, const float* sl, const float* pr,
+                                          const float* sr) {
+  float U[NFE], f[NFE], el, er, q[NFE];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) q[l] = pl[l] + 0.5f * sl[l];
+  state_flux<D>(P, q, U, f, el);
+#pragma unroll
+  for (int l = 0; l < NFE; l++) q[l] = pr[l] - 0.5f * sr[l];
+  state_flux<D>(P, q, U, f, er);
+  return fmaxf(el, er);
+}
+
+}  // namespace f32
+}  // namespace amrh

@@ -1,0 +1,43 @@
+// Fused Kurganov-Tadmor substep kernel and its CFL twin.
+//
+// One workgroup per block, one thread per (i, j) column, N*N threads.  The block is swept plane by
+// plane in z:
+//   * each thread keeps its column's primitive states for planes k, k+1, k+2 in registers, the
+//     z-slope of plane k and the z-face flux at k-1/2, so every z-face is evaluated once;
+//   * the primitives of plane k, with their x/y ghost cells, are staged in LDS, and the (N+1)*N
+//     x-faces and N*(N+1) y-faces of the plane are spread over all threads and written to LDS;
+//   * the cell (i, j, k) then accumulates D = -(dFx + dFy + dFz)/h in that order
+//     (grid/grid.cpp:304-344), forms U = (U + D*dt)*beta + U0*(1-beta), floors it
+//     (grid/grid.cpp:406-420) and stores it.
+// Each cell's primitives are formed once per substep (the reference redoes to_prim per sweep;
+// the value is the same).  HBM traffic per block is the algorithmic count: interior + face ghost
+// slabs read once, interior written once, U0 interior read when beta != 1.
+#pragma once
+#include "amrh_layout.h"
+#include "euler_phys.h"
+
+namespace amrh {
+
+struct SubstepArgs {
+  const double* Uin;       // current stage records
+  double* Uout;            // output records (may alias Uin)
+  const double* U0;        // RK base records (interior read when has_u0)
+  const double* h;         // per-block cell width
+  const double* dt_dev;    // device dt or nullptr
+  double dt, beta;
+  int nblocks;
+  int block0;              // first block handled by this launch
+  PhysParams P;
+  int* err;
+};
+
+struct MaxEigArgs {
+  const double* U;
+  const double* h;
+  double* block_eig;           // per block max eig (nullptr = skip)
+  unsigned long long* gmax;    // global max of eig/h as ordered bits (nullptr = skip)
+  int nblocks;
+  PhysParams P;
+};
+
+}  // namespace amrh

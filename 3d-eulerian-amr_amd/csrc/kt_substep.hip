@@ -1,0 +1,528 @@
+// Fused KT substep, CFL wavespeed scan, ghost fill and layout conversion kernels (gfx950).
+// See kt_kernels.h for the sweep scheme and amrh_layout.h for the record layout.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "amrh_internal.h"
+#include "kt_kernels.h"
+
+namespace amrh {
+
+namespace {
+
+constexpr int GW = 2;  // ghost width of the minmod scheme (euler/euler_defs.h:25)
+
+template <bool MIXED>
+struct Prec {
+  using T = double;
+};
+template <>
+struct Prec<true> {
+  using T = float;
+};
+
+// conserved FP64 -> primitive in the compute type
+template <bool MIXED>
+__device__ __forceinline__ void make_prim(const double* u, typename Prec<MIXED>::T* p) {
+  if constexpr (!MIXED) {
+#pragma unroll
+    for (int l = 0; l < NFE; l++) p[l] = u[l];
+    x64::to_prim(p);
+  } else {
+    const double inv = 1.0 / u[RHO];
+    const double vx = u[SX] * inv, vy = u[SY] * inv, vz = u[SZ] * inv;
+    const double ekin = 0.5 * (u[SX] * vx + u[SY] * vy + u[SZ] * vz);
+    p[RHO] = (float)u[RHO];
+    p[SX] = (float)vx;
+    p[SY] = (float)vy;
+    p[SZ] = (float)vz;
+    p[ET] = (float)(u[ET] - ekin);
+    p[TAU] = (float)u[TAU];
+  }
+}
+
+template <bool MIXED, int D>
+__device__ __forceinline__ void face_flux(const PhysParams& P, const typename Prec<MIXED>::T* pm2,
+                                          const typename Prec<MIXED>::T* pm1,
+                                          const typename Prec<MIXED>::T* p0,
+                                          const typename Prec<MIXED>::T* pp1,
+                                          typename Prec<MIXED>::T* F) {
+  using T = typename Prec<MIXED>::T;
+  T sl[NFE], sr[NFE];
+  if constexpr (!MIXED) {
+    x64::slope(pm2, pm1, p0, sl, P.theta);
+    x64::slope(pm1, p0, pp1, sr, P.theta);
+    double e;
+    x64::kt_face<D>(P, pm1, sl, p0, sr, F, e);
+  } else {
+    const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+    f32::slope(pm2, pm1, p0, sl, PF.theta);
+    f32::slope(pm1, p0, pp1, sr, PF.theta);
+    float e;
+    f32::kt_face<D>(PF, pm1, sl, p0, sr, F, e);
+  }
+}
+
+template <bool MIXED, int D>
+__device__ __forceinline__ double face_eig(const PhysParams& P, const typename Prec<MIXED>::T* pm2,
+                                           const typename Prec<MIXED>::T* pm1,
+                                           const typename Prec<MIXED>::T* p0,
+                                           const typename Prec<MIXED>::T* pp1) {
+  using T = typename Prec<MIXED>::T;
+  T sl[NFE], sr[NFE];
+  if constexpr (!MIXED) {
+    x64::slope(pm2, pm1, p0, sl, P.theta);
+    x64::slope(pm1, p0, pp1, sr, P.theta);
+    return x64::face_eig<D>(P, pm1, sl, p0, sr);
+  } else {
+    const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+    f32::slope(pm2, pm1, p0, sl, PF.theta);
+    f32::slope(pm1, p0, pp1, sr, PF.theta);
+    return (double)f32::face_eig<D>(PF, pm1, sl, p0, sr);
+  }
+}
+
+// element offset of the column cell (i, j, kk), kk in [-G, N+G)
+template <int N>
+__device__ __forceinline__ long col_elem(const Layout& L, int i, int j, int kk) {
+  if (kk < 0) return L.slab(ZL) + ((long)(kk + GW) * N + j) * N + i;
+  if (kk >= N) return L.slab(ZU) + ((long)(kk - N) * N + j) * N + i;
+  return L.interior(i, j, kk);
+}
+
+// MODE 0: substep (flux + divergence + update).  MODE 1: max wavespeed scan only.
+template <int N, bool MIXED, int MODE>
+__global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArgs m) {
+  using T = typename Prec<MIXED>::T;
+  constexpr int PW = N + 2 * GW;  // padded plane edge
+  constexpr int NT = N * N;
+  constexpr int NFX = N * (N + 1);
+  constexpr int HN = GW * N;  // halo cells per face per plane
+  constexpr Layout L{N, GW, NFE};
+  constexpr long R = L.rec();
+
+  __shared__ T plane[NFE][PW][PW];
+  __shared__ T Fx[MODE == 0 ? NFE : 1][N][N + 1];
+  __shared__ T Fy[MODE == 0 ? NFE : 1][N + 1][N];
+  __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+
+  const int b = (MODE == 0 ? a.block0 : 0) + blockIdx.x;
+  const int tid = threadIdx.x;
+  const int i = tid % N, j = tid / N;
+  const PhysParams P = MODE == 0 ? a.P : m.P;
+  const double* Ub = (MODE == 0 ? a.Uin : m.U) + (long)b * NFE * R;
+
+  double dt = a.dt;
+  const double beta = a.beta;
+  if (MODE == 0 && a.dt_dev) dt = *a.dt_dev;
+  const bool has_u0 = MODE == 0 && beta != 1.0;
+  const double hinv = MODE == 0 ? 1.0 / a.h[b] : 0.0;
+
+  // ---- prologue: column cells k = -2 .. 2 ------------------------------------------------
+  double ucur[NFE], uc1[NFE], uc2[NFE];
+  T cm1[NFE], c0[NFE], cp1[NFE], cp2[NFE], s0[NFE], Fzlo[NFE];
+  double emax = 0.0;
+  {
+    double u[NFE];
+    T cm2[NFE], sm1[NFE];
+    long e = col_elem<N>(L, i, j, -2);
+#pragma unroll
+    for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
+    make_prim<MIXED>(u, cm2);
+    e = col_elem<N>(L, i, j, -1);
+#pragma unroll
+    for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
+    make_prim<MIXED>(u, cm1);
+    e = col_elem<N>(L, i, j, 0);
+#pragma unroll
+    for (int l = 0; l < NFE; l++) ucur[l] = Ub[l * R + e];
+    make_prim<MIXED>(ucur, c0);
+    e = col_elem<N>(L, i, j, 1);
+#pragma unroll
+    for (int l = 0; l < NFE; l++) uc1[l] = Ub[l * R + e];
+    make_prim<MIXED>(uc1, cp1);
+    e = col_elem<N>(L, i, j, 2);
+#pragma unroll
+    for (int l = 0; l < NFE; l++) uc2[l] = Ub[l * R + e];
+    make_prim<MIXED>(uc2, cp2);
+    if constexpr (!MIXED) {
+      x64::slope(cm2, cm1, c0, sm1, P.theta);
+      x64::slope(cm1, c0, cp1, s0, P.theta);
+    } else {
+      f32::slope(cm2, cm1, c0, sm1, (float)P.theta);
+      f32::slope(cm1, c0, cp1, s0, (float)P.theta);
+    }
+    if constexpr (MODE == 0) {
+      T sl[NFE];
+      // face k = 0 from cells -1 (slope sm1) and 0 (slope s0)
+      if constexpr (!MIXED) {
+        double e2;
+        x64::kt_face<2>(P, cm1, sm1, c0, s0, Fzlo, e2);
+      } else {
+        const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+        float e2;
+        f32::kt_face<2>(PF, cm1, sm1, c0, s0, Fzlo, e2);
+      }
+      (void)sl;
+    } else {
+      // eigenvalue of face 0 (cells -2..1)
+      emax = x64::rmax(emax, face_eig<MIXED, 2>(P, cm2, cm1, c0, cp1));
+    }
+  }
+
+  // halo raw values for plane 0
+  double hraw[NFE];
+  const bool is_halo = tid < 4 * HN;
+  int hpx = 0, hpy = 0;
+  long hbase = 0, hstride = 0;
+  if (is_halo) {
+    const int face = tid / HN, r = tid % HN;
+    if (face < 2) {
+      const int jj = r / GW, t = r % GW;
+      hpx = face == XL ? t : N + GW + t;
+      hpy = jj + GW;
+    } else {
+      const int t = r / N, ii = r % N;
+      hpx = ii + GW;
+      hpy = face == YL ? t : N + GW + t;
+    }
+    hbase = L.slab(face) + r;  // plane k adds k*HN (both slab kinds are plane-major)
+    hstride = HN;
+#pragma unroll
+    for (int l = 0; l < NFE; l++) hraw[l] = Ub[l * R + hbase];
+  }
+
+  for (int k = 0; k < N; k++) {
+    // 1. stage plane k (interior from registers, halo from the prefetched raw values)
+#pragma unroll
+    for (int l = 0; l < NFE; l++) plane[l][j + GW][i + GW] = c0[l];
+    if (is_halo) {
+      T hp[NFE];
+      make_prim<MIXED>(hraw, hp);
+#pragma unroll
+      for (int l = 0; l < NFE; l++) plane[l][hpy][hpx] = hp[l];
+    }
+    // 2. prefetch: halo of plane k+1, column cell k+3, U0 at k
+    if (is_halo && k + 1 < N) {
+#pragma unroll
+      for (int l = 0; l < NFE; l++) hraw[l] = Ub[l * R + hbase + (long)(k + 1) * hstride];
+    }
+    double craw[NFE];
+    const bool have_next = (k + 3) < N + GW;
+    if (have_next) {
+      const long e = col_elem<N>(L, i, j, k + 3);
+#pragma unroll
+      for (int l = 0; l < NFE; l++) craw[l] = Ub[l * R + e];
+    }
+    double u0v[NFE];
+    if (MODE == 0 && has_u0) {
+      const double* U0b = a.U0 + (long)b * NFE * R;
+      const long e = L.interior(i, j, k);
+#pragma unroll
+      for (int l = 0; l < NFE; l++) u0v[l] = U0b[l * R + e];
+    }
+    __syncthreads();
+
+    // 3. x- and y-faces of plane k
+    for (int it = tid; it < 2 * NFX; it += NT) {
+      T pm2[NFE], pm1[NFE], p0[NFE], pp1[NFE];
+      if (it < NFX) {
+        const int fi = it % (N + 1), jj = it / (N + 1);
+        const int py = jj + GW;
+#pragma unroll
+        for (int l = 0; l < NFE; l++) {
+          pm2[l] = plane[l][py][fi];
+          pm1[l] = plane[l][py][fi + 1];
+          p0[l] = plane[l][py][fi + 2];
+          pp1[l] = plane[l][py][fi + 3];
+        }
+        if constexpr (MODE == 0) {
+          T F[NFE];
+          face_flux<MIXED, 0>(P, pm2, pm1, p0, pp1, F);
+#pragma unroll
+          for (int l = 0; l < NFE; l++) Fx[l][jj][fi] = F[l];
+        } else {
+          emax = x64::rmax(emax, face_eig<MIXED, 0>(P, pm2, pm1, p0, pp1));
+        }
+      } else {
+        const int it2 = it - NFX;
+        const int ii = it2 % N, fj = it2 / N;
+        const int px = ii + GW;
+#pragma unroll
+        for (int l = 0; l < NFE; l++) {
+          pm2[l] = plane[l][fj][px];
+          pm1[l] = plane[l][fj + 1][px];
+          p0[l] = plane[l][fj + 2][px];
+          pp1[l] = plane[l][fj + 3][px];
+        }
+        if constexpr (MODE == 0) {
+          T F[NFE];
+          face_flux<MIXED, 1>(P, pm2, pm1, p0, pp1, F);
+#pragma unroll
+          for (int l = 0; l < NFE; l++) Fy[l][fj][ii] = F[l];
+        } else {
+          emax = x64::rmax(emax, face_eig<MIXED, 1>(P, pm2, pm1, p0, pp1));
+        }
+      }
+    }
+
+    // 4. z-face k+1 (cells k, k+1 with slopes s0, sp1)
+    T sp1[NFE], Fzhi[NFE];
+    if constexpr (!MIXED)
+      x64::slope(c0, cp1, cp2, sp1, P.theta);
+    else
+      f32::slope(c0, cp1, cp2, sp1, (float)P.theta);
+    if constexpr (MODE == 0) {
+      if constexpr (!MIXED) {
+        double e2;
+        x64::kt_face<2>(P, c0, s0, cp1, sp1, Fzhi, e2);
+      } else {
+        const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+        float e2;
+        f32::kt_face<2>(PF, c0, s0, cp1, sp1, Fzhi, e2);
+      }
+    } else {
+      if constexpr (!MIXED)
+        emax = x64::rmax(emax, x64::face_eig<2>(P, c0, s0, cp1, sp1));
+      else {
+        const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+        emax = x64::rmax(emax, (double)f32::face_eig<2>(PF, c0, s0, cp1, sp1));
+      }
+    }
+    __syncthreads();
+
+    // 5. divergence (x, then y, then z) and the RK update of cell (i, j, k)
+    if constexpr (MODE == 0) {
+      double u[NFE];
+#pragma unroll
+      for (int l = 0; l < NFE; l++) {
+        double D = 0.0 - ((double)Fx[l][j][i + 1] - (double)Fx[l][j][i]) * hinv;
+        D -= ((double)Fy[l][j + 1][i] - (double)Fy[l][j][i]) * hinv;
+        D -= ((double)Fzhi[l] - (double)Fzlo[l]) * hinv;
+        D += 0.0;  // State::source is zero for the ideal gas (euler/state.cpp:19-22)
+        if (has_u0)
+          u[l] = (ucur[l] + D * dt) * beta + u0v[l] * (1.0 - beta);
+        else
+          u[l] = (ucur[l] + D * dt) * beta;
+      }
+      x64::floor_state(P, u);
+      if (!(u[RHO] > 0.0) && a.err) atomicOr(a.err, 1);
+      double* Uo = a.Uout + (long)b * NFE * R;
+      const long e = L.interior(i, j, k);
+#pragma unroll
+      for (int l = 0; l < NFE; l++) Uo[l * R + e] = u[l];
+    }
+
+    // 6. advance the column window
+#pragma unroll
+    for (int l = 0; l < NFE; l++) {
+      c0[l] = cp1[l];
+      cp1[l] = cp2[l];
+      s0[l] = sp1[l];
+      Fzlo[l] = Fzhi[l];
+      ucur[l] = uc1[l];
+      uc1[l] = uc2[l];
+    }
+    if (have_next) {
+#pragma unroll
+      for (int l = 0; l < NFE; l++) uc2[l] = craw[l];
+      make_prim<MIXED>(craw, cp2);
+    }
+  }
+
+  if constexpr (MODE == 1) {
+    // block max: wavefront reduction, then across waves through LDS
+    for (int off = 32; off > 0; off >>= 1) emax = x64::rmax(emax, __shfl_xor(emax, off, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = emax;
+    __syncthreads();
+    if (tid == 0) {
+      double v = red[0];
+      for (int w = 1; w < NT / 64; w++) v = x64::rmax(v, red[w]);
+      if (m.block_eig) m.block_eig[b] = v;
+      if (m.gmax) {
+        const double r = v / m.h[b];
+        atomicMax(m.gmax, (unsigned long long)__double_as_longlong(r));
+      }
+    }
+  }
+  (void)cm1;
+}
+
+// Ghost fill of isolated blocks: every face slab cell copies the adjacent interior cell and
+// removes inflow momentum (grid/grid_phys_bound.cpp:27-54, euler/state.cpp:45-84); the lower z face
+// mirrors with sz -> -sz when z_reflect (grid/grid_phys_bound.cpp:298-313, euler/state.cpp:94-96).
+template <int N>
+__global__ __launch_bounds__(256) void outflow_ghost_kernel(double* U, int nblocks, int z_reflect) {
+  constexpr Layout L{N, GW, NFE};
+  constexpr long R = L.rec();
+  constexpr int SC = GW * N * N;  // cells per slab
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)nblocks * 6 * SC;
+  if (gid >= total) return;
+  const int b = (int)(gid / (6 * SC));
+  const int rem = (int)(gid % (6 * SC));
+  const int face = rem / SC, r = rem % SC;
+  int ii, jj, kk, t;  // interior coordinates of the face cell and the ghost depth
+  if (face < 2) {
+    t = r % GW;
+    jj = (r / GW) % N;
+    kk = r / (GW * N);
+    ii = face == XL ? 0 : N - 1;
+  } else if (face < 4) {
+    ii = r % N;
+    t = (r / N) % GW;
+    kk = r / (GW * N);
+    jj = face == YL ? 0 : N - 1;
+  } else {
+    ii = r % N;
+    jj = (r / N) % N;
+    t = r / (N * N);
+    kk = face == ZL ? 0 : N - 1;
+  }
+  const bool reflect = z_reflect && face == ZL;
+  if (reflect) kk = GW - 1 - t;  // padded source 2G-1-t
+  double* Ub = U + (long)b * NFE * R;
+  const long src = L.interior(ii, jj, kk);
+  double u[NFE];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + src];
+  if (reflect) {
+    u[SZ] = -u[SZ];
+  } else {
+    // the boundary object's face is the node face inverted: lower node faces test v > 0
+    const int d = face >> 1;
+    const bool node_upper = face & 1;
+    const double v = (d == 2) ? u[SZ] : u[SX + d] / u[RHO];
+    const bool inflow = node_upper ? (v < 0.0) : (v > 0.0);
+    if (inflow) {
+      const double s = u[SX + d];
+      u[ET] = u[ET] - 0.5 * s * s / u[RHO];
+      u[SX + d] = 0.0;
+    }
+  }
+  const long dst = L.slab(face) + r;
+#pragma unroll
+  for (int l = 0; l < NFE; l++) Ub[l * R + dst] = u[l];
+}
+
+// padded SoA <-> records (one thread per record element)
+template <int N>
+__global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ pad, double* __restrict__ rec,
+                                                  long nrec_elems, int to_rec) {
+  constexpr Layout L{N, GW, NFE};
+  constexpr long R = L.rec();
+  constexpr int PW = N + 2 * GW;
+  constexpr long P3 = (long)PW * PW * PW;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nrec_elems) return;
+  const long bf = gid / R;  // block*NF + field
+  const int e = (int)(gid % R);
+  int pi, pj, pk;
+  if (e < N * N * N) {
+    pi = e % N + GW;
+    pj = (e / N) % N + GW;
+    pk = e / (N * N) + GW;
+  } else {
+    const int s = (e - N * N * N) / (GW * N * N);
+    const int r = (e - N * N * N) % (GW * N * N);
+    const int lo = (s & 1) ? N + GW : 0;
+    if (s < 2) {
+      pi = lo + r % GW;
+      pj = (r / GW) % N + GW;
+      pk = r / (GW * N) + GW;
+    } else if (s < 4) {
+      pi = r % N + GW;
+      pj = lo + (r / N) % GW;
+      pk = r / (GW * N) + GW;
+    } else {
+      pi = r % N + GW;
+      pj = (r / N) % N + GW;
+      pk = lo + r / (N * N);
+    }
+  }
+  const long pidx = bf * P3 + ((long)pk * PW + pj) * PW + pi;
+  if (to_rec)
+    rec[gid] = pad[pidx];
+  else
+    const_cast<double*>(pad)[pidx] = rec[gid];
+}
+
+__global__ void dt_from_gmax_kernel(const unsigned long long* gmax, double cfl, double scale, double* dt) {
+  const double m = __longlong_as_double((long long)*gmax);
+  double dtinv = m;
+  if (dtinv == 0.0) dtinv = 1.0e-10;
+  dt[0] = (1.0 / dtinv) * cfl * scale;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+template <int N>
+static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, hipStream_t s) {
+  MaxEigArgs m{};
+  if (a.nblocks <= 0) return hipSuccess;
+  if (mixed)
+    kt_sweep_kernel<N, true, 0><<<a.nblocks, N * N, 0, s>>>(a, m);
+  else
+    kt_sweep_kernel<N, false, 0><<<a.nblocks, N * N, 0, s>>>(a, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, hipStream_t s) {
+  if (n == 8) return launch_substep_n<8>(a, mixed, s);
+  if (n == 16) return launch_substep_n<16>(a, mixed, s);
+  return hipErrorInvalidValue;
+}
+
+template <int N>
+static hipError_t launch_maxeig_n(const MaxEigArgs& m, bool mixed, hipStream_t s) {
+  SubstepArgs a{};
+  if (m.nblocks <= 0) return hipSuccess;
+  if (mixed)
+    kt_sweep_kernel<N, true, 1><<<m.nblocks, N * N, 0, s>>>(a, m);
+  else
+    kt_sweep_kernel<N, false, 1><<<m.nblocks, N * N, 0, s>>>(a, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, hipStream_t s) {
+  if (n == 8) return launch_maxeig_n<8>(m, mixed, s);
+  if (n == 16) return launch_maxeig_n<16>(m, mixed, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_outflow_ghosts(int n, double* U, int nblocks, int z_reflect, hipStream_t s) {
+  const long total = (long)nblocks * 6 * GW * n * n;
+  const unsigned grid = (unsigned)((total + 255) / 256);
+  if (total == 0) return hipSuccess;
+  if (n == 8)
+    outflow_ghost_kernel<8><<<grid, 256, 0, s>>>(U, nblocks, z_reflect);
+  else if (n == 16)
+    outflow_ghost_kernel<16><<<grid, 256, 0, s>>>(U, nblocks, z_reflect);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(int n, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s) {
+  const unsigned grid = (unsigned)((nrec_elems + 255) / 256);
+  if (nrec_elems == 0) return hipSuccess;
+  if (n == 8)
+    pack_kernel<8><<<grid, 256, 0, s>>>(pad, rec, nrec_elems, to_rec ? 1 : 0);
+  else if (n == 16)
+    pack_kernel<16><<<grid, 256, 0, s>>>(pad, rec, nrec_elems, to_rec ? 1 : 0);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_dt_from_gmax(const unsigned long long* gmax, double cfl, double scale, double* dt,
+                               hipStream_t s) {
+  dt_from_gmax_kernel<<<1, 1, 0, s>>>(gmax, cfl, scale, dt);
+  return hipGetLastError();
+}
+
+}  // namespace amrh

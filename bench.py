@@ -1,0 +1,257 @@
+#!/usr/bin/env python3
+"""Throughput of the hydro right-hand side + RK3 update on one MI355X.
+
+Workload: the batched flat stencil -- 4096 independent blocks of 16^3 interior cells plus a 2-cell
+ghost layer, 6 conserved fields, lognormal density with 5% planar jumps (workloads.flat_batch).
+A "step" is one full RK3 time step over the batch: CFL wavespeed reduction to a device dt, then
+three fused substeps (reconstruction + KT flux + divergence + RK combine + floor), each preceded by
+the outflow ghost refresh (grid/grid_phys_bound.cpp semantics).  `value` is leaf cell-updates/s
+(3 substeps x cells / wall time of the whole step, ghost refresh included) of the mixed-precision
+form; the FP64 form is measured the same way and reported beside it.
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import statistics
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "3d-eulerian-amr_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "leaf cell-updates/s (KT RHS) mixed vs FP64 at ≤1e-6 rel L1; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+BETAS = (1.0, 0.25, 2.0 / 3.0)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
+    cfg = amrhydro.euler_config(n=n, precision=precision)
+    B = amrhydro.Batch(cfg, nb, 1.0 / n)
+    stream = torch.cuda.current_stream()
+    B.set_stream(stream.cuda_stream)
+    B.upload_device(U_dev_ptr)
+    dt = torch.zeros(1, dtype=torch.float64, device="cuda")
+    B.fill_outflow_ghosts()
+    cells = nb * n ** 3
+
+    def step(events=None):
+        B.dt_device(0.5, dt.data_ptr())
+        B.store()
+        for s, beta in enumerate(BETAS):
+            if s > 0:
+                B.fill_outflow_ghosts()
+            if events is not None:
+                events.append((torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True), beta))
+                events[-1][0].record(stream)
+            B.substep_dev(dt.data_ptr(), beta)
+            if events is not None:
+                events[-1][1].record(stream)
+        B.fill_outflow_ghosts()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(events)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    k_ms = [(a.elapsed_time(b), beta) for a, b, beta in events]
+    bytes_one = B.substep_bytes(True)
+    bytes_other = B.substep_bytes(False)
+    tot_bytes = sum(bytes_one if beta == 1.0 else bytes_other for _, beta in k_ms)
+    tot_ms = sum(t for t, _ in k_ms)
+    flags = B.error_flags()
+    res = {
+        "cell_updates_per_s": 3 * steps * cells / wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "substep_kernel_ms_avg": tot_ms / len(k_ms),
+        "substep_kernel_ms_beta1": statistics.mean(t for t, b in k_ms if b == 1.0),
+        "substep_kernel_ms_beta_ne1": statistics.mean(t for t, b in k_ms if b != 1.0),
+        "rhs_cell_updates_per_s": len(k_ms) * cells / (tot_ms * 1e-3),
+        "achieved_GBs": tot_bytes / (tot_ms * 1e-3) / 1e9,
+        "bytes_per_launch_avg": tot_bytes / len(k_ms),
+        "device_error_flags": flags,
+    }
+    B.close()
+    return res
+
+
+def accuracy(amrhydro, torch, U_dev_ptr, nb, n, steps):
+    """Mixed vs FP64 GPU after `steps` RK3 steps on the full workload (same dt sequence: the FP64
+    form's CFL dt).  The FP64 form is pinned to the CPU oracle at 1e-12 by the test suite."""
+    forms = {}
+    for prec in (amrhydro.AMRH_FP64, amrhydro.AMRH_MIXED):
+        B = amrhydro.Batch(amrhydro.euler_config(n=n, precision=prec), nb, 1.0 / n)
+        B.set_stream(torch.cuda.current_stream().cuda_stream)
+        B.upload_device(U_dev_ptr)
+        B.fill_outflow_ghosts()
+        forms[prec] = B
+    B64, B32 = forms[amrhydro.AMRH_FP64], forms[amrhydro.AMRH_MIXED]
+    for s in range(steps):
+        m = B64.max_speed()
+        dt = 0.5 * 0.4 / m
+        for B in (B64, B32):
+            B.rk_step(dt, 3)
+            B.fill_outflow_ghosts()
+    g = 2
+    out = {}
+    buf = torch.empty(nb * 6 * (n + 4) ** 3, dtype=torch.float64, device="cuda")
+    res = []
+    for B in (B64, B32):
+        B.download_device(buf.data_ptr())
+        torch.cuda.synchronize()
+        res.append(buf.view(nb, 6, n + 4, n + 4, n + 4)[:, :, g:-g, g:-g, g:-g].clone())
+    a, b = res[1], res[0]
+    rel = ((a - b).abs().sum(dim=(0, 2, 3, 4)) / b.abs().sum(dim=(0, 2, 3, 4))).cpu().numpy()
+    for B in (B64, B32):
+        B.close()
+    out["rel_l1_per_field"] = [float(x) for x in rel]
+    out["steps"] = steps
+    return out
+
+
+def cpu_baseline(n, target_s=8.0):
+    """The FP64 CPU oracle (OpenMP over blocks) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    import workloads
+
+    cfg = orc.euler_cfg(n=n)
+    threads = os.environ.get("OMP_NUM_THREADS")
+    nb = 32
+    U = workloads.flat_batch(nb, n=n, seed=5)
+    orc.outflow_ghosts(cfg, U)
+    h = np.full(nb, 1.0 / n)
+    dt = 0.2 * orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
+    t0 = time.perf_counter()
+    orc.batch_substep(cfg, U, U.copy(), h, dt, 1.0)
+    t1 = time.perf_counter()
+    rate = nb * n ** 3 / (t1 - t0)
+    nb = max(32, min(4096, int(rate * target_s / 3 / n ** 3)))
+    U = workloads.flat_batch(nb, n=n, seed=6)
+    orc.outflow_ghosts(cfg, U)
+    h = np.full(nb, 1.0 / n)
+    U0 = U.copy()
+    times = []
+    for beta in BETAS:
+        orc.outflow_ghosts(cfg, U)
+        t0 = time.perf_counter()
+        orc.batch_substep(cfg, U, U0, h, dt, beta)
+        times.append(time.perf_counter() - t0)
+    rate = 3 * nb * n ** 3 / sum(times)
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+        cores = gomp.omp_get_max_threads()
+    except OSError:
+        cores = int(threads) if threads else os.cpu_count()
+    return {"value": rate, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": "%d blocks of 16^3 (+2 ghosts) x one RK3 step (3 substeps), FP64 oracle, "
+                      "OpenMP over blocks, %.1f s" % (nb, sum(times))}
+
+
+def load_traffic(workload_key):
+    p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload_key)
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nblocks", type=int, default=4096)
+    ap.add_argument("--n", type=int, default=16)
+    ap.add_argument("--accuracy-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["mixed", "fp64", "both"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import amrhydro
+    import workloads
+
+    amrhydro.lib()
+    n, nb = args.n, args.nblocks
+    log("generating workload: %d blocks of %d^3 (+2 ghosts)" % (nb, n))
+    U = workloads.flat_batch(nb, n=n)
+    U_dev = torch.from_numpy(U).to("cuda")
+    del U
+    torch.cuda.synchronize()
+
+    results = {}
+    forms = {"mixed": amrhydro.AMRH_MIXED, "fp64": amrhydro.AMRH_FP64}
+    for name in (["mixed", "fp64"] if args.only == "both" else [args.only]):
+        log("timing %s form: %d RK3 steps after %d warmup" % (name, args.steps, args.warmup))
+        results[name] = run_form(amrhydro, torch, U_dev.data_ptr(), nb, n, forms[name], args.steps,
+                                 args.warmup)
+        log("  %s: %.3e cell-updates/s, substep kernel %.3f ms, %.0f GB/s"
+            % (name, results[name]["cell_updates_per_s"], results[name]["substep_kernel_ms_avg"],
+               results[name]["achieved_GBs"]))
+
+    acc = None
+    if args.accuracy_steps > 0 and args.only == "both":
+        log("accuracy: %d RK3 steps mixed vs FP64" % args.accuracy_steps)
+        acc = accuracy(amrhydro, torch, U_dev.data_ptr(), nb, n, args.accuracy_steps)
+        log("  rel-L1 per field: %s" % acc["rel_l1_per_field"])
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (FP64 oracle)")
+        cpu = cpu_baseline(n)
+        log("  %.3e cell-updates/s on %d threads" % (cpu["value"], cpu["cores"]))
+
+    head = results.get("mixed") or results.get("fp64")
+    head_name = "mixed" if "mixed" in results else "fp64"
+    achieved = head["achieved_GBs"]
+    out = {
+        "metric": METRIC,
+        "value": head["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 flux / f64 accumulate" if head_name == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "flat batch: %d blocks x %d^3 interior (+2 ghost), 6 fields, RK3 "
+                               "(lognormal rho, 5%% planar jumps)" % (nb, n),
+                   "form": head_name, "cells": nb * n ** 3, "parallelism": "single GPU"},
+        "forms": results,
+        "mixed_over_fp64": (results["mixed"]["cell_updates_per_s"] / results["fp64"]["cell_updates_per_s"]
+                            if "mixed" in results and "fp64" in results else None),
+        "accuracy_mixed_vs_fp64": acc,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic("flat%d_%s" % (n, head_name)),
+                     "kernel": "kt_sweep_kernel<%d,%s,0>" % (n, "true" if head_name == "mixed" else "false"),
+                     "bytes_per_launch": head["bytes_per_launch_avg"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+/* amrhydro -- MI355X (gfx950) hydro right-hand side of an octree-AMR Eulerian code.
+ *
+ * C ABI between host code (the tree / driver) and the HIP kernels.  Plain pointers and sizes; no
+ * exceptions; every entry point returns 0 on success or an AMRH_E* code (amrh_last_error() gives
+ * the message).  One context = one device stream; calls on a context are asynchronous with
+ * respect to the host except where stated, and a context must not be used from two host threads
+ * at once.  The device owns the state buffers; the host owns the tree.
+ *
+ * Each entry point replaces a method of the reference's Grid / GridNode interface
+ * (grid/grid.h:81-96, grid/grid_node.h:47-80), applied to every block of the batch at once.
+ *
+ * Host-side array layout ("padded SoA"): per block, per field, a GNX^3 cube, GNX = n + 2g, x
+ * fastest: element (b, f, i, j, k) at ((b*nf + f)*GNX + k)*GNX*GNX + j*GNX + i.  It is the
+ * reference's Array3d<State,GNX,GNX,GNX> (array3d.h:11) with the fields split out.  Only the
+ * interior and the six face ghost slabs are meaningful (the reference never reads edge or corner
+ * ghosts, grid/grid_node.cpp:59-112); downloads write zeros there.
+ */
+#ifndef AMRHYDRO_H
+#define AMRHYDRO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMRH_OK 0
+#define AMRH_EINVAL 1   /* bad argument */
+#define AMRH_EHIP 2     /* HIP runtime error */
+#define AMRH_ESTATE 3   /* call out of order (e.g. no tree, no upload) */
+#define AMRH_EDEVICE 4  /* device-side error flag raised (see amrh_error) */
+
+#define AMRH_PHYS_EULER 0 /* ideal gas, fields rho sx sy sz E tau (euler/state.cpp:9-17) */
+
+#define AMRH_RECON_MINMOD 0 /* minmod-theta slopes (reconstruct.cpp:60-71) */
+
+#define AMRH_FP64 0  /* FP64 end to end; matches the FP64 CPU path to roundoff */
+#define AMRH_MIXED 1 /* FP32 reconstruction / fluxes / wavespeeds, FP64 accumulation + RK + floor */
+
+/* device error flags (amrh_error) */
+#define AMRH_DERR_NEG_DENSITY 1 /* a face or cell state with rho <= 0 reached the flux */
+
+typedef struct {
+  int n;         /* interior cells per block edge; 8 or 16 (euler/euler_defs.h:27) */
+  int g;         /* ghost width; 2 (euler/euler_defs.h:25) */
+  int nf;        /* fields; 6 for AMRH_PHYS_EULER (euler/state.h:16) */
+  int physics;   /* AMRH_PHYS_* */
+  int recon;     /* AMRH_RECON_* */
+  int precision; /* AMRH_FP64 or AMRH_MIXED */
+  int z_reflect; /* lower z boundary reflects (euler/euler_defs.h:21) */
+  double gamma;      /* 5/3 (euler/state.cpp:9) */
+  double rho_floor;  /* 1e-20 (euler/state.cpp:10) */
+  double ei_floor;   /* 1e-20 (euler/state.cpp:11) */
+  double theta;      /* 1.3 (euler/euler_defs.h:20) */
+  double cfl;        /* 0.4 (euler/euler_defs.h:46) */
+  double grid_dim;   /* half edge of the root domain (grid/grid.cpp:70,512-526) */
+} amrh_config;
+
+typedef struct amrh_ctx amrh_ctx;
+
+/* Create a context for `nblocks` blocks on the current HIP device.  Allocates two state buffers
+ * (the current stage and the RK base U0), per-block metadata and reduction scratch.
+ * Replaces the storage of Grid (grid/grid.h:26-32) for every node of the tree. */
+int amrh_create(const amrh_config* cfg, int nblocks, amrh_ctx** out);
+void amrh_destroy(amrh_ctx* ctx);
+const char* amrh_last_error(void);
+
+/* Run all further work of this context on `stream` (a hipStream_t); NULL = the context's own. */
+int amrh_set_stream(amrh_ctx* ctx, void* stream);
+/* Block until all work of the context has finished; returns a pending HIP error if any. */
+int amrh_sync(amrh_ctx* ctx);
+
+/* Per-block cell width h (Grid::set_dx, grid/grid.cpp:536-539).  Host array of nblocks. */
+int amrh_set_block_h(amrh_ctx* ctx, const double* h);
+
+/* Copy the padded SoA state of every block host->device (the whole current stage).
+ * Replaces direct writes into Grid::U (grid/grid.cpp:153-159). */
+int amrh_upload(amrh_ctx* ctx, const double* U_padded);
+/* Same, from a device pointer holding the padded SoA layout. */
+int amrh_upload_device(amrh_ctx* ctx, const void* U_padded_dev);
+/* Copy the current stage device->host, padded SoA (interior + face ghost slabs). */
+int amrh_download(amrh_ctx* ctx, double* U_padded);
+int amrh_download_device(amrh_ctx* ctx, void* U_padded_dev);
+
+/* U0 = U for every block (Grid::store, grid/grid.cpp:173-185).  No copy is made: the current
+ * buffer becomes the RK base and the next substep writes the other buffer. */
+int amrh_store(amrh_ctx* ctx);
+
+/* Ghost fill of isolated blocks whose six neighbours are physical outflow boundaries
+ * (GridPhysBound, grid/grid_phys_bound.cpp:27-54 with State::enforce_outflow,
+ * euler/state.cpp:45-84; z-mirror on the lower z face when z_reflect). */
+int amrh_fill_outflow_ghosts(amrh_ctx* ctx);
+
+/* One RK substep over every block, fused: clear_difs, compute_{x,y,z}_flux, sum_{x,y,z}_difs,
+ * add_difs with floor (grid/grid.cpp:201-344, 406-420):
+ *   U <- (U + dt*D(U))*beta + U0*(1-beta),  D = -sum_d (F_{d,i+1}-F_{d,i})/h.
+ * Ghost slabs of the current stage must be filled. */
+int amrh_substep(amrh_ctx* ctx, double dt, double beta);
+/* Same, with dt read from device memory when the kernel runs (e.g. written by amrh_dt_device, or
+ * replayed from a captured graph); dt_dev points to one double in device memory. */
+int amrh_substep_dev(amrh_ctx* ctx, const double* dt_dev, double beta);
+
+/* Grid::max_dt scan (grid/grid.cpp:422-473): per block, the maximum over all interior faces of
+ * the three sweeps of |v_d| + c_s of both reconstructed face states.  Writes nblocks doubles to
+ * the host array `max_eig` (synchronous). */
+int amrh_block_max_eig(amrh_ctx* ctx, double* max_eig);
+/* Device reduction of max_b(max_eig_b / h_b) (a wavefront + atomic max over the bit pattern of
+ * non-negative doubles, order independent).  Synchronous; writes one double. */
+int amrh_max_speed(amrh_ctx* ctx, double* max_eig_over_h);
+/* dt = cfl / max_b(max_eig_b/h_b) computed on the device into dt_dev[0] (no host round trip),
+ * scaled by `scale` (e.g. MAXINITDT at t = 0, euler/hydro_driver.cpp:169-173). */
+int amrh_dt_device(amrh_ctx* ctx, double scale, double* dt_dev);
+
+/* Device error flags raised since the last call (AMRH_DERR_*); clears them.  Synchronous. */
+int amrh_error(amrh_ctx* ctx, int* flags);
+
+/* Bytes of HBM one substep over the batch moves by the algorithmic count (interior + face
+ * ghost slabs read, interior written, U0 read when beta != 1). */
+double amrh_substep_bytes(const amrh_ctx* ctx, int beta_is_one);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,40 @@
+"""The C-ABI library loads (no GPU call is made) and exports every entry point include/*.h
+declares; the Python binding covers the same surface."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import orc
+
+ROOT = orc.ROOT
+PKG = os.path.join(ROOT, "3d-eulerian-amr_amd")
+HEADER = os.path.join(ROOT, "include", "amrhydro.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(amrh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "amrh_substep" in names and "amrh_create" in names and len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    so = os.path.join(PKG, "libamrhydro.so")
+    if not os.path.exists(so):
+        pytest.fail("libamrhydro.so not built (run __graft_entry__.build())")
+    L = C.CDLL(so)
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    import sys
+    sys.path.insert(0, PKG)
+    import amrhydro
+    assert set(declared_functions()) == set(amrhydro.SIGNATURES)

@@ -1,0 +1,174 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle on the same seeded
+inputs.  FP64 form: <= 1e-12 relative L1 per conserved field (the expected difference is libm vs
+device pow, one ulp on a few cells).  Mixed form: error per field stated, density <= 1e-6."""
+import numpy as np
+import pytest
+
+import cases
+import orc
+from gpu_util import amrhydro, rel_l1, say
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def make_blocks(n, nb, seed):
+    U = cases.c2_fast(nb, n=n, g=2, seed=seed)
+    orc.outflow_ghosts(orc.euler_cfg(n=n), U)
+    return U
+
+
+def gpu_batch(n, nb, h, precision=amrhydro.AMRH_FP64, z_reflect=0):
+    cfg = amrhydro.euler_config(n=n, precision=precision, z_reflect=z_reflect)
+    return amrhydro.Batch(cfg, nb, h)
+
+
+@pytest.mark.parametrize("n", [8, 16])
+@pytest.mark.parametrize("z_reflect", [0, 1])
+def test_outflow_ghosts_bitexact(n, z_reflect):
+    say("outflow ghosts n=%d z_reflect=%d" % (n, z_reflect))
+    nb = 6
+    U = cases.c2_fast(nb, n=n, seed=3)
+    ref = U.copy()
+    orc.outflow_ghosts(orc.euler_cfg(n=n, z_reflect=z_reflect), ref)
+    B = gpu_batch(n, nb, 0.1, z_reflect=z_reflect)
+    B.upload(U)
+    B.fill_outflow_ghosts()
+    out = B.download()
+    g = 2
+    G = n + 2 * g
+    # compare every face-slab cell and the interior; edges/corners are not represented
+    mask = np.zeros((G, G, G), bool)
+    inner = slice(g, G - g)
+    mask[inner, inner, inner] = True
+    mask[:, inner, inner] = True
+    mask[inner, :, inner] = True
+    mask[inner, inner, :] = True
+    np.testing.assert_array_equal(out[:, :, mask], ref[:, :, mask])
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_substep_fp64_matches_oracle(n):
+    nb = 12 if n == 16 else 40
+    say("fp64 substeps n=%d nb=%d" % (n, nb))
+    U = make_blocks(n, nb, seed=10 + n)
+    h = np.full(nb, 1.0 / n)
+    cfg = orc.euler_cfg(n=n)
+    dt = 0.2 * orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
+    B = gpu_batch(n, nb, h)
+    B.upload(U)
+    B.store()
+    ref = U.copy()
+    ref0 = U.copy()
+    for beta in (1.0, 0.25, 2.0 / 3.0):
+        orc.outflow_ghosts(cfg, ref)
+        orc.batch_substep(cfg, ref, ref0, h, dt, beta)
+        B.fill_outflow_ghosts()
+        B.substep(dt, beta)
+        out = B.download()
+        e = rel_l1(out, ref, 2)
+        same = np.mean(out[:, :, 2:-2, 2:-2, 2:-2] == ref[:, :, 2:-2, 2:-2, 2:-2])
+        say("  beta=%.4f rel-L1 per field %s  bit-identical %.6f" % (beta, np.array2string(e, precision=2), same))
+        assert np.all(e <= FP64_TOL), e
+        assert B.error_flags() == 0
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_block_max_eig_matches_oracle(n):
+    nb = 10
+    U = make_blocks(n, nb, seed=20 + n)
+    cfg = orc.euler_cfg(n=n)
+    ref = orc.batch_max_eig(cfg, U)
+    B = gpu_batch(n, nb, 1.0 / n)
+    B.upload(U)
+    got = B.block_max_eig()
+    np.testing.assert_allclose(got, ref, rtol=1e-14, atol=0)
+    m = B.max_speed()
+    assert abs(m - (ref * n).max()) <= 1e-14 * m
+
+
+def test_rk3_fp64_trajectory_matches_oracle():
+    """C2-shaped batch (n=16, 5% planar jumps), 3 full RK3 steps with outflow ghost refresh."""
+    n, nb = 16, 16
+    U = make_blocks(n, nb, seed=77)
+    h = np.full(nb, 1.0 / n)
+    cfg = orc.euler_cfg(n=n)
+    dt = orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
+    ref = U.copy()
+    orc.rk_steps(cfg, ref, h, dt, 3, 3)
+    B = gpu_batch(n, nb, h)
+    B.upload(U)
+    for _ in range(3):
+        B.rk_step(dt, 3)
+    B.fill_outflow_ghosts()
+    out = B.download()
+    e = rel_l1(out, ref, 2)
+    say("RK3 x3 fp64 rel-L1 %s" % np.array2string(e, precision=2))
+    assert np.all(e <= FP64_TOL), e
+
+
+def test_mixed_precision_error_after_rk3_steps():
+    """Mixed form (FP32 reconstruction/flux/wavespeed, FP64 accumulate): 20 RK3 steps of a C2-shaped
+    batch against the FP64 oracle.  Density rel-L1 must stay <= 1e-6."""
+    n, nb, steps = 16, 8, 20
+    U = make_blocks(n, nb, seed=91)
+    h = np.full(nb, 1.0 / n)
+    cfg = orc.euler_cfg(n=n)
+    dt = orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
+    ref = U.copy()
+    orc.rk_steps(cfg, ref, h, dt, 3, steps)
+    B = gpu_batch(n, nb, h, precision=amrhydro.AMRH_MIXED)
+    B.upload(U)
+    for _ in range(steps):
+        B.rk_step(dt, 3)
+    B.fill_outflow_ghosts()
+    out = B.download()
+    e = rel_l1(out, ref, 2)
+    say("mixed %d RK3 steps rel-L1 per field %s" % (steps, np.array2string(e, precision=2)))
+    assert e[0] <= 1e-6, e
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_uniform_state_is_stationary(precision):
+    """Identical face fluxes cancel exactly: a uniform state does not change."""
+    n, nb = 8, 3
+    G = n + 4
+    rho, v, p = 0.7, (0.3, -0.2, 0.1), 0.4
+    cell = cases.prim_to_con(np.array(rho), *[np.array(x) for x in v], np.array(p))
+    orc.lib().orc_floor(orc.C.byref(orc.euler_cfg()), orc.dptr(cell))
+    U = np.empty((nb, 6, G, G, G))
+    U[:] = cell[None, :, None, None, None]
+    B = gpu_batch(n, nb, 0.125, precision=precision)
+    B.upload(U)
+    B.store()
+    B.substep(1e-3, 1.0)
+    out = B.download()
+    np.testing.assert_array_equal(out[:, :, 2:-2, 2:-2, 2:-2], U[:, :, 2:-2, 2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_periodic_block_conserves_mass_and_energy(precision):
+    """Ghosts filled periodically from the block's own interior make each block closed: the flux
+    divergence telescopes, so total mass and energy change only by roundoff."""
+    n, nb, g = 16, 4, 2
+    U = cases.c2_fast(nb, n=n, seed=5)
+    I = U[:, :, g:-g, g:-g, g:-g]
+    U[:, :, :g, g:-g, g:-g] = I[:, :, -g:, :, :]
+    U[:, :, -g:, g:-g, g:-g] = I[:, :, :g, :, :]
+    U[:, :, g:-g, :g, g:-g] = I[:, :, :, -g:, :]
+    U[:, :, g:-g, -g:, g:-g] = I[:, :, :, :g, :]
+    U[:, :, g:-g, g:-g, :g] = I[:, :, :, :, -g:]
+    U[:, :, g:-g, g:-g, -g:] = I[:, :, :, :, :g]
+    B = gpu_batch(n, nb, 1.0 / n, precision=precision)
+    B.upload(U)
+    B.store()
+    B.substep(2e-3, 1.0)
+    out = B.download()
+    for f in (0, 4):
+        before = U[:, f, g:-g, g:-g, g:-g].sum(axis=(1, 2, 3))
+        after = out[:, f, g:-g, g:-g, g:-g].sum(axis=(1, 2, 3))
+        scale = np.abs(U[:, f, g:-g, g:-g, g:-g]).sum(axis=(1, 2, 3))
+        rel = np.abs(after - before) / scale
+        say("field %d conservation defect %s" % (f, rel))
+        assert np.all(rel < 1e-13), rel
