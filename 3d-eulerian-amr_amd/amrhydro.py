@@ -18,6 +18,7 @@ AMRH_FP64 = 0
 AMRH_MIXED = 1
 
 DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
 
 
 class Config(C.Structure):
@@ -52,9 +53,16 @@ SIGNATURES = {
     "amrh_substep_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
     "amrh_block_max_eig": (C.c_int, [C.c_void_p, DP]),
     "amrh_max_speed": (C.c_int, [C.c_void_p, DP]),
-    "amrh_dt_device": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
+    "amrh_dt_device": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
     "amrh_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "amrh_substep_bytes": (C.c_double, [C.c_void_p, C.c_int]),
+    "amrh_tree_set": (C.c_int, [C.c_void_p, IP, IP]),
+    "amrh_tree_fill_ghosts": (C.c_int, [C.c_void_p]),
+    "amrh_tree_substep": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "amrh_tree_substep_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
+    "amrh_tree_inject": (C.c_int, [C.c_void_p]),
+    "amrh_tree_max_dt": (C.c_int, [C.c_void_p, DP]),
+    "amrh_tree_flow_off": (C.c_int, [C.c_void_p, DP]),
 }
 
 _lib = None
@@ -160,8 +168,8 @@ class Batch:
         _ck(lib().amrh_max_speed(self.ctx, C.byref(v)))
         return v.value
 
-    def dt_device(self, scale, ptr):
-        _ck(lib().amrh_dt_device(self.ctx, scale, C.c_void_p(ptr)))
+    def dt_device(self, scale, growth, ptr):
+        _ck(lib().amrh_dt_device(self.ctx, scale, growth, C.c_void_p(ptr)))
 
     def sync(self):
         _ck(lib().amrh_sync(self.ctx))
@@ -173,6 +181,53 @@ class Batch:
 
     def substep_bytes(self, beta_is_one):
         return lib().amrh_substep_bytes(self.ctx, 1 if beta_is_one else 0)
+
+    def cfl_dt(self, last_dt=None, maxinitdt=1e-2, maxdtinc=1.25):
+        """Host-side dt of HydroDriver::step_to_time (euler/hydro_driver.cpp:168-173): the batch's
+        CFL dt, times MAXINITDT on the first step, capped at MAXDTINC * the last dt after."""
+        m = self.max_speed()
+        dtinv = m if m != 0.0 else 1.0e-10
+        dt = (1.0 / dtinv) * self.cfg.cfl
+        if last_dt is None:
+            return maxinitdt * dt
+        return min(maxdtinc * last_dt, dt)
+
+    # ---- octree ------------------------------------------------------------------------------
+    def tree_set(self, parent, child):
+        p = np.ascontiguousarray(parent, dtype=np.int32)
+        c = np.ascontiguousarray(child, dtype=np.int32)
+        assert len(p) == self.nb and len(c) == self.nb
+        _ck(lib().amrh_tree_set(self.ctx, p.ctypes.data_as(IP), c.ctypes.data_as(IP)))
+
+    def tree_fill_ghosts(self):
+        _ck(lib().amrh_tree_fill_ghosts(self.ctx))
+
+    def tree_substep(self, dt, beta):
+        _ck(lib().amrh_tree_substep(self.ctx, dt, beta))
+
+    def tree_substep_dev(self, dt_ptr, beta):
+        _ck(lib().amrh_tree_substep_dev(self.ctx, C.c_void_p(dt_ptr), beta))
+
+    def tree_inject(self):
+        _ck(lib().amrh_tree_inject(self.ctx))
+
+    def tree_max_dt(self):
+        v = C.c_double()
+        _ck(lib().amrh_tree_max_dt(self.ctx, C.byref(v)))
+        return v.value
+
+    def tree_flow_off(self):
+        fo = np.zeros(self.cfg.nf)
+        _ck(lib().amrh_tree_flow_off(self.ctx, _dp(fo)))
+        return fo
+
+    def tree_step(self, dt, rk=3):
+        """HydroDriver::step on the octree (euler/hydro_driver.cpp:231-254, regrid excluded)."""
+        betas = {1: (1.0,), 2: (1.0, 0.5), 3: (1.0, 0.25, 2.0 / 3.0)}[rk]
+        self.store()
+        for b in betas:
+            self.tree_substep(dt, b)
+        self.tree_fill_ghosts()
 
     def rk_step(self, dt, rk=3, fill_ghosts=True):
         """HydroDriver::step for a flat batch (euler/hydro_driver.cpp:231-249)."""

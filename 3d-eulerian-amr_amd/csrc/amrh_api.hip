@@ -6,34 +6,19 @@
 #include <cstring>
 #include <string>
 
+#include "amrh_ctx.h"
 #include "amrh_internal.h"
 #include "amrh_layout.h"
 #include "amrhydro.h"
 
 using namespace amrh;
 
-struct amrh_ctx {
-  amrh_config cfg;
-  int nb = 0;
-  Layout L{8, 2, 6};
-  double* buf[2] = {nullptr, nullptr};
-  int cur = 0;     // buffer holding the current stage
-  int u0buf = -1;  // buffer holding the RK base, -1 = none stored
-  double* h_dev = nullptr;
-  double* eig_dev = nullptr;
-  unsigned long long* gmax_dev = nullptr;
-  int* err_dev = nullptr;
-  double* pad_stage = nullptr;  // padded SoA staging for upload/download
-  hipStream_t stream = nullptr;
-  hipStream_t own_stream = nullptr;
-  bool have_h = false, have_u = false;
-};
+namespace amrh {
 
 namespace {
-
 thread_local std::string g_err;
+}
 
-int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -44,11 +29,19 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                     \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess) return fail(AMRH_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
+PhysParams params(const amrh_config& c) {
+  PhysParams P;
+  P.gamma = c.gamma;
+  P.rho_floor = c.rho_floor;
+  P.ei_floor = c.ei_floor;
+  P.theta = c.theta;
+  P.cfl = c.cfl;
+  return P;
+}
+
+}  // namespace amrh
+
+namespace {
 
 long padded_elems(const amrh_ctx* c) {
   const long P = c->cfg.n + 2 * c->cfg.g;
@@ -67,16 +60,6 @@ int check_cfg(const amrh_config* c) {
     return fail(AMRH_EINVAL, "precision %d unknown", c->precision);
   if (!(c->gamma > 1.0)) return fail(AMRH_EINVAL, "gamma must exceed 1");
   return AMRH_OK;
-}
-
-PhysParams params(const amrh_config& c) {
-  PhysParams P;
-  P.gamma = c.gamma;
-  P.rho_floor = c.rho_floor;
-  P.ei_floor = c.ei_floor;
-  P.theta = c.theta;
-  P.cfl = c.cfl;
-  return P;
 }
 
 }  // namespace
@@ -103,6 +86,7 @@ int amrh_create(const amrh_config* cfg, int nblocks, amrh_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->err_dev, sizeof(int));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMemset(c->err_dev, 0, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->gmax_dev, 0, sizeof(unsigned long long) * 2);
   if (e == hipSuccess) e = hipMemset(c->buf[0], 0, rec_bytes);
   if (e == hipSuccess) e = hipMemset(c->buf[1], 0, rec_bytes);
   if (e != hipSuccess) {
@@ -125,6 +109,10 @@ void amrh_destroy(amrh_ctx* c) {
   if (c->err_dev) (void)hipFree(c->err_dev);
   if (c->pad_stage) (void)hipFree(c->pad_stage);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  if (c->tree) {
+    c->tree->free_dev();
+    delete c->tree;
+  }
   delete c;
 }
 
@@ -196,6 +184,7 @@ int amrh_store(amrh_ctx* c) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
   c->u0buf = c->cur;
+  if (c->tree) HIPCHK(launch_ledger_store(c->tree->fo_dev, c->stream));  // FO0 = FO
   return AMRH_OK;
 }
 
@@ -206,7 +195,11 @@ int amrh_fill_outflow_ghosts(amrh_ctx* c) {
   return AMRH_OK;
 }
 
-static int substep_impl(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
+}  // extern "C"
+
+// The fused substep over every block; with `tree` the reflux overrides and the root's flow-off
+// ledger of the octree are applied (the callers have exported the fine faces first).
+int amrh::substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool tree) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
   const bool need_u0 = beta != 1.0;
@@ -225,16 +218,28 @@ static int substep_impl(amrh_ctx* c, double dt, double beta, const double* dt_de
   a.block0 = 0;
   a.P = params(c->cfg);
   a.err = c->err_dev;
+  a.slot = nullptr;
+  a.ovr = nullptr;
+  a.ledger_block = -1;
+  a.ledger = nullptr;
+  if (tree && c->tree) {
+    a.slot = c->tree->slot_dev;
+    a.ovr = c->tree->ovr_dev;
+    a.ledger_block = 0;
+    a.ledger = c->tree->ledger_dev;
+  }
   HIPCHK(launch_substep(c->cfg.n, a, c->cfg.precision == AMRH_MIXED, c->stream));
   c->cur = out;
   return AMRH_OK;
 }
 
-int amrh_substep(amrh_ctx* c, double dt, double beta) { return substep_impl(c, dt, beta, nullptr); }
+extern "C" {
+
+int amrh_substep(amrh_ctx* c, double dt, double beta) { return substep_core(c, dt, beta, nullptr, false); }
 
 int amrh_substep_dev(amrh_ctx* c, const double* dt_dev, double beta) {
   if (!dt_dev) return fail(AMRH_EINVAL, "null dt_dev");
-  return substep_impl(c, 0.0, beta, dt_dev);
+  return substep_core(c, 0.0, beta, dt_dev, false);
 }
 
 static int maxeig_impl(amrh_ctx* c, bool per_block, bool global) {
@@ -247,7 +252,6 @@ static int maxeig_impl(amrh_ctx* c, bool per_block, bool global) {
   m.gmax = global ? c->gmax_dev : nullptr;
   m.nblocks = c->nb;
   m.P = params(c->cfg);
-  if (global) HIPCHK(hipMemsetAsync(c->gmax_dev, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(launch_maxeig(c->cfg.n, m, c->cfg.precision == AMRH_MIXED, c->stream));
   return AMRH_OK;
 }
@@ -265,15 +269,16 @@ int amrh_max_speed(amrh_ctx* c, double* out) {
   if (int rc = maxeig_impl(c, false, true)) return rc;
   unsigned long long bits = 0;
   HIPCHK(hipMemcpyAsync(&bits, c->gmax_dev, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(launch_reset_u64(c->gmax_dev, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   std::memcpy(out, &bits, sizeof(double));
   return AMRH_OK;
 }
 
-int amrh_dt_device(amrh_ctx* c, double scale, double* dt_dev) {
+int amrh_dt_device(amrh_ctx* c, double scale, double growth, double* dt_dev) {
   if (!dt_dev) return fail(AMRH_EINVAL, "null dt_dev");
   if (int rc = maxeig_impl(c, false, true)) return rc;
-  HIPCHK(launch_dt_from_gmax(c->gmax_dev, c->cfg.cfl, scale, dt_dev, c->stream));
+  HIPCHK(launch_dt_from_gmax(c->gmax_dev, c->cfg.cfl, scale, growth, dt_dev, c->stream));
   return AMRH_OK;
 }
 

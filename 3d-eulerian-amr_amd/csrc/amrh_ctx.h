@@ -1,0 +1,72 @@
+// The context behind the C ABI, shared by the translation units that implement it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <vector>
+
+#include "amr_kernels.h"
+#include "amrh_layout.h"
+#include "amrhydro.h"
+#include "euler_phys.h"
+
+namespace amrh {
+
+// Octree topology and the device tables derived from it (amrh_tree.hip).
+struct TreeState {
+  int nn = 0, maxlevel = 0;
+  std::vector<int> parent, cidx, level;
+  std::vector<std::array<int, 3>> loc, off;
+  std::vector<std::array<int, 8>> child;
+  std::vector<double> h;
+  // host copies of the tables
+  std::vector<int> level_nodes, level_start, level_count;
+  std::vector<int> item_start, item_count;  // export items per fine level
+  std::vector<int> pair_start, pair_count;  // restriction pairs per parent level
+  int nslots = 0;
+  // device tables
+  int* ghost_dev = nullptr;
+  int* rel_dev = nullptr;
+  int* level_nodes_dev = nullptr;
+  int* slot_dev = nullptr;
+  double* ovr_dev = nullptr;
+  ExportItem* items_dev = nullptr;
+  int2* pairs_dev = nullptr;
+  double* ledger_dev = nullptr;
+  double* fo_dev = nullptr;  // [FO | FO0]
+  void free_dev();
+};
+
+}  // namespace amrh
+
+struct amrh_ctx {
+  amrh_config cfg;
+  int nb = 0;
+  amrh::Layout L{8, 2, 6};
+  double* buf[2] = {nullptr, nullptr};
+  int cur = 0;     // buffer holding the current stage
+  int u0buf = -1;  // buffer holding the RK base, -1 = none stored
+  double* h_dev = nullptr;
+  double* eig_dev = nullptr;
+  unsigned long long* gmax_dev = nullptr;
+  int* err_dev = nullptr;
+  double* pad_stage = nullptr;  // padded SoA staging for upload/download
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  bool have_h = false, have_u = false;
+  amrh::TreeState* tree = nullptr;
+};
+
+namespace amrh {
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+PhysParams params(const amrh_config& c);
+int substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool tree);
+
+}  // namespace amrh
+
+#define HIPCHK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) return ::amrh::fail(AMRH_EHIP, "%s: %s", #expr, hipGetErrorString(e_));       \
+  } while (0)

@@ -10,7 +10,8 @@ hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, hipStream_t s
 hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, hipStream_t s);
 hipError_t launch_outflow_ghosts(int n, double* U, int nblocks, int z_reflect, hipStream_t s);
 hipError_t launch_pack(int n, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s);
-hipError_t launch_dt_from_gmax(const unsigned long long* gmax, double cfl, double scale, double* dt,
+hipError_t launch_dt_from_gmax(unsigned long long* gmax, double cfl, double scale, double growth, double* dt,
                                hipStream_t s);
+hipError_t launch_reset_u64(unsigned long long* p, hipStream_t s);
 
 }  // namespace amrh

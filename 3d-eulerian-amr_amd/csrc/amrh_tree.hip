@@ -1,0 +1,305 @@
+// Octree topology on the host and the device tables the coupling kernels read.
+//
+// The reference wires OctNode siblings pointer by pointer as children are created
+// (oct_node/oct_node.cpp:172-240) and puts GridAMRBound / GridPhysBound objects where a same-level
+// neighbour is missing (grid/grid_node.cpp:271-304).  Here a node is identified by (level,
+// location); a face's ghost source is the same-level node at location +/- 1 if it exists, the
+// physical boundary if the location is at the domain edge (oct_node/oct_node.cpp:86-110), else the
+// parent (coarse->fine interpolation).  Offsets and widths follow grid/grid_node.cpp:276-279.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <tuple>
+
+#include "amrh_ctx.h"
+#include "amrh_internal.h"
+
+using namespace amrh;
+
+void TreeState::free_dev() {
+  for (void* p : {(void*)ghost_dev, (void*)rel_dev, (void*)level_nodes_dev, (void*)slot_dev, (void*)ovr_dev,
+                  (void*)items_dev, (void*)pairs_dev, (void*)ledger_dev, (void*)fo_dev})
+    if (p) (void)hipFree(p);
+  ghost_dev = rel_dev = level_nodes_dev = slot_dev = nullptr;
+  ovr_dev = ledger_dev = fo_dev = nullptr;
+  items_dev = nullptr;
+  pairs_dev = nullptr;
+}
+
+namespace {
+
+template <typename T>
+hipError_t upload_vec(T** dst, const std::vector<T>& v) {
+  const size_t bytes = sizeof(T) * std::max<size_t>(v.size(), 1);
+  hipError_t e = hipMalloc(dst, bytes);
+  if (e != hipSuccess) return e;
+  if (!v.empty()) e = hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+  return e;
+}
+
+int build(amrh_ctx* c, const int* parent, const int* child) {
+  const int n = c->cfg.n, g = c->cfg.g, G = n + 2 * g;
+  TreeState* t = new TreeState;
+  t->nn = c->nb;
+  t->parent.assign(t->nn, -1);
+  t->cidx.assign(t->nn, -1);
+  t->level.assign(t->nn, 0);
+  t->loc.assign(t->nn, {0, 0, 0});
+  t->off.assign(t->nn, {0, 0, 0});
+  t->child.assign(t->nn, {-1, -1, -1, -1, -1, -1, -1, -1});
+  t->h.assign(t->nn, 0.0);
+  std::map<std::tuple<int, int, int, int>, int> by_loc;
+  t->h[0] = 2.0 * c->cfg.grid_dim / double(n);  // grid/grid.cpp:70
+  by_loc[std::make_tuple(0, 0, 0, 0)] = 0;
+  if (parent[0] != -1) {
+    delete t;
+    return fail(AMRH_EINVAL, "node 0 must be the root (parent -1)");
+  }
+  for (int b = 1; b < t->nn; b++) {
+    const int p = parent[b], ci = child[b];
+    if (p < 0 || p >= b || ci < 0 || ci > 7 || t->child[p][ci] >= 0) {
+      delete t;
+      return fail(AMRH_EINVAL, "node %d: bad parent %d / child index %d", b, p, ci);
+    }
+    t->parent[b] = p;
+    t->cidx[b] = ci;
+    t->child[p][ci] = b;
+    t->level[b] = t->level[p] + 1;
+    const int cv[3] = {ci & 1, (ci >> 1) & 1, (ci >> 2) & 1};
+    for (int d = 0; d < 3; d++) {
+      t->loc[b][d] = 2 * t->loc[p][d] + cv[d];
+      t->off[b][d] = 2 * t->off[p][d] + g + cv[d] * (G - 2 * g);
+    }
+    t->h[b] = t->h[p] * 0.5;
+    by_loc[std::make_tuple(t->level[b], t->loc[b][0], t->loc[b][1], t->loc[b][2])] = b;
+    t->maxlevel = std::max(t->maxlevel, t->level[b]);
+  }
+  auto sibling = [&](int b, int face) {
+    const int d = face >> 1;
+    std::array<int, 3> l = t->loc[b];
+    l[d] += (face & 1) ? 1 : -1;
+    if (l[d] < 0 || l[d] >= (1 << t->level[b])) return -1;
+    auto it = by_loc.find(std::make_tuple(t->level[b], l[0], l[1], l[2]));
+    return it == by_loc.end() ? -1 : it->second;
+  };
+  auto phys = [&](int b, int face) {
+    const int d = face >> 1;
+    return (face & 1) ? t->loc[b][d] == (1 << t->level[b]) - 1 : t->loc[b][d] == 0;
+  };
+  // ghost sources
+  std::vector<int> ghost(t->nn * 12), rel(t->nn * 3, 0);
+  for (int b = 0; b < t->nn; b++) {
+    for (int f = 0; f < 6; f++) {
+      const int s = sibling(b, f);
+      int type, src;
+      if (s >= 0) {
+        type = GHOST_SIB;
+        src = s;
+      } else if (phys(b, f)) {
+        type = GHOST_PHYS;
+        src = b;
+      } else {
+        type = GHOST_COARSE;
+        src = t->parent[b];
+        if (src < 0) {
+          delete t;
+          return fail(AMRH_EINVAL, "node %d face %d has no neighbour and no parent", b, f);
+        }
+      }
+      ghost[(b * 6 + f) * 2] = type;
+      ghost[(b * 6 + f) * 2 + 1] = src;
+    }
+    if (t->parent[b] >= 0)
+      for (int d = 0; d < 3; d++) rel[b * 3 + d] = t->off[b][d] - 2 * t->off[t->parent[b]][d];
+  }
+  // nodes per level
+  t->level_start.assign(t->maxlevel + 1, 0);
+  t->level_count.assign(t->maxlevel + 1, 0);
+  for (int l = 0; l <= t->maxlevel; l++) {
+    t->level_start[l] = (int)t->level_nodes.size();
+    for (int b = 0; b < t->nn; b++)
+      if (t->level[b] == l) t->level_nodes.push_back(b);
+    t->level_count[l] = (int)t->level_nodes.size() - t->level_start[l];
+  }
+  // reflux slots and export items (grid/grid_node.cpp:1076-1274)
+  std::vector<int> slot(t->nn * 36, -1);
+  std::vector<std::vector<ExportItem>> by_level(t->maxlevel + 1);
+  auto child_of = [&](int node, int c3x, int c3y, int c3z) {
+    if (node < 0) return -1;
+    return t->child[node][c3x | (c3y << 1) | (c3z << 2)];
+  };
+  for (int b = 0; b < t->nn; b++)
+    for (int d = 0; d < 3; d++) {
+      const int sL = sibling(b, 2 * d), sU = sibling(b, 2 * d + 1);
+      for (int pl = 0; pl < 3; pl++)
+        for (int ck = 0; ck < 2; ck++)
+          for (int cj = 0; cj < 2; cj++) {
+            auto ci = [&](int node, int nb) {
+              if (d == 0) return child_of(node, nb, cj, ck);
+              if (d == 1) return child_of(node, cj, nb, ck);
+              return child_of(node, cj, ck, nb);
+            };
+            int cl, cr;
+            if (pl == 0) { cr = ci(b, 0); cl = ci(sL, 1); }
+            else if (pl == 1) { cl = ci(b, 0); cr = ci(b, 1); }
+            else { cr = ci(sU, 0); cl = ci(b, 1); }
+            int fine, side;
+            if (cr < 0 && cl >= 0) { fine = cl; side = 1; }
+            else if (cl < 0 && cr >= 0) { fine = cr; side = 0; }
+            else continue;
+            const int s = t->nslots++;
+            slot[((b * 3 + d) * 3 + pl) * 4 + cj + 2 * ck] = s;
+            by_level[t->level[fine]].push_back(ExportItem{fine, d, side, s});
+          }
+    }
+  std::vector<ExportItem> items;
+  t->item_start.assign(t->maxlevel + 1, 0);
+  t->item_count.assign(t->maxlevel + 1, 0);
+  for (int l = 0; l <= t->maxlevel; l++) {
+    t->item_start[l] = (int)items.size();
+    items.insert(items.end(), by_level[l].begin(), by_level[l].end());
+    t->item_count[l] = (int)by_level[l].size();
+  }
+  // restriction pairs per parent level
+  std::vector<int2> pairs;
+  t->pair_start.assign(t->maxlevel + 1, 0);
+  t->pair_count.assign(t->maxlevel + 1, 0);
+  for (int l = 0; l <= t->maxlevel; l++) {
+    t->pair_start[l] = (int)pairs.size();
+    for (int b = 0; b < t->nn; b++)
+      if (t->level[b] == l)
+        for (int ci = 0; ci < 8; ci++)
+          if (t->child[b][ci] >= 0) pairs.push_back(make_int2(b, (ci << 24) | t->child[b][ci]));
+    t->pair_count[l] = (int)pairs.size() - t->pair_start[l];
+  }
+  hipError_t e = upload_vec(&t->ghost_dev, ghost);
+  if (e == hipSuccess) e = upload_vec(&t->rel_dev, rel);
+  if (e == hipSuccess) e = upload_vec(&t->level_nodes_dev, t->level_nodes);
+  if (e == hipSuccess) e = upload_vec(&t->slot_dev, slot);
+  if (e == hipSuccess) e = upload_vec(&t->items_dev, items);
+  if (e == hipSuccess) e = upload_vec(&t->pairs_dev, pairs);
+  const long half = n / 2;
+  if (e == hipSuccess) e = hipMalloc(&t->ovr_dev, sizeof(double) * std::max<long>(1, t->nslots) * half * half * 6);
+  if (e == hipSuccess) e = hipMalloc(&t->ledger_dev, sizeof(double) * 3 * 2 * n * n * 6);
+  if (e == hipSuccess) e = hipMalloc(&t->fo_dev, sizeof(double) * 12);
+  if (e == hipSuccess) e = hipMemset(t->fo_dev, 0, sizeof(double) * 12);
+  if (e == hipSuccess) e = hipMemset(t->ledger_dev, 0, sizeof(double) * 3 * 2 * n * n * 6);
+  if (e != hipSuccess) {
+    t->free_dev();
+    delete t;
+    return fail(AMRH_EHIP, "tree tables: %s", hipGetErrorString(e));
+  }
+  if (c->tree) {
+    c->tree->free_dev();
+    delete c->tree;
+  }
+  c->tree = t;
+  return AMRH_OK;
+}
+
+int fill_ghosts(amrh_ctx* c) {
+  TreeState* t = c->tree;
+  for (int l = 0; l <= t->maxlevel; l++)
+    HIPCHK(launch_tree_ghosts(c->cfg.n, c->buf[c->cur], t->level_nodes_dev + t->level_start[l], t->level_count[l],
+                              t->ghost_dev, t->rel_dev, c->cfg.z_reflect, c->err_dev, c->stream));
+  return AMRH_OK;
+}
+
+int inject(amrh_ctx* c) {
+  TreeState* t = c->tree;
+  const PhysParams P = params(c->cfg);
+  for (int l = t->maxlevel - 1; l >= 0; l--)
+    HIPCHK(launch_restrict(c->cfg.n, c->buf[c->cur], t->pairs_dev + t->pair_start[l], t->pair_count[l], P,
+                           c->stream));
+  return AMRH_OK;
+}
+
+int tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
+  TreeState* t = c->tree;
+  if (int rc = fill_ghosts(c)) return rc;
+  const PhysParams P = params(c->cfg);
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  for (int l = t->maxlevel; l >= 1; l--)
+    HIPCHK(launch_reflux_export(c->cfg.n, mixed, c->buf[c->cur], t->items_dev + t->item_start[l],
+                                t->item_count[l], t->slot_dev, t->ovr_dev, P, c->stream));
+  if (int rc = substep_core(c, dt, beta, dt_dev, true)) return rc;
+  HIPCHK(launch_ledger(c->cfg.n, t->ledger_dev, t->h[0], dt_dev, dt, beta, c->cfg.z_reflect, t->fo_dev,
+                       c->stream));
+  return inject(c);
+}
+
+double block_dt(double m, double h, double cfl) {  // grid/grid.cpp:474-481
+  double dtinv = m / h;
+  if (dtinv == 0.0) dtinv = 1.0e-10;
+  double dt = 1.0 / dtinv;
+  dt *= cfl;
+  return dt;
+}
+
+double max_dt_rec(const TreeState* t, const std::vector<double>& eig, double cfl, int b) {
+  double dtinv = 0.0;  // grid/grid_node.cpp:163-174
+  for (int ci = 0; ci < 8; ci++)
+    if (t->child[b][ci] >= 0) {
+      const double cdtinv = 1.0 / max_dt_rec(t, eig, cfl, t->child[b][ci]);
+      dtinv = cdtinv > dtinv ? cdtinv : dtinv;
+    }
+  const double own = 1.0 / block_dt(eig[b], t->h[b], cfl);
+  dtinv = own > dtinv ? own : dtinv;
+  return 1.0 / dtinv;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amrh_tree_set(amrh_ctx* c, const int* parent, const int* child) {
+  if (!c || !parent || !child) return fail(AMRH_EINVAL, "null argument");
+  if (int rc = build(c, parent, child)) return rc;
+  HIPCHK(hipMemcpy(c->h_dev, c->tree->h.data(), sizeof(double) * c->nb, hipMemcpyHostToDevice));
+  c->have_h = true;
+  return AMRH_OK;
+}
+
+int amrh_tree_fill_ghosts(amrh_ctx* c) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (!c->tree || !c->have_u) return fail(AMRH_ESTATE, "tree or state not set");
+  return fill_ghosts(c);
+}
+
+int amrh_tree_inject(amrh_ctx* c) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (!c->tree || !c->have_u) return fail(AMRH_ESTATE, "tree or state not set");
+  return inject(c);
+}
+
+int amrh_tree_substep(amrh_ctx* c, double dt, double beta) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (!c->tree || !c->have_u) return fail(AMRH_ESTATE, "tree or state not set");
+  return tree_substep(c, dt, beta, nullptr);
+}
+
+int amrh_tree_substep_dev(amrh_ctx* c, const double* dt_dev, double beta) {
+  if (!c || !dt_dev) return fail(AMRH_EINVAL, "null argument");
+  if (!c->tree || !c->have_u) return fail(AMRH_ESTATE, "tree or state not set");
+  return tree_substep(c, 0.0, beta, dt_dev);
+}
+
+int amrh_tree_max_dt(amrh_ctx* c, double* dt) {
+  if (!c || !dt) return fail(AMRH_EINVAL, "null argument");
+  if (!c->tree) return fail(AMRH_ESTATE, "tree not set");
+  std::vector<double> eig(c->nb);
+  if (int rc = amrh_block_max_eig(c, eig.data())) return rc;
+  *dt = max_dt_rec(c->tree, eig, c->cfg.cfl, 0);
+  return AMRH_OK;
+}
+
+int amrh_tree_flow_off(amrh_ctx* c, double* fo) {
+  if (!c || !fo) return fail(AMRH_EINVAL, "null argument");
+  if (!c->tree) return fail(AMRH_ESTATE, "tree not set");
+  HIPCHK(hipMemcpyAsync(fo, c->tree->fo_dev, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMRH_OK;
+}
+
+}  // extern "C"

@@ -29,6 +29,11 @@ struct SubstepArgs {
   int block0;              // first block handled by this launch
   PhysParams P;
   int* err;
+  // octree coupling (all null / -1 for a flat batch)
+  const int* slot;         // [nb][3 dir][3 plane][4 quadrant] reflux override slot or -1
+  const double* ovr;       // override face fluxes [slot][N/2][N/2][NF] (fine-face averages)
+  int ledger_block;        // block whose domain-boundary fluxes feed the flow-off ledger
+  double* ledger;          // [3 dir][2 side][N][N][NF] boundary face fluxes of that block
 };
 
 struct MaxEigArgs {

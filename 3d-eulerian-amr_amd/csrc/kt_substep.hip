@@ -83,6 +83,23 @@ __device__ __forceinline__ double face_eig(const PhysParams& P, const typename P
   }
 }
 
+// Reflux: a face on plane 0, N/2 or N of direction d whose quadrant has a fine neighbour on exactly
+// one side takes the mean of the four fine faces over it (grid/grid_node.cpp:1076-1140); the means
+// were written to a.ovr by the export kernel.  (ta, tb): the face's tangential cell indices in
+// (x, y, z) order with d left out.
+template <int N, typename T>
+__device__ __forceinline__ void apply_override(const SubstepArgs& a, int b, int d, int fi, int ta, int tb,
+                                               T* F) {
+  const int p = fi == 0 ? 0 : (fi == N / 2 ? 1 : (fi == N ? 2 : -1));
+  if (p < 0) return;
+  const int qa = ta >= N / 2, qb = tb >= N / 2;
+  const int s = a.slot[((b * 3 + d) * 3 + p) * 4 + qa + 2 * qb];
+  if (s < 0) return;
+  const double* o = a.ovr + (((long)s * (N / 2) + (tb - qb * (N / 2))) * (N / 2) + (ta - qa * (N / 2))) * NFE;
+#pragma unroll
+  for (int l = 0; l < NFE; l++) F[l] = (T)o[l];
+}
+
 // element offset of the column cell (i, j, kk), kk in [-G, N+G)
 template <int N>
 __device__ __forceinline__ long col_elem(const Layout& L, int i, int j, int kk) {
@@ -164,6 +181,11 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
         float e2;
         f32::kt_face<2>(PF, cm1, sm1, c0, s0, Fzlo, e2);
       }
+      if (a.slot) apply_override<N>(a, b, 2, 0, i, j, Fzlo);
+      if (b == a.ledger_block) {
+#pragma unroll
+        for (int l = 0; l < NFE; l++) a.ledger[(((2 * 2 + 0) * N + j) * N + i) * NFE + l] = (double)Fzlo[l];
+      }
       (void)sl;
     } else {
       // eigenvalue of face 0 (cells -2..1)
@@ -240,6 +262,7 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
         if constexpr (MODE == 0) {
           T F[NFE];
           face_flux<MIXED, 0>(P, pm2, pm1, p0, pp1, F);
+          if (a.slot) apply_override<N>(a, b, 0, fi, jj, k, F);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fx[l][jj][fi] = F[l];
         } else {
@@ -259,6 +282,7 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
         if constexpr (MODE == 0) {
           T F[NFE];
           face_flux<MIXED, 1>(P, pm2, pm1, p0, pp1, F);
+          if (a.slot) apply_override<N>(a, b, 1, fj, ii, k, F);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fy[l][fj][ii] = F[l];
         } else {
@@ -282,6 +306,7 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
         float e2;
         f32::kt_face<2>(PF, c0, s0, cp1, sp1, Fzhi, e2);
       }
+      if (a.slot) apply_override<N>(a, b, 2, k + 1, i, j, Fzhi);
     } else {
       if constexpr (!MIXED)
         emax = x64::rmax(emax, x64::face_eig<2>(P, c0, s0, cp1, sp1));
@@ -308,6 +333,28 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
       }
       x64::floor_state(P, u);
       if (!(u[RHO] > 0.0) && a.err) atomicOr(a.err, 1);
+      if (b == a.ledger_block) {
+        // boundary faces of the ledger block (grid/grid.cpp:350-404 sums them per pencil)
+        double* lg = a.ledger;
+        if (i == 0) {
+#pragma unroll
+          for (int l = 0; l < NFE; l++) {
+            lg[(((0 * 2 + 0) * N + k) * N + j) * NFE + l] = (double)Fx[l][j][0];
+            lg[(((0 * 2 + 1) * N + k) * N + j) * NFE + l] = (double)Fx[l][j][N];
+          }
+        }
+        if (j == 0) {
+#pragma unroll
+          for (int l = 0; l < NFE; l++) {
+            lg[(((1 * 2 + 0) * N + k) * N + i) * NFE + l] = (double)Fy[l][0][i];
+            lg[(((1 * 2 + 1) * N + k) * N + i) * NFE + l] = (double)Fy[l][N][i];
+          }
+        }
+        if (k == N - 1) {
+#pragma unroll
+          for (int l = 0; l < NFE; l++) lg[(((2 * 2 + 1) * N + j) * N + i) * NFE + l] = (double)Fzhi[l];
+        }
+      }
       double* Uo = a.Uout + (long)b * NFE * R;
       const long e = L.interior(i, j, k);
 #pragma unroll
@@ -448,12 +495,24 @@ __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ pa
     const_cast<double*>(pad)[pidx] = rec[gid];
 }
 
-__global__ void dt_from_gmax_kernel(const unsigned long long* gmax, double cfl, double scale, double* dt) {
+// dt = scale * cfl / max_b(eig_b/h_b), optionally capped at growth * (previous dt)
+// (euler/hydro_driver.cpp:168-173); resets the reduction word so the next scan starts at zero
+// without a memset (keeps the step capturable in a graph).
+__global__ void dt_from_gmax_kernel(unsigned long long* gmax, double cfl, double scale, double growth,
+                                    double* dt) {
   const double m = __longlong_as_double((long long)*gmax);
   double dtinv = m;
   if (dtinv == 0.0) dtinv = 1.0e-10;
-  dt[0] = (1.0 / dtinv) * cfl * scale;
+  double d = (1.0 / dtinv) * cfl * scale;
+  if (growth > 0.0) {
+    const double cap = growth * dt[0];
+    d = cap < d ? cap : d;
+  }
+  dt[0] = d;
+  *gmax = 0ull;
 }
+
+__global__ void reset_u64_kernel(unsigned long long* p) { *p = 0ull; }
 
 }  // namespace
 
@@ -519,9 +578,14 @@ hipError_t launch_pack(int n, const double* pad, double* rec, long nrec_elems, b
   return hipGetLastError();
 }
 
-hipError_t launch_dt_from_gmax(const unsigned long long* gmax, double cfl, double scale, double* dt,
+hipError_t launch_dt_from_gmax(unsigned long long* gmax, double cfl, double scale, double growth, double* dt,
                                hipStream_t s) {
-  dt_from_gmax_kernel<<<1, 1, 0, s>>>(gmax, cfl, scale, dt);
+  dt_from_gmax_kernel<<<1, 1, 0, s>>>(gmax, cfl, scale, growth, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_u64(unsigned long long* p, hipStream_t s) {
+  reset_u64_kernel<<<1, 1, 0, s>>>(p);
   return hipGetLastError();
 }
 

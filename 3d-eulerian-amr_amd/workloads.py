@@ -44,6 +44,64 @@ def flat_batch(nb, n=16, g=2, seed=SEED, jump_frac=0.05):
     return U
 
 
-def flat_batch_sample_rate():
-    """Cells per block of the flat batch, for throughput arithmetic."""
-    return 16 ** 3
+# Uneven multilevel octree leaves: 8 at level 0 up to 30k at the finest, 59,518 in all.  The
+# endpoints and the ~60k total are not one geometric series, so the counts are fixed here.
+LEVEL_COUNTS = (8, 60, 450, 2000, 7000, 20000, 30000)
+
+
+def multilevel_layout(counts=LEVEL_COUNTS, seed=SEED):
+    """Leaf positions per level.  Level l tiles the cube [-1, 1]^3 with 2^(l+1) blocks per edge; the
+    leaves of a level are the blocks nearest the centre (ties broken by a seeded jitter), so finer
+    levels sit on the density core.  Returns (level, origin[3]) arrays."""
+    rng = np.random.default_rng(seed)
+    lev, org = [], []
+    for l, cnt in enumerate(counts):
+        m = 2 ** (l + 1)
+        if cnt > m ** 3:
+            raise ValueError("level %d has only %d block positions" % (l, m ** 3))
+        edge = 2.0 / m
+        idx = np.arange(m)
+        c = -1.0 + (idx + 0.5) * edge
+        Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+        r = np.sqrt(X * X + Y * Y + Z * Z).ravel() + rng.uniform(0, 1e-6, m ** 3)
+        pick = np.argsort(r, kind="stable")[:cnt]
+        o = np.stack([X.ravel()[pick], Y.ravel()[pick], Z.ravel()[pick]], axis=1) - 0.5 * edge
+        lev.append(np.full(cnt, l))
+        org.append(o)
+    return np.concatenate(lev), np.concatenate(org)
+
+
+def multilevel(counts=LEVEL_COUNTS, n=8, g=2, sigma=0.2, omega=0.5, seed=SEED):
+    """Uneven multilevel batch: leaves of 7 levels, n^3 interior + g ghosts, cell width
+    h_l = 2/(n 2^(l+1)).  Smooth Gaussian density core rho = 1e-6 + exp(-r^2/(2 sigma^2)), solid-body
+    rotation v = omega z_hat x r, pressure p = 0.1 rho^gamma + 1e-6 (subsonic everywhere).  Ghost
+    cells hold the same analytic field.  Returns (U, h, level)."""
+    lev, org = multilevel_layout(counts, seed)
+    nb = len(lev)
+    G = n + 2 * g
+    U = np.empty((nb, 6, G, G, G))
+    h = 2.0 / (n * 2.0 ** (lev + 1))
+    off = (np.arange(G) - g + 0.5)
+    for s in range(0, nb, 4096):
+        e = min(nb, s + 4096)
+        hh = h[s:e, None]
+        x = org[s:e, 0:1] + off[None, :] * hh
+        y = org[s:e, 1:2] + off[None, :] * hh
+        z = org[s:e, 2:3] + off[None, :] * hh
+        X = x[:, None, None, :]
+        Y = y[:, None, :, None]
+        Z = z[:, :, None, None]
+        r2 = X * X + Y * Y + Z * Z
+        rho = 1e-6 + np.exp(-r2 / (2 * sigma * sigma))
+        vx = -omega * Y + 0 * X
+        vy = omega * X + 0 * Y
+        vz = 0 * r2
+        p = 0.1 * rho ** GAMMA + 1e-6
+        eps = p / (GAMMA - 1.0)
+        U[s:e, 0] = rho
+        U[s:e, 1] = rho * vx
+        U[s:e, 2] = rho * vy
+        U[s:e, 3] = rho * vz
+        U[s:e, 4] = eps + 0.5 * rho * (vx * vx + vy * vy)
+        U[s:e, 5] = eps ** (1.0 / GAMMA)
+    return U, h, lev

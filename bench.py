@@ -44,8 +44,15 @@ def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
     B.fill_outflow_ghosts()
     cells = nb * n ** 3
 
+    first = [True]
+
     def step(events=None):
-        B.dt_device(0.5, dt.data_ptr())
+        # HydroDriver::step_to_time's dt control on the device (euler/hydro_driver.cpp:168-173)
+        if first[0]:
+            B.dt_device(1e-2, 0.0, dt.data_ptr())
+            first[0] = False
+        else:
+            B.dt_device(1.0, 1.25, dt.data_ptr())
         B.store()
         for s, beta in enumerate(BETAS):
             if s > 0:
@@ -90,7 +97,8 @@ def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
 
 def accuracy(amrhydro, torch, U_dev_ptr, nb, n, steps):
     """Mixed vs FP64 GPU after `steps` RK3 steps on the full workload (same dt sequence: the FP64
-    form's CFL dt).  The FP64 form is pinned to the CPU oracle at 1e-12 by the test suite."""
+    form's CFL dt under the reference's control, euler/hydro_driver.cpp:168-173).  The FP64 form is
+    pinned to the CPU oracle at 1e-12 by the test suite."""
     forms = {}
     for prec in (amrhydro.AMRH_FP64, amrhydro.AMRH_MIXED):
         B = amrhydro.Batch(amrhydro.euler_config(n=n, precision=prec), nb, 1.0 / n)
@@ -99,20 +107,22 @@ def accuracy(amrhydro, torch, U_dev_ptr, nb, n, steps):
         B.fill_outflow_ghosts()
         forms[prec] = B
     B64, B32 = forms[amrhydro.AMRH_FP64], forms[amrhydro.AMRH_MIXED]
+    dt = None
     for s in range(steps):
-        m = B64.max_speed()
-        dt = 0.5 * 0.4 / m
+        dt = B64.cfl_dt(dt)
         for B in (B64, B32):
             B.rk_step(dt, 3)
             B.fill_outflow_ghosts()
     g = 2
     out = {}
-    buf = torch.empty(nb * 6 * (n + 4) ** 3, dtype=torch.float64, device="cuda")
     res = []
     for B in (B64, B32):
-        B.download_device(buf.data_ptr())
+        buf = torch.empty(nb * 6 * (n + 4) ** 3, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        res.append(buf.view(nb, 6, n + 4, n + 4, n + 4)[:, :, g:-g, g:-g, g:-g].clone())
+        B.download_device(buf.data_ptr())
+        B.sync()
+        res.append(buf.view(nb, 6, n + 4, n + 4, n + 4)[:, :, g:-g, g:-g, g:-g])
+    torch.cuda.synchronize()
     a, b = res[1], res[0]
     rel = ((a - b).abs().sum(dim=(0, 2, 3, 4)) / b.abs().sum(dim=(0, 2, 3, 4))).cpu().numpy()
     for B in (B64, B32):
@@ -122,8 +132,9 @@ def accuracy(amrhydro, torch, U_dev_ptr, nb, n, steps):
     return out
 
 
-def cpu_baseline(n, target_s=8.0):
-    """The FP64 CPU oracle (OpenMP over blocks) on a bounded sample of the same workload."""
+def cpu_baseline(n, target_s=10.0):
+    """The FP64 CPU oracle (OpenMP over blocks) on a bounded sample of the same workload: whole
+    RK3 steps (3 substeps each, ghost refresh excluded from the timing) until ~target_s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
     import workloads
@@ -134,23 +145,26 @@ def cpu_baseline(n, target_s=8.0):
     U = workloads.flat_batch(nb, n=n, seed=5)
     orc.outflow_ghosts(cfg, U)
     h = np.full(nb, 1.0 / n)
-    dt = 0.2 * orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
+    dt = 1e-2 * orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
     t0 = time.perf_counter()
     orc.batch_substep(cfg, U, U.copy(), h, dt, 1.0)
     t1 = time.perf_counter()
     rate = nb * n ** 3 / (t1 - t0)
-    nb = max(32, min(4096, int(rate * target_s / 3 / n ** 3)))
+    nb = max(32, min(1024, int(rate * target_s / 3 / n ** 3)))
     U = workloads.flat_batch(nb, n=n, seed=6)
     orc.outflow_ghosts(cfg, U)
     h = np.full(nb, 1.0 / n)
-    U0 = U.copy()
     times = []
-    for beta in BETAS:
-        orc.outflow_ghosts(cfg, U)
-        t0 = time.perf_counter()
-        orc.batch_substep(cfg, U, U0, h, dt, beta)
-        times.append(time.perf_counter() - t0)
-    rate = 3 * nb * n ** 3 / sum(times)
+    steps = 0
+    while sum(times) < target_s and steps < 50:
+        U0 = U.copy()
+        for beta in BETAS:
+            orc.outflow_ghosts(cfg, U)
+            t0 = time.perf_counter()
+            orc.batch_substep(cfg, U, U0, h, dt, beta)
+            times.append(time.perf_counter() - t0)
+        steps += 1
+    rate = 3 * steps * nb * n ** 3 / sum(times)
     try:
         import ctypes
         gomp = ctypes.CDLL("libgomp.so.1")
@@ -158,8 +172,8 @@ def cpu_baseline(n, target_s=8.0):
     except OSError:
         cores = int(threads) if threads else os.cpu_count()
     return {"value": rate, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": "%d blocks of 16^3 (+2 ghosts) x one RK3 step (3 substeps), FP64 oracle, "
-                      "OpenMP over blocks, %.1f s" % (nb, sum(times))}
+            "sample": "%d blocks of %d^3 (+2 ghosts) x %d RK3 steps (3 substeps each), FP64 oracle, "
+                      "OpenMP over blocks, %.1f s timed" % (nb, n, steps, sum(times))}
 
 
 def load_traffic(workload_key):
@@ -191,6 +205,8 @@ def main():
     import workloads
 
     amrhydro.lib()
+    # a real (non-null) stream shared by the library and the timing events
+    torch.cuda.set_stream(torch.cuda.Stream())
     n, nb = args.n, args.nblocks
     log("generating workload: %d blocks of %d^3 (+2 ghosts)" % (nb, n))
     U = workloads.flat_batch(nb, n=n)

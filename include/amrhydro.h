@@ -105,9 +105,11 @@ int amrh_block_max_eig(amrh_ctx* ctx, double* max_eig);
 /* Device reduction of max_b(max_eig_b / h_b) (a wavefront + atomic max over the bit pattern of
  * non-negative doubles, order independent).  Synchronous; writes one double. */
 int amrh_max_speed(amrh_ctx* ctx, double* max_eig_over_h);
-/* dt = cfl / max_b(max_eig_b/h_b) computed on the device into dt_dev[0] (no host round trip),
- * scaled by `scale` (e.g. MAXINITDT at t = 0, euler/hydro_driver.cpp:169-173). */
-int amrh_dt_device(amrh_ctx* ctx, double scale, double* dt_dev);
+/* dt = scale * cfl / max_b(max_eig_b/h_b) computed on the device into dt_dev[0] (no host round
+ * trip).  With growth > 0 the result is also capped at growth * dt_dev[0] (the previous dt).  The
+ * reference's control is scale = MAXINITDT = 1e-2, growth = 0 at t = 0, then scale = 1,
+ * growth = MAXDTINC = 1.25 (euler/hydro_driver.cpp:168-173, euler/euler_defs.h:45,57). */
+int amrh_dt_device(amrh_ctx* ctx, double scale, double growth, double* dt_dev);
 
 /* Device error flags raised since the last call (AMRH_DERR_*); clears them.  Synchronous. */
 int amrh_error(amrh_ctx* ctx, int* flags);
@@ -115,6 +117,32 @@ int amrh_error(amrh_ctx* ctx, int* flags);
 /* Bytes of HBM one substep over the batch moves by the algorithmic count (interior + face
  * ghost slabs read, interior written, U0 read when beta != 1). */
 double amrh_substep_bytes(const amrh_ctx* ctx, int beta_is_one);
+
+/* ---- octree coupling (GridNode recursions, grid/grid_node.cpp) -------------------------------
+ * Block b of the context is node b of the tree.  Node 0 is the root (parent[0] = -1); every other
+ * node has a parent with a smaller index and child[b] in 0..7 (bit0 x, bit1 y, bit2 z;
+ * oct_node/child_index.cpp:66-76).  Locations, offsets (grid/grid_node.cpp:276), cell widths
+ * (h0 = 2*grid_dim/n, halved per level; grid/grid.cpp:70, grid/grid_node.cpp:279) and the ghost,
+ * reflux and restriction tables are derived here.  Replaces OctNode::create_child's wiring
+ * (oct_node/oct_node.cpp:172-240) and GridNode::create_child's boundary objects
+ * (grid/grid_node.cpp:271-304) for a tree built on the host. */
+int amrh_tree_set(amrh_ctx* ctx, const int* parent, const int* child);
+/* GridNode::enforce_boundaries (grid/grid_node.cpp:59-112), level by level: same-level copy,
+ * coarse->fine interpolation (grid/grid_amr_bound.cpp:27-78), physical boundary
+ * (grid/grid_phys_bound.cpp:27-54). */
+int amrh_tree_fill_ghosts(amrh_ctx* ctx);
+/* HydroDriver::sub_step + inject_from_children (euler/hydro_driver.cpp:243-274): ghost fill,
+ * fluxes with reflux (grid/grid_node.cpp:1076-1274), divergence, add_difs on every node, the root's
+ * flow-off ledger (grid/grid.cpp:350-404, 419), then restriction (grid/grid_node.cpp:1035-1074). */
+int amrh_tree_substep(amrh_ctx* ctx, double dt, double beta);
+int amrh_tree_substep_dev(amrh_ctx* ctx, const double* dt_dev, double beta);
+/* GridNode::inject_from_children (grid/grid_node.cpp:1035-1074). */
+int amrh_tree_inject(amrh_ctx* ctx);
+/* GridNode::max_dt (grid/grid_node.cpp:163-174): per-node Grid::max_dt from the device scan,
+ * combined on the host in the reference's recursion.  Synchronous. */
+int amrh_tree_max_dt(amrh_ctx* ctx, double* dt);
+/* The root's flow-off ledger FO (Grid::flow_off, grid/grid.cpp:346-348): nf doubles. */
+int amrh_tree_flow_off(amrh_ctx* ctx, double* fo);
 
 #ifdef __cplusplus
 }

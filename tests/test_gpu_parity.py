@@ -88,18 +88,26 @@ def test_block_max_eig_matches_oracle(n):
     assert abs(m - (ref * n).max()) <= 1e-14 * m
 
 
+def oracle_dt(cfg, U, h, last):
+    """HydroDriver::step_to_time's dt (euler/hydro_driver.cpp:168-173) on the oracle state."""
+    dt = orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h)
+    return 1e-2 * dt if last is None else min(1.25 * last, dt)
+
+
 def test_rk3_fp64_trajectory_matches_oracle():
-    """C2-shaped batch (n=16, 5% planar jumps), 3 full RK3 steps with outflow ghost refresh."""
-    n, nb = 16, 16
+    """C2-shaped batch (n=16, 5% planar jumps), 12 RK3 steps with outflow ghost refresh and the
+    reference's dt control (dt from the oracle state, used by both)."""
+    n, nb, steps = 16, 16, 12
     U = make_blocks(n, nb, seed=77)
     h = np.full(nb, 1.0 / n)
     cfg = orc.euler_cfg(n=n)
-    dt = orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
     ref = U.copy()
-    orc.rk_steps(cfg, ref, h, dt, 3, 3)
     B = gpu_batch(n, nb, h)
     B.upload(U)
-    for _ in range(3):
+    dt = None
+    for _ in range(steps):
+        dt = oracle_dt(cfg, ref, h[0], dt)
+        orc.rk_steps(cfg, ref, h, dt, 3, 1)
         B.rk_step(dt, 3)
     B.fill_outflow_ghosts()
     out = B.download()
@@ -109,18 +117,19 @@ def test_rk3_fp64_trajectory_matches_oracle():
 
 
 def test_mixed_precision_error_after_rk3_steps():
-    """Mixed form (FP32 reconstruction/flux/wavespeed, FP64 accumulate): 20 RK3 steps of a C2-shaped
+    """Mixed form (FP32 reconstruction/flux/wavespeed, FP64 accumulate): 30 RK3 steps of a C2-shaped
     batch against the FP64 oracle.  Density rel-L1 must stay <= 1e-6."""
-    n, nb, steps = 16, 8, 20
+    n, nb, steps = 16, 8, 30
     U = make_blocks(n, nb, seed=91)
     h = np.full(nb, 1.0 / n)
     cfg = orc.euler_cfg(n=n)
-    dt = orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
     ref = U.copy()
-    orc.rk_steps(cfg, ref, h, dt, 3, steps)
     B = gpu_batch(n, nb, h, precision=amrhydro.AMRH_MIXED)
     B.upload(U)
+    dt = None
     for _ in range(steps):
+        dt = oracle_dt(cfg, ref, h[0], dt)
+        orc.rk_steps(cfg, ref, h, dt, 3, 1)
         B.rk_step(dt, 3)
     B.fill_outflow_ghosts()
     out = B.download()
