@@ -25,14 +25,20 @@ class Config(C.Structure):
     _fields_ = [("n", C.c_int), ("g", C.c_int), ("nf", C.c_int), ("physics", C.c_int),
                 ("recon", C.c_int), ("precision", C.c_int), ("z_reflect", C.c_int),
                 ("gamma", C.c_double), ("rho_floor", C.c_double), ("ei_floor", C.c_double),
-                ("theta", C.c_double), ("cfl", C.c_double), ("grid_dim", C.c_double)]
+                ("theta", C.c_double), ("cfl", C.c_double), ("grid_dim", C.c_double),
+                ("boundary", C.c_int)]
 
 
-def euler_config(n=8, g=2, precision=AMRH_FP64, z_reflect=0, grid_dim=1.0):
+AMRH_BOUNDARY_EXTERNAL = 0
+AMRH_BOUNDARY_OUTFLOW = 1
+
+
+def euler_config(n=8, g=2, precision=AMRH_FP64, z_reflect=0, grid_dim=1.0,
+                 boundary=AMRH_BOUNDARY_EXTERNAL):
     """Ideal-gas constants (euler/state.cpp:9-11, euler/euler_defs.h:20,46)."""
     return Config(n=n, g=g, nf=6, physics=AMRH_PHYS_EULER, recon=AMRH_RECON_MINMOD,
                   precision=precision, z_reflect=z_reflect, gamma=5.0 / 3.0, rho_floor=1e-20,
-                  ei_floor=1e-20, theta=1.3, cfl=0.4, grid_dim=grid_dim)
+                  ei_floor=1e-20, theta=1.3, cfl=0.4, grid_dim=grid_dim, boundary=boundary)
 
 
 # name -> (restype, argtypes); the full exported surface of include/amrhydro.h
@@ -56,6 +62,9 @@ SIGNATURES = {
     "amrh_dt_device": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
     "amrh_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "amrh_substep_bytes": (C.c_double, [C.c_void_p, C.c_int]),
+    "amrh_autotune": (C.c_int, [C.c_void_p, IP, IP]),
+    "amrh_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "amrh_run_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "amrh_tree_set": (C.c_int, [C.c_void_p, IP, IP]),
     "amrh_tree_fill_ghosts": (C.c_int, [C.c_void_p]),
     "amrh_tree_substep": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
@@ -192,6 +201,17 @@ class Batch:
             return maxinitdt * dt
         return min(maxdtinc * last_dt, dt)
 
+    def autotune(self):
+        gs, gm = C.c_int(), C.c_int()
+        _ck(lib().amrh_autotune(self.ctx, C.byref(gs), C.byref(gm)))
+        return gs.value, gm.value
+
+    def set_launch(self, grid_substep, grid_maxeig):
+        _ck(lib().amrh_set_launch(self.ctx, grid_substep, grid_maxeig))
+
+    def run_steps(self, nsteps, dt_ptr, rk=3, use_graph=True):
+        _ck(lib().amrh_run_steps(self.ctx, rk, nsteps, C.c_void_p(dt_ptr), 1 if use_graph else 0))
+
     # ---- octree ------------------------------------------------------------------------------
     def tree_set(self, parent, child):
         p = np.ascontiguousarray(parent, dtype=np.int32)
@@ -232,8 +252,10 @@ class Batch:
     def rk_step(self, dt, rk=3, fill_ghosts=True):
         """HydroDriver::step for a flat batch (euler/hydro_driver.cpp:231-249)."""
         betas = {1: (1.0,), 2: (1.0, 0.5), 3: (1.0, 0.25, 2.0 / 3.0)}[rk]
+        # in AMRH_BOUNDARY_OUTFLOW mode each substep refreshes the ghosts itself
+        fill = fill_ghosts and self.cfg.boundary != AMRH_BOUNDARY_OUTFLOW
         self.store()
         for b in betas:
-            if fill_ghosts:
+            if fill:
                 self.fill_outflow_ghosts()
             self.substep(dt, b)

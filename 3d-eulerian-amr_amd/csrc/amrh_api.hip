@@ -59,6 +59,8 @@ int check_cfg(const amrh_config* c) {
   if (c->precision != AMRH_FP64 && c->precision != AMRH_MIXED)
     return fail(AMRH_EINVAL, "precision %d unknown", c->precision);
   if (!(c->gamma > 1.0)) return fail(AMRH_EINVAL, "gamma must exceed 1");
+  if (c->boundary != AMRH_BOUNDARY_EXTERNAL && c->boundary != AMRH_BOUNDARY_OUTFLOW)
+    return fail(AMRH_EINVAL, "boundary %d unknown", c->boundary);
   return AMRH_OK;
 }
 
@@ -222,13 +224,15 @@ int amrh::substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev
   a.ovr = nullptr;
   a.ledger_block = -1;
   a.ledger = nullptr;
+  a.outflow = (!tree && c->cfg.boundary == AMRH_BOUNDARY_OUTFLOW) ? 1 : 0;
+  a.z_reflect = c->cfg.z_reflect;
   if (tree && c->tree) {
     a.slot = c->tree->slot_dev;
     a.ovr = c->tree->ovr_dev;
     a.ledger_block = 0;
     a.ledger = c->tree->ledger_dev;
   }
-  HIPCHK(launch_substep(c->cfg.n, a, c->cfg.precision == AMRH_MIXED, c->stream));
+  HIPCHK(launch_substep(c->cfg.n, a, c->cfg.precision == AMRH_MIXED, c->grid_substep, c->stream));
   c->cur = out;
   return AMRH_OK;
 }
@@ -252,7 +256,7 @@ static int maxeig_impl(amrh_ctx* c, bool per_block, bool global) {
   m.gmax = global ? c->gmax_dev : nullptr;
   m.nblocks = c->nb;
   m.P = params(c->cfg);
-  HIPCHK(launch_maxeig(c->cfg.n, m, c->cfg.precision == AMRH_MIXED, c->stream));
+  HIPCHK(launch_maxeig(c->cfg.n, m, c->cfg.precision == AMRH_MIXED, c->grid_maxeig, c->stream));
   return AMRH_OK;
 }
 
@@ -297,7 +301,9 @@ double amrh_substep_bytes(const amrh_ctx* c, int beta_is_one) {
   const double n = c->cfg.n, g = c->cfg.g, nf = c->cfg.nf;
   const double n3 = n * n * n;
   const double read_u = n3 + 6.0 * g * n * n;
-  const double per_block = 8.0 * nf * (read_u + n3 + (beta_is_one ? 0.0 : n3));
+  // with AMRH_BOUNDARY_OUTFLOW the substep also writes the next stage's face ghost slabs
+  const double ghost_w = c->cfg.boundary == AMRH_BOUNDARY_OUTFLOW ? 6.0 * g * n * n : 0.0;
+  const double per_block = 8.0 * nf * (read_u + n3 + ghost_w + (beta_is_one ? 0.0 : n3));
   return per_block * c->nb;
 }
 

@@ -55,6 +55,12 @@ struct amrh_ctx {
   hipStream_t own_stream = nullptr;
   bool have_h = false, have_u = false;
   amrh::TreeState* tree = nullptr;
+  int grid_substep = 0;  // persistent grid of the substep kernel (0: one workgroup per block)
+  int grid_maxeig = 0;   // same for the CFL scan
+  // captured RK steps, one graph per parity of the ping-pong buffers (amrh_graph.hip)
+  hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+  int step_exec_rk = 0;
+  const double* step_exec_dt = nullptr;
 };
 
 namespace amrh {

@@ -6,8 +6,9 @@
 
 namespace amrh {
 
-hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, hipStream_t s);
-hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, hipStream_t s);
+// grid <= 0: one workgroup per block; otherwise a persistent grid of that many workgroups
+hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s);
+hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s);
 hipError_t launch_outflow_ghosts(int n, double* U, int nblocks, int z_reflect, hipStream_t s);
 hipError_t launch_pack(int n, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s);
 hipError_t launch_dt_from_gmax(unsigned long long* gmax, double cfl, double scale, double growth, double* dt,

@@ -128,16 +128,19 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // x^y for x > 0 via the hardware log2/exp2 (about 1-2 ulp in FP32)
 __device__ __forceinline__ float fast_pow(float x, float y) {
+#pragma clang fp contract(fast)
   return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
 }
 
 // minmod(theta*a, theta*b, (a+b)/2): nonzero only when all three agree in sign
 __device__ __forceinline__ float minmod_theta(float a, float b, float th) {
+#pragma clang fp contract(fast)
   const float c = 0.5f * (a + b);
   const float m = fminf(fminf(fabsf(th * a), fabsf(th * b)), fabsf(c));
   return (a * b > 0.0f) ? copysignf(m, a) : 0.0f;
 }
 __device__ __forceinline__ void slope(const float* a, const float* b, const float* c, float* s, float th) {
+#pragma clang fp contract(fast)
 #pragma unroll
   for (int l = 0; l < NFE; l++) s[l] = minmod_theta(c[l] - b[l], b[l] - a[l], th);
 }
@@ -145,6 +148,7 @@ __device__ __forceinline__ void slope(const float* a, const float* b, const floa
 // one reconstructed face state (primitive q: rho, vx, vy, vz, eps, tau) -> conserved U, flux, eig
 template <int D>
 __device__ __forceinline__ void state_flux(const PhysF& P, const float* q, float* U, float* f, float& e) {
+#pragma clang fp contract(fast)
   const float rho = q[RHO];
   const float v2 = q[SX] * q[SX] + q[SY] * q[SY] + q[SZ] * q[SZ];
   const float ekin = 0.5f * rho * v2;
@@ -168,6 +172,7 @@ __device__ __forceinline__ void state_flux(const PhysF& P, const float* q, float
 template <int D>
 __device__ __forceinline__ void kt_face(const PhysF& P, const float* pl, const float* sl, const float* pr,
                                         const float* sr, float* F, float& eig) {
+#pragma clang fp contract(fast)
   float ql[NFE], qr[NFE], UL[NFE], UR[NFE], fl[NFE], fr[NFE], el, er;
 #pragma unroll
   for (int l = 0; l < NFE; l++) {
@@ -185,6 +190,7 @@ __device__ __forceinline__ void kt_face(const PhysF& P, const float* pl, const f
 template <int D>
 __device__ __forceinline__ float face_eig(const PhysF& P, const float* pl, const float* sl, const float* pr,
                                           const float* sr) {
+#pragma clang fp contract(fast)
   float U[NFE], f[NFE], el, er, q[NFE];
 #pragma unroll
   for (int l = 0; l < NFE; l++) q[l] = pl[l] + 0.5f * sl[l];
@@ -193,6 +199,24 @@ __device__ __forceinline__ float face_eig(const PhysF& P, const float* pl, const
   for (int l = 0; l < NFE; l++) q[l] = pr[l] - 0.5f * sr[l];
   state_flux<D>(P, q, U, f, er);
   return fmaxf(el, er);
+}
+
+// FP64 helpers of the mixed form's conversion and update (not bit-matched to the reference)
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);  // ~2^-26 relative
+  const double e = fma(-x, r, 1.0);
+  return fma(r, e, r);  // one Newton step: ~2^-52
+}
+
+// x^(1/gamma) for x >= 1e-20: FP32 estimate, then for gamma = 5/3 one FP64 Newton step on
+// y^5 = x^3 (relative error ~1e-13); other gamma fall back to pow.
+__device__ __forceinline__ double pow_inv_gamma(double x, double gamma) {
+  if (gamma != 5.0 / 3.0) return pow(x, 1.0 / gamma);
+  const float xf = (float)x;
+  const double y0 = (double)__builtin_amdgcn_exp2f(0.6f * __builtin_amdgcn_logf(xf));
+  const double y2 = y0 * y0;
+  const double x3 = x * x * x;
+  return fma(0.8, y0, 0.2 * x3 * rcp64(y2 * y2));
 }
 
 }  // namespace f32

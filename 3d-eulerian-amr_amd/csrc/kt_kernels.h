@@ -34,6 +34,9 @@ struct SubstepArgs {
   const double* ovr;       // override face fluxes [slot][N/2][N/2][NF] (fine-face averages)
   int ledger_block;        // block whose domain-boundary fluxes feed the flow-off ledger
   double* ledger;          // [3 dir][2 side][N][N][NF] boundary face fluxes of that block
+  // isolated outflow blocks: write the next stage's ghosts with the interior
+  int outflow;
+  int z_reflect;
 };
 
 struct MaxEigArgs {

@@ -30,7 +30,7 @@ __device__ __forceinline__ void make_prim(const double* u, typename Prec<MIXED>:
     for (int l = 0; l < NFE; l++) p[l] = u[l];
     x64::to_prim(p);
   } else {
-    const double inv = 1.0 / u[RHO];
+    const double inv = f32::rcp64(u[RHO]);
     const double vx = u[SX] * inv, vy = u[SY] * inv, vz = u[SZ] * inv;
     const double ekin = 0.5 * (u[SX] * vx + u[SY] * vy + u[SZ] * vz);
     p[RHO] = (float)u[RHO];
@@ -40,6 +40,15 @@ __device__ __forceinline__ void make_prim(const double* u, typename Prec<MIXED>:
     p[ET] = (float)(u[ET] - ekin);
     p[TAU] = (float)u[TAU];
   }
+}
+
+// State::floor of the mixed form (euler/state.cpp:86-92): same tests in FP64, the entropy tracer
+// from pow_inv_gamma instead of libm pow
+__device__ __forceinline__ void floor_mixed(const PhysParams& P, double* u) {
+  u[RHO] = x64::rmax(u[RHO], P.rho_floor);
+  const double ek = 0.5 * (u[SX] * u[SX] + u[SY] * u[SY] + u[SZ] * u[SZ]) * f32::rcp64(u[RHO]);
+  const double ei0 = u[ET] - ek;
+  if (ei0 > 0.1 * u[ET]) u[TAU] = f32::pow_inv_gamma(x64::rmax(ei0, P.ei_floor), P.gamma);
 }
 
 template <bool MIXED, int D>
@@ -108,23 +117,103 @@ __device__ __forceinline__ long col_elem(const Layout& L, int i, int j, int kk) 
   return L.interior(i, j, kk);
 }
 
+template <bool MIXED>
+__device__ __forceinline__ void slope_of(const PhysParams& P, const typename Prec<MIXED>::T* a,
+                                         const typename Prec<MIXED>::T* b, const typename Prec<MIXED>::T* c,
+                                         typename Prec<MIXED>::T* s) {
+  if constexpr (!MIXED)
+    x64::slope(a, b, c, s, P.theta);
+  else
+    f32::slope(a, b, c, s, (float)P.theta);
+}
+
+template <bool MIXED, int D>
+__device__ __forceinline__ void face_of(const PhysParams& P, const typename Prec<MIXED>::T* pl,
+                                        const typename Prec<MIXED>::T* sl, const typename Prec<MIXED>::T* pr,
+                                        const typename Prec<MIXED>::T* sr, typename Prec<MIXED>::T* F) {
+  if constexpr (!MIXED) {
+    double e;
+    x64::kt_face<D>(P, pl, sl, pr, sr, F, e);
+  } else {
+    const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+    float e;
+    f32::kt_face<D>(PF, pl, sl, pr, sr, F, e);
+  }
+}
+
+template <bool MIXED, int D>
+__device__ __forceinline__ double eig_of(const PhysParams& P, const typename Prec<MIXED>::T* pl,
+                                         const typename Prec<MIXED>::T* sl, const typename Prec<MIXED>::T* pr,
+                                         const typename Prec<MIXED>::T* sr) {
+  if constexpr (!MIXED) {
+    return x64::face_eig<D>(P, pl, sl, pr, sr);
+  } else {
+    const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+    return (double)f32::face_eig<D>(PF, pl, sl, pr, sr);
+  }
+}
+
+// outflow ghost of an updated boundary cell u across node face `face` (the boundary object's face
+// is the node's inverted: a lower node face removes v > 0; the z test uses sz,
+// euler/state.cpp:45-84)
+__device__ __forceinline__ void outflow_fix(double* u, int face) {
+  const int d = face >> 1;
+  const bool upper = face & 1;
+  const double v = (d == 2) ? u[SZ] : u[SX + d] / u[RHO];
+  const bool inflow = upper ? (v < 0.0) : (v > 0.0);
+  if (inflow) {
+    const double s = u[SX + d];
+    u[ET] = u[ET] - 0.5 * s * s / u[RHO];
+    u[SX + d] = 0.0;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void put_ghost(double* Uo, const Layout& L, int face, int t, int i, int j, int k,
+                                          const double* u, bool fix) {
+  constexpr long R = (long)N * N * N + 6L * GW * N * N;
+  double g[NFE];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) g[l] = u[l];
+  if (fix) outflow_fix(g, face);
+  long e;
+  if (face < 2)
+    e = L.slab(face) + ((long)k * N + j) * GW + t;
+  else if (face < 4)
+    e = L.slab(face) + ((long)k * GW + t) * N + i;
+  else
+    e = L.slab(face) + ((long)t * N + j) * N + i;
+#pragma unroll
+  for (int l = 0; l < NFE; l++) Uo[l * R + e] = g[l];
+}
+
 // MODE 0: substep (flux + divergence + update).  MODE 1: max wavespeed scan only.
+// Per plane k: stage the plane's primitives (with x/y ghosts) in LDS; x- and y-slopes of every
+// cell once; the x/y faces from two cells and their slopes; the z-face from the column window;
+// then each thread updates its cell.  Three barriers per plane.
 template <int N, bool MIXED, int MODE>
 __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArgs m) {
   using T = typename Prec<MIXED>::T;
   constexpr int PW = N + 2 * GW;  // padded plane edge
   constexpr int NT = N * N;
-  constexpr int NFX = N * (N + 1);
-  constexpr int HN = GW * N;  // halo cells per face per plane
+  constexpr int NFX = N * (N + 1);  // x faces of a plane (= y faces)
+  constexpr int NSX = N * (N + 2);  // x slopes of a plane (= y slopes)
+  constexpr int HN = GW * N;        // halo cells per face per plane
   constexpr Layout L{N, GW, NFE};
   constexpr long R = L.rec();
 
   __shared__ T plane[NFE][PW][PW];
+  __shared__ T Sx[NFE][N][N + 2];  // x-slope of plane cell (padded x = idx+1, y = row+G)
+  __shared__ T Sy[NFE][N + 2][N];  // y-slope of plane cell (padded y = idx+1, x = col+G)
   __shared__ T Fx[MODE == 0 ? NFE : 1][N][N + 1];
   __shared__ T Fy[MODE == 0 ? NFE : 1][N + 1][N];
   __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
 
-  const int b = (MODE == 0 ? a.block0 : 0) + blockIdx.x;
+  // persistent: a grid smaller than the batch walks it with a stride (launch shape chosen by the
+  // autotuner); with one workgroup per block the loop runs once
+  const int nbl = MODE == 0 ? a.nblocks : m.nblocks;
+  for (int bb = blockIdx.x; bb < nbl; bb += gridDim.x) {
+  const int b = (MODE == 0 ? a.block0 : 0) + bb;
   const int tid = threadIdx.x;
   const int i = tid % N, j = tid / N;
   const PhysParams P = MODE == 0 ? a.P : m.P;
@@ -138,11 +227,11 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
 
   // ---- prologue: column cells k = -2 .. 2 ------------------------------------------------
   double ucur[NFE], uc1[NFE], uc2[NFE];
-  T cm1[NFE], c0[NFE], cp1[NFE], cp2[NFE], s0[NFE], Fzlo[NFE];
+  T c0[NFE], cp1[NFE], cp2[NFE], s0[NFE], Fzlo[NFE];
   double emax = 0.0;
   {
     double u[NFE];
-    T cm2[NFE], sm1[NFE];
+    T cm2[NFE], cm1[NFE], sm1[NFE];
     long e = col_elem<N>(L, i, j, -2);
 #pragma unroll
     for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
@@ -163,33 +252,17 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
 #pragma unroll
     for (int l = 0; l < NFE; l++) uc2[l] = Ub[l * R + e];
     make_prim<MIXED>(uc2, cp2);
-    if constexpr (!MIXED) {
-      x64::slope(cm2, cm1, c0, sm1, P.theta);
-      x64::slope(cm1, c0, cp1, s0, P.theta);
-    } else {
-      f32::slope(cm2, cm1, c0, sm1, (float)P.theta);
-      f32::slope(cm1, c0, cp1, s0, (float)P.theta);
-    }
+    slope_of<MIXED>(P, cm2, cm1, c0, sm1);
+    slope_of<MIXED>(P, cm1, c0, cp1, s0);
     if constexpr (MODE == 0) {
-      T sl[NFE];
-      // face k = 0 from cells -1 (slope sm1) and 0 (slope s0)
-      if constexpr (!MIXED) {
-        double e2;
-        x64::kt_face<2>(P, cm1, sm1, c0, s0, Fzlo, e2);
-      } else {
-        const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
-        float e2;
-        f32::kt_face<2>(PF, cm1, sm1, c0, s0, Fzlo, e2);
-      }
+      face_of<MIXED, 2>(P, cm1, sm1, c0, s0, Fzlo);  // face k = 0
       if (a.slot) apply_override<N>(a, b, 2, 0, i, j, Fzlo);
       if (b == a.ledger_block) {
 #pragma unroll
         for (int l = 0; l < NFE; l++) a.ledger[(((2 * 2 + 0) * N + j) * N + i) * NFE + l] = (double)Fzlo[l];
       }
-      (void)sl;
     } else {
-      // eigenvalue of face 0 (cells -2..1)
-      emax = x64::rmax(emax, face_eig<MIXED, 2>(P, cm2, cm1, c0, cp1));
+      emax = x64::rmax(emax, eig_of<MIXED, 2>(P, cm1, sm1, c0, s0));
     }
   }
 
@@ -197,7 +270,7 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
   double hraw[NFE];
   const bool is_halo = tid < 4 * HN;
   int hpx = 0, hpy = 0;
-  long hbase = 0, hstride = 0;
+  long hbase = 0;
   if (is_halo) {
     const int face = tid / HN, r = tid % HN;
     if (face < 2) {
@@ -210,13 +283,12 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
       hpy = face == YL ? t : N + GW + t;
     }
     hbase = L.slab(face) + r;  // plane k adds k*HN (both slab kinds are plane-major)
-    hstride = HN;
 #pragma unroll
     for (int l = 0; l < NFE; l++) hraw[l] = Ub[l * R + hbase];
   }
 
   for (int k = 0; k < N; k++) {
-    // 1. stage plane k (interior from registers, halo from the prefetched raw values)
+    // A. stage plane k (interior from registers, halo from the prefetched raw values)
 #pragma unroll
     for (int l = 0; l < NFE; l++) plane[l][j + GW][i + GW] = c0[l];
     if (is_halo) {
@@ -225,10 +297,10 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
 #pragma unroll
       for (int l = 0; l < NFE; l++) plane[l][hpy][hpx] = hp[l];
     }
-    // 2. prefetch: halo of plane k+1, column cell k+3, U0 at k
+    // prefetch: halo of plane k+1, column cell k+3, U0 at k
     if (is_halo && k + 1 < N) {
 #pragma unroll
-      for (int l = 0; l < NFE; l++) hraw[l] = Ub[l * R + hbase + (long)(k + 1) * hstride];
+      for (int l = 0; l < NFE; l++) hraw[l] = Ub[l * R + hbase + (long)(k + 1) * HN];
     }
     double craw[NFE];
     const bool have_next = (k + 3) < N + GW;
@@ -246,27 +318,61 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
     }
     __syncthreads();
 
-    // 3. x- and y-faces of plane k
+    // B. x- and y-slopes of every plane cell that a face uses (reconstruct.cpp:81-85)
+#pragma unroll 1
+    for (int q = tid; q < 2 * NSX; q += NT) {
+      T pa[NFE], pb[NFE], pc[NFE], sl[NFE];
+      if (q < NSX) {
+        const int jj = q / (N + 2), sx = q % (N + 2);
+        const int py = jj + GW;
+#pragma unroll
+        for (int l = 0; l < NFE; l++) {
+          pa[l] = plane[l][py][sx];
+          pb[l] = plane[l][py][sx + 1];
+          pc[l] = plane[l][py][sx + 2];
+        }
+        slope_of<MIXED>(P, pa, pb, pc, sl);
+#pragma unroll
+        for (int l = 0; l < NFE; l++) Sx[l][jj][sx] = sl[l];
+      } else {
+        const int q2 = q - NSX;
+        const int ii = q2 % N, sy = q2 / N;
+        const int px = ii + GW;
+#pragma unroll
+        for (int l = 0; l < NFE; l++) {
+          pa[l] = plane[l][sy][px];
+          pb[l] = plane[l][sy + 1][px];
+          pc[l] = plane[l][sy + 2][px];
+        }
+        slope_of<MIXED>(P, pa, pb, pc, sl);
+#pragma unroll
+        for (int l = 0; l < NFE; l++) Sy[l][sy][ii] = sl[l];
+      }
+    }
+    __syncthreads();
+
+    // C. x- and y-faces of plane k, then the z-face k+1 of the column
+#pragma unroll 1
     for (int it = tid; it < 2 * NFX; it += NT) {
-      T pm2[NFE], pm1[NFE], p0[NFE], pp1[NFE];
+      T pl[NFE], pr[NFE], sl[NFE], sr[NFE];
       if (it < NFX) {
         const int fi = it % (N + 1), jj = it / (N + 1);
         const int py = jj + GW;
 #pragma unroll
         for (int l = 0; l < NFE; l++) {
-          pm2[l] = plane[l][py][fi];
-          pm1[l] = plane[l][py][fi + 1];
-          p0[l] = plane[l][py][fi + 2];
-          pp1[l] = plane[l][py][fi + 3];
+          pl[l] = plane[l][py][fi + 1];
+          pr[l] = plane[l][py][fi + 2];
+          sl[l] = Sx[l][jj][fi];
+          sr[l] = Sx[l][jj][fi + 1];
         }
         if constexpr (MODE == 0) {
           T F[NFE];
-          face_flux<MIXED, 0>(P, pm2, pm1, p0, pp1, F);
+          face_of<MIXED, 0>(P, pl, sl, pr, sr, F);
           if (a.slot) apply_override<N>(a, b, 0, fi, jj, k, F);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fx[l][jj][fi] = F[l];
         } else {
-          emax = x64::rmax(emax, face_eig<MIXED, 0>(P, pm2, pm1, p0, pp1));
+          emax = x64::rmax(emax, eig_of<MIXED, 0>(P, pl, sl, pr, sr));
         }
       } else {
         const int it2 = it - NFX;
@@ -274,64 +380,58 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
         const int px = ii + GW;
 #pragma unroll
         for (int l = 0; l < NFE; l++) {
-          pm2[l] = plane[l][fj][px];
-          pm1[l] = plane[l][fj + 1][px];
-          p0[l] = plane[l][fj + 2][px];
-          pp1[l] = plane[l][fj + 3][px];
+          pl[l] = plane[l][fj + 1][px];
+          pr[l] = plane[l][fj + 2][px];
+          sl[l] = Sy[l][fj][ii];
+          sr[l] = Sy[l][fj + 1][ii];
         }
         if constexpr (MODE == 0) {
           T F[NFE];
-          face_flux<MIXED, 1>(P, pm2, pm1, p0, pp1, F);
+          face_of<MIXED, 1>(P, pl, sl, pr, sr, F);
           if (a.slot) apply_override<N>(a, b, 1, fj, ii, k, F);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fy[l][fj][ii] = F[l];
         } else {
-          emax = x64::rmax(emax, face_eig<MIXED, 1>(P, pm2, pm1, p0, pp1));
+          emax = x64::rmax(emax, eig_of<MIXED, 1>(P, pl, sl, pr, sr));
         }
       }
     }
-
-    // 4. z-face k+1 (cells k, k+1 with slopes s0, sp1)
     T sp1[NFE], Fzhi[NFE];
-    if constexpr (!MIXED)
-      x64::slope(c0, cp1, cp2, sp1, P.theta);
-    else
-      f32::slope(c0, cp1, cp2, sp1, (float)P.theta);
+    slope_of<MIXED>(P, c0, cp1, cp2, sp1);
     if constexpr (MODE == 0) {
-      if constexpr (!MIXED) {
-        double e2;
-        x64::kt_face<2>(P, c0, s0, cp1, sp1, Fzhi, e2);
-      } else {
-        const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
-        float e2;
-        f32::kt_face<2>(PF, c0, s0, cp1, sp1, Fzhi, e2);
-      }
+      face_of<MIXED, 2>(P, c0, s0, cp1, sp1, Fzhi);
       if (a.slot) apply_override<N>(a, b, 2, k + 1, i, j, Fzhi);
     } else {
-      if constexpr (!MIXED)
-        emax = x64::rmax(emax, x64::face_eig<2>(P, c0, s0, cp1, sp1));
-      else {
-        const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
-        emax = x64::rmax(emax, (double)f32::face_eig<2>(PF, c0, s0, cp1, sp1));
-      }
+      emax = x64::rmax(emax, eig_of<MIXED, 2>(P, c0, s0, cp1, sp1));
     }
     __syncthreads();
 
-    // 5. divergence (x, then y, then z) and the RK update of cell (i, j, k)
+    // D. divergence (x, then y, then z) and the RK update of cell (i, j, k)
     if constexpr (MODE == 0) {
       double u[NFE];
 #pragma unroll
       for (int l = 0; l < NFE; l++) {
-        double D = 0.0 - ((double)Fx[l][j][i + 1] - (double)Fx[l][j][i]) * hinv;
-        D -= ((double)Fy[l][j + 1][i] - (double)Fy[l][j][i]) * hinv;
-        D -= ((double)Fzhi[l] - (double)Fzlo[l]) * hinv;
+        double D;
+        if constexpr (!MIXED) {
+          D = 0.0 - (Fx[l][j][i + 1] - Fx[l][j][i]) * hinv;
+          D -= (Fy[l][j + 1][i] - Fy[l][j][i]) * hinv;
+          D -= (Fzhi[l] - Fzlo[l]) * hinv;
+        } else {
+          // FP32 differences of FP32 fluxes, accumulated in FP64
+          D = 0.0 - (double)(Fx[l][j][i + 1] - Fx[l][j][i]) * hinv;
+          D -= (double)(Fy[l][j + 1][i] - Fy[l][j][i]) * hinv;
+          D -= (double)(Fzhi[l] - Fzlo[l]) * hinv;
+        }
         D += 0.0;  // State::source is zero for the ideal gas (euler/state.cpp:19-22)
         if (has_u0)
           u[l] = (ucur[l] + D * dt) * beta + u0v[l] * (1.0 - beta);
         else
           u[l] = (ucur[l] + D * dt) * beta;
       }
-      x64::floor_state(P, u);
+      if constexpr (!MIXED)
+        x64::floor_state(P, u);
+      else
+        floor_mixed(P, u);
       if (!(u[RHO] > 0.0) && a.err) atomicOr(a.err, 1);
       if (b == a.ledger_block) {
         // boundary faces of the ledger block (grid/grid.cpp:350-404 sums them per pencil)
@@ -359,9 +459,31 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
       const long e = L.interior(i, j, k);
 #pragma unroll
       for (int l = 0; l < NFE; l++) Uo[l * R + e] = u[l];
+      if (a.outflow) {
+        // The next stage's ghosts of an isolated outflow block are functions of its own boundary
+        // cells (grid/grid_phys_bound.cpp:27-54): write them now instead of in a separate pass.
+        // One rolled loop over the faces this cell touches keeps a single copy of the code.
+        const unsigned touch = (i == 0 ? 1u << XL : 0u) | (i == N - 1 ? 1u << XU : 0u) |
+                               (j == 0 ? 1u << YL : 0u) | (j == N - 1 ? 1u << YU : 0u) |
+                               ((a.z_reflect ? k < GW : k == 0) ? 1u << ZL : 0u) | (k == N - 1 ? 1u << ZU : 0u);
+#pragma unroll 1
+        for (int f = 0; f < 6; f++) {
+          if (!((touch >> f) & 1u)) continue;
+          const bool mirror = a.z_reflect && f == ZL;
+          double g[NFE];
+#pragma unroll
+          for (int l = 0; l < NFE; l++) g[l] = u[l];
+          if (mirror)
+            g[SZ] = -g[SZ];  // ghost G-1-k holds interior cell k mirrored (euler/state.cpp:94-96)
+          else
+            outflow_fix(g, f);
+#pragma unroll 1
+          for (int t = mirror ? GW - 1 - k : 0; t < (mirror ? GW - k : GW); t++) put_ghost<N>(Uo, L, f, t, i, j, k, g, false);
+        }
+      }
     }
 
-    // 6. advance the column window
+    // advance the column window
 #pragma unroll
     for (int l = 0; l < NFE; l++) {
       c0[l] = cp1[l];
@@ -393,7 +515,7 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
       }
     }
   }
-  (void)cm1;
+  }  // block loop
 }
 
 // Ghost fill of isolated blocks: every face slab cell copies the adjacent interior cell and
@@ -520,36 +642,38 @@ __global__ void reset_u64_kernel(unsigned long long* p) { *p = 0ull; }
 // launchers
 // ---------------------------------------------------------------------------------------------
 template <int N>
-static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, hipStream_t s) {
+static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, int grid, hipStream_t s) {
   MaxEigArgs m{};
   if (a.nblocks <= 0) return hipSuccess;
+  const int g = (grid > 0 && grid < a.nblocks) ? grid : a.nblocks;
   if (mixed)
-    kt_sweep_kernel<N, true, 0><<<a.nblocks, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, true, 0><<<g, N * N, 0, s>>>(a, m);
   else
-    kt_sweep_kernel<N, false, 0><<<a.nblocks, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, false, 0><<<g, N * N, 0, s>>>(a, m);
   return hipGetLastError();
 }
 
-hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, hipStream_t s) {
-  if (n == 8) return launch_substep_n<8>(a, mixed, s);
-  if (n == 16) return launch_substep_n<16>(a, mixed, s);
+hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s) {
+  if (n == 8) return launch_substep_n<8>(a, mixed, grid, s);
+  if (n == 16) return launch_substep_n<16>(a, mixed, grid, s);
   return hipErrorInvalidValue;
 }
 
 template <int N>
-static hipError_t launch_maxeig_n(const MaxEigArgs& m, bool mixed, hipStream_t s) {
+static hipError_t launch_maxeig_n(const MaxEigArgs& m, bool mixed, int grid, hipStream_t s) {
   SubstepArgs a{};
   if (m.nblocks <= 0) return hipSuccess;
+  const int g = (grid > 0 && grid < m.nblocks) ? grid : m.nblocks;
   if (mixed)
-    kt_sweep_kernel<N, true, 1><<<m.nblocks, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, true, 1><<<g, N * N, 0, s>>>(a, m);
   else
-    kt_sweep_kernel<N, false, 1><<<m.nblocks, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, false, 1><<<g, N * N, 0, s>>>(a, m);
   return hipGetLastError();
 }
 
-hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, hipStream_t s) {
-  if (n == 8) return launch_maxeig_n<8>(m, mixed, s);
-  if (n == 16) return launch_maxeig_n<16>(m, mixed, s);
+hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s) {
+  if (n == 8) return launch_maxeig_n<8>(m, mixed, grid, s);
+  if (n == 16) return launch_maxeig_n<16>(m, mixed, grid, s);
   return hipErrorInvalidValue;
 }
 

@@ -35,6 +35,12 @@ extern "C" {
 #define AMRH_FP64 0  /* FP64 end to end; matches the FP64 CPU path to roundoff */
 #define AMRH_MIXED 1 /* FP32 reconstruction / fluxes / wavespeeds, FP64 accumulation + RK + floor */
 
+/* ghost layers of a flat batch (amrh_config.boundary) */
+#define AMRH_BOUNDARY_EXTERNAL 0 /* the caller fills the ghost slabs before every substep */
+#define AMRH_BOUNDARY_OUTFLOW 1  /* every block is an isolated domain with outflow faces
+                                    (z-mirror below with z_reflect): each substep writes the next
+                                    stage's ghost slabs as it stores the boundary cells */
+
 /* device error flags (amrh_error) */
 #define AMRH_DERR_NEG_DENSITY 1 /* a face or cell state with rho <= 0 reached the flux */
 
@@ -52,6 +58,7 @@ typedef struct {
   double theta;      /* 1.3 (euler/euler_defs.h:20) */
   double cfl;        /* 0.4 (euler/euler_defs.h:46) */
   double grid_dim;   /* half edge of the root domain (grid/grid.cpp:70,512-526) */
+  int boundary;      /* AMRH_BOUNDARY_* (flat batches; the octree path fills its own ghosts) */
 } amrh_config;
 
 typedef struct amrh_ctx amrh_ctx;
@@ -115,8 +122,25 @@ int amrh_dt_device(amrh_ctx* ctx, double scale, double growth, double* dt_dev);
 int amrh_error(amrh_ctx* ctx, int* flags);
 
 /* Bytes of HBM one substep over the batch moves by the algorithmic count (interior + face
- * ghost slabs read, interior written, U0 read when beta != 1). */
+ * ghost slabs read, interior written, U0 read when beta != 1; plus the face ghost slabs written
+ * when boundary = AMRH_BOUNDARY_OUTFLOW). */
 double amrh_substep_bytes(const amrh_ctx* ctx, int beta_is_one);
+
+/* ---- launch shape and whole steps (no counterpart in the reference: its loops are OpenMP) -----
+ * The sweep kernels walk the batch with a grid stride.  amrh_autotune times one workgroup per
+ * block against persistent grids of k workgroups per CU on the context's own state and keeps the
+ * fastest for the substep and for the CFL scan (writes the spare state buffer: call it before
+ * amrh_store).  amrh_set_launch sets them by hand (0 = one workgroup per block). */
+int amrh_autotune(amrh_ctx* ctx, int* grid_substep, int* grid_maxeig);
+int amrh_set_launch(amrh_ctx* ctx, int grid_substep, int grid_maxeig);
+/* HydroDriver::step repeated nsteps times (euler/hydro_driver.cpp:231-254, regrid excluded) with
+ * the reference's dt control on the device: dt_dev[0] holds the previous dt and becomes
+ * min(MAXDTINC*dt, CFL dt).  For the octree (after amrh_tree_set) each step fills ghosts, scans
+ * the CFL speed, stores U0 and runs rk tree substeps; for a flat batch it needs
+ * boundary = AMRH_BOUNDARY_OUTFLOW.  With use_graph each step is one launch of a captured
+ * hipGraph (two graphs, one per ping-pong parity, captured on first use; a new dt_dev, rk or
+ * launch shape re-captures).  Asynchronous. */
+int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_graph);
 
 /* ---- octree coupling (GridNode recursions, grid/grid_node.cpp) -------------------------------
  * Block b of the context is node b of the tree.  Node 0 is the root (parent[0] = -1); every other
