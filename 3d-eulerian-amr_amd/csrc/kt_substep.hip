@@ -417,10 +417,11 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
           D -= (Fy[l][j + 1][i] - Fy[l][j][i]) * hinv;
           D -= (Fzhi[l] - Fzlo[l]) * hinv;
         } else {
-          // FP32 differences of FP32 fluxes, accumulated in FP64
-          D = 0.0 - (double)(Fx[l][j][i + 1] - Fx[l][j][i]) * hinv;
-          D -= (double)(Fy[l][j + 1][i] - Fy[l][j][i]) * hinv;
-          D -= (double)(Fzhi[l] - Fzlo[l]) * hinv;
+          // FP32 fluxes, differenced in FP64: the difference of two FP32 numbers is exact in FP64,
+          // so the divergence still telescopes and mass and energy are conserved to roundoff
+          D = 0.0 - ((double)Fx[l][j][i + 1] - (double)Fx[l][j][i]) * hinv;
+          D -= ((double)Fy[l][j + 1][i] - (double)Fy[l][j][i]) * hinv;
+          D -= ((double)Fzhi[l] - (double)Fzlo[l]) * hinv;
         }
         D += 0.0;  // State::source is zero for the ideal gas (euler/state.cpp:19-22)
         if (has_u0)
