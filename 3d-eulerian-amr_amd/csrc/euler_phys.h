@@ -153,7 +153,10 @@ __device__ __forceinline__ void state_flux(const PhysF& P, const float* q, float
   const float v2 = q[SX] * q[SX] + q[SY] * q[SY] + q[SZ] * q[SZ];
   const float ekin = 0.5f * rho * v2;
   const float E = q[ET] + ekin;
-  const float p = (q[ET] < 0.001f * E) ? P.gm1 * fast_pow(q[TAU], P.gamma) : P.gm1 * q[ET];
+  float p = P.gm1 * q[ET];
+  // dual-energy branch (rare): a real branch, so the transcendental pair is skipped by whole
+  // wavefronts that do not need it
+  if (__builtin_expect(q[ET] < 0.001f * E, 0)) p = P.gm1 * fast_pow(q[TAU], P.gamma);
   const float vd = q[SX + D];
   const float cs = fast_sqrt(P.gamma * p * fast_rcp(rho));
   e = fabsf(vd) + cs;

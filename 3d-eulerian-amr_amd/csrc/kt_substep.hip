@@ -208,6 +208,7 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
   __shared__ T Fx[MODE == 0 ? NFE : 1][N][N + 1];
   __shared__ T Fy[MODE == 0 ? NFE : 1][N + 1][N];
   __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+  __shared__ double gb[MODE == 0 ? 4 : 1][N][NFE];  // updated x/y boundary cells of the last plane
 
   // persistent: a grid smaller than the batch walks it with a stride (launch shape chosen by the
   // autotuner); with one workgroup per block the loop runs once
@@ -223,6 +224,23 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
   const double beta = a.beta;
   if (MODE == 0 && a.dt_dev) dt = *a.dt_dev;
   const bool has_u0 = MODE == 0 && beta != 1.0;
+  // x/y ghost-slab writers of the outflow epilogue: the last 4*G*N threads, one slab cell each
+  constexpr int NGW = 4 * GW * N;
+  const int gw = tid - (NT - NGW);
+  const bool is_gw = MODE == 0 && a.outflow && gw >= 0;
+  double* const Uob = MODE == 0 ? a.Uout + (long)b * NFE * R : nullptr;
+  auto write_xy_ghost = [&](int kk) {
+    const int face = gw / (GW * N), r = gw % (GW * N);
+    const double* src = face < 2 ? gb[face][r / GW] : gb[face][r % N];
+    double g[NFE];
+#pragma unroll
+    for (int l = 0; l < NFE; l++) g[l] = src[l];
+    outflow_fix(g, face);
+    // X slabs: (kk*N + jj)*G + t = kk*G*N + r;  Y slabs: (kk*G + t)*N + ii = kk*G*N + r
+    const long e = L.slab(face) + (long)kk * GW * N + r;
+#pragma unroll
+    for (int l = 0; l < NFE; l++) Uob[l * R + e] = g[l];
+  };
   const double hinv = MODE == 0 ? 1.0 / a.h[b] : 0.0;
 
   // ---- prologue: column cells k = -2 .. 2 ------------------------------------------------
@@ -317,6 +335,9 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
       for (int l = 0; l < NFE; l++) u0v[l] = U0b[l * R + e];
     }
     __syncthreads();
+
+    // the previous plane's x/y outflow ghosts (stashed in phase D, fenced by the barrier above)
+    if (is_gw && k > 0) write_xy_ghost(k - 1);
 
     // B. x- and y-slopes of every plane cell that a face uses (reconstruct.cpp:81-85)
 #pragma unroll 1
@@ -416,18 +437,20 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
           D = 0.0 - (Fx[l][j][i + 1] - Fx[l][j][i]) * hinv;
           D -= (Fy[l][j + 1][i] - Fy[l][j][i]) * hinv;
           D -= (Fzhi[l] - Fzlo[l]) * hinv;
+          D += 0.0;  // State::source is zero for the ideal gas (euler/state.cpp:19-22)
+          if (has_u0)
+            u[l] = (ucur[l] + D * dt) * beta + u0v[l] * (1.0 - beta);
+          else
+            u[l] = (ucur[l] + D * dt) * beta;
         } else {
           // FP32 fluxes, differenced in FP64: the difference of two FP32 numbers is exact in FP64,
           // so the divergence still telescopes and mass and energy are conserved to roundoff
-          D = 0.0 - ((double)Fx[l][j][i + 1] - (double)Fx[l][j][i]) * hinv;
-          D -= ((double)Fy[l][j + 1][i] - (double)Fy[l][j][i]) * hinv;
-          D -= ((double)Fzhi[l] - (double)Fzlo[l]) * hinv;
+          const double dsum = (((double)Fx[l][j][i + 1] - (double)Fx[l][j][i]) +
+                               ((double)Fy[l][j + 1][i] - (double)Fy[l][j][i])) +
+                              ((double)Fzhi[l] - (double)Fzlo[l]);
+          const double un = fma(-dt * hinv, dsum, ucur[l]);
+          u[l] = has_u0 ? fma(beta, un, u0v[l] * (1.0 - beta)) : un * beta;
         }
-        D += 0.0;  // State::source is zero for the ideal gas (euler/state.cpp:19-22)
-        if (has_u0)
-          u[l] = (ucur[l] + D * dt) * beta + u0v[l] * (1.0 - beta);
-        else
-          u[l] = (ucur[l] + D * dt) * beta;
       }
       if constexpr (!MIXED)
         x64::floor_state(P, u);
@@ -456,21 +479,33 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
           for (int l = 0; l < NFE; l++) lg[(((2 * 2 + 1) * N + j) * N + i) * NFE + l] = (double)Fzhi[l];
         }
       }
-      double* Uo = a.Uout + (long)b * NFE * R;
+      double* Uo = Uob;
       const long e = L.interior(i, j, k);
 #pragma unroll
       for (int l = 0; l < NFE; l++) Uo[l * R + e] = u[l];
       if (a.outflow) {
         // The next stage's ghosts of an isolated outflow block are functions of its own boundary
-        // cells (grid/grid_phys_bound.cpp:27-54): write them now instead of in a separate pass.
-        // One rolled loop over the faces this cell touches keeps a single copy of the code.
-        const unsigned touch = (i == 0 ? 1u << XL : 0u) | (i == N - 1 ? 1u << XU : 0u) |
-                               (j == 0 ? 1u << YL : 0u) | (j == N - 1 ? 1u << YU : 0u) |
-                               ((a.z_reflect ? k < GW : k == 0) ? 1u << ZL : 0u) | (k == N - 1 ? 1u << ZU : 0u);
-#pragma unroll 1
-        for (int f = 0; f < 6; f++) {
-          if (!((touch >> f) & 1u)) continue;
-          const bool mirror = a.z_reflect && f == ZL;
+        // cells (grid/grid_phys_bound.cpp:27-54), so they are written here instead of in a separate
+        // pass.  x/y boundary cells stash their state in LDS; a fixed set of threads writes those
+        // slabs during the next plane's slope phase (no extra barrier, no divergent tail in every
+        // wavefront).  The z slabs belong to whole planes and are written by every thread.
+        if (i == 0)
+#pragma unroll
+          for (int l = 0; l < NFE; l++) gb[0][j][l] = u[l];
+        if (i == N - 1)
+#pragma unroll
+          for (int l = 0; l < NFE; l++) gb[1][j][l] = u[l];
+        if (j == 0)
+#pragma unroll
+          for (int l = 0; l < NFE; l++) gb[2][i][l] = u[l];
+        if (j == N - 1)
+#pragma unroll
+          for (int l = 0; l < NFE; l++) gb[3][i][l] = u[l];
+        // one rolled copy of the z-slab store serves both faces (keeps the register count down)
+        const bool zl = a.z_reflect ? k < GW : k == 0;
+        if (zl || k == N - 1) {
+          const int f = zl ? ZL : ZU;
+          const bool mirror = a.z_reflect && zl;
           double g[NFE];
 #pragma unroll
           for (int l = 0; l < NFE; l++) g[l] = u[l];
@@ -479,7 +514,8 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
           else
             outflow_fix(g, f);
 #pragma unroll 1
-          for (int t = mirror ? GW - 1 - k : 0; t < (mirror ? GW - k : GW); t++) put_ghost<N>(Uo, L, f, t, i, j, k, g, false);
+          for (int t = mirror ? GW - 1 - k : 0; t < (mirror ? GW - k : GW); t++)
+            put_ghost<N>(Uo, L, f, t, i, j, k, g, false);
         }
       }
     }
@@ -498,6 +534,14 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc2[l] = craw[l];
       make_prim<MIXED>(craw, cp2);
+    }
+  }
+
+  if constexpr (MODE == 0) {
+    if (a.outflow) {  // the last plane's x/y ghosts
+      __syncthreads();
+      if (is_gw) write_xy_ghost(N - 1);
+      __syncthreads();  // gb is rewritten by the next block of a persistent grid
     }
   }
 
