@@ -192,7 +192,7 @@ class Batch:
         return lib().amrh_substep_bytes(self.ctx, 1 if beta_is_one else 0)
 
     def cfl_dt(self, last_dt=None, maxinitdt=1e-2, maxdtinc=1.25):
-        """Host-side dt of HydroDriver::step_to_time (euler/hydro_driver.cpp:168-173): the batch's
+        """Host-side dt of HydroDriver::step_to_time (euler/hydro_driver.cpp:107-113): the batch's
         CFL dt, times MAXINITDT on the first step, capped at MAXDTINC * the last dt after."""
         m = self.max_speed()
         dtinv = m if m != 0.0 else 1.0e-10
@@ -242,7 +242,7 @@ class Batch:
         return fo
 
     def tree_step(self, dt, rk=3):
-        """HydroDriver::step on the octree (euler/hydro_driver.cpp:231-254, regrid excluded)."""
+        """HydroDriver::step on the octree (euler/hydro_driver.cpp:171-194, regrid excluded)."""
         betas = {1: (1.0,), 2: (1.0, 0.5), 3: (1.0, 0.25, 2.0 / 3.0)}[rk]
         self.store()
         for b in betas:
@@ -250,7 +250,7 @@ class Batch:
         self.tree_fill_ghosts()
 
     def rk_step(self, dt, rk=3, fill_ghosts=True):
-        """HydroDriver::step for a flat batch (euler/hydro_driver.cpp:231-249)."""
+        """HydroDriver::step for a flat batch (euler/hydro_driver.cpp:171-194)."""
         betas = {1: (1.0,), 2: (1.0, 0.5), 3: (1.0, 0.25, 2.0 / 3.0)}[rk]
         # in AMRH_BOUNDARY_OUTFLOW mode each substep refreshes the ghosts itself
         fill = fill_ghosts and self.cfg.boundary != AMRH_BOUNDARY_OUTFLOW

@@ -663,7 +663,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ pa
 }
 
 // dt = scale * cfl / max_b(eig_b/h_b), optionally capped at growth * (previous dt)
-// (euler/hydro_driver.cpp:168-173); resets the reduction word so the next scan starts at zero
+// (euler/hydro_driver.cpp:107-113); resets the reduction word so the next scan starts at zero
 // without a memset (keeps the step capturable in a graph).
 __global__ void dt_from_gmax_kernel(unsigned long long* gmax, double cfl, double scale, double growth,
                                     double* dt) {

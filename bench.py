@@ -50,7 +50,7 @@ def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
     first = [True]
 
     def step(events=None):
-        # HydroDriver::step_to_time's dt control on the device (euler/hydro_driver.cpp:168-173)
+        # HydroDriver::step_to_time's dt control on the device (euler/hydro_driver.cpp:107-113)
         if first[0]:
             B.dt_device(1e-2, 0.0, dt.data_ptr())
             first[0] = False
@@ -97,7 +97,7 @@ def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
 
 def accuracy(amrhydro, torch, U_dev_ptr, nb, n, steps):
     """Mixed vs FP64 GPU after `steps` RK3 steps on the full workload (same dt sequence: the FP64
-    form's CFL dt under the reference's control, euler/hydro_driver.cpp:168-173).  The FP64 form is
+    form's CFL dt under the reference's control, euler/hydro_driver.cpp:107-113).  The FP64 form is
     pinned to the CPU oracle at 1e-12 by the test suite."""
     forms = {}
     for prec in (amrhydro.AMRH_FP64, amrhydro.AMRH_MIXED):

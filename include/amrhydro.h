@@ -115,7 +115,7 @@ int amrh_max_speed(amrh_ctx* ctx, double* max_eig_over_h);
 /* dt = scale * cfl / max_b(max_eig_b/h_b) computed on the device into dt_dev[0] (no host round
  * trip).  With growth > 0 the result is also capped at growth * dt_dev[0] (the previous dt).  The
  * reference's control is scale = MAXINITDT = 1e-2, growth = 0 at t = 0, then scale = 1,
- * growth = MAXDTINC = 1.25 (euler/hydro_driver.cpp:168-173, euler/euler_defs.h:45,57). */
+ * growth = MAXDTINC = 1.25 (euler/hydro_driver.cpp:107-113, euler/euler_defs.h:45,57). */
 int amrh_dt_device(amrh_ctx* ctx, double scale, double growth, double* dt_dev);
 
 /* Device error flags raised since the last call (AMRH_DERR_*); clears them.  Synchronous. */
@@ -133,7 +133,7 @@ double amrh_substep_bytes(const amrh_ctx* ctx, int beta_is_one);
  * amrh_store).  amrh_set_launch sets them by hand (0 = one workgroup per block). */
 int amrh_autotune(amrh_ctx* ctx, int* grid_substep, int* grid_maxeig);
 int amrh_set_launch(amrh_ctx* ctx, int grid_substep, int grid_maxeig);
-/* HydroDriver::step repeated nsteps times (euler/hydro_driver.cpp:231-254, regrid excluded) with
+/* HydroDriver::step repeated nsteps times (euler/hydro_driver.cpp:171-194, regrid excluded) with
  * the reference's dt control on the device: dt_dev[0] holds the previous dt and becomes
  * min(MAXDTINC*dt, CFL dt).  For the octree (after amrh_tree_set) each step fills ghosts, scans
  * the CFL speed, stores U0 and runs rk tree substeps; for a flat batch it needs
@@ -155,7 +155,7 @@ int amrh_tree_set(amrh_ctx* ctx, const int* parent, const int* child);
  * coarse->fine interpolation (grid/grid_amr_bound.cpp:27-78), physical boundary
  * (grid/grid_phys_bound.cpp:27-54). */
 int amrh_tree_fill_ghosts(amrh_ctx* ctx);
-/* HydroDriver::sub_step + inject_from_children (euler/hydro_driver.cpp:243-274): ghost fill,
+/* HydroDriver::sub_step + inject_from_children (euler/hydro_driver.cpp:171-214): ghost fill,
  * fluxes with reflux (grid/grid_node.cpp:1076-1274), divergence, add_difs on every node, the root's
  * flow-off ledger (grid/grid.cpp:350-404, 419), then restriction (grid/grid_node.cpp:1035-1074). */
 int amrh_tree_substep(amrh_ctx* ctx, double dt, double beta);

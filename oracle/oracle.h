@@ -42,7 +42,7 @@ double orc_block_dt(const orc_cfg* c, double max_eig, double h);
 /* Ghost fill of isolated blocks whose six neighbours are all outflow boundaries
  * (grid/grid_phys_bound.cpp:27-54 with euler/state.cpp:45-84); z lower face reflects if z_reflect. */
 int orc_batch_outflow_ghosts(const orc_cfg* c, int nb, double* U);
-/* n RK steps of order rk (euler/hydro_driver.cpp:231-249) on a flat batch with outflow ghost
+/* n RK steps of order rk (euler/hydro_driver.cpp:171-194) on a flat batch with outflow ghost
  * refresh before every substep; dt fixed.  U0 scratch of the same size as U. */
 int orc_batch_rk_steps(const orc_cfg* c, int nb, double* U, double* U0, const double* h, double dt,
                        int rk, int nsteps);
@@ -62,7 +62,7 @@ int orc_tree_enforce_boundaries(orc_tree* t);   /* grid/grid_node.cpp:59-112 */
 int orc_tree_inject_from_children(orc_tree* t); /* grid/grid_node.cpp:1035-1074 */
 int orc_tree_store(orc_tree* t);                /* grid/grid_node.cpp:495-515 */
 double orc_tree_max_dt(const orc_tree* t);      /* grid/grid_node.cpp:163-174 */
-/* HydroDriver::sub_step (euler/hydro_driver.cpp:260-274) followed by inject_from_children */
+/* HydroDriver::sub_step (euler/hydro_driver.cpp:200-214) followed by inject_from_children */
 int orc_tree_substep(orc_tree* t, double dt, double beta);
 /* sum over uncovered cells of U*h^3 (grid/grid_node.cpp:1489-1529, a0 only) and the root ledger */
 int orc_tree_state_sum(const orc_tree* t, double* sum);

@@ -108,7 +108,7 @@ def test_full_multilevel_mixed_vs_fp64():
         B.autotune()
     dt = None
     for _ in range(20):
-        dt = B64.cfl_dt(dt)  # the FP64 form's dt drives both (euler/hydro_driver.cpp:168-173)
+        dt = B64.cfl_dt(dt)  # the FP64 form's dt drives both (euler/hydro_driver.cpp:107-113)
         for B in (B64, B32):
             B.rk_step(dt, 3)
     res = []

@@ -89,7 +89,7 @@ def test_block_max_eig_matches_oracle(n):
 
 
 def oracle_dt(cfg, U, h, last):
-    """HydroDriver::step_to_time's dt (euler/hydro_driver.cpp:168-173) on the oracle state."""
+    """HydroDriver::step_to_time's dt (euler/hydro_driver.cpp:107-113) on the oracle state."""
     dt = orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h)
     return 1e-2 * dt if last is None else min(1.25 * last, dt)
 

@@ -46,7 +46,7 @@ def run_and_compare(cfg, spec, ic, nsub, label):
     dt = None
     done = 0
     while done < nsub:
-        # HydroDriver::step_to_time's dt (euler/hydro_driver.cpp:168-173) on the oracle tree
+        # HydroDriver::step_to_time's dt (euler/hydro_driver.cpp:107-113) on the oracle tree
         T.enforce_boundaries()
         m = T.max_dt()
         mg = B.tree_max_dt()
