@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libamrhydro.so")
 
 AMRH_PHYS_EULER = 0
 AMRH_RECON_MINMOD = 0
+AMRH_RECON_PPM = 1
 AMRH_FP64 = 0
 AMRH_MIXED = 1
 
@@ -34,9 +35,10 @@ AMRH_BOUNDARY_OUTFLOW = 1
 
 
 def euler_config(n=8, g=2, precision=AMRH_FP64, z_reflect=0, grid_dim=1.0,
-                 boundary=AMRH_BOUNDARY_EXTERNAL):
-    """Ideal-gas constants (euler/state.cpp:9-11, euler/euler_defs.h:20,46)."""
-    return Config(n=n, g=g, nf=6, physics=AMRH_PHYS_EULER, recon=AMRH_RECON_MINMOD,
+                 boundary=AMRH_BOUNDARY_EXTERNAL, recon=AMRH_RECON_MINMOD):
+    """Ideal-gas constants (euler/state.cpp:9-11, euler/euler_defs.h:20,46).  PPM
+    (reconstruct.cpp:22-58) takes g = 3 and n = 8."""
+    return Config(n=n, g=g, nf=6, physics=AMRH_PHYS_EULER, recon=recon,
                   precision=precision, z_reflect=z_reflect, gamma=5.0 / 3.0, rho_floor=1e-20,
                   ei_floor=1e-20, theta=1.3, cfl=0.4, grid_dim=grid_dim, boundary=boundary)
 

@@ -54,8 +54,14 @@ int check_cfg(const amrh_config* c) {
   if (c->physics != AMRH_PHYS_EULER) return fail(AMRH_EINVAL, "physics %d not supported", c->physics);
   if (c->nf != 6) return fail(AMRH_EINVAL, "EULER physics has 6 fields, got %d", c->nf);
   if (c->n != 8 && c->n != 16) return fail(AMRH_EINVAL, "n must be 8 or 16, got %d", c->n);
-  if (c->g != 2) return fail(AMRH_EINVAL, "minmod reconstruction needs g = 2, got %d", c->g);
-  if (c->recon != AMRH_RECON_MINMOD) return fail(AMRH_EINVAL, "recon %d not supported", c->recon);
+  if (c->recon == AMRH_RECON_MINMOD) {
+    if (c->g != 2) return fail(AMRH_EINVAL, "minmod reconstruction needs g = 2, got %d", c->g);
+  } else if (c->recon == AMRH_RECON_PPM) {
+    if (c->g != 3) return fail(AMRH_EINVAL, "PPM reconstruction needs g = 3, got %d", c->g);
+    if (c->n != 8) return fail(AMRH_EINVAL, "PPM reconstruction runs on n = 8 blocks, got n = %d", c->n);
+  } else {
+    return fail(AMRH_EINVAL, "recon %d unknown", c->recon);
+  }
   if (c->precision != AMRH_FP64 && c->precision != AMRH_MIXED)
     return fail(AMRH_EINVAL, "precision %d unknown", c->precision);
   if (!(c->gamma > 1.0)) return fail(AMRH_EINVAL, "gamma must exceed 1");
@@ -147,7 +153,7 @@ static int ensure_stage(amrh_ctx* c) {
 
 int amrh_upload_device(amrh_ctx* c, const void* pad) {
   if (!c || !pad) return fail(AMRH_EINVAL, "null argument");
-  HIPCHK(launch_pack(c->cfg.n, (const double*)pad, c->buf[c->cur], record_elems(c), true, c->stream));
+  HIPCHK(launch_pack(c->cfg.n, c->cfg.g, (const double*)pad, c->buf[c->cur], record_elems(c), true, c->stream));
   c->have_u = true;
   c->u0buf = -1;
   return AMRH_OK;
@@ -168,7 +174,7 @@ int amrh_download_device(amrh_ctx* c, void* pad) {
   if (!c || !pad) return fail(AMRH_EINVAL, "null argument");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
   HIPCHK(hipMemsetAsync(pad, 0, sizeof(double) * (size_t)padded_elems(c), c->stream));
-  HIPCHK(launch_pack(c->cfg.n, (const double*)pad, c->buf[c->cur], record_elems(c), false, c->stream));
+  HIPCHK(launch_pack(c->cfg.n, c->cfg.g, (const double*)pad, c->buf[c->cur], record_elems(c), false, c->stream));
   return AMRH_OK;
 }
 
@@ -193,11 +199,25 @@ int amrh_store(amrh_ctx* c) {
 int amrh_fill_outflow_ghosts(amrh_ctx* c) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
-  HIPCHK(launch_outflow_ghosts(c->cfg.n, c->buf[c->cur], c->nb, c->cfg.z_reflect, c->stream));
+  HIPCHK(launch_outflow_ghosts(c->cfg.n, c->cfg.g, c->buf[c->cur], c->nb, c->cfg.z_reflect, c->stream));
   return AMRH_OK;
 }
 
 }  // extern "C"
+
+hipError_t amrh::ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int grid) {
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  if (c->cfg.recon == AMRH_RECON_MINMOD) return launch_substep(c->cfg.n, a, mixed, grid, c->stream);
+  hipError_t e = launch_block_substep(c->cfg.n, c->cfg.g, c->cfg.recon, a, mixed, grid, c->stream);
+  if (e == hipSuccess && a.outflow) e = launch_outflow_ghosts(c->cfg.n, c->cfg.g, a.Uout, c->nb, c->cfg.z_reflect, c->stream);
+  return e;
+}
+
+hipError_t amrh::ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid) {
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  if (c->cfg.recon == AMRH_RECON_MINMOD) return launch_maxeig(c->cfg.n, m, mixed, grid, c->stream);
+  return launch_block_maxeig(c->cfg.n, c->cfg.g, c->cfg.recon, m, mixed, grid, c->stream);
+}
 
 // The fused substep over every block; with `tree` the reflux overrides and the root's flow-off
 // ledger of the octree are applied (the callers have exported the fine faces first).
@@ -232,7 +252,7 @@ int amrh::substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev
     a.ledger_block = 0;
     a.ledger = c->tree->ledger_dev;
   }
-  HIPCHK(launch_substep(c->cfg.n, a, c->cfg.precision == AMRH_MIXED, c->grid_substep, c->stream));
+  HIPCHK(ctx_substep_launch(c, a, c->grid_substep));
   c->cur = out;
   return AMRH_OK;
 }
@@ -256,7 +276,7 @@ static int maxeig_impl(amrh_ctx* c, bool per_block, bool global) {
   m.gmax = global ? c->gmax_dev : nullptr;
   m.nblocks = c->nb;
   m.P = params(c->cfg);
-  HIPCHK(launch_maxeig(c->cfg.n, m, c->cfg.precision == AMRH_MIXED, c->grid_maxeig, c->stream));
+  HIPCHK(ctx_maxeig_launch(c, m, c->grid_maxeig));
   return AMRH_OK;
 }
 

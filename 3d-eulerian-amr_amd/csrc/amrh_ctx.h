@@ -9,6 +9,7 @@
 #include "amrh_layout.h"
 #include "amrhydro.h"
 #include "euler_phys.h"
+#include "kt_kernels.h"
 
 namespace amrh {
 
@@ -68,6 +69,10 @@ namespace amrh {
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 PhysParams params(const amrh_config& c);
 int substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool tree);
+// the substep / CFL-scan kernels of the context's scheme (the column-sweep kernel for minmod,
+// the block kernel for PPM); a block-kernel substep in outflow mode is followed by the ghost fill
+hipError_t ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int grid);
+hipError_t ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid);
 
 }  // namespace amrh
 

@@ -86,7 +86,6 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   std::vector<int> cand = {0};
   for (int k : {1, 2, 3, 4, 6, 8, 12, 16})
     if (k * cus < c->nb) cand.push_back(k * cus);
-  const bool mixed = c->cfg.precision == AMRH_MIXED;
   SubstepArgs a{};
   a.Uin = c->buf[c->cur];
   a.Uout = c->buf[other];
@@ -111,13 +110,13 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   for (int g : cand) {
     float ms = 0.f;
     if (int rc = time_launches(c, 3, [&]() -> int {
-          HIPCHK(launch_substep(c->cfg.n, a, mixed, g, c->stream));
+          HIPCHK(ctx_substep_launch(c, a, g));
           return AMRH_OK;
         }, &ms))
       return rc;
     if (ms < best_s) { best_s = ms; gs = g; }
     if (int rc = time_launches(c, 3, [&]() -> int {
-          HIPCHK(launch_maxeig(c->cfg.n, m, mixed, g, c->stream));
+          HIPCHK(ctx_maxeig_launch(c, m, g));
           return AMRH_OK;
         }, &ms))
       return rc;

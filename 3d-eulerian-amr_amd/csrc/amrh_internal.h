@@ -9,8 +9,13 @@ namespace amrh {
 // grid <= 0: one workgroup per block; otherwise a persistent grid of that many workgroups
 hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s);
 hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s);
-hipError_t launch_outflow_ghosts(int n, double* U, int nblocks, int z_reflect, hipStream_t s);
-hipError_t launch_pack(int n, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s);
+hipError_t launch_outflow_ghosts(int n, int g, double* U, int nblocks, int z_reflect, hipStream_t s);
+hipError_t launch_pack(int n, int g, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s);
+
+// the block kernel (kt_block.hip): n = 8 with PPM (g = 3) or minmod (g = 2)
+bool block_scheme_supported(int n, int g, int recon);
+hipError_t launch_block_substep(int n, int g, int recon, const SubstepArgs& a, bool mixed, int grid, hipStream_t s);
+hipError_t launch_block_maxeig(int n, int g, int recon, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s);
 hipError_t launch_dt_from_gmax(unsigned long long* gmax, double cfl, double scale, double growth, double* dt,
                                hipStream_t s);
 hipError_t launch_reset_u64(unsigned long long* p, hipStream_t s);

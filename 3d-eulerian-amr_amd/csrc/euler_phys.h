@@ -66,15 +66,9 @@ __device__ __forceinline__ void slope(const double* a, const double* b, const do
 }
 // KT face flux in direction D from the primitive states and slopes of the two adjacent cells
 // (reconstruct.cpp:86-89, grid/grid.cpp:233-239, state.cpp:171-220).  eig = max(eL, eR).
+// KT face flux from the two reconstructed primitive face states (converted in place)
 template <int D>
-__device__ __forceinline__ void kt_face(const PhysParams& P, const double* pl, const double* sl,
-                                        const double* pr, const double* sr, double* F, double& eig) {
-  double ql[NFE], qr[NFE];
-#pragma unroll
-  for (int l = 0; l < NFE; l++) {
-    ql[l] = pl[l] + sl[l] * 0.5;
-    qr[l] = pr[l] - sr[l] * 0.5;
-  }
+__device__ __forceinline__ void kt_flux(const PhysParams& P, double* ql, double* qr, double* F, double& eig) {
   to_con(ql);
   to_con(qr);
   const double vl = ql[SX + D] / ql[RHO], vr = qr[SX + D] / qr[RHO];
@@ -98,6 +92,26 @@ __device__ __forceinline__ void kt_face(const PhysParams& P, const double* pl, c
   }
 }
 template <int D>
+__device__ __forceinline__ void kt_face(const PhysParams& P, const double* pl, const double* sl,
+                                        const double* pr, const double* sr, double* F, double& eig) {
+  double ql[NFE], qr[NFE];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) {
+    ql[l] = pl[l] + sl[l] * 0.5;
+    qr[l] = pr[l] - sr[l] * 0.5;
+  }
+  kt_flux<D>(P, ql, qr, F, eig);
+}
+// max(|v_d| + c_s) of the two reconstructed primitive face states (converted in place)
+template <int D>
+__device__ __forceinline__ double eig_q(const PhysParams& P, double* ql, double* qr) {
+  to_con(ql);
+  to_con(qr);
+  const double el = fabs(ql[SX + D] / ql[RHO]) + sqrt(P.gamma * pg(P, ql) / ql[RHO]);
+  const double er = fabs(qr[SX + D] / qr[RHO]) + sqrt(P.gamma * pg(P, qr) / qr[RHO]);
+  return rmax(el, er);
+}
+template <int D>
 __device__ __forceinline__ double face_eig(const PhysParams& P, const double* pl, const double* sl,
                                            const double* pr, const double* sr) {
   double ql[NFE], qr[NFE];
@@ -106,11 +120,7 @@ __device__ __forceinline__ double face_eig(const PhysParams& P, const double* pl
     ql[l] = pl[l] + sl[l] * 0.5;
     qr[l] = pr[l] - sr[l] * 0.5;
   }
-  to_con(ql);
-  to_con(qr);
-  const double el = fabs(ql[SX + D] / ql[RHO]) + sqrt(P.gamma * pg(P, ql) / ql[RHO]);
-  const double er = fabs(qr[SX + D] / qr[RHO]) + sqrt(P.gamma * pg(P, qr) / qr[RHO]);
-  return rmax(el, er);
+  return eig_q<D>(P, ql, qr);
 }
 
 }  // namespace x64
@@ -172,22 +182,37 @@ __device__ __forceinline__ void state_flux(const PhysF& P, const float* q, float
   f[ET] += vd * p;
 }
 
+// KT face flux from the two reconstructed primitive face states
 template <int D>
-__device__ __forceinline__ void kt_face(const PhysF& P, const float* pl, const float* sl, const float* pr,
-                                        const float* sr, float* F, float& eig) {
+__device__ __forceinline__ void kt_flux(const PhysF& P, const float* ql, const float* qr, float* F, float& eig) {
 #pragma clang fp contract(fast)
-  float ql[NFE], qr[NFE], UL[NFE], UR[NFE], fl[NFE], fr[NFE], el, er;
-#pragma unroll
-  for (int l = 0; l < NFE; l++) {
-    ql[l] = pl[l] + 0.5f * sl[l];
-    qr[l] = pr[l] - 0.5f * sr[l];
-  }
+  float UL[NFE], UR[NFE], fl[NFE], fr[NFE], el, er;
   state_flux<D>(P, ql, UL, fl, el);
   state_flux<D>(P, qr, UR, fr, er);
   const float a = fmaxf(el, er);
   eig = a;
 #pragma unroll
   for (int l = 0; l < NFE; l++) F[l] = 0.5f * ((fl[l] + fr[l]) - a * (UR[l] - UL[l]));
+}
+template <int D>
+__device__ __forceinline__ void kt_face(const PhysF& P, const float* pl, const float* sl, const float* pr,
+                                        const float* sr, float* F, float& eig) {
+#pragma clang fp contract(fast)
+  float ql[NFE], qr[NFE];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) {
+    ql[l] = pl[l] + 0.5f * sl[l];
+    qr[l] = pr[l] - 0.5f * sr[l];
+  }
+  kt_flux<D>(P, ql, qr, F, eig);
+}
+template <int D>
+__device__ __forceinline__ float eig_q(const PhysF& P, const float* ql, const float* qr) {
+#pragma clang fp contract(fast)
+  float U[NFE], f[NFE], el, er;
+  state_flux<D>(P, ql, U, f, el);
+  state_flux<D>(P, qr, U, f, er);
+  return fmaxf(el, er);
 }
 
 template <int D>

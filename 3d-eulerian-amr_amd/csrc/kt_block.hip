@@ -1,0 +1,268 @@
+// Block kernel: the fused KT substep and its CFL twin for schemes with a wide stencil (PPM, g = 3)
+// on the reference's 8^3 blocks.
+//
+// One workgroup per block and one thread per interior cell (N^3 = 512 threads).  The block's
+// primitives (interior + the six face ghost slabs, the record layout) are staged in LDS once; then
+// for each direction in x, y, z order every face of the block is reconstructed and fluxed (a
+// thread per face, the (N+1)*N^2 faces spread over the workgroup) into an LDS face buffer, and
+// every cell subtracts its flux difference from its D held in registers, so D accumulates in the
+// reference's order (grid/grid.cpp:304-344).  The cell then forms U = (U + D*dt)*beta +
+// U0*(1-beta), floors it and stores it (grid/grid.cpp:406-420).  HBM traffic per block is the
+// algorithmic count: interior + face ghost slabs read once, interior written once, U0 read when
+// beta != 1.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "amrh_internal.h"
+#include "amrhydro.h"
+#include "cell_ops.h"
+#include "kt_kernels.h"
+#include "recon.h"
+
+namespace amrh {
+
+namespace {
+
+// record element of pencil position p in [-G, N+G) along direction d, at tangential cell
+// indices (ta, tb): x pencils (j, k), y pencils (i, k), z pencils (i, j)
+template <int N, int G>
+__device__ __forceinline__ int pencil_elem(int d, int p, int ta, int tb) {
+  constexpr int N3 = N * N * N, S = G * N * N;
+  if (p >= 0 && p < N) {
+    if (d == 0) return (tb * N + ta) * N + p;
+    if (d == 1) return (tb * N + p) * N + ta;
+    return (p * N + tb) * N + ta;
+  }
+  const int up = p >= N ? 1 : 0;
+  const int t = up ? p - N : p + G;
+  const int base = N3 + (2 * d + up) * S;
+  if (d == 0) return base + (tb * N + ta) * G + t;
+  if (d == 1) return base + (tb * G + t) * N + ta;
+  return base + (t * N + tb) * N + ta;
+}
+
+template <int N, int G, int RECON, bool MIXED>
+__device__ __forceinline__ void reconstruct(const typename Prec<MIXED>::T (*P)[N * N * N + 6 * G * N * N], int d,
+                                            int f, int ta, int tb, typename Prec<MIXED>::T* ql,
+                                            typename Prec<MIXED>::T* qr, double theta) {
+  using T = typename Prec<MIXED>::T;
+  if constexpr (RECON == AMRH_RECON_PPM) {
+    int e[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) e[m] = pencil_elem<N, G>(d, f - 3 + m, ta, tb);
+#pragma unroll
+    for (int l = 0; l < NFE; l++) {
+      T q[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) q[m] = P[l][e[m]];
+      // every ideal-gas field is neither low-order nor smooth (euler/state.h:37-42)
+      if constexpr (MIXED)
+        f32::ppm_face(q, ql[l], qr[l], false, false);
+      else
+        x64::ppm_face(q, ql[l], qr[l], false, false);
+    }
+  } else {
+    int e[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) e[m] = pencil_elem<N, G>(d, f - 2 + m, ta, tb);
+#pragma unroll
+    for (int l = 0; l < NFE; l++) {
+      T q[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) q[m] = P[l][e[m]];
+      if constexpr (MIXED)
+        f32::minmod_face(q, ql[l], qr[l], (float)theta);
+      else
+        x64::minmod_face(q, ql[l], qr[l], theta);
+    }
+  }
+}
+
+template <bool MIXED, int D>
+__device__ __forceinline__ void flux_q(const PhysParams& P, typename Prec<MIXED>::T* ql,
+                                       typename Prec<MIXED>::T* qr, typename Prec<MIXED>::T* F) {
+  if constexpr (!MIXED) {
+    double e;
+    x64::kt_flux<D>(P, ql, qr, F, e);
+  } else {
+    const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+    float e;
+    f32::kt_flux<D>(PF, ql, qr, F, e);
+  }
+}
+
+template <bool MIXED, int D>
+__device__ __forceinline__ double eig_of_q(const PhysParams& P, typename Prec<MIXED>::T* ql,
+                                           typename Prec<MIXED>::T* qr) {
+  if constexpr (!MIXED) {
+    return x64::eig_q<D>(P, ql, qr);
+  } else {
+    const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+    return (double)f32::eig_q<D>(PF, ql, qr);
+  }
+}
+
+// MODE 0: substep.  MODE 1: per-block max wavespeed (Grid::max_dt's scan, grid/grid.cpp:422-473).
+template <int N, int G, int RECON, bool MIXED, int MODE>
+__global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
+  using T = typename Prec<MIXED>::T;
+  constexpr int NT = N * N * N;
+  constexpr int R = N * N * N + 6 * G * N * N;
+  constexpr int NFACE = (N + 1) * N * N;
+  __shared__ T P[NFE][R];
+  __shared__ T Fb[MODE == 0 ? NFE : 1][MODE == 0 ? NFACE : 1];
+  __shared__ double red[NT / 64];
+
+  const int tid = threadIdx.x;
+  const PhysParams PP = MODE == 0 ? a.P : m.P;
+  const int nbl = MODE == 0 ? a.nblocks : m.nblocks;
+  double dt = a.dt;
+  if (MODE == 0 && a.dt_dev) dt = *a.dt_dev;
+  const double beta = a.beta;
+  const bool has_u0 = MODE == 0 && beta != 1.0;
+  const int ci = tid % N, cj = (tid / N) % N, ck = tid / (N * N);
+
+  for (int bb = blockIdx.x; bb < nbl; bb += gridDim.x) {
+    const int b = MODE == 0 ? a.block0 + bb : bb;
+    const double* Ub = (MODE == 0 ? a.Uin : m.U) + (long)b * NFE * R;
+
+    // stage the primitives; each thread keeps its own interior cell's conserved state
+    double ucur[NFE];
+#pragma unroll 1
+    for (int e = tid; e < R; e += NT) {
+      double u[NFE];
+#pragma unroll
+      for (int l = 0; l < NFE; l++) u[l] = Ub[(long)l * R + e];
+      if (e == tid) {  // the first pass: e = tid < N^3 is this thread's interior cell
+#pragma unroll
+        for (int l = 0; l < NFE; l++) ucur[l] = u[l];
+      }
+      T p[NFE];
+      make_prim<MIXED>(u, p);
+#pragma unroll
+      for (int l = 0; l < NFE; l++) P[l][e] = p[l];
+    }
+    __syncthreads();
+
+    double D[NFE];
+#pragma unroll
+    for (int l = 0; l < NFE; l++) D[l] = 0.0;
+    double emax = 0.0;
+    const double hinv = MODE == 0 ? 1.0 / a.h[b] : 0.0;
+
+    auto sweep = [&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+#pragma unroll 1
+      for (int q = tid; q < NFACE; q += NT) {
+        const int f = q % (N + 1), t = q / (N + 1), ta = t % N, tb = t / N;
+        T ql[NFE], qr[NFE];
+        reconstruct<N, G, RECON, MIXED>(P, d, f, ta, tb, ql, qr, PP.theta);
+        if constexpr (MODE == 0) {
+          T Fo[NFE];
+          flux_q<MIXED, d>(PP, ql, qr, Fo);
+#pragma unroll
+          for (int l = 0; l < NFE; l++) Fb[l][q] = Fo[l];
+        } else {
+          emax = x64::rmax(emax, eig_of_q<MIXED, d>(PP, ql, qr));
+        }
+      }
+      if constexpr (MODE == 0) {
+        __syncthreads();
+        const int p = d == 0 ? ci : (d == 1 ? cj : ck);
+        const int ta = d == 0 ? cj : ci;
+        const int tb = d == 2 ? cj : ck;
+        const int q0 = p + (N + 1) * (ta + N * tb);
+#pragma unroll
+        for (int l = 0; l < NFE; l++) {
+          if constexpr (!MIXED)
+            D[l] -= (Fb[l][q0 + 1] - Fb[l][q0]) * hinv;
+          else  // FP32 fluxes differenced in FP64 (exact), so the divergence telescopes
+            D[l] -= ((double)Fb[l][q0 + 1] - (double)Fb[l][q0]) * hinv;
+        }
+        __syncthreads();  // the next direction rewrites the face buffer
+      }
+    };
+    sweep(std::integral_constant<int, 0>{});
+    sweep(std::integral_constant<int, 1>{});
+    sweep(std::integral_constant<int, 2>{});
+
+    if constexpr (MODE == 0) {
+      const long e = (long)tid;  // interior element of cell (ci, cj, ck)
+      double u0v[NFE];
+      if (has_u0) {
+        const double* U0b = a.U0 + (long)b * NFE * R;
+#pragma unroll
+        for (int l = 0; l < NFE; l++) u0v[l] = U0b[(long)l * R + e];
+      }
+      double u[NFE];
+#pragma unroll
+      for (int l = 0; l < NFE; l++) {
+        const double Dl = D[l] + 0.0;  // State::source is zero for the ideal gas (euler/state.cpp:19-22)
+        if (has_u0)
+          u[l] = (ucur[l] + Dl * dt) * beta + u0v[l] * (1.0 - beta);
+        else
+          u[l] = (ucur[l] + Dl * dt) * beta;
+      }
+      if constexpr (!MIXED)
+        x64::floor_state(PP, u);
+      else
+        floor_mixed(PP, u);
+      if (!(u[RHO] > 0.0) && a.err) atomicOr(a.err, 1);
+      double* Uo = a.Uout + (long)b * NFE * R;
+#pragma unroll
+      for (int l = 0; l < NFE; l++) Uo[(long)l * R + e] = u[l];
+    } else {
+      for (int off = 32; off > 0; off >>= 1) emax = x64::rmax(emax, __shfl_xor(emax, off, 64));
+      if ((tid & 63) == 0) red[tid >> 6] = emax;
+      __syncthreads();
+      if (tid == 0) {
+        double v = red[0];
+        for (int w = 1; w < NT / 64; w++) v = x64::rmax(v, red[w]);
+        if (m.block_eig) m.block_eig[b] = v;
+        if (m.gmax) {
+          const double r = v / m.h[b];
+          atomicMax(m.gmax, (unsigned long long)__double_as_longlong(r));
+        }
+      }
+    }
+    __syncthreads();  // P (and red) are restaged by the next block of a persistent grid
+  }
+}
+
+template <int N, int G, int RECON, bool MIXED, int MODE>
+hipError_t launch_one(const SubstepArgs& a, const MaxEigArgs& m, int nblocks, int grid, hipStream_t s) {
+  const int g = grid > 0 && grid < nblocks ? grid : nblocks;
+  if (nblocks <= 0) return hipSuccess;
+  kt_block_kernel<N, G, RECON, MIXED, MODE><<<g, N * N * N, 0, s>>>(a, m);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool block_scheme_supported(int n, int g, int recon) {
+  return n == 8 && ((recon == AMRH_RECON_PPM && g == 3) || (recon == AMRH_RECON_MINMOD && g == 2));
+}
+
+hipError_t launch_block_substep(int n, int g, int recon, const SubstepArgs& a, bool mixed, int grid,
+                                hipStream_t s) {
+  const MaxEigArgs m{};
+  if (!block_scheme_supported(n, g, recon)) return hipErrorInvalidValue;
+  if (recon == AMRH_RECON_PPM)
+    return mixed ? launch_one<8, 3, AMRH_RECON_PPM, true, 0>(a, m, a.nblocks, grid, s)
+                 : launch_one<8, 3, AMRH_RECON_PPM, false, 0>(a, m, a.nblocks, grid, s);
+  return mixed ? launch_one<8, 2, AMRH_RECON_MINMOD, true, 0>(a, m, a.nblocks, grid, s)
+               : launch_one<8, 2, AMRH_RECON_MINMOD, false, 0>(a, m, a.nblocks, grid, s);
+}
+
+hipError_t launch_block_maxeig(int n, int g, int recon, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s) {
+  const SubstepArgs a{};
+  if (!block_scheme_supported(n, g, recon)) return hipErrorInvalidValue;
+  if (recon == AMRH_RECON_PPM)
+    return mixed ? launch_one<8, 3, AMRH_RECON_PPM, true, 1>(a, m, m.nblocks, grid, s)
+                 : launch_one<8, 3, AMRH_RECON_PPM, false, 1>(a, m, m.nblocks, grid, s);
+  return mixed ? launch_one<8, 2, AMRH_RECON_MINMOD, true, 1>(a, m, m.nblocks, grid, s)
+               : launch_one<8, 2, AMRH_RECON_MINMOD, false, 1>(a, m, m.nblocks, grid, s);
+}
+
+}  // namespace amrh

@@ -5,6 +5,7 @@
 #include <type_traits>
 
 #include "amrh_internal.h"
+#include "cell_ops.h"
 #include "kt_kernels.h"
 
 namespace amrh {
@@ -12,44 +13,6 @@ namespace amrh {
 namespace {
 
 constexpr int GW = 2;  // ghost width of the minmod scheme (euler/euler_defs.h:25)
-
-template <bool MIXED>
-struct Prec {
-  using T = double;
-};
-template <>
-struct Prec<true> {
-  using T = float;
-};
-
-// conserved FP64 -> primitive in the compute type
-template <bool MIXED>
-__device__ __forceinline__ void make_prim(const double* u, typename Prec<MIXED>::T* p) {
-  if constexpr (!MIXED) {
-#pragma unroll
-    for (int l = 0; l < NFE; l++) p[l] = u[l];
-    x64::to_prim(p);
-  } else {
-    const double inv = f32::rcp64(u[RHO]);
-    const double vx = u[SX] * inv, vy = u[SY] * inv, vz = u[SZ] * inv;
-    const double ekin = 0.5 * (u[SX] * vx + u[SY] * vy + u[SZ] * vz);
-    p[RHO] = (float)u[RHO];
-    p[SX] = (float)vx;
-    p[SY] = (float)vy;
-    p[SZ] = (float)vz;
-    p[ET] = (float)(u[ET] - ekin);
-    p[TAU] = (float)u[TAU];
-  }
-}
-
-// State::floor of the mixed form (euler/state.cpp:86-92): same tests in FP64, the entropy tracer
-// from pow_inv_gamma instead of libm pow
-__device__ __forceinline__ void floor_mixed(const PhysParams& P, double* u) {
-  u[RHO] = x64::rmax(u[RHO], P.rho_floor);
-  const double ek = 0.5 * (u[SX] * u[SX] + u[SY] * u[SY] + u[SZ] * u[SZ]) * f32::rcp64(u[RHO]);
-  const double ei0 = u[ET] - ek;
-  if (ei0 > 0.1 * u[ET]) u[TAU] = f32::pow_inv_gamma(x64::rmax(ei0, P.ei_floor), P.gamma);
-}
 
 template <bool MIXED, int D>
 __device__ __forceinline__ void face_flux(const PhysParams& P, const typename Prec<MIXED>::T* pm2,
@@ -566,7 +529,7 @@ __global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArg
 // Ghost fill of isolated blocks: every face slab cell copies the adjacent interior cell and
 // removes inflow momentum (grid/grid_phys_bound.cpp:27-54, euler/state.cpp:45-84); the lower z face
 // mirrors with sz -> -sz when z_reflect (grid/grid_phys_bound.cpp:298-313, euler/state.cpp:94-96).
-template <int N>
+template <int N, int GW>
 __global__ __launch_bounds__(256) void outflow_ghost_kernel(double* U, int nblocks, int z_reflect) {
   constexpr Layout L{N, GW, NFE};
   constexpr long R = L.rec();
@@ -621,7 +584,7 @@ __global__ __launch_bounds__(256) void outflow_ghost_kernel(double* U, int nbloc
 }
 
 // padded SoA <-> records (one thread per record element)
-template <int N>
+template <int N, int GW>
 __global__ __launch_bounds__(256) void pack_kernel(const double* __restrict__ pad, double* __restrict__ rec,
                                                   long nrec_elems, int to_rec) {
   constexpr Layout L{N, GW, NFE};
@@ -722,26 +685,30 @@ hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, int grid, hipSt
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_outflow_ghosts(int n, double* U, int nblocks, int z_reflect, hipStream_t s) {
-  const long total = (long)nblocks * 6 * GW * n * n;
+hipError_t launch_outflow_ghosts(int n, int g, double* U, int nblocks, int z_reflect, hipStream_t s) {
+  const long total = (long)nblocks * 6 * g * n * n;
   const unsigned grid = (unsigned)((total + 255) / 256);
   if (total == 0) return hipSuccess;
-  if (n == 8)
-    outflow_ghost_kernel<8><<<grid, 256, 0, s>>>(U, nblocks, z_reflect);
-  else if (n == 16)
-    outflow_ghost_kernel<16><<<grid, 256, 0, s>>>(U, nblocks, z_reflect);
+  if (n == 8 && g == 2)
+    outflow_ghost_kernel<8, 2><<<grid, 256, 0, s>>>(U, nblocks, z_reflect);
+  else if (n == 8 && g == 3)
+    outflow_ghost_kernel<8, 3><<<grid, 256, 0, s>>>(U, nblocks, z_reflect);
+  else if (n == 16 && g == 2)
+    outflow_ghost_kernel<16, 2><<<grid, 256, 0, s>>>(U, nblocks, z_reflect);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_pack(int n, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s) {
+hipError_t launch_pack(int n, int g, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s) {
   const unsigned grid = (unsigned)((nrec_elems + 255) / 256);
   if (nrec_elems == 0) return hipSuccess;
-  if (n == 8)
-    pack_kernel<8><<<grid, 256, 0, s>>>(pad, rec, nrec_elems, to_rec ? 1 : 0);
-  else if (n == 16)
-    pack_kernel<16><<<grid, 256, 0, s>>>(pad, rec, nrec_elems, to_rec ? 1 : 0);
+  if (n == 8 && g == 2)
+    pack_kernel<8, 2><<<grid, 256, 0, s>>>(pad, rec, nrec_elems, to_rec ? 1 : 0);
+  else if (n == 8 && g == 3)
+    pack_kernel<8, 3><<<grid, 256, 0, s>>>(pad, rec, nrec_elems, to_rec ? 1 : 0);
+  else if (n == 16 && g == 2)
+    pack_kernel<16, 2><<<grid, 256, 0, s>>>(pad, rec, nrec_elems, to_rec ? 1 : 0);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -1,0 +1,114 @@
+// Face reconstruction along one pencil, per field, from the primitive values of the cells around
+// a face.  Face f lies between cells f-1 and f; "ql" is the state on its low side (cell f-1's
+// right edge) and "qr" the state on its high side (cell f's left edge), the caller's convention
+// in grid/grid.cpp:229-239.
+//
+// PPM (reconstruct.cpp:22-58, called with its ql/qr arguments swapped at :94): theta = 2
+// minmod-theta slopes, the interface value a_{i+1/2} = (q_i + q_{i+1})/2 + (s_i - s_{i+1})/6
+// (plain average for a low-order field), then the Colella-Woodward monotonisation of each cell's
+// pair of edges (skipped for a smooth field).  A face needs the six cells f-3 .. f+2.
+//
+// The FP64 functions keep the reference's operation order (build without FP contraction); the
+// FP32 ones are the mixed form's.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "euler_phys.h"
+
+namespace amrh {
+
+namespace x64 {
+
+// a_{i+1/2} from cells i, i+1 and their slopes (reconstruct.cpp:31-37)
+__device__ __forceinline__ double ppm_iface(double qa, double qb, double sa, double sb, bool low_order) {
+  double v = (qa + qb) * 0.5;
+  if (!low_order) v += (sa - sb) * (1.0 / 6.0);
+  return v;
+}
+// monotonise cell value q with edges A (left) and B (right) (reconstruct.cpp:39-53)
+__device__ __forceinline__ void ppm_mono(double q, double& A, double& B) {
+  const double tmp1 = B - A;
+  const double tmp2 = B + A;
+  if ((B - q) * (q - A) <= 0.0) {
+    B = A = q;
+  } else if (tmp1 * (q - 0.5 * tmp2) > (1.0 / 6.0) * tmp1 * tmp1) {
+    A = 3.0 * q - 2.0 * B;
+  } else if (-(1.0 / 6.0) * tmp1 * tmp1 > tmp1 * (q - 0.5 * tmp2)) {
+    B = 3.0 * q - 2.0 * A;
+  }
+}
+// q[0..5] = cells f-3 .. f+2
+__device__ __forceinline__ void ppm_face(const double* q, double& ql, double& qr, bool low_order, bool smooth) {
+  const double s1 = minmod_theta(q[2] - q[1], q[1] - q[0], 2.0);  // cell f-2
+  const double s2 = minmod_theta(q[3] - q[2], q[2] - q[1], 2.0);  // cell f-1
+  const double s3 = minmod_theta(q[4] - q[3], q[3] - q[2], 2.0);  // cell f
+  const double s4 = minmod_theta(q[5] - q[4], q[4] - q[3], 2.0);  // cell f+1
+  double a1 = ppm_iface(q[1], q[2], s1, s2, low_order);           // between f-2 and f-1
+  double a2 = ppm_iface(q[2], q[3], s2, s3, low_order);           // face f itself
+  double a3 = ppm_iface(q[3], q[4], s3, s4, low_order);           // between f and f+1
+  double a2r = a2;
+  if (!smooth) {
+    ppm_mono(q[2], a1, a2);   // cell f-1: edges (a1, a2) -> its right edge is ql
+    ppm_mono(q[3], a2r, a3);  // cell f: edges (a2, a3) -> its left edge is qr
+  }
+  ql = a2;
+  qr = a2r;
+}
+// minmod-theta: q[0..3] = cells f-2 .. f+1 (reconstruct.cpp:60-71)
+__device__ __forceinline__ void minmod_face(const double* q, double& ql, double& qr, double theta) {
+  const double sl = minmod_theta(q[2] - q[1], q[1] - q[0], theta);
+  const double sr = minmod_theta(q[3] - q[2], q[2] - q[1], theta);
+  ql = q[1] + sl / 2.0;
+  qr = q[2] - sr / 2.0;
+}
+
+}  // namespace x64
+
+namespace f32 {
+
+__device__ __forceinline__ float ppm_iface(float qa, float qb, float sa, float sb, bool low_order) {
+#pragma clang fp contract(fast)
+  float v = 0.5f * (qa + qb);
+  if (!low_order) v += (1.0f / 6.0f) * (sa - sb);
+  return v;
+}
+__device__ __forceinline__ void ppm_mono(float q, float& A, float& B) {
+#pragma clang fp contract(fast)
+  const float tmp1 = B - A;
+  const float c = tmp1 * (q - 0.5f * (B + A));
+  const float t6 = (1.0f / 6.0f) * tmp1 * tmp1;
+  if ((B - q) * (q - A) <= 0.0f) {
+    B = A = q;
+  } else if (c > t6) {
+    A = 3.0f * q - 2.0f * B;
+  } else if (-t6 > c) {
+    B = 3.0f * q - 2.0f * A;
+  }
+}
+__device__ __forceinline__ void ppm_face(const float* q, float& ql, float& qr, bool low_order, bool smooth) {
+#pragma clang fp contract(fast)
+  const float s1 = minmod_theta(q[2] - q[1], q[1] - q[0], 2.0f);
+  const float s2 = minmod_theta(q[3] - q[2], q[2] - q[1], 2.0f);
+  const float s3 = minmod_theta(q[4] - q[3], q[3] - q[2], 2.0f);
+  const float s4 = minmod_theta(q[5] - q[4], q[4] - q[3], 2.0f);
+  float a1 = ppm_iface(q[1], q[2], s1, s2, low_order);
+  float a2 = ppm_iface(q[2], q[3], s2, s3, low_order);
+  float a3 = ppm_iface(q[3], q[4], s3, s4, low_order);
+  float a2r = a2;
+  if (!smooth) {
+    ppm_mono(q[2], a1, a2);
+    ppm_mono(q[3], a2r, a3);
+  }
+  ql = a2;
+  qr = a2r;
+}
+__device__ __forceinline__ void minmod_face(const float* q, float& ql, float& qr, float theta) {
+#pragma clang fp contract(fast)
+  const float sl = minmod_theta(q[2] - q[1], q[1] - q[0], theta);
+  const float sr = minmod_theta(q[3] - q[2], q[2] - q[1], theta);
+  ql = q[1] + 0.5f * sl;
+  qr = q[2] - 0.5f * sr;
+}
+
+}  // namespace f32
+}  // namespace amrh

@@ -30,7 +30,9 @@ extern "C" {
 
 #define AMRH_PHYS_EULER 0 /* ideal gas, fields rho sx sy sz E tau (euler/state.cpp:9-17) */
 
-#define AMRH_RECON_MINMOD 0 /* minmod-theta slopes (reconstruct.cpp:60-71) */
+#define AMRH_RECON_MINMOD 0 /* minmod-theta slopes (reconstruct.cpp:60-71); g = 2 */
+#define AMRH_RECON_PPM 1    /* piecewise parabolic, theta = 2 slopes + monotonisation
+                               (reconstruct.cpp:22-58 as dispatched at :92-102); g = 3, n = 8 */
 
 #define AMRH_FP64 0  /* FP64 end to end; matches the FP64 CPU path to roundoff */
 #define AMRH_MIXED 1 /* FP32 reconstruction / fluxes / wavespeeds, FP64 accumulation + RK + floor */
@@ -46,7 +48,7 @@ extern "C" {
 
 typedef struct {
   int n;         /* interior cells per block edge; 8 or 16 (euler/euler_defs.h:27) */
-  int g;         /* ghost width; 2 (euler/euler_defs.h:25) */
+  int g;         /* ghost width: 2 for minmod (euler/euler_defs.h:25), 3 for PPM (binary/binary_defs.h:30) */
   int nf;        /* fields; 6 for AMRH_PHYS_EULER (euler/state.h:16) */
   int physics;   /* AMRH_PHYS_* */
   int recon;     /* AMRH_RECON_* */

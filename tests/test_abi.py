@@ -38,3 +38,22 @@ def test_python_binding_covers_header():
     sys.path.insert(0, PKG)
     import amrhydro
     assert set(declared_functions()) == set(amrhydro.SIGNATURES)
+
+
+@pytest.mark.parametrize("n,g,recon,msg", [
+    (8, 2, 1, "PPM reconstruction needs g = 3"),
+    (16, 3, 1, "PPM reconstruction runs on n = 8"),
+    (8, 3, 0, "minmod reconstruction needs g = 2"),
+    (8, 2, 7, "recon 7 unknown"),
+    (12, 2, 0, "n must be 8 or 16"),
+])
+def test_create_rejects_unsupported_schemes(n, g, recon, msg):
+    """Configuration checks run before any device call, so they are exercised without a GPU."""
+    import sys
+    sys.path.insert(0, PKG)
+    import amrhydro
+    cfg = amrhydro.euler_config(n=n, g=g, recon=recon)
+    ctx = C.c_void_p()
+    rc = amrhydro.lib().amrh_create(C.byref(cfg), 4, C.byref(ctx))
+    assert rc == 1 and not ctx
+    assert msg in amrhydro.lib().amrh_last_error().decode()
