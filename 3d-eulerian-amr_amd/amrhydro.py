@@ -74,6 +74,13 @@ SIGNATURES = {
     "amrh_tree_inject": (C.c_int, [C.c_void_p]),
     "amrh_tree_max_dt": (C.c_int, [C.c_void_p, DP]),
     "amrh_tree_flow_off": (C.c_int, [C.c_void_p, DP]),
+    "amrh_eos_create": (C.c_int, [DP, C.POINTER(C.c_void_p)]),
+    "amrh_eos_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "amrh_eos_destroy": (None, [C.c_void_p]),
+    "amrh_eos_set_units": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "amrh_eos_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "amrh_eos_compute_T": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "amrh_eos_error": (C.c_int, [C.c_void_p, IP]),
 }
 
 _lib = None
@@ -261,3 +268,45 @@ class Batch:
             if fill:
                 self.fill_outflow_ghosts()
             self.substep(dt, b)
+
+
+EOS_FIELDS = ("p", "e", "cs", "cv", "abar", "zbar", "rho", "T", "s", "dsdt")  # eos_t order
+
+
+class HelmholtzEos:
+    """The device Helmholtz EOS (helmholtz/helmeos.cpp).  States are (n, 10) float64 arrays in
+    eos_t order in device memory, passed by pointer."""
+
+    def __init__(self, rows=None, path=None):
+        self.h = C.c_void_p()
+        if path is not None:
+            _ck(lib().amrh_eos_load(path.encode(), C.byref(self.h)))
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.float64)
+            assert rows.shape == (1081 * 401, 9), rows.shape
+            _ck(lib().amrh_eos_create(_dp(rows), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().amrh_eos_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_units(self, cm, g, s, K=1.0):
+        _ck(lib().amrh_eos_set_units(self.h, cm, g, s, K))
+
+    def eval(self, n, st_ptr, stream=0):
+        _ck(lib().amrh_eos_eval(self.h, int(n), C.c_void_p(st_ptr), C.c_void_p(stream)))
+
+    def compute_T(self, n, st_ptr, stream=0):
+        _ck(lib().amrh_eos_compute_T(self.h, int(n), C.c_void_p(st_ptr), C.c_void_p(stream)))
+
+    def error_flags(self):
+        f = C.c_int()
+        _ck(lib().amrh_eos_error(self.h, C.byref(f)))
+        return f.value

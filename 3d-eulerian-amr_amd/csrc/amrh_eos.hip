@@ -1,0 +1,211 @@
+// C ABI of the Helmholtz EOS: the device table and batch evaluation of helmholtz_eos /
+// helmholtz_compute_T (helmholtz/helmeos.cpp).  The kernels of the rotating-frame state use the
+// same device table through amrh_eos_device().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "amrh_ctx.h"
+#include "amrh_eos.h"
+#include "amrhydro.h"
+#include "helmholtz.h"
+
+using namespace amrh;
+
+struct amrh_eos {
+  helm::HelmDev H{};
+  double* cell = nullptr;
+  double* small = nullptr;  // t, d and the eight delta arrays
+  int* err = nullptr;
+  double cm = 1.0, g = 1.0, s = 1.0, K = 1.0;
+};
+
+namespace {
+
+using helm::IMAX;
+using helm::JMAX;
+
+void set_factors(amrh_eos* E) {
+  // helmholtz_set_cgs_units (helmholtz/helmeos.cpp:133-142) and the factors eos_from_code applies
+  const double cm = E->cm, g = E->g, s = E->s, K = E->K;
+  const double erg = g * cm * cm / s / s;
+  const double dyne = g * cm / s / s;
+  E->H.f_p = dyne / pow(cm, 2);
+  E->H.f_e = erg / g;
+  E->H.f_T = K;
+  E->H.f_cs = cm / s;
+  E->H.f_cv = erg / g / K;
+  E->H.f_rho = g / pow(cm, 3);
+  E->H.f_s = erg / K / g;
+  E->H.f_dsdt = erg / K / K / g;
+}
+
+__global__ void eos_kernel(helm::HelmDev H, helm::Eos* st, int n, int solve_T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  helm::Eos q = st[i];
+  if (solve_T)
+    helm::compute_T(H, q);
+  else
+    helm::eos(H, q);
+  st[i] = q;
+}
+
+int eos_batch(amrh_eos* E, int n, void* st_dev, void* stream, int solve_T) {
+  if (!E || !st_dev) return fail(AMRH_EINVAL, "null argument");
+  if (n < 0) return fail(AMRH_EINVAL, "negative count");
+  if (n == 0) return AMRH_OK;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  eos_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(E->H, (helm::Eos*)st_dev, n, solve_T);
+  HIPCHK(hipGetLastError());
+  return AMRH_OK;
+}
+
+}  // namespace
+
+const helm::HelmDev* amrh::eos_device(const amrh_eos* E) { return E ? &E->H : nullptr; }
+
+extern "C" {
+
+int amrh_eos_create(const double* rows, amrh_eos** out) {
+  if (!rows || !out) return fail(AMRH_EINVAL, "null argument");
+  *out = nullptr;
+  // node coordinates and deltas exactly as read_helm_table (helmholtz/helmeos.cpp:68-131)
+  const double tlo = 3.0, thi = 13.0, dlo = -12.0, dhi = 15.0;
+  const double tstp = (thi - tlo) / double(JMAX - 1), tstpi = 1.0 / tstp;
+  const double dstp = (dhi - dlo) / double(IMAX - 1), dstpi = 1.0 / dstp;
+  std::vector<double> small((size_t)JMAX * 5 + (size_t)IMAX * 5 + 2, 0.0);
+  double* t = small.data();
+  double* dt_sav = t + JMAX;
+  double* dt2_sav = dt_sav + JMAX;
+  double* dti_sav = dt2_sav + JMAX;
+  double* dt2i_sav = dti_sav + JMAX;
+  double* d = dt2i_sav + JMAX;
+  double* dd_sav = d + IMAX;
+  double* dd2_sav = dd_sav + IMAX;
+  double* ddi_sav = dd2_sav + IMAX;
+  double* dd2i_sav = ddi_sav + IMAX;
+  for (int j = 0; j < JMAX; j++) t[j] = pow(10.0, tlo + j * tstp);
+  for (int i = 0; i < IMAX; i++) d[i] = pow(10.0, dlo + i * dstp);
+  for (int j = 0; j < JMAX - 1; j++) {
+    const double dth = t[j + 1] - t[j];
+    const double dt2 = dth * dth;
+    dt_sav[j] = dth;
+    dt2_sav[j] = dt2;
+    dti_sav[j] = 1.0 / dth;
+    dt2i_sav[j] = 1.0 / dt2;
+  }
+  for (int i = 0; i < IMAX - 1; i++) {
+    const double dd = d[i + 1] - d[i];
+    const double dd2 = dd * dd;
+    dd_sav[i] = dd;
+    dd2_sav[i] = dd2;
+    ddi_sav[i] = 1.0 / dd;
+    dd2i_sav[i] = 1.0 / dd2;
+  }
+  // per interpolation cell, the 36 coefficients in the order the biquintic sum reads them
+  // (helmholtz/helmeos.cpp:218-253): quantity q in (f, ft, ftt, fd, fdd, fdt, fddt, fdtt, fddtt),
+  // corners (i,j), (i+1,j), (i,j+1), (i+1,j+1).  The file's columns are
+  // f, fd, ft, fdd, ftt, fdt, fddt, fdtt, fddtt (:91-92).
+  static const int col_of_q[9] = {0, 2, 4, 1, 3, 5, 6, 7, 8};
+  const size_t ncell = (size_t)(IMAX - 1) * (JMAX - 1);
+  std::vector<double> cell(ncell * 36);
+  for (int j = 0; j < JMAX - 1; j++)
+    for (int i = 0; i < IMAX - 1; i++) {
+      double* c = &cell[((size_t)j * (IMAX - 1) + i) * 36];
+      for (int q = 0; q < 9; q++) {
+        const int col = col_of_q[q];
+        c[4 * q + 0] = rows[((size_t)j * IMAX + i) * 9 + col];
+        c[4 * q + 1] = rows[((size_t)j * IMAX + i + 1) * 9 + col];
+        c[4 * q + 2] = rows[((size_t)(j + 1) * IMAX + i) * 9 + col];
+        c[4 * q + 3] = rows[((size_t)(j + 1) * IMAX + i + 1) * 9 + col];
+      }
+    }
+  amrh_eos* E = new amrh_eos;
+  hipError_t e = hipMalloc(&E->cell, sizeof(double) * cell.size());
+  if (e == hipSuccess) e = hipMalloc(&E->small, sizeof(double) * small.size());
+  if (e == hipSuccess) e = hipMalloc(&E->err, sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(E->cell, cell.data(), sizeof(double) * cell.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(E->small, small.data(), sizeof(double) * small.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(E->err, 0, sizeof(int));
+  if (e != hipSuccess) {
+    amrh_eos_destroy(E);
+    return fail(AMRH_EHIP, "EOS table allocation failed: %s", hipGetErrorString(e));
+  }
+  helm::HelmDev& H = E->H;
+  H.cell = E->cell;
+  const long off_dt = JMAX, off_d = 5L * JMAX;
+  H.t = E->small;
+  H.dt_sav = E->small + off_dt;
+  H.dt2_sav = H.dt_sav + JMAX;
+  H.dti_sav = H.dt2_sav + JMAX;
+  H.dt2i_sav = H.dti_sav + JMAX;
+  H.d = E->small + off_d;
+  H.dd_sav = H.d + IMAX;
+  H.dd2_sav = H.dd_sav + IMAX;
+  H.ddi_sav = H.dd2_sav + IMAX;
+  H.dd2i_sav = H.ddi_sav + IMAX;
+  H.tlo = tlo;
+  H.tstpi = tstpi;
+  H.dlo = dlo;
+  H.dstpi = dstpi;
+  H.err = E->err;
+  set_factors(E);
+  *out = E;
+  return AMRH_OK;
+}
+
+int amrh_eos_load(const char* path, amrh_eos** out) {
+  if (!path || !out) return fail(AMRH_EINVAL, "null argument");
+  FILE* fp = fopen(path, "rt");
+  if (!fp) return fail(AMRH_EINVAL, "cannot open EOS table %s", path);
+  std::vector<double> rows((size_t)IMAX * JMAX * 9);
+  for (size_t r = 0; r < (size_t)IMAX * JMAX; r++) {
+    double* v = &rows[r * 9];
+    if (fscanf(fp, "%le %le %le %le %le %le %le %le %le\n", v, v + 1, v + 2, v + 3, v + 4, v + 5, v + 6, v + 7,
+               v + 8) != 9) {
+      fclose(fp);
+      return fail(AMRH_EINVAL, "EOS table %s: row %zu unreadable (need %d x %d rows of 9 values)", path, r, JMAX,
+                  IMAX);
+    }
+  }
+  fclose(fp);
+  return amrh_eos_create(rows.data(), out);
+}
+
+void amrh_eos_destroy(amrh_eos* E) {
+  if (!E) return;
+  if (E->cell) (void)hipFree(E->cell);
+  if (E->small) (void)hipFree(E->small);
+  if (E->err) (void)hipFree(E->err);
+  delete E;
+}
+
+int amrh_eos_set_units(amrh_eos* E, double cm, double g, double s, double K) {
+  if (!E) return fail(AMRH_EINVAL, "null EOS");
+  if (!(cm > 0.0 && g > 0.0 && s > 0.0 && K > 0.0)) return fail(AMRH_EINVAL, "unit factors must be positive");
+  E->cm = cm;
+  E->g = g;
+  E->s = s;
+  E->K = K;
+  set_factors(E);
+  return AMRH_OK;
+}
+
+int amrh_eos_eval(amrh_eos* E, int n, void* st_dev, void* stream) { return eos_batch(E, n, st_dev, stream, 0); }
+
+int amrh_eos_compute_T(amrh_eos* E, int n, void* st_dev, void* stream) { return eos_batch(E, n, st_dev, stream, 1); }
+
+int amrh_eos_error(amrh_eos* E, int* flags) {
+  if (!E || !flags) return fail(AMRH_EINVAL, "null argument");
+  int v = 0;
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(&v, E->err, sizeof v, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(E->err, 0, sizeof(int)));
+  *flags = v;
+  return AMRH_OK;
+}
+
+}  // extern "C"

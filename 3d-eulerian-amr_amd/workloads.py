@@ -105,3 +105,44 @@ def multilevel(counts=LEVEL_COUNTS, n=8, g=2, sigma=0.2, omega=0.5, seed=SEED):
         U[s:e, 4] = eps + 0.5 * rho * (vx * vx + vy * vy)
         U[s:e, 5] = eps ** (1.0 / GAMMA)
     return U, h, lev
+
+
+# ---- Helmholtz EOS table ---------------------------------------------------------------------
+HELM_IMAX, HELM_JMAX = 1081, 401  # helmholtz/helmeos.cpp:7-8
+HELM_AVO, HELM_KERG = 6.0221417930e23, 1.380650424e-16  # helmholtz/const.h
+HELM_H, HELM_ME = 6.6260689633e-27, 9.1093821545e-28
+
+
+def helmholtz_table():
+    """Synthetic electron free-energy table at the reference's dimensions and ranges
+    (log10(ye*rho) in [-12, 15] over 1081 nodes, log10 T in [3, 13] over 401 nodes,
+    helmholtz/helmeos.cpp:7-8, :78-86), built from the free energy of a non-degenerate ideal
+    electron gas per unit ye*rho:
+        f(d, T) = A T (ln d - 1.5 ln T + C),  A = N_A k,  C = ln(N_A/2 (h^2/(2 pi m_e k))^1.5) - 1,
+    with its derivatives in closed form.  Rows are (f, fd, ft, fdd, ftt, fdt, fddt, fdtt, fddtt),
+    temperature outer, density inner: the helmholtz.table layout.  The EOS built on it is an ideal
+    gas of ions and electrons, p = N_A k T rho (1/abar + ye), to the biquintic interpolation error."""
+    I, J = HELM_IMAX, HELM_JMAX
+    d = 10.0 ** (-12.0 + np.arange(I) * (27.0 / (I - 1)))
+    T = 10.0 ** (3.0 + np.arange(J) * (10.0 / (J - 1)))
+    A = HELM_AVO * HELM_KERG
+    C = np.log(HELM_AVO / 2.0 * (HELM_H ** 2 / (2.0 * np.pi * HELM_ME * HELM_KERG)) ** 1.5) - 1.0
+    dd, TT = np.meshgrid(d, T)  # shape (J, I): temperature outer
+    L = np.log(dd) - 1.5 * np.log(TT) + C
+    rows = np.empty((J, I, 9))
+    rows[..., 0] = A * TT * L               # f
+    rows[..., 1] = A * TT / dd              # f_d
+    rows[..., 2] = A * (L - 1.5)            # f_t
+    rows[..., 3] = -A * TT / dd ** 2        # f_dd
+    rows[..., 4] = -1.5 * A / TT            # f_tt
+    rows[..., 5] = A / dd                   # f_dt
+    rows[..., 6] = -A / dd ** 2             # f_ddt
+    rows[..., 7] = 0.0                      # f_dtt
+    rows[..., 8] = 0.0                      # f_ddtt
+    return np.ascontiguousarray(rows.reshape(J * I, 9))
+
+
+def write_helmholtz_table(path, rows):
+    """The rows as a helmholtz.table text file (9 values per line, 17 significant digits, so
+    %le parsing restores them exactly)."""
+    np.savetxt(path, rows, fmt="%.17e")

@@ -170,6 +170,35 @@ int amrh_tree_max_dt(amrh_ctx* ctx, double* dt);
 /* The root's flow-off ledger FO (Grid::flow_off, grid/grid.cpp:346-348): nf doubles. */
 int amrh_tree_flow_off(amrh_ctx* ctx, double* fo);
 
+/* ---- Helmholtz EOS (helmholtz/helmeos.cpp) -------------------------------------------------
+ * The electron-positron free-energy table and the EOS built on it, on the device, FP64, in the
+ * reference's operation order: pressure, energy, sound speed and heat capacity are bit-identical
+ * to the FP64 CPU path (entropy to an ulp of log).  A state is the reference's eos_t
+ * (helmholtz/helmholtz.h:4-6): 10 doubles p, e, cs, cv, abar, zbar, rho, T, s, dsdt. */
+typedef struct amrh_eos amrh_eos;
+
+/* Device table from `rows`: IMAX*JMAX = 1081*401 records of 9 doubles (f, fd, ft, fdd, ftt, fdt,
+ * fddt, fdtt, fddtt), temperature outer, density inner -- the layout of the helmholtz.table file
+ * that read_helm_table parses (helmholtz/helmeos.cpp:68-131); node coordinates and spacings are
+ * formed as there.  The device keeps, per interpolation cell, the 36 coefficients one evaluation
+ * reads, contiguous (125 MB).  Synchronous. */
+int amrh_eos_create(const double* rows, amrh_eos** out);
+/* Same, parsing a helmholtz.table text file. */
+int amrh_eos_load(const char* path, amrh_eos** out);
+void amrh_eos_destroy(amrh_eos* eos);
+/* helmholtz_set_cgs_units (helmholtz/helmeos.cpp:133-142): code units in cgs; default all 1. */
+int amrh_eos_set_units(amrh_eos* eos, double cm, double g, double s, double K);
+/* helmholtz_eos (helmholtz/helmeos.cpp:164-421) on n states in device memory (st_dev: n * 10
+ * doubles), in place, on `stream` (a hipStream_t; NULL = the null stream).  Asynchronous. */
+int amrh_eos_eval(amrh_eos* eos, int n, void* st_dev, void* stream);
+/* helmholtz_compute_T, energy form (helmholtz/helmeos.cpp:436-486): damped Newton for the T at
+ * which e(rho, T) equals the state's e, starting from the state's T; in place.  Asynchronous. */
+int amrh_eos_compute_T(amrh_eos* eos, int n, void* st_dev, void* stream);
+/* Device error flags raised since the last call, where the reference aborts: 8 state outside
+ * the table, 16 negative heat capacity, 32 negative squared sound speed, 64 Newton did not
+ * converge in 100 iterations, 128 T fell below TMIN.  Synchronises the device; clears them. */
+int amrh_eos_error(amrh_eos* eos, int* flags);
+
 #ifdef __cplusplus
 }
 #endif

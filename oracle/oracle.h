@@ -70,6 +70,16 @@ int orc_tree_flow_off(const orc_tree* t, double* fo);
 /* face flux array of a node for the last computed direction (debug) */
 const double* orc_tree_node_F(const orc_tree* t, int node);
 
+/* ---- Helmholtz EOS (helmholtz/helmeos.cpp), helmholtz.cpp --------------------------------
+ * rows: 1081*401 records of 9 doubles in the helmholtz.table layout (temperature outer).
+ * States: 10 doubles each in eos_t order (p, e, cs, cv, abar, zbar, rho, T, s, dsdt), in place;
+ * flags[k] gets the error bits where the reference aborts (8 range, 16 cv < 0, 32 cs^2 < 0,
+ * 64 Newton did not converge, 128 T < TMIN). */
+void orc_helm_table(const double* rows);
+void orc_helm_set_units(double cm, double g, double s, double K);
+int orc_helm_eos(int n, double* st, int* flags);
+int orc_helm_compute_T(int n, double* st, int* flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,12 @@ def lib():
         L.orc_tree_substep.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_tree_state_sum.argtypes = [C.c_void_p, DP]
         L.orc_tree_flow_off.argtypes = [C.c_void_p, DP]
+        L.orc_helm_table.argtypes = [DP]
+        L.orc_helm_table.restype = None
+        L.orc_helm_set_units.argtypes = [C.c_double] * 4
+        L.orc_helm_set_units.restype = None
+        L.orc_helm_eos.argtypes = [C.c_int, DP, IP]
+        L.orc_helm_compute_T.argtypes = [C.c_int, DP, IP]
         _lib = L
     return _lib
 
@@ -203,3 +209,46 @@ class Tree:
         s = np.zeros(self.cfg.nf)
         self.L.orc_tree_flow_off(self.t, dptr(s))
         return s
+
+
+# ---- Helmholtz EOS ---------------------------------------------------------------------------
+
+def helm_table(rows):
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    lib().orc_helm_table(dptr(rows))
+
+
+def helm_set_units(cm=1.0, g=1.0, s=1.0, K=1.0):
+    lib().orc_helm_set_units(cm, g, s, K)
+
+
+def helm_eos(st, solve_T=False):
+    """States (n, 10) in eos_t order, updated in place; returns the per-state error flags."""
+    assert st.dtype == np.float64 and st.flags["C_CONTIGUOUS"] and st.shape[1] == 10
+    flags = np.zeros(st.shape[0], dtype=np.int32)
+    fn = lib().orc_helm_compute_T if solve_T else lib().orc_helm_eos
+    assert fn(st.shape[0], dptr(st), flags.ctypes.data_as(IP)) == 0
+    return flags
+
+
+def ref_helm_lib(table_dir):
+    """The reference's helmeos.cpp (oracle/_ref/libref_helm.so), which reads ./helmholtz.table when
+    it is loaded: table_dir must hold that file.  None where the harness is not built."""
+    path = os.path.join(REF_DIR, "libref_helm.so")
+    if not os.path.exists(path):
+        return None
+    if not os.path.exists(os.path.join(table_dir, "helmholtz.table")):
+        raise RuntimeError("helmholtz.table missing in %s" % table_dir)
+    cwd = os.getcwd()
+    os.chdir(table_dir)
+    try:
+        L = C.CDLL(path)
+    finally:
+        os.chdir(cwd)
+    L.ref_helm_set_units.argtypes = [C.c_double] * 4
+    L.ref_helm_set_units.restype = None
+    L.ref_helm_eos.argtypes = [C.c_int, DP]
+    L.ref_helm_eos.restype = None
+    L.ref_helm_compute_T.argtypes = [C.c_int, DP]
+    L.ref_helm_compute_T.restype = None
+    return L

@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "amrhydro.h")
 def declared_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(amrh_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(amrh_[A-Za-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_declares_entry_points():
