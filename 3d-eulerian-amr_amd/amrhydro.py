@@ -13,6 +13,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libamrhydro.so")
 
 AMRH_PHYS_EULER = 0
+AMRH_PHYS_ROTATING = 1
+AMRH_EOS_IDEAL = 0
+AMRH_EOS_HELMHOLTZ = 1
+ROT_FIELDS = ("rho", "rho_acc", "rho_don", "s_R", "l_z", "s_z", "pot", "E", "tau", "T",
+              "X_he", "X_co", "X_ne")  # binary/state_ztwd.cpp:15-28 (species by abar 4, 14.12, 17.52)
 AMRH_RECON_MINMOD = 0
 AMRH_RECON_PPM = 1
 AMRH_FP64 = 0
@@ -27,7 +32,9 @@ class Config(C.Structure):
                 ("recon", C.c_int), ("precision", C.c_int), ("z_reflect", C.c_int),
                 ("gamma", C.c_double), ("rho_floor", C.c_double), ("ei_floor", C.c_double),
                 ("theta", C.c_double), ("cfl", C.c_double), ("grid_dim", C.c_double),
-                ("boundary", C.c_int)]
+                ("boundary", C.c_int), ("eos", C.c_int), ("omega", C.c_double),
+                ("omega0", C.c_double), ("f_r", C.c_double), ("f_theta", C.c_double),
+                ("abar", C.c_double * 3)]
 
 
 AMRH_BOUNDARY_EXTERNAL = 0
@@ -41,6 +48,21 @@ def euler_config(n=8, g=2, precision=AMRH_FP64, z_reflect=0, grid_dim=1.0,
     return Config(n=n, g=g, nf=6, physics=AMRH_PHYS_EULER, recon=recon,
                   precision=precision, z_reflect=z_reflect, gamma=5.0 / 3.0, rho_floor=1e-20,
                   ei_floor=1e-20, theta=1.3, cfl=0.4, grid_dim=grid_dim, boundary=boundary)
+
+
+def rotating_config(eos=AMRH_EOS_IDEAL, recon=AMRH_RECON_PPM, precision=AMRH_FP64, omega=0.0, omega0=None,
+                    f_r=0.0, f_theta=0.0, z_reflect=0, boundary=AMRH_BOUNDARY_EXTERNAL, gamma=5.0 / 3.0,
+                    cfl=0.3):
+    """Rotating-frame binary-star state (binary/state_ztwd.cpp) on 8^3 blocks: 13 fields, PPM with
+    g = 3 (binary/binary_defs.h:29-30) or minmod with g = 2; rho_floor 1e-10 (:31), species mass
+    numbers 4, 14.12, 17.52 (:28), GRID_CFL_FACTOR 0.3 (binary/binary_defs.h:58)."""
+    g = 3 if recon == AMRH_RECON_PPM else 2
+    cfg = Config(n=8, g=g, nf=13, physics=AMRH_PHYS_ROTATING, recon=recon, precision=precision,
+                 z_reflect=z_reflect, gamma=gamma, rho_floor=1.0e-10, ei_floor=1.0e-20, theta=1.3,
+                 cfl=cfl, grid_dim=1.0, boundary=boundary, eos=eos, omega=omega,
+                 omega0=omega if omega0 is None else omega0, f_r=f_r, f_theta=f_theta)
+    cfg.abar[0], cfg.abar[1], cfg.abar[2] = 4.0, 14.12, 17.52
+    return cfg
 
 
 # name -> (restype, argtypes); the full exported surface of include/amrhydro.h
@@ -81,6 +103,10 @@ SIGNATURES = {
     "amrh_eos_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amrh_eos_compute_T": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amrh_eos_error": (C.c_int, [C.c_void_p, IP]),
+    "amrh_attach_eos": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "amrh_set_block_origin": (C.c_int, [C.c_void_p, DP]),
+    "amrh_set_potential": (C.c_int, [C.c_void_p, DP]),
+    "amrh_set_potential_device": (C.c_int, [C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -220,6 +246,22 @@ class Batch:
 
     def run_steps(self, nsteps, dt_ptr, rk=3, use_graph=True):
         _ck(lib().amrh_run_steps(self.ctx, rk, nsteps, C.c_void_p(dt_ptr), 1 if use_graph else 0))
+
+    # ---- rotating-frame state ----------------------------------------------------------------
+    def set_block_origin(self, x0):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        assert x0.shape == (self.nb, 3)
+        _ck(lib().amrh_set_block_origin(self.ctx, _dp(x0)))
+
+    def set_potential(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        P = self.cfg.n + 2
+        assert phi.shape == (self.nb, P, P, P), phi.shape
+        _ck(lib().amrh_set_potential(self.ctx, _dp(phi)))
+
+    def attach_eos(self, eos):
+        self._eos = eos  # the table must outlive the context
+        _ck(lib().amrh_attach_eos(self.ctx, eos.h))
 
     # ---- octree ------------------------------------------------------------------------------
     def tree_set(self, parent, child):

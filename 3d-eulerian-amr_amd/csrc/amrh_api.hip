@@ -7,8 +7,10 @@
 #include <string>
 
 #include "amrh_ctx.h"
+#include "amrh_eos.h"
 #include "amrh_internal.h"
 #include "amrh_layout.h"
+#include "kt_rot.h"
 #include "amrhydro.h"
 
 using namespace amrh;
@@ -51,8 +53,18 @@ long record_elems(const amrh_ctx* c) { return (long)c->nb * c->L.block(); }
 
 int check_cfg(const amrh_config* c) {
   if (!c) return fail(AMRH_EINVAL, "null config");
-  if (c->physics != AMRH_PHYS_EULER) return fail(AMRH_EINVAL, "physics %d not supported", c->physics);
-  if (c->nf != 6) return fail(AMRH_EINVAL, "EULER physics has 6 fields, got %d", c->nf);
+  if (c->physics == AMRH_PHYS_EULER) {
+    if (c->nf != 6) return fail(AMRH_EINVAL, "EULER physics has 6 fields, got %d", c->nf);
+  } else if (c->physics == AMRH_PHYS_ROTATING) {
+    if (c->nf != rot::NF) return fail(AMRH_EINVAL, "ROTATING physics has %d fields, got %d", rot::NF, c->nf);
+    if (c->n != 8) return fail(AMRH_EINVAL, "ROTATING physics runs on n = 8 blocks, got n = %d", c->n);
+    if (c->eos != AMRH_EOS_IDEAL && c->eos != AMRH_EOS_HELMHOLTZ) return fail(AMRH_EINVAL, "eos %d unknown", c->eos);
+    for (int i = 0; i < 3; i++)
+      if (!(c->abar[i] > 0.0)) return fail(AMRH_EINVAL, "abar[%d] must be positive", i);
+    if (!(c->rho_floor > 0.0)) return fail(AMRH_EINVAL, "rho_floor must be positive");
+  } else {
+    return fail(AMRH_EINVAL, "physics %d not supported", c->physics);
+  }
   if (c->n != 8 && c->n != 16) return fail(AMRH_EINVAL, "n must be 8 or 16, got %d", c->n);
   if (c->recon == AMRH_RECON_MINMOD) {
     if (c->g != 2) return fail(AMRH_EINVAL, "minmod reconstruction needs g = 2, got %d", c->g);
@@ -92,6 +104,11 @@ int amrh_create(const amrh_config* cfg, int nblocks, amrh_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->eig_dev, sizeof(double) * nblocks);
   if (e == hipSuccess) e = hipMalloc(&c->gmax_dev, sizeof(unsigned long long) * 2);
   if (e == hipSuccess) e = hipMalloc(&c->err_dev, sizeof(int));
+  if (e == hipSuccess && cfg->physics == AMRH_PHYS_ROTATING) {
+    const size_t P = (size_t)cfg->n + 2;
+    e = hipMalloc(&c->x0_dev, sizeof(double) * 3 * nblocks);
+    if (e == hipSuccess) e = hipMalloc(&c->phi_dev, sizeof(double) * P * P * P * nblocks);
+  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMemset(c->err_dev, 0, sizeof(int));
   if (e == hipSuccess) e = hipMemset(c->gmax_dev, 0, sizeof(unsigned long long) * 2);
@@ -116,6 +133,8 @@ void amrh_destroy(amrh_ctx* c) {
   if (c->gmax_dev) (void)hipFree(c->gmax_dev);
   if (c->err_dev) (void)hipFree(c->err_dev);
   if (c->pad_stage) (void)hipFree(c->pad_stage);
+  if (c->x0_dev) (void)hipFree(c->x0_dev);
+  if (c->phi_dev) (void)hipFree(c->phi_dev);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (c->tree) {
     c->tree->free_dev();
@@ -199,14 +218,108 @@ int amrh_store(amrh_ctx* c) {
 int amrh_fill_outflow_ghosts(amrh_ctx* c) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
-  HIPCHK(launch_outflow_ghosts(c->cfg.n, c->cfg.g, c->buf[c->cur], c->nb, c->cfg.z_reflect, c->stream));
+  if (int rc = amrh::ctx_ready(c)) return rc;
+  HIPCHK(ctx_outflow_ghosts(c, c->buf[c->cur]));
+  return AMRH_OK;
+}
+
+int amrh_set_block_origin(amrh_ctx* c, const double* x0) {
+  if (!c || !x0) return fail(AMRH_EINVAL, "null argument");
+  if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "block origins belong to ROTATING physics");
+  HIPCHK(hipMemcpyAsync(c->x0_dev, x0, sizeof(double) * 3 * c->nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_x0 = true;
+  return AMRH_OK;
+}
+
+static size_t phi_bytes(const amrh_ctx* c) {
+  const size_t P = (size_t)c->cfg.n + 2;
+  return sizeof(double) * P * P * P * c->nb;
+}
+
+int amrh_set_potential(amrh_ctx* c, const double* phi) {
+  if (!c || !phi) return fail(AMRH_EINVAL, "null argument");
+  if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "the potential belongs to ROTATING physics");
+  HIPCHK(hipMemcpyAsync(c->phi_dev, phi, phi_bytes(c), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_phi = true;
+  return AMRH_OK;
+}
+
+int amrh_set_potential_device(amrh_ctx* c, const void* phi) {
+  if (!c || !phi) return fail(AMRH_EINVAL, "null argument");
+  if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "the potential belongs to ROTATING physics");
+  HIPCHK(hipMemcpyAsync(c->phi_dev, phi, phi_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  c->have_phi = true;
+  return AMRH_OK;
+}
+
+int amrh_attach_eos(amrh_ctx* c, amrh_eos* eos) {
+  if (!c || !eos) return fail(AMRH_EINVAL, "null argument");
+  if (c->cfg.physics != AMRH_PHYS_ROTATING || c->cfg.eos != AMRH_EOS_HELMHOLTZ)
+    return fail(AMRH_EINVAL, "an EOS table belongs to ROTATING physics with eos = AMRH_EOS_HELMHOLTZ");
+  c->eos = eos;
   return AMRH_OK;
 }
 
 }  // extern "C"
 
+namespace {
+rot::RotParams rot_params(const amrh_config& g) {
+  rot::RotParams r{};
+  r.Omega = g.omega;
+  r.Omega0 = g.omega0;
+  r.fR = g.f_r;
+  r.ftheta = g.f_theta;
+  r.rho_floor = g.rho_floor;
+  r.gamma = g.gamma;
+  r.theta = g.theta;
+  for (int i = 0; i < 3; i++) r.abar[i] = g.abar[i];
+  return r;
+}
+
+RotArgs rot_args(const amrh_ctx* c) {
+  RotArgs r{};
+  r.phi = c->phi_dev;
+  r.x0 = c->x0_dev;
+  r.h = c->h_dev;
+  r.nblocks = c->nb;
+  r.err = c->err_dev;
+  r.c = rot_params(c->cfg);
+  if (c->eos) r.H = *eos_device(c->eos);
+  r.H.err = c->err_dev;
+  return r;
+}
+}  // namespace
+
+int amrh::ctx_ready(const amrh_ctx* c) {
+  if (c->cfg.physics != AMRH_PHYS_ROTATING) return AMRH_OK;
+  if (!c->have_x0) return fail(AMRH_ESTATE, "ROTATING physics needs amrh_set_block_origin");
+  if (!c->have_phi) return fail(AMRH_ESTATE, "ROTATING physics needs amrh_set_potential");
+  if (c->cfg.eos == AMRH_EOS_HELMHOLTZ && !c->eos) return fail(AMRH_ESTATE, "eos = HELMHOLTZ needs amrh_attach_eos");
+  return AMRH_OK;
+}
+
+hipError_t amrh::ctx_outflow_ghosts(const amrh_ctx* c, double* U) {
+  if (c->cfg.physics == AMRH_PHYS_ROTATING)
+    return launch_rot_ghosts(c->cfg.g, U, c->x0_dev, c->h_dev, c->nb, c->cfg.z_reflect, rot_params(c->cfg), c->stream);
+  return launch_outflow_ghosts(c->cfg.n, c->cfg.g, U, c->nb, c->cfg.z_reflect, c->stream);
+}
+
 hipError_t amrh::ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int grid) {
   const bool mixed = c->cfg.precision == AMRH_MIXED;
+  if (c->cfg.physics == AMRH_PHYS_ROTATING) {
+    RotArgs r = rot_args(c);
+    r.Uin = a.Uin;
+    r.Uout = a.Uout;
+    r.U0 = a.U0;
+    r.dt_dev = a.dt_dev;
+    r.dt = a.dt;
+    r.beta = a.beta;
+    hipError_t e = launch_rot_substep(c->cfg.g, c->cfg.recon, c->cfg.eos, r, grid, c->stream);
+    if (e == hipSuccess && a.outflow) e = ctx_outflow_ghosts(c, a.Uout);
+    return e;
+  }
   if (c->cfg.recon == AMRH_RECON_MINMOD) return launch_substep(c->cfg.n, a, mixed, grid, c->stream);
   hipError_t e = launch_block_substep(c->cfg.n, c->cfg.g, c->cfg.recon, a, mixed, grid, c->stream);
   if (e == hipSuccess && a.outflow) e = launch_outflow_ghosts(c->cfg.n, c->cfg.g, a.Uout, c->nb, c->cfg.z_reflect, c->stream);
@@ -215,6 +328,13 @@ hipError_t amrh::ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int
 
 hipError_t amrh::ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid) {
   const bool mixed = c->cfg.precision == AMRH_MIXED;
+  if (c->cfg.physics == AMRH_PHYS_ROTATING) {
+    RotArgs r = rot_args(c);
+    r.Uin = m.U;
+    r.block_eig = m.block_eig;
+    r.gmax = m.gmax;
+    return launch_rot_maxeig(c->cfg.g, c->cfg.recon, c->cfg.eos, r, grid, c->stream);
+  }
   if (c->cfg.recon == AMRH_RECON_MINMOD) return launch_maxeig(c->cfg.n, m, mixed, grid, c->stream);
   return launch_block_maxeig(c->cfg.n, c->cfg.g, c->cfg.recon, m, mixed, grid, c->stream);
 }
@@ -224,6 +344,7 @@ hipError_t amrh::ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int g
 int amrh::substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool tree) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
+  if (int rc = ctx_ready(c)) return rc;
   const bool need_u0 = beta != 1.0;
   if (need_u0 && c->u0buf < 0) return fail(AMRH_ESTATE, "beta != 1 needs amrh_store() first");
   SubstepArgs a{};
@@ -269,6 +390,7 @@ int amrh_substep_dev(amrh_ctx* c, const double* dt_dev, double beta) {
 static int maxeig_impl(amrh_ctx* c, bool per_block, bool global) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
+  if (int rc = ctx_ready(c)) return rc;
   MaxEigArgs m{};
   m.U = c->buf[c->cur];
   m.h = c->h_dev;
@@ -323,7 +445,9 @@ double amrh_substep_bytes(const amrh_ctx* c, int beta_is_one) {
   const double read_u = n3 + 6.0 * g * n * n;
   // with AMRH_BOUNDARY_OUTFLOW the substep also writes the next stage's face ghost slabs
   const double ghost_w = c->cfg.boundary == AMRH_BOUNDARY_OUTFLOW ? 6.0 * g * n * n : 0.0;
-  const double per_block = 8.0 * nf * (read_u + n3 + ghost_w + (beta_is_one ? 0.0 : n3));
+  double per_block = 8.0 * nf * (read_u + n3 + ghost_w + (beta_is_one ? 0.0 : n3));
+  // the rotating state also reads the potential with its ghost layer
+  if (c->cfg.physics == AMRH_PHYS_ROTATING) per_block += 8.0 * (n + 2.0) * (n + 2.0) * (n + 2.0);
   return per_block * c->nb;
 }
 

@@ -56,6 +56,11 @@ struct amrh_ctx {
   hipStream_t own_stream = nullptr;
   bool have_h = false, have_u = false;
   amrh::TreeState* tree = nullptr;
+  // AMRH_PHYS_ROTATING
+  double* x0_dev = nullptr;    // [nb][3] block origins
+  double* phi_dev = nullptr;   // [nb][(n+2)^3] potential
+  const amrh_eos* eos = nullptr;
+  bool have_x0 = false, have_phi = false;
   int grid_substep = 0;  // persistent grid of the substep kernel (0: one workgroup per block)
   int grid_maxeig = 0;   // same for the CFL scan
   // captured RK steps, one graph per parity of the ping-pong buffers (amrh_graph.hip)
@@ -73,6 +78,10 @@ int substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool
 // the block kernel for PPM); a block-kernel substep in outflow mode is followed by the ghost fill
 hipError_t ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int grid);
 hipError_t ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid);
+// the physics' extra inputs are in place (0) or an AMRH_ESTATE failure
+int ctx_ready(const amrh_ctx* c);
+// ghost fill of isolated outflow blocks for the context's physics
+hipError_t ctx_outflow_ghosts(const amrh_ctx* c, double* U);
 
 }  // namespace amrh
 

@@ -80,6 +80,7 @@ extern "C" {
 int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
+  if (int rc = ctx_ready(c)) return rc;
   const int other = 1 - c->cur;
   if (c->u0buf == other) return fail(AMRH_ESTATE, "autotune writes the spare buffer: call it before amrh_store");
   const int cus = n_cus();

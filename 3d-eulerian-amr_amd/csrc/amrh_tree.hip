@@ -255,8 +255,8 @@ extern "C" {
 
 int amrh_tree_set(amrh_ctx* c, const int* parent, const int* child) {
   if (!c || !parent || !child) return fail(AMRH_EINVAL, "null argument");
-  if (c->cfg.recon != AMRH_RECON_MINMOD)
-    return fail(AMRH_EINVAL, "the octree coupling runs the minmod scheme (g = 2) only");
+  if (c->cfg.recon != AMRH_RECON_MINMOD || c->cfg.physics != AMRH_PHYS_EULER)
+    return fail(AMRH_EINVAL, "the octree coupling runs the ideal-gas minmod scheme (g = 2) only");
   if (int rc = build(c, parent, child)) return rc;
   HIPCHK(hipMemcpy(c->h_dev, c->tree->h.data(), sizeof(double) * c->nb, hipMemcpyHostToDevice));
   c->have_h = true;

@@ -1,0 +1,403 @@
+// Rotating-frame binary-star substep (the reference's BinaryDriver::sub_step with a given
+// gravitational potential), its CFL scan and its outflow ghost fill, on 8^3 blocks, FP64.
+//
+// One workgroup of N^3 = 512 threads per block, a thread per interior cell.  For each direction
+// in x, y, z order the primitives of the direction's pencils (interior + its two face ghost slabs)
+// are staged in LDS, every face of the block is reconstructed (PPM with the potential field
+// low-order and smooth, or minmod), converted back at the face position, given to the EOS, and
+// its KT flux and the normal velocity of the mean face state (SAVE_RECONSTRUCTIONS,
+// grid/grid.cpp:235-237) are written to LDS; each cell then subtracts its flux difference from D
+// (grid/grid.cpp:304-344).  The cell finally adds the gravity source from the face-averaged force
+// of the potential and the -p div u source (binary/binary.cpp:205-254, poisson/poisson.cpp:223-247),
+// zeroes D[pot] and D[T], adds the geometric source and forms U = (U + D dt) beta + U0 (1 - beta)
+// (grid/grid.cpp:406-420), floors it with the Newton solve for T (binary/state_ztwd.cpp:482-554) and
+// resets pot = rho (phi - R^2 Omega0^2 / 2) (binary/binary.cpp:825-849).
+// LDS: 13 x 896 staged primitives + 13 x 576 face fluxes + 576 face velocities, FP64: 154 KB.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "amrh_internal.h"
+#include "amrhydro.h"
+#include "kt_rot.h"
+#include "recon.h"
+#include "rot_phys.h"
+
+namespace amrh {
+
+namespace {
+
+using namespace rot;
+
+constexpr int N = 8;
+constexpr int N2 = N * N, N3 = N * N * N;
+constexpr int NFACE = (N + 1) * N * N;
+
+// record element -> interior-relative coordinates (i, j, k), any of them in [-G, N+G)
+template <int G>
+__device__ __forceinline__ void elem_ijk(int e, int& i, int& j, int& k) {
+  constexpr int S = G * N2;
+  if (e < N3) {
+    i = e % N;
+    j = (e / N) % N;
+    k = e / N2;
+    return;
+  }
+  const int s = (e - N3) / S, r = (e - N3) % S;
+  const int up = s & 1;
+  if (s < 2) {
+    const int t = r % G;
+    j = (r / G) % N;
+    k = r / (G * N);
+    i = up ? N + t : t - G;
+  } else if (s < 4) {
+    i = r % N;
+    const int t = (r / N) % G;
+    k = r / (G * N);
+    j = up ? N + t : t - G;
+  } else {
+    i = r % N;
+    j = (r / N) % N;
+    const int t = r / N2;
+    k = up ? N + t : t - G;
+  }
+}
+
+// the region staged for direction d: interior, then the low and high slabs of d
+template <int G>
+__device__ __forceinline__ int region_to_elem(int d, int r) {
+  constexpr int S = G * N2;
+  return r < N3 ? r : r + 2 * d * S;
+}
+template <int G>
+__device__ __forceinline__ int pencil_region(int d, int p, int ta, int tb) {
+  constexpr int S = G * N2;
+  if (p >= 0 && p < N) {
+    if (d == 0) return (tb * N + ta) * N + p;
+    if (d == 1) return (tb * N + p) * N + ta;
+    return (p * N + tb) * N + ta;
+  }
+  const int up = p >= N ? 1 : 0;
+  const int t = up ? p - N : p + G;
+  const int base = N3 + up * S;
+  if (d == 0) return base + (tb * N + ta) * G + t;
+  if (d == 1) return base + (tb * G + t) * N + ta;
+  return base + (t * N + tb) * N + ta;
+}
+
+__device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int j, int k, double* X) {
+  const int q[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double xf = x0[d] + (double)q[d] * h;
+    X[d] = xf + 0.5 * h;
+  }
+}
+
+template <int G, int RECON, int EOS, int MODE>
+__global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
+  constexpr int RS = N3 + 2 * G * N2;  // staged elements per field
+  constexpr int R = N3 + 6 * G * N2;   // record elements per field
+  __shared__ double P[NF][RS];
+  __shared__ double Fb[MODE == 0 ? NF : 1][MODE == 0 ? NFACE : 1];
+  __shared__ double Vf[MODE == 0 ? NFACE : 1];
+  __shared__ double red[N3 / 64];
+
+  const int tid = threadIdx.x;
+  const RotParams& c = a.c;
+  const helm::HelmDev& H = a.H;
+  double dt = a.dt;
+  if (MODE == 0 && a.dt_dev) dt = *a.dt_dev;
+  const double beta = a.beta;
+  const bool has_u0 = MODE == 0 && beta != 1.0;
+  const int ci = tid % N, cj = (tid / N) % N, ck = tid / N2;
+
+  for (int b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+    const double* Ub = a.Uin + (long)b * NF * R;
+    const double h = a.h[b];
+    const double x0[3] = {a.x0[3 * b], a.x0[3 * b + 1], a.x0[3 * b + 2]};
+    double D[NF];
+#pragma unroll
+    for (int l = 0; l < NF; l++) D[l] = 0.0;
+    double vlo[3], vhi[3];
+    double emax = 0.0;
+    const double hinv = 1.0 / h;
+
+    auto sweep = [&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      // stage the primitives of this direction's pencils
+#pragma unroll 1
+      for (int r = tid; r < RS; r += N3) {
+        const int e = region_to_elem<G>(d, r);
+        double u[NF];
+#pragma unroll
+        for (int l = 0; l < NF; l++) u[l] = Ub[(long)l * R + e];
+        int i, j, k;
+        elem_ijk<G>(e, i, j, k);
+        double X[3];
+        cell_pos(x0, h, i, j, k, X);
+        to_prim(c, u, X);
+#pragma unroll
+        for (int l = 0; l < NF; l++) P[l][r] = u[l];
+      }
+      __syncthreads();
+#pragma unroll 1
+      for (int q = tid; q < NFACE; q += N3) {
+        const int f = q % (N + 1), t = q / (N + 1), ta = t % N, tb = t / N;
+        double ql[NF], qr[NF];
+        if constexpr (RECON == AMRH_RECON_PPM) {
+          int e6[6];
+#pragma unroll
+          for (int m = 0; m < 6; m++) e6[m] = pencil_region<G>(d, f - 3 + m, ta, tb);
+#pragma unroll
+          for (int l = 0; l < NF; l++) {
+            double v[6];
+#pragma unroll
+            for (int m = 0; m < 6; m++) v[m] = P[l][e6[m]];
+            // the potential is low-order and smooth (binary/state_ztwd.cpp:50-56)
+            x64::ppm_face(v, ql[l], qr[l], l == POT, l == POT);
+          }
+        } else {
+          int e4[4];
+#pragma unroll
+          for (int m = 0; m < 4; m++) e4[m] = pencil_region<G>(d, f - 2 + m, ta, tb);
+#pragma unroll
+          for (int l = 0; l < NF; l++) {
+            double v[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) v[m] = P[l][e4[m]];
+            x64::minmod_face(v, ql[l], qr[l], c.theta);
+          }
+        }
+        // face position: cell centres across, the face itself along d
+        double X[3];
+        {
+          const int ii = d == 0 ? f : ta;
+          const int jj = d == 0 ? ta : (d == 1 ? f : tb);
+          const int kk = d == 2 ? f : tb;
+          cell_pos(x0, h, ii, jj, kk, X);
+          X[d] = x0[d] + (double)f * h;
+        }
+        to_con(c, ql, X);
+        to_con(c, qr, X);
+        if constexpr (MODE == 0) {
+          double uf[NF];
+#pragma unroll
+          for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5;
+          Vf[q] = d == 0 ? vx(c, uf, X) : (d == 1 ? vy(c, uf, X) : vz(uf));
+          double fl[NF], fr[NF];
+          const double el = flux<EOS, d>(H, c, ql, X, fl);
+          const double er = flux<EOS, d>(H, c, qr, X, fr);
+          const double amax = rmax(el, er);
+#pragma unroll
+          for (int l = 0; l < NF; l++) Fb[l][q] = ((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5;
+        } else {
+          double F[NF];
+          const double el = flux<EOS, d>(H, c, ql, X, F);
+          const double er = flux<EOS, d>(H, c, qr, X, F);
+          emax = rmax(emax, rmax(el, er));
+        }
+      }
+      __syncthreads();
+      if constexpr (MODE == 0) {
+        const int p = d == 0 ? ci : (d == 1 ? cj : ck);
+        const int ta = d == 0 ? cj : ci;
+        const int tb = d == 2 ? cj : ck;
+        const int q0 = p + (N + 1) * (ta + N * tb);
+#pragma unroll
+        for (int l = 0; l < NF; l++) D[l] -= (Fb[l][q0 + 1] - Fb[l][q0]) * hinv;
+        vlo[d] = Vf[q0];
+        vhi[d] = Vf[q0 + 1];
+        __syncthreads();  // P, Fb and Vf are restaged for the next direction
+      }
+    };
+    sweep(std::integral_constant<int, 0>{});
+    sweep(std::integral_constant<int, 1>{});
+    sweep(std::integral_constant<int, 2>{});
+
+    if constexpr (MODE == 0) {
+      const long e = (long)tid;
+      double s[NF];
+#pragma unroll
+      for (int l = 0; l < NF; l++) s[l] = Ub[(long)l * R + e];
+      double X[3];
+      cell_pos(x0, h, ci, cj, ck, X);
+      // face-averaged force of the potential (poisson/poisson.cpp:223-247, binary/binary.cpp:225-227)
+      constexpr int PP = N + 2;
+      const double* ph = a.phi + (long)b * PP * PP * PP;
+      auto phi_at = [&](int i, int j, int k) { return ph[((k + 1) * PP + (j + 1)) * PP + (i + 1)]; };
+      const double pc = phi_at(ci, cj, ck);
+      double gf[3];
+      {
+        const double fxl = -(pc - phi_at(ci - 1, cj, ck)) / h;
+        const double fxh = -(phi_at(ci + 1, cj, ck) - pc) / h;
+        const double fyl = -(pc - phi_at(ci, cj - 1, ck)) / h;
+        const double fyh = -(phi_at(ci, cj + 1, ck) - pc) / h;
+        const double fzl = -(pc - phi_at(ci, cj, ck - 1)) / h;
+        const double fzh = -(phi_at(ci, cj, ck + 1) - pc) / h;
+        gf[0] = 0.5 * (fxl + fxh);
+        gf[1] = 0.5 * (fyl + fyh);
+        gf[2] = 0.5 * (fzl + fzh);
+      }
+      // gravity_source (binary/state_ztwd.cpp:277-288); the zero components add exactly as
+      // Vector += of a zero-initialised source
+      double sv[NF];
+#pragma unroll
+      for (int l = 0; l < NF; l++) sv[l] = 0.0;
+      const double dd = s[D0];
+      sv[SR] += dd * (X[0] * gf[0] + X[1] * gf[1]) / Rc(X);
+      sv[LZ] += dd * (X[0] * gf[1] - X[1] * gf[0]);
+      sv[rot::SZ] += dd * gf[2];
+      sv[rot::ET] += D[POT];
+      sv[rot::ET] -= D[D0] * s[POT] / dd;
+      sv[rot::ET] += s[D0] * (dvx(c, X) * gf[0] + dvy(c, X) * gf[1]);
+#pragma unroll
+      for (int l = 0; l < NF; l++) D[l] += sv[l];
+      // internal_energy_source with the divergence of the saved face velocities
+      double du = 0.0;
+      du += vhi[0];
+      du += vhi[1];
+      du += vhi[2];
+      du -= vlo[0];
+      du -= vlo[1];
+      du -= vlo[2];
+      du /= h;
+      double p, cs;
+      eos_p_cs<EOS>(H, c, s, p, cs);
+#pragma unroll
+      for (int l = 0; l < NF; l++) sv[l] = 0.0;
+      sv[rot::TAU] = -p * du;
+#pragma unroll
+      for (int l = 0; l < NF; l++) D[l] += sv[l];
+      D[POT] = 0.0;
+      D[TT] = 0.0;
+      // State::source (binary/state_ztwd.cpp:347-358)
+#pragma unroll
+      for (int l = 0; l < NF; l++) sv[l] = 0.0;
+      {
+        const double Rr = Rc(X);
+        sv[SR] += (p + pow(s[LZ] / Rr, 2.0) / s[D0]) / Rr;
+        const double th = atan2(X[1], X[0]);
+        sv[SR] += s[D0] * c.fR * cos(th - c.ftheta);
+        sv[LZ] -= s[D0] * c.fR * sin(th - c.ftheta) * Rr;
+        sv[rot::ET] += s[SR] * c.fR * cos(th - c.ftheta);
+        sv[rot::ET] -= st(c, s, X) * c.fR * sin(th - c.ftheta) * Rr;
+      }
+#pragma unroll
+      for (int l = 0; l < NF; l++) D[l] += sv[l];
+      double u[NF];
+      if (has_u0) {
+        const double* U0b = a.U0 + (long)b * NF * R;
+#pragma unroll
+        for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + U0b[(long)l * R + e] * (1.0 - beta);
+      } else {
+#pragma unroll
+        for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + s[l] * (1.0 - beta);
+      }
+      floor_state<EOS>(H, c, u, X);
+      if (!(u[D0] > 0.0)) atomicOr(a.err, 1);
+      const double R2 = X[0] * X[0] + X[1] * X[1];
+      u[POT] = u[D0] * (pc - 0.5 * R2 * c.Omega0 * c.Omega0);
+      double* Uo = a.Uout + (long)b * NF * R;
+#pragma unroll
+      for (int l = 0; l < NF; l++) Uo[(long)l * R + e] = u[l];
+    } else {
+      for (int off = 32; off > 0; off >>= 1) emax = rmax(emax, __shfl_xor(emax, off, 64));
+      if ((tid & 63) == 0) red[tid >> 6] = emax;
+      __syncthreads();
+      if (tid == 0) {
+        double v = red[0];
+        for (int w = 1; w < N3 / 64; w++) v = rmax(v, red[w]);
+        if (a.block_eig) a.block_eig[b] = v;
+        if (a.gmax) atomicMax(a.gmax, (unsigned long long)__double_as_longlong(v / h));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// GridPhysBound for isolated blocks (grid/grid_phys_bound.cpp:27-54): every face ghost copies the
+// boundary cell next to it and applies enforce_outflow at the boundary face's position; the lower
+// z face mirrors (s_z -> -s_z) with z_reflect
+template <int G>
+__global__ __launch_bounds__(256) void rot_ghost_kernel(double* U, const double* x0s, const double* hs, int nblocks,
+                                                        int z_reflect, RotParams c) {
+  constexpr int S = G * N2, R = N3 + 6 * S;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)nblocks * 6 * S) return;
+  const int b = (int)(gid / (6 * S));
+  const int e = N3 + (int)(gid % (6 * S));
+  int i, j, k;
+  elem_ijk<G>(e, i, j, k);
+  const int nf = (e - N3) / S;
+  const int d = nf >> 1, up = nf & 1;
+  int si = i, sj = j, sk = k;  // source cell
+  const bool mirror = z_reflect && nf == 4;
+  int* pd = d == 0 ? &si : (d == 1 ? &sj : &sk);
+  const int pg = *pd;  // ghost coordinate along d
+  if (mirror)
+    *pd = -1 - pg;  // interior cell G-1-t for ghost t (padded 2G-1-t)
+  else
+    *pd = up ? N - 1 : 0;
+  double* Ub = U + (long)b * NF * R;
+  const long src = ((long)sk * N + sj) * N + si;
+  double u[NF];
+#pragma unroll
+  for (int l = 0; l < NF; l++) u[l] = Ub[(long)l * R + src];
+  if (mirror) {
+    u[rot::SZ] = -u[rot::SZ];
+  } else {
+    const double h = hs[b];
+    double X[3];
+    cell_pos(x0s + 3L * b, h, si, sj, sk, X);
+    X[d] = x0s[3L * b + d] + (double)(up ? N : 0) * h;  // the boundary face
+    enforce_outflow(c, u, X, nf);
+  }
+#pragma unroll
+  for (int l = 0; l < NF; l++) Ub[(long)l * R + e] = u[l];
+}
+
+template <int G, int RECON, int EOS, int MODE>
+hipError_t launch_one(const RotArgs& a, int grid, hipStream_t s) {
+  if (a.nblocks <= 0) return hipSuccess;
+  const int g = grid > 0 && grid < a.nblocks ? grid : a.nblocks;
+  kt_rot_kernel<G, RECON, EOS, MODE><<<g, N3, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t dispatch(int g, int recon, int eos, const RotArgs& a, int grid, hipStream_t s) {
+  if (g == 3 && recon == AMRH_RECON_PPM)
+    return eos == EOS_HELMHOLTZ ? launch_one<3, AMRH_RECON_PPM, EOS_HELMHOLTZ, MODE>(a, grid, s)
+                                : launch_one<3, AMRH_RECON_PPM, EOS_IDEAL, MODE>(a, grid, s);
+  if (g == 2 && recon == AMRH_RECON_MINMOD)
+    return eos == EOS_HELMHOLTZ ? launch_one<2, AMRH_RECON_MINMOD, EOS_HELMHOLTZ, MODE>(a, grid, s)
+                                : launch_one<2, AMRH_RECON_MINMOD, EOS_IDEAL, MODE>(a, grid, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_rot_substep(int g, int recon, int eos, const RotArgs& a, int grid, hipStream_t s) {
+  return dispatch<0>(g, recon, eos, a, grid, s);
+}
+
+hipError_t launch_rot_maxeig(int g, int recon, int eos, const RotArgs& a, int grid, hipStream_t s) {
+  return dispatch<1>(g, recon, eos, a, grid, s);
+}
+
+hipError_t launch_rot_ghosts(int g, double* U, const double* x0, const double* h, int nblocks, int z_reflect,
+                             const RotParams& c, hipStream_t s) {
+  const long total = (long)nblocks * 6 * g * N2;
+  if (total == 0) return hipSuccess;
+  const unsigned grid = (unsigned)((total + 255) / 256);
+  if (g == 3)
+    rot_ghost_kernel<3><<<grid, 256, 0, s>>>(U, x0, h, nblocks, z_reflect, c);
+  else if (g == 2)
+    rot_ghost_kernel<2><<<grid, 256, 0, s>>>(U, x0, h, nblocks, z_reflect, c);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace amrh

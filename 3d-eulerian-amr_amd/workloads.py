@@ -146,3 +146,116 @@ def write_helmholtz_table(path, rows):
     """The rows as a helmholtz.table text file (9 values per line, 17 significant digits, so
     %le parsing restores them exactly)."""
     np.savetxt(path, rows, fmt="%.17e")
+
+
+# ---- rotating binary (13-field state of binary/state_ztwd.cpp) -------------------------------
+BINARY_COUNTS = (8, 64, 400, 2000, 6000, 11528)  # 20,000 leaves of 8^3 over 6 levels
+ROT_ABAR = (4.0, 14.12, 17.52)                   # binary/state_ztwd.cpp:28
+# code time unit (s) of the Helmholtz runs: with cm = g = 1 it puts the stars' centres at ~1e7 K
+HELM_UNITS = (1.0, 1.0, 4.9e-9, 1.0)
+
+
+def lane_emden(n=1.5, dxi=1e-3):
+    """theta(xi) of the Lane-Emden equation of index n by RK4; returns (xi, theta, xi1, omega)
+    with xi1 the first zero and omega = -xi1^2 theta'(xi1)."""
+    def rhs(x, y):
+        th, dth = y
+        return np.array([dth, -max(th, 0.0) ** n - 2.0 * dth / x])
+    x, y = dxi, np.array([1.0 - dxi * dxi / 6.0, -dxi / 3.0])
+    xs, ts = [0.0, x], [1.0, y[0]]
+    while y[0] > 0.0:
+        k1 = rhs(x, y)
+        k2 = rhs(x + dxi / 2, y + dxi / 2 * k1)
+        k3 = rhs(x + dxi / 2, y + dxi / 2 * k2)
+        k4 = rhs(x + dxi, y + dxi * k3)
+        yn = y + dxi / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        if yn[0] <= 0.0:
+            f = y[0] / (y[0] - yn[0])
+            xi1 = x + f * dxi
+            return np.array(xs), np.array(ts), xi1, -xi1 * xi1 * (y[1] + f * (yn[1] - y[1]))
+        x, y = x + dxi, yn
+        xs.append(x)
+        ts.append(y[0])
+    raise RuntimeError("no surface")
+
+
+def binary_batch(counts=BINARY_COUNTS, n=8, g=3, eos="ideal", q=0.7, r_donor=0.2, seed=SEED):
+    """Two n = 1.5 polytropes (Lane-Emden, one K) with mass ratio q = M_donor/M_accretor, separation
+    2.5 R_donor, centre of mass at the origin, at rest in the frame rotating at the Keplerian
+    Omega (G = 1); a hot tenuous background.  Accretor C/O (species 1), donor and background He
+    (species 0).  Leaves as in multilevel_layout (nearest the origin finest).  The potential is the
+    two stars' Plummer-softened point-mass potential (softening R/2), given on (n+2)^3 per block.
+    T from p with the ideal EOS (p = rho T (1 + zbar)/abar) or the synthetic Helmholtz table in
+    HELM_UNITS.  Returns a dict with U (nb, 13, G, G, G), phi, x0 (nb, 3), h (nb), level, omega."""
+    lev, org = multilevel_layout(counts, seed)
+    nb = len(lev)
+    G = n + 2 * g
+    xi, th, xi1, om = lane_emden(1.5)
+    a = 2.5 * r_donor
+    r_acc = r_donor * q ** (1.0 / 3.0)
+    rc_d, rc_a = 1.0, 1.0 / (q * q)
+    m_d = 4.0 * np.pi * rc_d * r_donor ** 3 * om / xi1 ** 3
+    m_a = 4.0 * np.pi * rc_a * r_acc ** 3 * om / xi1 ** 3
+    K = 4.0 * np.pi * r_donor ** 2 * rc_d ** (1.0 - 1.0 / 1.5) / (2.5 * xi1 ** 2)
+    omega = np.sqrt((m_a + m_d) / a ** 3)
+    xa, xd = -a * m_d / (m_a + m_d), a * m_a / (m_a + m_d)
+    rho_bg, p_bg = 1e-4 * rc_d, 1e-3 * K * rc_d ** (5.0 / 3.0)
+
+    def star(r, R, rc):
+        x = np.clip(r / R, 0.0, 1.0) * xi1
+        t = np.interp(x, xi, th)
+        return np.where(r < R, rc * np.maximum(t, 0.0) ** 1.5, 0.0)
+
+    h = 2.0 / (n * 2.0 ** (lev + 1))
+    U = np.empty((nb, 13, G, G, G))
+    P = n + 2
+    phi = np.empty((nb, P, P, P))
+    for s in range(0, nb, 2048):
+        e = min(nb, s + 2048)
+        hh = h[s:e, None]
+
+        def grid(off):
+            x = org[s:e, 0:1] + off[None, :] * hh
+            y = org[s:e, 1:2] + off[None, :] * hh
+            z = org[s:e, 2:3] + off[None, :] * hh
+            return x[:, None, None, :], y[:, None, :, None], z[:, :, None, None]
+
+        X, Y, Z = grid(np.arange(G) - g + 0.5)
+        ra = np.sqrt((X - xa) ** 2 + Y ** 2 + Z ** 2)
+        rd = np.sqrt((X - xd) ** 2 + Y ** 2 + Z ** 2)
+        rho_a, rho_d = star(ra, r_acc, rc_a), star(rd, r_donor, rc_d)
+        rho = rho_a + rho_d + rho_bg
+        p = K * (rho_a + rho_d) ** (5.0 / 3.0) + p_bg
+        w_a = rho_a / rho
+        R2 = X * X + Y * Y
+        pot_phi = -m_a / np.sqrt(ra ** 2 + (0.5 * r_acc) ** 2) - m_d / np.sqrt(rd ** 2 + (0.5 * r_donor) ** 2)
+        abar = 1.0 / ((1.0 - w_a) / ROT_ABAR[0] + w_a / ROT_ABAR[1])
+        zbar = abar / 2.0
+        eps = 1.5 * p
+        if eos == "ideal":
+            T = p * abar / (rho * (1.0 + zbar))
+        else:
+            cm, gr, sec, _ = HELM_UNITS
+            p_cgs = p * (gr / (cm * sec * sec))
+            rho_cgs = rho * (gr / cm ** 3)
+            T = p_cgs / (HELM_AVO * HELM_KERG * rho_cgs * (1.0 / abar + zbar / abar))
+        u = U[s:e]
+        u[:, 0] = rho
+        u[:, 1] = rho * w_a
+        u[:, 2] = rho * (1.0 - w_a)
+        u[:, 3] = 0.0
+        u[:, 4] = rho * R2 * omega
+        u[:, 5] = 0.0
+        u[:, 6] = rho * (pot_phi - 0.5 * R2 * omega * omega)
+        u[:, 7] = eps
+        u[:, 8] = eps
+        u[:, 9] = T
+        u[:, 10] = rho * (1.0 - w_a)
+        u[:, 11] = rho * w_a
+        u[:, 12] = 0.0
+        Xp, Yp, Zp = grid(np.arange(P) - 1 + 0.5)
+        ra = np.sqrt((Xp - xa) ** 2 + Yp ** 2 + Zp ** 2)
+        rd = np.sqrt((Xp - xd) ** 2 + Yp ** 2 + Zp ** 2)
+        phi[s:e] = -m_a / np.sqrt(ra ** 2 + (0.5 * r_acc) ** 2) - m_d / np.sqrt(rd ** 2 + (0.5 * r_donor) ** 2)
+    x0 = np.ascontiguousarray(org)
+    return dict(U=U, phi=phi, x0=x0, h=h, level=lev, omega=omega)

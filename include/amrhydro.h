@@ -28,7 +28,15 @@ extern "C" {
 #define AMRH_ESTATE 3   /* call out of order (e.g. no tree, no upload) */
 #define AMRH_EDEVICE 4  /* device-side error flag raised (see amrh_error) */
 
-#define AMRH_PHYS_EULER 0 /* ideal gas, fields rho sx sy sz E tau (euler/state.cpp:9-17) */
+#define AMRH_PHYS_EULER 0    /* ideal gas, fields rho sx sy sz E tau (euler/state.cpp:9-17) */
+#define AMRH_PHYS_ROTATING 1 /* rotating-frame binary-star state (binary/state_ztwd.cpp:15-28): 13 fields
+                                rho, rho_acc, rho_don, s_R, l_z, s_z, pot, E, tau, T, 3 species;
+                                n = 8; a given gravitational potential (amrh_set_potential) and
+                                block positions (amrh_set_block_origin) */
+
+/* equation of state of AMRH_PHYS_ROTATING (amrh_config.eos) */
+#define AMRH_EOS_IDEAL 0      /* p = rho T (1 + zbar)/abar, e = p/((gamma-1) rho) */
+#define AMRH_EOS_HELMHOLTZ 1  /* helmholtz/helmeos.cpp on a table attached with amrh_attach_eos */
 
 #define AMRH_RECON_MINMOD 0 /* minmod-theta slopes (reconstruct.cpp:60-71); g = 2 */
 #define AMRH_RECON_PPM 1    /* piecewise parabolic, theta = 2 slopes + monotonisation
@@ -61,6 +69,13 @@ typedef struct {
   double cfl;        /* 0.4 (euler/euler_defs.h:46) */
   double grid_dim;   /* half edge of the root domain (grid/grid.cpp:70,512-526) */
   int boundary;      /* AMRH_BOUNDARY_* (flat batches; the octree path fills its own ghosts) */
+  /* AMRH_PHYS_ROTATING only */
+  int eos;           /* AMRH_EOS_* */
+  double omega;      /* frame angular velocity Binary::Omega (binary/binary.h:43) */
+  double omega0;     /* Binary::Omega0, the potential's reference frame (binary/binary.cpp:24) */
+  double f_r;        /* State::fR, centre-of-mass correction force (binary/binary_driver.cpp:194) */
+  double f_theta;    /* State::ftheta (binary/binary_driver.cpp:195) */
+  double abar[3];    /* species mass numbers (binary/state_ztwd.cpp:28) */
 } amrh_config;
 
 typedef struct amrh_ctx amrh_ctx;
@@ -170,6 +185,18 @@ int amrh_tree_max_dt(amrh_ctx* ctx, double* dt);
 /* The root's flow-off ledger FO (Grid::flow_off, grid/grid.cpp:346-348): nf doubles. */
 int amrh_tree_flow_off(amrh_ctx* ctx, double* fo);
 
+/* ---- rotating-frame state (AMRH_PHYS_ROTATING) ------------------------------------------------
+ * Block b's interior spans [x0, x0 + n h) on each axis (x0: nblocks * 3 doubles, host): the cell
+ * and face positions of Grid::X / Xfx (grid/grid.cpp:13-45, :500-526).  Synchronous. */
+int amrh_set_block_origin(amrh_ctx* ctx, const double* x0);
+/* The gravitational potential phi of Poisson::compute_forces (poisson/poisson.cpp:223-247): per
+ * block (n+2)^3 doubles, x fastest, cell (i, j, k) of the interior at ((k+1)(n+2) + j+1)(n+2) + i+1
+ * (one ghost layer).  It stays fixed across substeps: after every substep the pot field is reset to
+ * rho (phi - R^2 omega0^2 / 2), Binary::pot_to_grid (binary/binary.cpp:825-849).  Host array,
+ * synchronous; the _device form takes a device pointer in the same layout. */
+int amrh_set_potential(amrh_ctx* ctx, const double* phi);
+int amrh_set_potential_device(amrh_ctx* ctx, const void* phi_dev);
+
 /* ---- Helmholtz EOS (helmholtz/helmeos.cpp) -------------------------------------------------
  * The electron-positron free-energy table and the EOS built on it, on the device, FP64, in the
  * reference's operation order: pressure, energy, sound speed and heat capacity are bit-identical
@@ -198,6 +225,10 @@ int amrh_eos_compute_T(amrh_eos* eos, int n, void* st_dev, void* stream);
  * the table, 16 negative heat capacity, 32 negative squared sound speed, 64 Newton did not
  * converge in 100 iterations, 128 T fell below TMIN.  Synchronises the device; clears them. */
 int amrh_eos_error(amrh_eos* eos, int* flags);
+/* The EOS table used by a context with eos = AMRH_EOS_HELMHOLTZ (State::compute_eos,
+ * binary/state_ztwd.cpp:168-176); the table must outlive the context.  Its error flags are raised
+ * in the context's (amrh_error). */
+int amrh_attach_eos(amrh_ctx* ctx, amrh_eos* eos);
 
 #ifdef __cplusplus
 }

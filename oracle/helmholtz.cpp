@@ -222,6 +222,24 @@ int compute_T(St& q) {
 
 }  // namespace
 
+namespace orc {
+// one eos_t state (10 doubles) for the rotating-state restatement
+int helm_eos_state(double* r) {
+  St q{r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9]};
+  const int e = eos(q);
+  const double o[10] = {q.p, q.e, q.cs, q.cv, q.abar, q.zbar, q.rho, q.T, q.s, q.dsdt};
+  for (int c = 0; c < 10; c++) r[c] = o[c];
+  return e;
+}
+int helm_compute_T_state(double* r) {
+  St q{r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9]};
+  const int e = compute_T(q);
+  const double o[10] = {q.p, q.e, q.cs, q.cv, q.abar, q.zbar, q.rho, q.T, q.s, q.dsdt};
+  for (int c = 0; c < 10; c++) r[c] = o[c];
+  return e;
+}
+}  // namespace orc
+
 extern "C" {
 
 void orc_helm_table(const double* rows) {

@@ -11,6 +11,7 @@
 // Build: oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
 #include "oracle.h"
 #include "euler.hpp"
+#include "recon.hpp"
 
 #include <cstdlib>
 #include <cstring>
@@ -22,8 +23,6 @@ using namespace orc;
 
 namespace {
 
-constexpr int MAXG = 40;  // longest pencil supported
-constexpr int MAXF = 16;  // most fields supported
 
 struct Geo {
   int n, g, nf, G;  // G = GNX
@@ -50,64 +49,13 @@ inline long pencil_at(const Geo& o, int d, int a, int b, int p) {
   return o.at(a, b, p);
 }
 
-// reconstruct.cpp:79-90  (slopes on 1..len-2, faces 2..len-2)
 void recon_minmod(const Geo& o, int len, double q0[][MAXF], double ql[][MAXF], double qr[][MAXF]) {
-  static thread_local double slope[MAXG][MAXF];
-  for (int i = 1; i < len - 1; i++) {
-    for (int l = 0; l < o.nf; l++) {
-      const double up = q0[i + 1][l] - q0[i][l];
-      const double um = q0[i][l] - q0[i - 1][l];
-      slope[i][l] = minmod_theta(up, um, o.P.theta);
-    }
-  }
-  for (int i = 2; i < len - 1; i++) {
-    for (int l = 0; l < o.nf; l++) {
-      ql[i][l] = q0[i - 1][l] + slope[i - 1][l] / 2.0;
-      qr[i][l] = q0[i][l] - slope[i][l] / 2.0;
-    }
-  }
+  orc::recon_minmod(o.nf, o.P.theta, len, q0, ql, qr);
 }
 
-// reconstruct.cpp:22-58 as called at :94 -- the routine's "ql" argument receives the caller's qr
-// and vice versa.  Here A is the routine's ql (= caller qr) and B its qr (= caller ql).
-// low_order / smooth flags are all false for the ideal-gas State (euler/state.h:37-42).
 void recon_ppm(const Geo& o, int len, double q0[][MAXF], double qlc[][MAXF], double qrc[][MAXF],
                const unsigned low_order_mask, const unsigned smooth_mask) {
-  static thread_local double s[MAXG][MAXF];
-  double (*A)[MAXF] = qrc;
-  double (*B)[MAXF] = qlc;
-  for (int i = 1; i < len - 1; i++)
-    for (int l = 0; l < o.nf; l++) {
-      const double up = q0[i + 1][l] - q0[i][l];
-      const double um = q0[i][l] - q0[i - 1][l];
-      s[i][l] = minmod_theta(up, um, 2.0);
-    }
-  for (int i = 1; i < len - 2; i++) {
-    for (int l = 0; l < o.nf; l++) {
-      B[i][l] = (q0[i][l] + q0[i + 1][l]) * 0.5;
-      if (!((low_order_mask >> l) & 1u)) B[i][l] += (s[i][l] - s[i + 1][l]) * (1.0 / 6.0);
-    }
-    // A[1] = B[0] reads a value the reference never initialises; it is never used for a face
-    // the caller keeps, so it is left as is here.
-    if (i > 1)
-      for (int l = 0; l < o.nf; l++) A[i][l] = B[i - 1][l];
-  }
-  for (int i = 2; i < len - 2; i++) {
-    for (int l = 0; l < o.nf; l++) {
-      if ((smooth_mask >> l) & 1u) continue;
-      const double tmp1 = B[i][l] - A[i][l];
-      const double tmp2 = B[i][l] + A[i][l];
-      if ((B[i][l] - q0[i][l]) * (q0[i][l] - A[i][l]) <= 0.0) {
-        B[i][l] = A[i][l] = q0[i][l];
-      } else if (tmp1 * (q0[i][l] - 0.5 * tmp2) > (1.0 / 6.0) * tmp1 * tmp1) {
-        A[i][l] = 3.0 * q0[i][l] - 2.0 * B[i][l];
-      } else if (-(1.0 / 6.0) * tmp1 * tmp1 > tmp1 * (q0[i][l] - 0.5 * tmp2)) {
-        B[i][l] = 3.0 * q0[i][l] - 2.0 * A[i][l];
-      }
-    }
-  }
-  for (int i = len - 3; i > 2; i--)
-    for (int l = 0; l < o.nf; l++) B[i][l] = B[i - 1][l];
+  orc::recon_ppm(o.nf, len, q0, qlc, qrc, low_order_mask, smooth_mask);
 }
 
 void reconstruct(const Geo& o, int len, double q0[][MAXF], double ql[][MAXF], double qr[][MAXF]) {

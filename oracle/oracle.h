@@ -80,6 +80,28 @@ void orc_helm_set_units(double cm, double g, double s, double K);
 int orc_helm_eos(int n, double* st, int* flags);
 int orc_helm_compute_T(int n, double* st, int* flags);
 
+/* ---- rotating-frame binary-star state (binary/state_ztwd.cpp), rotating.cpp ----------------
+ * 13 fields: rho, rho_acc, rho_don, s_R, l_z, s_z, pot, E, tau, T, 3 species.  Blocks are padded
+ * SoA (nb, 13, G, G, G), G = n + 2g; phi (nb, n+2, n+2, n+2) is the given gravitational potential
+ * with one ghost layer; x0 (nb, 3) the low corner of each block's interior, h (nb) its cell width. */
+typedef struct {
+  int n, g, recon, z_reflect;
+  int eos;            /* 0 ideal gas p = rho T (1 + zbar)/abar, 1 Helmholtz (orc_helm_table first) */
+  double theta;       /* minmod theta (PPM uses 2) */
+  double rho_floor;   /* binary/state_ztwd.cpp:31 */
+  double Omega, Omega0, fR, ftheta;  /* Binary::Omega, Omega0; State::fR, ftheta */
+  double gamma;       /* ideal-gas EOS only */
+  double cfl;
+  double abar[3];     /* species mass numbers (binary/state_ztwd.cpp:28) */
+} orc_rot_cfg;
+
+int orc_rot_batch_substep(const orc_rot_cfg* c, int nb, double* U, const double* U0, const double* phi,
+                          const double* x0, const double* h, double dt, double beta, int* flags);
+int orc_rot_batch_max_eig(const orc_rot_cfg* c, int nb, const double* U, const double* x0, const double* h,
+                          double* eig);
+int orc_rot_batch_outflow_ghosts(const orc_rot_cfg* c, int nb, double* U, const double* x0, const double* h);
+int orc_rot_state(const orc_rot_cfg* c, int op, double* u, const double* X, double* F);
+
 #ifdef __cplusplus
 }
 #endif

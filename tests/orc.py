@@ -252,3 +252,67 @@ def ref_helm_lib(table_dir):
     L.ref_helm_compute_T.argtypes = [C.c_int, DP]
     L.ref_helm_compute_T.restype = None
     return L
+
+
+# ---- rotating-frame state --------------------------------------------------------------------
+
+class OrcRotCfg(C.Structure):
+    _fields_ = [("n", C.c_int), ("g", C.c_int), ("recon", C.c_int), ("z_reflect", C.c_int),
+                ("eos", C.c_int), ("theta", C.c_double), ("rho_floor", C.c_double),
+                ("Omega", C.c_double), ("Omega0", C.c_double), ("fR", C.c_double),
+                ("ftheta", C.c_double), ("gamma", C.c_double), ("cfl", C.c_double),
+                ("abar", C.c_double * 3)]
+
+
+def rot_cfg(eos=0, recon=1, omega=0.0, omega0=None, f_r=0.0, f_theta=0.0, z_reflect=0):
+    """The constants of binary/state_ztwd.cpp:28-31 and binary/binary_defs.h:29-30."""
+    c = OrcRotCfg(n=8, g=3 if recon == 1 else 2, recon=recon, z_reflect=z_reflect, eos=eos, theta=1.3,
+                  rho_floor=1.0e-10, Omega=omega, Omega0=omega if omega0 is None else omega0, fR=f_r,
+                  ftheta=f_theta, gamma=5.0 / 3.0, cfl=0.3)
+    c.abar[0], c.abar[1], c.abar[2] = 4.0, 14.12, 17.52
+    return c
+
+
+def _rot_lib():
+    L = lib()
+    if not getattr(L, "_rot_ready", False):
+        cp = C.POINTER(OrcRotCfg)
+        L.orc_rot_batch_substep.argtypes = [cp, C.c_int, DP, DP, DP, DP, DP, C.c_double, C.c_double, IP]
+        L.orc_rot_batch_max_eig.argtypes = [cp, C.c_int, DP, DP, DP, DP]
+        L.orc_rot_batch_outflow_ghosts.argtypes = [cp, C.c_int, DP, DP, DP]
+        L.orc_rot_state.argtypes = [cp, C.c_int, DP, DP, DP]
+        L._rot_ready = True
+    return L
+
+
+def rot_substep(cfg, U, U0, phi, x0, h, dt, beta):
+    nb = U.shape[0]
+    h = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (nb,)))
+    flags = np.zeros(nb, dtype=np.int32)
+    assert _rot_lib().orc_rot_batch_substep(C.byref(cfg), nb, dptr(U), dptr(U0), dptr(phi), dptr(x0), dptr(h),
+                                            dt, beta, flags.ctypes.data_as(IP)) == 0
+    return flags
+
+
+def rot_max_eig(cfg, U, x0, h):
+    nb = U.shape[0]
+    h = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (nb,)))
+    eig = np.empty(nb)
+    _rot_lib().orc_rot_batch_max_eig(C.byref(cfg), nb, dptr(U), dptr(x0), dptr(h), dptr(eig))
+    return eig
+
+
+def rot_outflow_ghosts(cfg, U, x0, h):
+    nb = U.shape[0]
+    h = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (nb,)))
+    _rot_lib().orc_rot_batch_outflow_ghosts(C.byref(cfg), nb, dptr(U), dptr(x0), dptr(h))
+
+
+def rot_state(cfg, op, u, X, F=None):
+    """op 0 to_prim, 1 to_con, 2-4 flux in x/y/z (returns F with the eigenvalue in F[13]), 5 floor,
+    6 outflow on node face F[0].  u is updated in place."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    F = np.zeros(14) if F is None else np.ascontiguousarray(F, dtype=np.float64)
+    err = _rot_lib().orc_rot_state(C.byref(cfg), op, dptr(u), dptr(X), dptr(F))
+    return u, F, err
