@@ -40,13 +40,19 @@ def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
     # boundary cells, so no separate ghost pass runs between substeps
     cfg = amrhydro.euler_config(n=n, precision=precision, boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW)
     B = amrhydro.Batch(cfg, nb, 1.0 / n)
-    stream = torch.cuda.current_stream()
-    B.set_stream(stream.cuda_stream)
+    B.set_stream(torch.cuda.current_stream().cuda_stream)
     B.upload_device(U_dev_ptr)
-    dt = torch.zeros(1, dtype=torch.float64, device="cuda")
     B.fill_outflow_ghosts()
-    cells = nb * n ** 3
+    res = time_steps(B, torch, nb * n ** 3, steps, warmup)
+    B.close()
+    return res
 
+
+def time_steps(B, torch, cells, steps, warmup):
+    """RK3 steps on a prepared batch (state uploaded, ghosts filled, outflow boundary mode): the
+    device CFL dt, store, three substeps; HIP events around every substep launch."""
+    stream = torch.cuda.current_stream()
+    dt = torch.zeros(1, dtype=torch.float64, device="cuda")
     first = [True]
 
     def step(events=None):
@@ -91,8 +97,91 @@ def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
         "bytes_per_launch_avg": tot_bytes / len(k_ms),
         "device_error_flags": flags,
     }
-    B.close()
     return res
+
+
+def extra_configs(amrhydro, torch, steps, warmup):
+    """The other workloads of the hot path, one line each (FP64 and, where built, mixed): the
+    uneven multilevel leaf batch (59,518 leaves of 8^3), PPM on 8^3 blocks (g = 3), the rotating
+    binary (20,000 leaves, PPM) with the ideal-gas and the Helmholtz EOS, and the 73-node octree
+    of the Sedov blast (ghost exchange, reflux and restriction included, one captured graph per
+    step)."""
+    import workloads
+    out = {}
+    OUT = amrhydro.AMRH_BOUNDARY_OUTFLOW
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def flat(name, cfg, U, h, extra=None):
+        B = amrhydro.Batch(cfg, U.shape[0], h)
+        B.set_stream(stream)
+        if extra:
+            extra(B)
+        B.upload(U)
+        B.fill_outflow_ghosts()
+        cells = U.shape[0] * cfg.n ** 3
+        r = time_steps(B, torch, cells, steps, warmup)
+        B.close()
+        log("  %s: %.3e cell-updates/s, substep kernel %.3f ms, %.0f GB/s"
+            % (name, r["cell_updates_per_s"], r["substep_kernel_ms_avg"], r["achieved_GBs"]))
+        return r
+
+    log("multilevel batch (59,518 leaves of 8^3)")
+    U, h, lev = workloads.multilevel()
+    for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+        cfg = amrhydro.euler_config(n=8, precision=prec, boundary=OUT)
+        out["multilevel_8_%s" % tag] = flat("multilevel %s" % tag, cfg, U, h)
+    del U
+
+    log("PPM flat batch (8192 blocks of 8^3, g = 3)")
+    U = workloads.flat_batch(8192, n=8, g=3)
+    for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+        cfg = amrhydro.euler_config(n=8, g=3, precision=prec, boundary=OUT, recon=amrhydro.AMRH_RECON_PPM)
+        out["ppm_flat_8_%s" % tag] = flat("ppm %s" % tag, cfg, U, 1.0 / 8)
+    del U
+
+    eos_tab = None
+    for eos, tag in ((amrhydro.AMRH_EOS_IDEAL, "ideal"), (amrhydro.AMRH_EOS_HELMHOLTZ, "helmholtz")):
+        log("rotating binary, %s EOS (20,000 leaves of 8^3, PPM, 13 fields)" % tag)
+        Bn = workloads.binary_batch(eos="ideal" if eos == amrhydro.AMRH_EOS_IDEAL else "helm")
+        cfg = amrhydro.rotating_config(eos=eos, omega=Bn["omega"], boundary=OUT)
+        if eos == amrhydro.AMRH_EOS_HELMHOLTZ:
+            eos_tab = amrhydro.HelmholtzEos(workloads.helmholtz_table())
+            eos_tab.set_units(*workloads.HELM_UNITS)
+
+        def prep(B, Bn=Bn, eos=eos):
+            B.set_block_origin(Bn["x0"])
+            B.set_potential(Bn["phi"])
+            if eos == amrhydro.AMRH_EOS_HELMHOLTZ:
+                B.attach_eos(eos_tab)
+        out["rotating_%s_fp64" % tag] = flat("rotating %s fp64" % tag, cfg, Bn["U"], Bn["h"], prep)
+        del Bn
+    if eos_tab is not None:
+        eos_tab.close()
+
+    log("Sedov octree (73 nodes of 8^3, fully refined to level 2)")
+    U, parent, child = workloads.sedov_tree(maxlevel=2)
+    for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+        cfg = amrhydro.euler_config(n=8, grid_dim=0.5, precision=prec)
+        B = amrhydro.Batch(cfg, len(parent), 1.0)
+        B.set_stream(stream)
+        B.tree_set(parent, child)
+        B.upload(U)
+        B.tree_fill_ghosts()
+        B.tree_inject()
+        dt = torch.full((1,), 1e-6, dtype=torch.float64, device="cuda")
+        B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        B.run_steps(steps, dt.data_ptr(), rk=3, use_graph=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        leaves = 64 * 8 ** 3
+        out["sedov_tree_73_%s" % tag] = {"cell_updates_per_s": 3 * steps * leaves / wall,
+                                         "us_per_step": 1e6 * wall / steps,
+                                         "device_error_flags": B.error_flags()}
+        log("  sedov tree %s: %.1f us per RK3 step" % (tag, 1e6 * wall / steps))
+        B.close()
+    return out
 
 
 def accuracy(amrhydro, torch, U_dev_ptr, nb, n, steps):
@@ -193,6 +282,7 @@ def main():
     ap.add_argument("--n", type=int, default=16)
     ap.add_argument("--accuracy-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other workloads' lines")
     ap.add_argument("--only", choices=["mixed", "fp64", "both"], default="both")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (scale for rocprof byte counters)")
@@ -242,6 +332,12 @@ def main():
         acc = accuracy(amrhydro, torch, U_dev.data_ptr(), nb, n, args.accuracy_steps)
         log("  rel-L1 per field: %s" % acc["rel_l1_per_field"])
 
+    extra = None
+    if not args.no_extra:
+        del U_dev
+        torch.cuda.empty_cache()
+        extra = extra_configs(amrhydro, torch, max(3, args.steps // 4), 1)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (FP64 oracle)")
@@ -277,6 +373,7 @@ def main():
                      "kernel": "kt_sweep_kernel<%d,%s,0>" % (n, "true" if head_name == "mixed" else "false"),
                      "bytes_per_launch": head["bytes_per_launch_avg"]},
         "cpu_baseline": cpu,
+        "other_workloads": extra,
     }
     print(json.dumps(out), flush=True)
 
