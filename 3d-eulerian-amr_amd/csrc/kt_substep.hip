@@ -154,8 +154,13 @@ __device__ __forceinline__ void put_ghost(double* Uo, const Layout& L, int face,
 // Per plane k: stage the plane's primitives (with x/y ghosts) in LDS; x- and y-slopes of every
 // cell once; the x/y faces from two cells and their slopes; the z-face from the column window;
 // then each thread updates its cell.  Three barriers per plane.
+// FP64: two waves per SIMD.  The FP64 substep would otherwise take more than 256 registers and run
+// one wave per SIMD; a small spill at two waves is measured 20% faster.  The mixed form is left to
+// the compiler: it fits two waves without spilling, and three (168 registers) spill ~300 B per lane
+// and run slower.
 template <int N, bool MIXED, int MODE>
-__global__ __launch_bounds__(N* N) void kt_sweep_kernel(SubstepArgs a, MaxEigArgs m) {
+__global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 2))) void kt_sweep_kernel(
+    SubstepArgs a, MaxEigArgs m) {
   using T = typename Prec<MIXED>::T;
   constexpr int PW = N + 2 * GW;  // padded plane edge
   constexpr int NT = N * N;
