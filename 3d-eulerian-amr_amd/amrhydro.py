@@ -107,6 +107,9 @@ SIGNATURES = {
     "amrh_set_block_origin": (C.c_int, [C.c_void_p, DP]),
     "amrh_set_potential": (C.c_int, [C.c_void_p, DP]),
     "amrh_set_potential_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "amrh_state_sum": (C.c_int, [C.c_void_p, DP]),
+    "amrh_state_max": (C.c_int, [C.c_void_p, DP]),
+    "amrh_state_min": (C.c_int, [C.c_void_p, DP]),
 }
 
 _lib = None
@@ -246,6 +249,23 @@ class Batch:
 
     def run_steps(self, nsteps, dt_ptr, rk=3, use_graph=True):
         _ck(lib().amrh_run_steps(self.ctx, rk, nsteps, C.c_void_p(dt_ptr), 1 if use_graph else 0))
+
+    # ---- diagnostics -------------------------------------------------------------------------
+    def state_sum(self):
+        """(4, nf): sum U h^3 and its x, y, z moments over uncovered cells (GridNode::state_sum)."""
+        s = np.empty(4 * self.cfg.nf)
+        _ck(lib().amrh_state_sum(self.ctx, _dp(s)))
+        return s.reshape(4, self.cfg.nf)
+
+    def state_max(self):
+        s = np.empty(self.cfg.nf)
+        _ck(lib().amrh_state_max(self.ctx, _dp(s)))
+        return s
+
+    def state_min(self):
+        s = np.empty(self.cfg.nf)
+        _ck(lib().amrh_state_min(self.ctx, _dp(s)))
+        return s
 
     # ---- rotating-frame state ----------------------------------------------------------------
     def set_block_origin(self, x0):

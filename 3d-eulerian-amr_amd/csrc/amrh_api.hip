@@ -134,6 +134,7 @@ void amrh_destroy(amrh_ctx* c) {
   if (c->err_dev) (void)hipFree(c->err_dev);
   if (c->pad_stage) (void)hipFree(c->pad_stage);
   if (c->x0_dev) (void)hipFree(c->x0_dev);
+  if (c->diag_part) (void)hipFree(c->diag_part);
   if (c->phi_dev) (void)hipFree(c->phi_dev);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (c->tree) {
@@ -225,7 +226,8 @@ int amrh_fill_outflow_ghosts(amrh_ctx* c) {
 
 int amrh_set_block_origin(amrh_ctx* c, const double* x0) {
   if (!c || !x0) return fail(AMRH_EINVAL, "null argument");
-  if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "block origins belong to ROTATING physics");
+  if (c->tree) return fail(AMRH_EINVAL, "an octree's positions follow from amrh_tree_set");
+  if (!c->x0_dev) HIPCHK(hipMalloc(&c->x0_dev, sizeof(double) * 3 * c->nb));
   HIPCHK(hipMemcpyAsync(c->x0_dev, x0, sizeof(double) * 3 * c->nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->have_x0 = true;

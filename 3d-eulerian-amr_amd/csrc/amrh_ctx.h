@@ -35,6 +35,7 @@ struct TreeState {
   int2* pairs_dev = nullptr;
   double* ledger_dev = nullptr;
   double* fo_dev = nullptr;  // [FO | FO0]
+  unsigned char* covered_dev = nullptr;  // per node: bit c set when child c exists
   void free_dev();
 };
 
@@ -61,6 +62,7 @@ struct amrh_ctx {
   double* phi_dev = nullptr;   // [nb][(n+2)^3] potential
   const amrh_eos* eos = nullptr;
   bool have_x0 = false, have_phi = false;
+  double* diag_part = nullptr;  // state_sum/max/min partials (allocated on first use)
   int grid_substep = 0;  // persistent grid of the substep kernel (0: one workgroup per block)
   int grid_maxeig = 0;   // same for the CFL scan
   // captured RK steps, one graph per parity of the ping-pong buffers (amrh_graph.hip)

@@ -19,10 +19,11 @@ using namespace amrh;
 
 void TreeState::free_dev() {
   for (void* p : {(void*)ghost_dev, (void*)rel_dev, (void*)level_nodes_dev, (void*)slot_dev, (void*)ovr_dev,
-                  (void*)items_dev, (void*)pairs_dev, (void*)ledger_dev, (void*)fo_dev})
+                  (void*)items_dev, (void*)pairs_dev, (void*)ledger_dev, (void*)fo_dev, (void*)covered_dev})
     if (p) (void)hipFree(p);
   ghost_dev = rel_dev = level_nodes_dev = slot_dev = nullptr;
   ovr_dev = ledger_dev = fo_dev = nullptr;
+  covered_dev = nullptr;
   items_dev = nullptr;
   pairs_dev = nullptr;
 }
@@ -179,6 +180,11 @@ int build(amrh_ctx* c, const int* parent, const int* child) {
   if (e == hipSuccess) e = upload_vec(&t->slot_dev, slot);
   if (e == hipSuccess) e = upload_vec(&t->items_dev, items);
   if (e == hipSuccess) e = upload_vec(&t->pairs_dev, pairs);
+  std::vector<unsigned char> covered(t->nn, 0);
+  for (int b = 0; b < t->nn; b++)
+    for (int ci = 0; ci < 8; ci++)
+      if (t->child[b][ci] >= 0) covered[b] |= (unsigned char)(1u << ci);
+  if (e == hipSuccess) e = upload_vec(&t->covered_dev, covered);
   const long half = n / 2;
   if (e == hipSuccess) e = hipMalloc(&t->ovr_dev, sizeof(double) * std::max<long>(1, t->nslots) * half * half * 6);
   if (e == hipSuccess) e = hipMalloc(&t->ledger_dev, sizeof(double) * 3 * 2 * n * n * 6);
@@ -260,6 +266,20 @@ int amrh_tree_set(amrh_ctx* c, const int* parent, const int* child) {
   if (int rc = build(c, parent, child)) return rc;
   HIPCHK(hipMemcpy(c->h_dev, c->tree->h.data(), sizeof(double) * c->nb, hipMemcpyHostToDevice));
   c->have_h = true;
+  // low interior face of every node, Grid::xf(bw) (grid/grid.cpp:512-526; no -GRID_DIM in z when
+  // the lower z boundary reflects)
+  const TreeState* t = c->tree;
+  const int g = c->cfg.g;
+  const double h0 = t->h[0];
+  std::vector<double> x0(3 * (size_t)c->nb);
+  for (int b = 0; b < c->nb; b++)
+    for (int d = 0; d < 3; d++) {
+      const double shift = (d == 2 && c->cfg.z_reflect) ? 0.0 : c->cfg.grid_dim;
+      x0[3 * b + d] = double(t->off[b][d] + g) * t->h[b] - shift - g * h0;
+    }
+  if (!c->x0_dev) HIPCHK(hipMalloc(&c->x0_dev, sizeof(double) * 3 * c->nb));
+  HIPCHK(hipMemcpy(c->x0_dev, x0.data(), sizeof(double) * x0.size(), hipMemcpyHostToDevice));
+  c->have_x0 = true;
   return AMRH_OK;
 }
 

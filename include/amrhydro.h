@@ -185,9 +185,21 @@ int amrh_tree_max_dt(amrh_ctx* ctx, double* dt);
 /* The root's flow-off ledger FO (Grid::flow_off, grid/grid.cpp:346-348): nf doubles. */
 int amrh_tree_flow_off(amrh_ctx* ctx, double* fo);
 
-/* ---- rotating-frame state (AMRH_PHYS_ROTATING) ------------------------------------------------
+/* ---- diagnostics (device reductions) ---------------------------------------------------------
+ * GridNode::state_sum (grid/grid_node.cpp:1489-1527) over the uncovered interior cells of every
+ * block (a cell is covered when its octant has a child, :1355-1362): sums[q*nf + f] for q = 0..3
+ * is sum(U_f h^3), sum(U_f h^3 x), sum(U_f h^3 y), sum(U_f h^3 z) at cell centres.  A flat batch
+ * needs amrh_set_block_origin first; an octree's positions follow from amrh_tree_set.  The block
+ * sums are combined in block order, so the result does not depend on the launch.  Synchronous. */
+int amrh_state_sum(amrh_ctx* ctx, double* sums);
+/* GridNode::state_max / state_min (grid/grid_node.cpp:1529-1589): per field over the same cells. */
+int amrh_state_max(amrh_ctx* ctx, double* out);
+int amrh_state_min(amrh_ctx* ctx, double* out);
+
+/* ---- block positions, rotating-frame state (AMRH_PHYS_ROTATING) --------------------------------
  * Block b's interior spans [x0, x0 + n h) on each axis (x0: nblocks * 3 doubles, host): the cell
- * and face positions of Grid::X / Xfx (grid/grid.cpp:13-45, :500-526).  Synchronous. */
+ * and face positions of Grid::X / Xfx (grid/grid.cpp:13-45, :500-526), used by the rotating state
+ * and by amrh_state_sum.  Flat batches only.  Synchronous. */
 int amrh_set_block_origin(amrh_ctx* ctx, const double* x0);
 /* The gravitational potential phi of Poisson::compute_forces (poisson/poisson.cpp:223-247): per
  * block (n+2)^3 doubles, x fastest, cell (i, j, k) of the interior at ((k+1)(n+2) + j+1)(n+2) + i+1
