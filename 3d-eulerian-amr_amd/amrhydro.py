@@ -110,6 +110,11 @@ SIGNATURES = {
     "amrh_state_sum": (C.c_int, [C.c_void_p, DP]),
     "amrh_state_max": (C.c_int, [C.c_void_p, DP]),
     "amrh_state_min": (C.c_int, [C.c_void_p, DP]),
+    "amrh_tree_repack": (C.c_int, [C.c_void_p, C.c_void_p, IP]),
+    "amrh_tree_density_flags": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_ubyte)]),
+    "amrh_tree_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, IP]),
+    "amrh_tree_checkpoint_read": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(C.c_void_p),
+                                            IP, DP, DP]),
 }
 
 _lib = None
@@ -311,6 +316,38 @@ class Batch:
         fo = np.zeros(self.cfg.nf)
         _ck(lib().amrh_tree_flow_off(self.ctx, _dp(fo)))
         return fo
+
+    def tree_repack_from(self, src, src_index):
+        """Fill this context (its tree already set) from `src` after a regrid; src_index[b] is the
+        src node of node b or -1 for a node created by refinement."""
+        idx = np.ascontiguousarray(src_index, dtype=np.int32)
+        assert len(idx) == self.nb
+        _ck(lib().amrh_tree_repack(self.ctx, src.ctx, idx.ctypes.data_as(IP)))
+
+    def tree_density_flags(self, min_density):
+        f = np.zeros(self.nb, dtype=np.uint8)
+        _ck(lib().amrh_tree_density_flags(self.ctx, min_density, f.ctypes.data_as(C.POINTER(C.c_ubyte))))
+        return f
+
+    def tree_checkpoint_write(self, path, time=0.0, refine_alpha=1.0, ages=None):
+        a = None if ages is None else np.ascontiguousarray(ages, dtype=np.int32)
+        _ck(lib().amrh_tree_checkpoint_write(self.ctx, path.encode(), time, refine_alpha,
+                                             a.ctypes.data_as(IP) if a is not None else None))
+
+    @classmethod
+    def tree_checkpoint_read(cls, path, cfg):
+        """A new Batch holding the checkpoint's tree and state; returns (batch, time, refine_alpha)."""
+        ctx = C.c_void_p()
+        nb = C.c_int()
+        t, a = C.c_double(), C.c_double()
+        _ck(lib().amrh_tree_checkpoint_read(path.encode(), C.byref(cfg), C.byref(ctx), C.byref(nb), C.byref(t),
+                                            C.byref(a)))
+        B = cls.__new__(cls)
+        B.cfg = cfg
+        B.ctx = ctx
+        B.G = cfg.n + 2 * cfg.g
+        B.nb = nb.value
+        return B, t.value, a.value
 
     def tree_step(self, dt, rk=3):
         """HydroDriver::step on the octree (euler/hydro_driver.cpp:171-194, regrid excluded)."""

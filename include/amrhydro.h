@@ -185,6 +185,31 @@ int amrh_tree_max_dt(amrh_ctx* ctx, double* dt);
 /* The root's flow-off ledger FO (Grid::flow_off, grid/grid.cpp:346-348): nf doubles. */
 int amrh_tree_flow_off(amrh_ctx* ctx, double* fo);
 
+/* ---- regrid and checkpoints (octree) ------------------------------------------------------------
+ * After the host changes the tree (GridNode::check_for_refine, grid/grid_node.cpp:957-1033), a
+ * context for the new tree (amrh_create + amrh_tree_set) is filled from the old one: node b takes
+ * the record of src node src_index[b], or, with src_index[b] = -1 (a node created by refinement),
+ * is filled from its parent by GridNode::inject_from_parent (grid/grid_node.cpp:193-233: minmod
+ * slopes per axis, then the floor), level by level; the root's flow-off ledger carries over; ghosts
+ * are filled at the end.  Nodes absent from the new tree are dropped (their parents already hold
+ * the restricted values).  Synchronous. */
+int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index);
+/* The density test of GridNode::set_refine_flags (grid/grid_node.cpp:647-667): flags[b] bit c is
+ * set when an interior cell of node b's octant c has rho > min_density.  The flag propagation,
+ * proper nesting and mirroring rules (:631-956) stay with the host's tree.  Synchronous. */
+int amrh_tree_density_flags(amrh_ctx* ctx, double min_density, unsigned char* flags);
+/* GridNode::write (grid/grid_node.cpp:29-43, grid/grid.cpp:104-124) of the whole tree to `path`, in
+ * the reference's byte layout: depth first, per node h, h0, time, offsets and origins, the ledger
+ * FO (the root's; zero elsewhere), the padded state (x slowest, field fastest; edge and corner
+ * ghosts as zeros), child flags, refine_alpha and age (ages: one per node, or NULL for -1). */
+int amrh_tree_checkpoint_write(amrh_ctx* ctx, const char* path, double time, double refine_alpha,
+                               const int* ages);
+/* GridNode::read (grid/grid_node.cpp:45-57): a new context (cfg gives n, g, nf and the constants)
+ * holding the tree and state of the checkpoint; its node count, the root's time and refine_alpha
+ * are returned. */
+int amrh_tree_checkpoint_read(const char* path, const amrh_config* cfg, amrh_ctx** out, int* nblocks,
+                              double* time, double* refine_alpha);
+
 /* ---- diagnostics (device reductions) ---------------------------------------------------------
  * GridNode::state_sum (grid/grid_node.cpp:1489-1527) over the uncovered interior cells of every
  * block (a cell is covered when its octant has a child, :1355-1362): sums[q*nf + f] for q = 0..3

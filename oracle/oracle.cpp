@@ -444,6 +444,44 @@ void inject_child(orc_tree* t, int pi, int c) {
       }
 }
 
+// GridNode::inject_from_parent (grid/grid_node.cpp:193-233): a new child's interior from the
+// parent's cells (ghosts filled) with minmod slopes per axis, then the floor
+void prolong_child(orc_tree* t, int ni) {
+  const Geo& o = t->o;
+  Node& C = t->nodes[ni];
+  const Node& P = t->nodes[C.parent];
+  int c = 0;
+  while (P.child[c] != ni) c++;
+  const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+  for (int k = o.g; k <= o.G - o.g - 1; k += 2)
+    for (int j = o.g; j <= o.G - o.g - 1; j += 2) {
+      const int k0 = (o.g + k) / 2 + cz * (o.G / 2 - o.g);
+      const int j0 = (o.g + j) / 2 + cy * (o.G / 2 - o.g);
+      int i0 = o.g + cx * (o.G / 2 - o.g);
+      for (int i = o.g; i < o.G - o.g; i += 2, i0++) {
+        double v[8][MAXF];
+        for (int l = 0; l < o.nf; l++) {
+          const double u = t->Uat(P, l, i0, j0, k0);
+          const double s1 = minmod2(t->Uat(P, l, i0 + 1, j0, k0) - u, u - t->Uat(P, l, i0 - 1, j0, k0));
+          const double s2 = minmod2(t->Uat(P, l, i0, j0 + 1, k0) - u, u - t->Uat(P, l, i0, j0 - 1, k0));
+          const double s3 = minmod2(t->Uat(P, l, i0, j0, k0 + 1) - u, u - t->Uat(P, l, i0, j0, k0 - 1));
+          v[0][l] = u - (s1 + s2 + s3) * 0.25;
+          v[1][l] = u + (s1 - s2 - s3) * 0.25;
+          v[2][l] = u - (s1 - s2 + s3) * 0.25;
+          v[3][l] = u + (s1 + s2 - s3) * 0.25;
+          v[4][l] = u - (s1 + s2 - s3) * 0.25;
+          v[5][l] = u + (s1 - s2 + s3) * 0.25;
+          v[6][l] = u - (s1 - s2 - s3) * 0.25;
+          v[7][l] = u + (s1 + s2 + s3) * 0.25;
+        }
+        for (int q = 0; q < 8; q++) {
+          floor_state(o.P, v[q]);
+          for (int l = 0; l < o.nf; l++) t->Uat(C, l, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2)) = v[q][l];
+        }
+      }
+    }
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -549,6 +587,12 @@ int orc_batch_rk_steps(const orc_cfg* c, int nb, double* U, double* U0, const do
     }
   }
   orc_batch_outflow_ghosts(c, nb, U);
+  return 0;
+}
+
+int orc_tree_prolong(orc_tree* t, int node) {
+  if (node <= 0 || node >= (int)t->nodes.size()) return 1;
+  prolong_child(t, node);
   return 0;
 }
 

@@ -67,6 +67,9 @@ int orc_tree_substep(orc_tree* t, double dt, double beta);
 /* sum over uncovered cells of U*h^3 (grid/grid_node.cpp:1489-1529, a0 only) and the root ledger */
 int orc_tree_state_sum(const orc_tree* t, double* sum);
 int orc_tree_flow_off(const orc_tree* t, double* fo);
+/* GridNode::inject_from_parent (grid/grid_node.cpp:193-233) into node from its parent (whose
+ * ghosts must be filled) */
+int orc_tree_prolong(orc_tree* t, int node);
 /* face flux array of a node for the last computed direction (debug) */
 const double* orc_tree_node_F(const orc_tree* t, int node);
 

@@ -70,6 +70,7 @@ def lib():
         L.orc_tree_max_dt.restype = C.c_double
         L.orc_tree_substep.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_tree_state_sum.argtypes = [C.c_void_p, DP]
+        L.orc_tree_prolong.argtypes = [C.c_void_p, C.c_int]
         L.orc_tree_flow_off.argtypes = [C.c_void_p, DP]
         L.orc_helm_table.argtypes = [DP]
         L.orc_helm_table.restype = None
@@ -196,6 +197,9 @@ class Tree:
 
     def max_dt(self):
         return self.L.orc_tree_max_dt(self.t)
+
+    def prolong(self, node):
+        assert self.L.orc_tree_prolong(self.t, node) == 0
 
     def substep(self, dt, beta):
         self.L.orc_tree_substep(self.t, dt, beta)
