@@ -115,7 +115,25 @@ SIGNATURES = {
     "amrh_tree_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, IP]),
     "amrh_tree_checkpoint_read": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(C.c_void_p),
                                             IP, DP, DP]),
+    "amrh_poisson_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, IP, IP, C.c_void_p,
+                                      C.POINTER(C.c_void_p)]),
+    "amrh_poisson_destroy": (None, [C.c_void_p]),
+    "amrh_poisson_upload": (C.c_int, [C.c_void_p, C.c_int, DP]),
+    "amrh_poisson_download": (C.c_int, [C.c_void_p, C.c_int, DP]),
+    "amrh_poisson_phi_device": (C.c_void_p, [C.c_void_p]),
+    "amrh_poisson_source_from_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
+    "amrh_poisson_prepare": (C.c_int, [C.c_void_p]),
+    "amrh_poisson_boundary_tables": (C.c_int, [C.c_void_p, DP, C.c_int]),
+    "amrh_poisson_phi_from_children": (C.c_int, [C.c_void_p]),
+    "amrh_poisson_phi_boundaries": (C.c_int, [C.c_void_p]),
+    "amrh_poisson_forces": (C.c_int, [C.c_void_p, DP]),
+    "amrh_poisson_vcycle": (C.c_int, [C.c_void_p, DP]),
+    "amrh_poisson_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, IP, DP]),
+    "amrh_poisson_store_phi": (C.c_int, [C.c_void_p, C.c_int]),
 }
+
+# amrh_poisson fields (include/amrhydro.h AMRH_PSN_*)
+PSN_FIELDS = {"phi": 0, "phi0": 1, "dphi": 2, "dphi1": 3, "S": 4, "fx": 5, "fy": 6, "fz": 7}
 
 _lib = None
 
@@ -284,6 +302,10 @@ class Batch:
         assert phi.shape == (self.nb, P, P, P), phi.shape
         _ck(lib().amrh_set_potential(self.ctx, _dp(phi)))
 
+    def set_potential_device(self, ptr):
+        """phi already on the device, e.g. Poisson.phi_device()"""
+        _ck(lib().amrh_set_potential_device(self.ctx, C.c_void_p(ptr)))
+
     def attach_eos(self, eos):
         self._eos = eos  # the table must outlive the context
         _ck(lib().amrh_attach_eos(self.ctx, eos.h))
@@ -409,3 +431,83 @@ class HelmholtzEos:
         f = C.c_int()
         _ck(lib().amrh_eos_error(self.h, C.byref(f)))
         return f.value
+
+
+class Poisson:
+    """The multigrid Poisson solver on an octree (poisson/poisson.h:14-69) on the GPU.  Fields are
+    (nnodes, n+2, n+2, n+2) float64 arrays, index [node, k, j, i], one ghost layer."""
+
+    def __init__(self, n, parent, child, grid_dim=1.0, z_reflect=0, bw=2):
+        self.n, self.P = int(n), int(n) + 2
+        parent = np.ascontiguousarray(parent, np.int32)
+        child = np.ascontiguousarray(child, np.int32)
+        self.nn = len(parent)
+        self.h = C.c_void_p()
+        _ck(lib().amrh_poisson_create(self.n, int(bw), float(grid_dim), int(z_reflect), self.nn,
+                                      parent.ctypes.data_as(IP), child.ctypes.data_as(IP), None,
+                                      C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().amrh_poisson_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (self.nn, self.P, self.P, self.P), a.shape
+        _ck(lib().amrh_poisson_upload(self.h, PSN_FIELDS[name], _dp(a)))
+
+    def download(self, name):
+        a = np.empty((self.nn, self.P, self.P, self.P))
+        _ck(lib().amrh_poisson_download(self.h, PSN_FIELDS[name], _dp(a)))
+        return a
+
+    def phi_device(self):
+        return lib().amrh_poisson_phi_device(self.h)
+
+    def source_from(self, batch, G=1.0):
+        _ck(lib().amrh_poisson_source_from_state(self.h, batch.ctx, float(G)))
+
+    def prepare(self):
+        _ck(lib().amrh_poisson_prepare(self.h))
+
+    def tables(self, new=None):
+        P = self.P
+        if new is not None:
+            new = np.ascontiguousarray(new, dtype=np.float64)
+            assert new.shape == (self.nn, 6, P, P)
+            _ck(lib().amrh_poisson_boundary_tables(self.h, _dp(new), 1))
+            return new
+        t = np.empty((self.nn, 6, P, P))
+        _ck(lib().amrh_poisson_boundary_tables(self.h, _dp(t), 0))
+        return t
+
+    def phi_from_children(self):
+        _ck(lib().amrh_poisson_phi_from_children(self.h))
+
+    def phi_boundaries(self):
+        _ck(lib().amrh_poisson_phi_boundaries(self.h))
+
+    def forces(self):
+        e = C.c_double()
+        _ck(lib().amrh_poisson_forces(self.h, C.byref(e)))
+        return e.value
+
+    def vcycle(self):
+        e = C.c_double()
+        _ck(lib().amrh_poisson_vcycle(self.h, C.byref(e)))
+        return e.value
+
+    def solve(self, toler, max_iter=10000, prepare=True):
+        it, e = C.c_int(), C.c_double()
+        _ck(lib().amrh_poisson_solve(self.h, float(toler), int(max_iter), int(prepare), C.byref(it), C.byref(e)))
+        return it.value, e.value
+
+    def store_phi(self, mode):
+        _ck(lib().amrh_poisson_store_phi(self.h, int(mode)))

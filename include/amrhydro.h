@@ -267,6 +267,67 @@ int amrh_eos_error(amrh_eos* eos, int* flags);
  * in the context's (amrh_error). */
 int amrh_attach_eos(amrh_ctx* ctx, amrh_eos* eos);
 
+/* ---- Self-gravity: the multigrid Poisson solver on the octree (poisson/) --------------------
+ * Poisson : GridNode (poisson/poisson.h:14-69) over every node of a tree given as (parent, child
+ * index) arrays, parent before child, node 0 the root -- the same arrays as amrh_tree_set.  Per
+ * node the solver keeps (n+2)^3 cubes with one ghost layer, x fastest (PNX = GNX - 2 BW + 2,
+ * poisson/poisson.h:4): the fields below.  phi's layout is amrh_set_potential's, so
+ * amrh_set_potential_device(rotating_ctx, amrh_poisson_phi_device(p)) hands the solved potential
+ * to a rotating-state batch whose blocks are the tree's nodes. */
+typedef struct amrh_poisson amrh_poisson;
+enum {
+  AMRH_PSN_PHI = 0,   /* the potential */
+  AMRH_PSN_PHI0 = 1,  /* store_phi's copy */
+  AMRH_PSN_DPHI = 2,  /* the V-cycle's correction; after a cycle, the residual of solution_error */
+  AMRH_PSN_DPHI1 = 3, /* the V-cycle's right-hand side */
+  AMRH_PSN_S = 4,     /* the source 4 pi G rho (poisson/poisson.cpp:956-972) */
+  AMRH_PSN_FX = 5,    /* face forces of compute_forces: fx(i) on the face between i-1 and i */
+  AMRH_PSN_FY = 6,
+  AMRH_PSN_FZ = 7
+};
+/* Builds the tree as GridNode::create_child does (offsets with ghost width bw, grid/grid_node.cpp:
+ * 270-299) and the neighbour objects of each face: a node, a PoissonPhysBound table (reflecting on
+ * the low z face when z_reflect, poisson/poisson_phys_bound.cpp:8-60) or a PoissonAMRBound on the
+ * parent.  n = 8, 12 or 16.  stream: a hipStream_t, or NULL for a private one.  Fields start at 0. */
+int amrh_poisson_create(int n, int bw, double grid_dim, int z_reflect, int nnodes, const int* parent,
+                        const int* child, void* stream, amrh_poisson** out);
+void amrh_poisson_destroy(amrh_poisson* p);
+/* Host copies of one field for every node (nnodes * (n+2)^3 doubles).  Synchronous. */
+int amrh_poisson_upload(amrh_poisson* p, int field, const double* host);
+int amrh_poisson_download(amrh_poisson* p, int field, double* host);
+/* Device pointer to phi (nnodes * (n+2)^3 doubles), valid while p lives. */
+const void* amrh_poisson_phi_device(const amrh_poisson* p);
+/* Poisson::set_source (poisson/poisson.cpp:956-972) from a hydro batch whose blocks are the same
+ * nodes (same n): S = 4 pi G rho on the interior (State::poisson_source,
+ * binary/state_ztwd.cpp:219-222).  Synchronous. */
+int amrh_poisson_source_from_state(amrh_poisson* p, const amrh_ctx* hydro, double G);
+/* Binary::compute_source_sums and compute_physical_boundaries (binary/binary.cpp:938-1019): per
+ * node and octant the source's monopole, and the boundary phi of every physical face as the
+ * tree's sum of -alpha m / r (alpha = 2 with z reflection).  Synchronous. */
+int amrh_poisson_prepare(amrh_poisson* p);
+/* The boundary phi tables as [nnodes][6][(n+2)^2] (index [a][c] over the face's other two axes in
+ * x, y, z order); entries of faces that are not physical are ignored / returned as 0.
+ * write = 1 replaces the tables (a caller's own boundary condition).  Synchronous. */
+int amrh_poisson_boundary_tables(amrh_poisson* p, double* tab, int write);
+/* Poisson::phi_from_children (poisson/poisson.cpp:707-743).  Synchronous. */
+int amrh_poisson_phi_from_children(amrh_poisson* p);
+/* enforce_phi_boundaries for every level, root first (poisson/poisson.cpp:745-902). */
+int amrh_poisson_phi_boundaries(amrh_poisson* p);
+/* compute_forces with the coarse-fine averaging of adjust_{x,y,z}_force (poisson/poisson.cpp:
+ * 223-247, 474-635) and solution_error (:637-668) into *error (may be NULL).  Synchronous. */
+int amrh_poisson_forces(amrh_poisson* p, double* error);
+/* Poisson::vcycle (poisson/poisson.cpp:1069-1074) with the source already set: vcycle0, forces,
+ * and the error sum |div f - S| h^3 over unrefined cells.  Needs the boundary tables. */
+int amrh_poisson_vcycle(amrh_poisson* p, double* error);
+/* BinaryDriver::solve (binary/binary_driver.cpp:294-344): prepare (unless prepare = 0, keeping
+ * tables set by amrh_poisson_boundary_tables), forces, then V-cycles until the error is at most
+ * toler; AMRH_ESTATE after max_iter cycles (the reference aborts).  The V-cycles start from the
+ * phi the solver holds. */
+int amrh_poisson_solve(amrh_poisson* p, double toler, int max_iter, int prepare, int* iters, double* error);
+/* store_phi / restore_phi / restore_half_phi (poisson/poisson.cpp:144-221) around the RK stages of
+ * BinaryDriver::sub_step (binary/binary_driver.cpp:362-373): mode 0, 1, 2. */
+int amrh_poisson_store_phi(amrh_poisson* p, int mode);
+
 #ifdef __cplusplus
 }
 #endif

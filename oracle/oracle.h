@@ -105,6 +105,30 @@ int orc_rot_batch_max_eig(const orc_rot_cfg* c, int nb, const double* U, const d
 int orc_rot_batch_outflow_ghosts(const orc_rot_cfg* c, int nb, double* U, const double* x0, const double* h);
 int orc_rot_state(const orc_rot_cfg* c, int op, double* u, const double* X, double* F);
 
+/* ---- self-gravity: the multigrid Poisson solver on the octree (poisson/), poisson.cpp ------
+ * Nodes are given as (parent, child index) arrays, parent before child, node 0 the root.  Every
+ * per-node array is (n+2)^3 doubles, x fastest, one ghost layer (Poisson's PNX = GNX - 2 BW + 2).
+ * field: 0 phi, 1 S (source, 4 pi G rho), 2 fx, 3 fy, 4 fz, 5 dphi, 6 dphi1, 7 phi0. */
+typedef struct orc_poisson orc_poisson;
+orc_poisson* orc_poisson_create(int n, int bw, double grid_dim, int z_reflect, int nn, const int* parent,
+                                const int* cidx);
+void orc_poisson_destroy(orc_poisson* p);
+double* orc_poisson_field(orc_poisson* p, int node, int field);
+/* source sums and the physical-boundary phi tables (binary/binary.cpp:938-1019) */
+int orc_poisson_prepare(orc_poisson* p);
+double* orc_poisson_boundary_table(orc_poisson* p, int node, int face); /* after prepare; P x P */
+double orc_poisson_boundary_phi(const orc_poisson* p, double x, double y, double z);
+double orc_poisson_cell_centre(const orc_poisson* p, int node, int axis, int i);
+int orc_poisson_phi_from_children(orc_poisson* p);
+int orc_poisson_enforce_phi(orc_poisson* p);              /* every level, root first */
+int orc_poisson_enforce_phi_level(orc_poisson* p, int level);
+int orc_poisson_forces(orc_poisson* p);                   /* compute_forces with adjust_force */
+double orc_poisson_error(orc_poisson* p);                 /* solution_error */
+double orc_poisson_vcycle(orc_poisson* p);                /* vcycle0 + forces + error */
+/* BinaryDriver::solve: vcycles until the error is at most toler (1 if max_iter is exceeded) */
+int orc_poisson_solve(orc_poisson* p, double toler, int max_iter, int prepare, int* iters, double* error);
+int orc_poisson_store_phi(orc_poisson* p, int mode); /* 0 store, 1 restore, 2 restore_half */
+
 #ifdef __cplusplus
 }
 #endif

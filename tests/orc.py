@@ -320,3 +320,117 @@ def rot_state(cfg, op, u, X, F=None):
     F = np.zeros(14) if F is None else np.ascontiguousarray(F, dtype=np.float64)
     err = _rot_lib().orc_rot_state(C.byref(cfg), op, dptr(u), dptr(X), dptr(F))
     return u, F, err
+
+
+# ---- self-gravity: the multigrid Poisson solver (poisson/) --------------------------------------
+_poisson_ready = False
+
+
+def _poisson_lib():
+    global _poisson_ready
+    L = lib()
+    if not _poisson_ready:
+        vp = C.c_void_p
+        L.orc_poisson_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, IP, IP]
+        L.orc_poisson_create.restype = vp
+        L.orc_poisson_destroy.argtypes = [vp]
+        L.orc_poisson_destroy.restype = None
+        L.orc_poisson_field.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_poisson_field.restype = DP
+        L.orc_poisson_prepare.argtypes = [vp]
+        L.orc_poisson_boundary_table.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_poisson_boundary_table.restype = DP
+        L.orc_poisson_boundary_phi.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+        L.orc_poisson_boundary_phi.restype = C.c_double
+        L.orc_poisson_cell_centre.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.orc_poisson_cell_centre.restype = C.c_double
+        for f in ("phi_from_children", "enforce_phi", "forces"):
+            getattr(L, "orc_poisson_" + f).argtypes = [vp]
+        L.orc_poisson_enforce_phi_level.argtypes = [vp, C.c_int]
+        L.orc_poisson_error.argtypes = [vp]
+        L.orc_poisson_error.restype = C.c_double
+        L.orc_poisson_vcycle.argtypes = [vp]
+        L.orc_poisson_vcycle.restype = C.c_double
+        L.orc_poisson_solve.argtypes = [vp, C.c_double, C.c_int, C.c_int, IP, DP]
+        L.orc_poisson_store_phi.argtypes = [vp, C.c_int]
+        _poisson_ready = True
+    return L
+
+
+POISSON_FIELDS = {"phi": 0, "S": 1, "fx": 2, "fy": 3, "fz": 4, "dphi": 5, "dphi1": 6, "phi0": 7}
+
+
+class Poisson:
+    """The oracle's octree Poisson solver; arrays are (n+2)^3 per node, indexed [k, j, i]."""
+
+    def __init__(self, n, parent, child, grid_dim=1.0, z_reflect=0, bw=2):
+        self.L = _poisson_lib()
+        self.n, self.P = n, n + 2
+        self.parent = np.ascontiguousarray(parent, np.int32)
+        self.child = np.ascontiguousarray(child, np.int32)
+        self.nn = len(self.parent)
+        self.p = self.L.orc_poisson_create(n, bw, grid_dim, z_reflect, self.nn,
+                                           self.parent.ctypes.data_as(IP), self.child.ctypes.data_as(IP))
+        if not self.p:
+            raise ValueError("bad Poisson tree")
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.L.orc_poisson_destroy(self.p)
+            self.p = None
+
+    def field(self, node, name):
+        P = self.P
+        ptr = self.L.orc_poisson_field(self.p, node, POISSON_FIELDS[name])
+        return np.ctypeslib.as_array(ptr, shape=(P, P, P))
+
+    def fields(self, name):
+        return np.stack([self.field(b, name).copy() for b in range(self.nn)])
+
+    def set_fields(self, name, a):
+        for b in range(self.nn):
+            self.field(b, name)[:] = a[b]
+
+    def table(self, node, face):
+        ptr = self.L.orc_poisson_boundary_table(self.p, node, face)
+        if not ptr:
+            return None
+        return np.ctypeslib.as_array(ptr, shape=(self.P, self.P))
+
+    def centre(self, node, axis, i):
+        return self.L.orc_poisson_cell_centre(self.p, node, axis, i)
+
+    def prepare(self):
+        self.L.orc_poisson_prepare(self.p)
+
+    def boundary_phi(self, x, y, z):
+        return self.L.orc_poisson_boundary_phi(self.p, x, y, z)
+
+    def phi_from_children(self):
+        self.L.orc_poisson_phi_from_children(self.p)
+
+    def enforce_phi(self, level=None):
+        if level is None:
+            self.L.orc_poisson_enforce_phi(self.p)
+        else:
+            assert self.L.orc_poisson_enforce_phi_level(self.p, level) == 0
+
+    def forces(self):
+        self.L.orc_poisson_forces(self.p)
+
+    def error(self):
+        return self.L.orc_poisson_error(self.p)
+
+    def vcycle(self):
+        return self.L.orc_poisson_vcycle(self.p)
+
+    def solve(self, toler, max_iter=10000, prepare=True):
+        it = C.c_int(0)
+        err = C.c_double(0.0)
+        rc = self.L.orc_poisson_solve(self.p, toler, max_iter, int(prepare), C.byref(it), C.byref(err))
+        if rc:
+            raise RuntimeError("Poisson solve did not converge in %d vcycles" % max_iter)
+        return it.value, err.value
+
+    def store_phi(self, mode):
+        self.L.orc_poisson_store_phi(self.p, mode)
