@@ -55,6 +55,17 @@ struct PArgs {
   const int* allref;      // per node: every octant refined
   const int* pre;         // nodes in pre-order
   const int2* slots;      // boundary table -> (node, face)
+  double* exps;           // per node the six interior layers next to its faces, two copies
+  struct LoopState* st;   // the residual-ratio loop of vcycle0 on the device
+};
+
+// One "resid0 = vcycle_relax(l); do { enforce_dphi_boundaries(l); resid = vcycle_relax(l); }
+// while (resid / resid0 > limit)" loop of vcycle0 (poisson/poisson.cpp:66-84), kept on the device.
+struct LoopState {
+  double resid0, resid;
+  int iters, done;
+  unsigned counter;
+  int pad;
 };
 
 __device__ __forceinline__ long cell(int P, int i, int j, int k) { return ((long)k * P + j) * P + i; }
@@ -329,19 +340,50 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
-// vcycle_relax for the nodes of one level (poisson/poisson.cpp:88-130): Jacobi sweeps of
-// dphi += dphi1 + div(dphi)/6 over the interior until the sum of squared updates falls to half of
-// the first sweep's, at most 16 sweeps; ghosts stay as they are.  The last sum goes to part[b].
-template <int N>
-__global__ __launch_bounds__(NTHREADS) void relax_kernel(PArgs A) {
-  constexpr int P = N + 2, M = N, CELLS = M * M * M, PER = (CELLS + NTHREADS - 1) / NTHREADS;
-  __shared__ double L[P * P * P];
-  __shared__ double red[NTHREADS / 64];
+// interior layer next to face f of a node: [copy][node][face][(c - 1) M + (a - 1)], (a, c) the
+// face's other two axes in x, y, z order
+__device__ __forceinline__ long exp_at(const PArgs& A, int copy, int b, int f, int a, int c) {
+  const int M = A.P - 2;
+  return (((long)copy * A.nn + b) * 6 + f) * M * M + (long)(c - 1) * M + (a - 1);
+}
+
+__device__ __forceinline__ long layer_cell(int P, int f, int a, int c, int t) {
+  const int d = f >> 1;
+  return d == 0 ? cell(P, t, a, c) : d == 1 ? cell(P, a, t, c) : cell(P, a, c, t);
+}
+
+// copy 1 of the layers from dphi, before a loop's first step
+__global__ __launch_bounds__(NTHREADS) void export_kernel(PArgs A) {
   const int b = A.list[blockIdx.x];
+  const int P = A.P, M = P - 2;
+  const double* d = A.f[DPHI] + (long)b * A.P3;
+  for (int t = threadIdx.x; t < 6 * M * M; t += blockDim.x) {
+    const int f = t / (M * M), r = t % (M * M), a = r % M + 1, c = r / M + 1;
+    A.exps[exp_at(A, 1, b, f, a, c)] = d[layer_cell(P, f, a, c, (f & 1) ? M : 1)];
+  }
+}
+
+// Step k of the loop for the nodes of one level: enforce_dphi_boundaries (neighbours' layers from
+// step k-1's copy, the parent's dphi, the mirror, or 0) and vcycle_relax (poisson/poisson.cpp:
+// 88-130): Jacobi sweeps of dphi += dphi1 + div(dphi)/6 over the interior until the sum of squared
+// updates falls to half of the first sweep's, at most 16.  The last workgroup to finish sums the
+// level's residuals and decides whether the loop goes on; once it stops, later steps return at once.
+template <int N>
+__global__ __launch_bounds__(NTHREADS) void relax_step_kernel(PArgs A, int k, double limit, int count) {
+  constexpr int P = N + 2, M = N, CELLS = M * M * M, PER = (CELLS + NTHREADS - 1) / NTHREADS;
+  constexpr long P3 = (long)P * P * P;
+  // two copies of the cube where they fit: a sweep reads one and writes the other, so one barrier
+  // per sweep suffices; otherwise the update waits for every thread's stencil
+  constexpr bool TWO = 2 * P3 * 8 <= 65536;
+  __shared__ double L[TWO ? 2 * P3 : P3];
+  __shared__ double red[2][NTHREADS / 64];
+  __shared__ int last;
+  if (A.st->done) return;
+  const int b = A.list[blockIdx.x];
+  const PNodeDev& nd = A.nd[b];
   const long base = (long)b * A.P3;
-  const double* g = A.f[DPHI] + base;
-  for (int t = threadIdx.x; t < P * P * P; t += NTHREADS) L[t] = g[t];
-  double d1[PER], tmp[PER];
+  double* g = A.f[DPHI] + base;
+  double d1[PER];
   long at[PER];
 #pragma unroll
   for (int c = 0; c < PER; c++) {
@@ -349,38 +391,98 @@ __global__ __launch_bounds__(NTHREADS) void relax_kernel(PArgs A) {
     const int tt = t < CELLS ? t : 0;
     at[c] = cell(P, tt % M + 1, (tt / M) % M + 1, tt / (M * M) + 1);
     d1[c] = t < CELLS ? A.f[DPHI1][base + at[c]] : 0.0;
+    if (t < CELLS) L[at[c]] = g[at[c]];
+  }
+  const int src = (k + 1) & 1;
+  for (int t = threadIdx.x; t < 6 * M * M; t += NTHREADS) {
+    const int f = t / (M * M), r = t % (M * M), a = r % M + 1, c = r / M + 1;
+    const int kind = nd.fkind[f], up = f & 1;
+    const long e = layer_cell(P, f, a, c, up ? P - 1 : 0);
+    double v;
+    if (kind == F_SIB) {
+      v = A.exps[exp_at(A, src, nd.fnode[f], f ^ 1, a, c)];
+    } else if (kind == F_TAB) {
+      v = 0.0;
+    } else if (kind == F_REFL) {
+      v = g[layer_cell(P, f, a, c, up ? P - 2 : 1)];
+    } else {
+      const int d = f >> 1;
+      int idx[3];
+      idx[d] = up ? 1 : P - 2;  // the sibling-frame index the node asks for
+      idx[d == 0 ? 1 : 0] = a;
+      idx[d == 2 ? 1 : 2] = c;
+      int q[3];
+      amr_q(A, b, f, idx, q);
+      v = A.f[DPHI][(long)nd.parent * A.P3 + cell(P, q[0] / 2, q[1] / 2, q[2] / 2)];
+    }
+    L[e] = v;
+    if (TWO) L[P3 + e] = v;  // ghosts stay fixed through the sweeps
+    g[e] = v;
   }
   __syncthreads();
   double resid = 0.0, resid0 = 0.0;
-  int iters = 0;
+  int iters = 0, cur = 0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (;;) {
-#pragma unroll
-    for (int c = 0; c < PER; c++) {
-      const long e = at[c];
-      tmp[c] = d1[c] + ((L[e + 1] + L[e - 1] + L[e + P] + L[e - P] + L[e + P * P] + L[e - P * P]) - (L[e] * 6.0)) *
-                           (1.0 / 6.0);
-    }
-    __syncthreads();
+    const double* X = L + (TWO ? cur * P3 : 0);
+    double* Y = L + (TWO ? (cur ^ 1) * P3 : 0);
+    double tmp[PER];
     double part = 0.0;
 #pragma unroll
     for (int c = 0; c < PER; c++) {
+      const long e = at[c];
+      tmp[c] = d1[c] + ((X[e + 1] + X[e - 1] + X[e + P] + X[e - P] + X[e + P * P] + X[e - P * P]) - (X[e] * 6.0)) *
+                           (1.0 / 6.0);
+    }
+    if (!TWO) __syncthreads();
+#pragma unroll
+    for (int c = 0; c < PER; c++) {
       if (threadIdx.x + c * NTHREADS < CELLS) {
-        L[at[c]] += tmp[c];
+        Y[at[c]] = X[at[c]] + tmp[c];
         part += tmp[c] * tmp[c];
       }
     }
-    resid = block_sum(part, red);
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+    if (lane == 0) red[iters & 1][wv] = part;
+    __syncthreads();
+    resid = 0.0;
+    for (int q = 0; q < NTHREADS / 64; q++) resid += red[iters & 1][q];
+    cur ^= 1;
     if (iters == 0) resid0 = resid;
     iters++;
     if (!(resid / resid0 > RELAX_RESID && iters < 16)) break;
-    __syncthreads();  // red[] is rewritten by the next sum
   }
-  __syncthreads();
-  double* out = A.f[DPHI] + base;
+  const double* X = L + (TWO ? cur * P3 : 0);
 #pragma unroll
   for (int c = 0; c < PER; c++)
-    if (threadIdx.x + c * NTHREADS < CELLS) out[at[c]] = L[at[c]];
-  if (threadIdx.x == 0) A.part[b] = resid;
+    if (threadIdx.x + c * NTHREADS < CELLS) g[at[c]] = X[at[c]];
+  for (int t = threadIdx.x; t < 6 * M * M; t += NTHREADS) {
+    const int f = t / (M * M), r = t % (M * M), a = r % M + 1, c = r / M + 1;
+    A.exps[exp_at(A, k & 1, b, f, a, c)] = X[layer_cell(P, f, a, c, (f & 1) ? M : 1)];
+  }
+  if (threadIdx.x == 0) {
+    A.part[b] = resid;
+    __threadfence();
+    last = atomicAdd(&A.st->counter, 1u) == (unsigned)(count - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double s = 0.0;
+  for (int q = threadIdx.x; q < count; q += NTHREADS)
+    s += __hip_atomic_load(&A.part[A.list[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double total = block_sum(s, red[0]);
+  if (threadIdx.x == 0) {
+    LoopState* st = A.st;
+    if (st->iters == 0)
+      st->resid0 = total;
+    else if (!(total / st->resid0 > limit))
+      st->done = 1;
+    st->resid = total;
+    st->iters++;
+    st->counter = 0;
+    __threadfence();
+  }
 }
 
 // vcycle_coarse_correction for the nodes of one level (poisson/poisson.cpp:14-55)
@@ -656,6 +758,10 @@ struct amrh_poisson {
   double* src_dev = nullptr;
   int* allref_dev = nullptr;
   double* part_dev = nullptr;
+  double* exps_dev = nullptr;
+  LoopState* st_dev = nullptr;
+  LoopState* st_host = nullptr;  // pinned
+  std::vector<int> steps_last;   // steps the last loop of each (sweep, level) took
   std::vector<double> part;
   bool have_tables = false;
 
@@ -673,12 +779,15 @@ struct amrh_poisson {
     a.allref = allref_dev;
     a.pre = pre_dev;
     a.slots = slot_dev;
+    a.exps = exps_dev;
+    a.st = st_dev;
     return a;
   }
   ~amrh_poisson() {
     for (void* p : {(void*)nd_dev, (void*)list_dev, (void*)pre_dev, (void*)slot_dev, (void*)fields, (void*)tab_dev,
-                    (void*)src_dev, (void*)allref_dev, (void*)part_dev})
+                    (void*)src_dev, (void*)allref_dev, (void*)part_dev, (void*)exps_dev, (void*)st_dev})
       if (p) (void)hipFree(p);
+    if (st_host) (void)hipHostFree(st_host);
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 };
@@ -698,24 +807,16 @@ hipError_t launch_ghosts(amrh_poisson* p, int l, bool dphi) {
   return hipGetLastError();
 }
 
-hipError_t launch_relax(amrh_poisson* p, int l) {
+hipError_t launch_step(amrh_poisson* p, int l, int k, double limit) {
   const PArgs A = p->args(p->lv_start[l]);
   const int nb = p->lv_count[l];
   if (p->n == 8)
-    relax_kernel<8><<<nb, NTHREADS, 0, p->stream>>>(A);
+    relax_step_kernel<8><<<nb, NTHREADS, 0, p->stream>>>(A, k, limit, nb);
   else if (p->n == 12)
-    relax_kernel<12><<<nb, NTHREADS, 0, p->stream>>>(A);
+    relax_step_kernel<12><<<nb, NTHREADS, 0, p->stream>>>(A, k, limit, nb);
   else
-    relax_kernel<16><<<nb, NTHREADS, 0, p->stream>>>(A);
+    relax_step_kernel<16><<<nb, NTHREADS, 0, p->stream>>>(A, k, limit, nb);
   return hipGetLastError();
-}
-
-double level_sum(const amrh_poisson* p, int b, int l) {  // vcycle_relax's recursion
-  if (p->level[b] == l) return p->part[b];
-  double r = 0.0;
-  for (int c = 0; c < 8; c++)
-    if (p->child[b][c] >= 0) r += level_sum(p, p->child[b][c], l);
-  return r;
 }
 
 double error_sum(const amrh_poisson* p, int b) {  // solution_error's recursion
@@ -732,19 +833,33 @@ int fetch_part(amrh_poisson* p) {
   return AMRH_OK;
 }
 
-int relax(amrh_poisson* p, int l, double* resid) {
-  hipError_t e = launch_relax(p, l);
-  if (e != hipSuccess) return fail(AMRH_EHIP, "poisson relax: %s", hipGetErrorString(e));
-  if (int rc = fetch_part(p)) return rc;
-  *resid = level_sum(p, 0, l);
-  return AMRH_OK;
-}
-
 #define PCHK(expr)                                                                             \
   do {                                                                                         \
     hipError_t e_ = (expr);                                                                    \
     if (e_ != hipSuccess) return fail(AMRH_EHIP, "poisson: %s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
+
+// resid0 = relax; do { ghosts; resid = relax } while (resid / resid0 > limit): steps are enqueued
+// in chunks sized by the last loop of the same (sweep, level); steps after the decision return
+// at once, and the host reads the loop state once per chunk.
+int relax_loop(amrh_poisson* p, int l, int sweep, double limit) {
+  PCHK(hipMemsetAsync(p->st_dev, 0, sizeof(LoopState), p->stream));
+  export_kernel<<<p->lv_count[l], NTHREADS, 0, p->stream>>>(p->args(p->lv_start[l]));
+  PCHK(hipGetLastError());
+  int& last = p->steps_last[sweep * (p->maxlevel + 1) + l];
+  int k = 0;
+  for (;;) {
+    const int chunk = std::max(2, last + 1);
+    for (int q = 0; q < chunk; q++) PCHK(launch_step(p, l, k++, limit));
+    PCHK(hipMemcpyAsync(p->st_host, p->st_dev, sizeof(LoopState), hipMemcpyDeviceToHost, p->stream));
+    PCHK(hipStreamSynchronize(p->stream));
+    if (p->st_host->done) break;
+    if (k > 200000) return fail(AMRH_ESTATE, "poisson: relaxation at level %d does not settle", l);
+    last = 2 * last + 2;  // grow the next chunk
+  }
+  last = p->st_host->iters;
+  return AMRH_OK;
+}
 
 int phi_from_children(amrh_poisson* p) {
   for (int l = p->maxlevel - 1; l >= 0; l--) {
@@ -776,20 +891,11 @@ int vcycle0(amrh_poisson* p) {
   if (int rc = phi_from_children(p)) return rc;
   for (int l = 0; l <= p->maxlevel; l++) PCHK(launch_ghosts(p, l, false));
   PCHK(hipMemsetAsync(p->fields + (long)DPHI * p->nn * p->P3, 0, sizeof(double) * 2 * p->nn * p->P3, p->stream));
-  const long cap = 100000;  // a sweep loop the reference would never leave
   for (int l = p->maxlevel; l >= 0; l--) {
     PCHK(launch_ghosts(p, l, true));
     down_kernel<<<p->lv_count[l], NTHREADS, 0, p->stream>>>(p->args(p->lv_start[l]));
     PCHK(hipGetLastError());
-    PCHK(launch_ghosts(p, l, true));
-    double resid0, resid;
-    if (int rc = relax(p, l, &resid0)) return rc;
-    long it = 0;
-    do {
-      PCHK(launch_ghosts(p, l, true));
-      if (int rc = relax(p, l, &resid)) return rc;
-      if (++it > cap) return fail(AMRH_ESTATE, "poisson: down sweep at level %d does not settle", l);
-    } while (resid / resid0 > VDOWN_RESID);
+    if (int rc = relax_loop(p, l, 0, VDOWN_RESID)) return rc;
   }
   for (int l = 0; l <= p->maxlevel; l++) {
     PCHK(launch_ghosts(p, l, true));
@@ -797,15 +903,7 @@ int vcycle0(amrh_poisson* p) {
       coarse_kernel<<<p->lv_count[l], NTHREADS, 0, p->stream>>>(p->args(p->lv_start[l]));
       PCHK(hipGetLastError());
     }
-    PCHK(launch_ghosts(p, l, true));
-    double resid0, resid;
-    if (int rc = relax(p, l, &resid0)) return rc;
-    long it = 0;
-    do {
-      PCHK(launch_ghosts(p, l, true));
-      if (int rc = relax(p, l, &resid)) return rc;
-      if (++it > cap) return fail(AMRH_ESTATE, "poisson: up sweep at level %d does not settle", l);
-    } while (resid / resid0 > VUP_RESID);
+    if (int rc = relax_loop(p, l, 1, VUP_RESID)) return rc;
     interior_kernel<<<p->lv_count[l], NTHREADS, 0, p->stream>>>(p->args(p->lv_start[l]), 3);
     PCHK(hipGetLastError());
     PCHK(launch_ghosts(p, l, false));
@@ -936,6 +1034,10 @@ int amrh_poisson_create(int n, int bw, double grid_dim, int z_reflect, int nnode
   alloc((void**)&p->src_dev, sizeof(double) * 32 * nnodes);
   alloc((void**)&p->allref_dev, sizeof(int) * nnodes);
   alloc((void**)&p->part_dev, sizeof(double) * nnodes);
+  alloc((void**)&p->exps_dev, sizeof(double) * 2 * nnodes * 6 * n * n);
+  alloc((void**)&p->st_dev, sizeof(LoopState));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&p->st_host, sizeof(LoopState), hipHostMallocDefault);
+  p->steps_last.assign(2 * (p->maxlevel + 1), 2);
   if (e == hipSuccess) e = hipMemcpy(p->nd_dev, nd.data(), sizeof(PNodeDev) * nnodes, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(p->list_dev, p->list_host.data(), sizeof(int) * p->list_host.size(), hipMemcpyHostToDevice);

@@ -303,3 +303,88 @@ def sedov_tree(maxlevel=2, n=8, g=2, grid_dim=0.5):
         U[b, 4] = E
         U[b, 5] = E ** (1.0 / GAMMA)
     return U, np.array(parent, np.int32), np.array(child, np.int32)
+
+
+# ---- self-gravity: the binary's density on a properly nested octree ---------------------------
+def _binary_stars(q=0.7, r_donor=0.2):
+    """centres, radii and central densities of the two n = 1.5 polytropes of binary_batch"""
+    xi, th, xi1, om = lane_emden(1.5)
+    a = 2.5 * r_donor
+    r_acc = r_donor * q ** (1.0 / 3.0)
+    rc_d, rc_a = 1.0, 1.0 / (q * q)
+    m_d = 4.0 * np.pi * rc_d * r_donor ** 3 * om / xi1 ** 3
+    m_a = 4.0 * np.pi * rc_a * r_acc ** 3 * om / xi1 ** 3
+    xa, xd = -a * m_d / (m_a + m_d), a * m_a / (m_a + m_d)
+    return (xi, th, xi1), ((xa, r_acc, rc_a, m_a), (xd, r_donor, rc_d, m_d))
+
+
+def binary_tree(maxlevel=5, n=8, margin=1.25):
+    """A properly nested octree over [-1, 1]^3 refined to maxlevel where a node's box comes within
+    margin radii of either star of binary_batch, closed so that every refined node's 26 same-level
+    neighbours inside the domain exist (the nesting the reference's regrid enforces,
+    grid/grid_node.cpp:687-900).  Breadth first, children in index order.  Returns
+    (parent, child, level, loc)."""
+    _, stars = _binary_stars()
+    refine = [set() for _ in range(maxlevel)]
+    for l in range(maxlevel):
+        w = 2.0 / 2 ** l
+        m = 2 ** l
+        lo = -1.0 + np.arange(m) * w
+        for (xc, R, _, _) in stars:
+            r = margin * R
+            # box-sphere distance per axis
+            dx = np.maximum(np.maximum(lo - xc, xc - (lo + w)), 0.0)
+            dy = np.maximum(np.maximum(lo, -(lo + w)), 0.0)
+            d2 = dx[None, None, :] ** 2 + dy[None, :, None] ** 2 + dy[:, None, None] ** 2
+            for k, j, i in zip(*np.nonzero(d2 <= r * r)):
+                refine[l].add((int(i), int(j), int(k)))
+    changed = True
+    while changed:
+        changed = False
+        for l in range(1, maxlevel):
+            m = 2 ** l
+            for loc in list(refine[l]):
+                for dz in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dx in (-1, 0, 1):
+                            nb = (loc[0] + dx, loc[1] + dy, loc[2] + dz)
+                            if min(nb) < 0 or max(nb) >= m:
+                                continue
+                            pa = (nb[0] // 2, nb[1] // 2, nb[2] // 2)
+                            if pa not in refine[l - 1]:
+                                refine[l - 1].add(pa)
+                                changed = True
+    parent, child, level, loc = [-1], [0], [0], [(0, 0, 0)]
+    frontier = [0]
+    for l in range(maxlevel):
+        new = []
+        for p in frontier:
+            if loc[p] not in refine[l]:
+                continue
+            for c in range(8):
+                parent.append(p)
+                child.append(c)
+                level.append(l + 1)
+                loc.append(tuple(2 * loc[p][d] + ((c >> d) & 1) for d in range(3)))
+                new.append(len(parent) - 1)
+        frontier = new
+    return np.array(parent, np.int32), np.array(child, np.int32), np.array(level), loc
+
+
+def binary_source(level, loc, n=8, G=1.0):
+    """S = 4 pi G rho of binary_batch's two stars and background at the (n+2)^3 cell centres of
+    every node (one ghost layer): (nodes, n+2, n+2, n+2)."""
+    (xi, th, xi1), stars = _binary_stars()
+    P = n + 2
+    S = np.empty((len(level), P, P, P))
+    for b, (l, L) in enumerate(zip(level, loc)):
+        h = 2.0 / n / 2 ** l
+        x, y, z = (-1.0 + (L[d] * n + np.arange(P) - 0.5) * h for d in range(3))
+        Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+        rho = np.full(X.shape, 1e-4)
+        for (xc, R, rc, _) in stars:
+            r = np.sqrt((X - xc) ** 2 + Y ** 2 + Z ** 2)
+            t = np.interp(np.clip(r / R, 0.0, 1.0) * xi1, xi, th)
+            rho += np.where(r < R, rc * np.maximum(t, 0.0) ** 1.5, 0.0)
+        S[b] = 4.0 * np.pi * G * rho
+    return S

@@ -29,6 +29,9 @@ sys.path.insert(0, PKG)
 METRIC = "leaf cell-updates/s (KT RHS) mixed vs FP64 at ≤1e-6 rel L1; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 BETAS = (1.0, 0.25, 2.0 / 3.0)
+POISSON_LEVELS = 5  # 2,473 nodes of 8^3 over 6 levels, refined around the two stars
+POISSON_TREE = "binary octree, 2473 nodes of 8^3, 6 levels"
+POISSON_TOLER = 1e-7  # ELL_TOLER (binary/binary_defs.h:64)
 
 
 def log(msg):
@@ -181,6 +184,29 @@ def extra_configs(amrhydro, torch, steps, warmup):
                                          "device_error_flags": B.error_flags()}
         log("  sedov tree %s: %.1f us per RK3 step" % (tag, 1e6 * wall / steps))
         B.close()
+
+    log("self-gravity: multigrid Poisson on the binary's octree (%s)" % POISSON_TREE)
+    parent, child, lv, loc = workloads.binary_tree(POISSON_LEVELS)
+    D = amrhydro.Poisson(8, parent, child)
+    D.upload("S", workloads.binary_source(lv, loc))
+    t0 = time.perf_counter()
+    D.prepare()
+    t_prep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    it, err = D.solve(POISSON_TOLER, prepare=False)
+    t_solve = time.perf_counter() - t0
+    ncyc = max(3, steps)
+    t0 = time.perf_counter()
+    for _ in range(ncyc):
+        D.vcycle()
+    t_cyc = (time.perf_counter() - t0) / ncyc
+    out["poisson_binary_tree"] = {
+        "workload": POISSON_TREE, "nodes": len(parent), "levels": int(lv.max()) + 1,
+        "boundary_ms": 1e3 * t_prep, "solve_ms": 1e3 * t_solve, "solve_vcycles": it,
+        "solve_error": err, "toler": POISSON_TOLER, "vcycle_ms": 1e3 * t_cyc}
+    log("  boundary %.2f ms; solve from phi = 0: %d V-cycles in %.1f ms; %.2f ms per V-cycle"
+        % (1e3 * t_prep, it, 1e3 * t_solve, 1e3 * t_cyc))
+    D.close()
     return out
 
 
@@ -265,6 +291,27 @@ def cpu_baseline(n, target_s=10.0):
                       "OpenMP over blocks, %.1f s timed" % (nb, n, steps, sum(times))}
 
 
+def cpu_poisson_baseline():
+    """One V-cycle of the FP64 oracle (one thread) on the same octree and source as the GPU's
+    Poisson line, after its boundary phi."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    import workloads
+
+    parent, child, lv, loc = workloads.binary_tree(POISSON_LEVELS)
+    T = orc.Poisson(8, parent, child)
+    T.set_fields("S", workloads.binary_source(lv, loc))
+    t0 = time.perf_counter()
+    T.prepare()
+    t_prep = time.perf_counter() - t0
+    T.forces()
+    t0 = time.perf_counter()
+    T.vcycle()
+    t_cyc = time.perf_counter() - t0
+    return {"vcycle_s": t_cyc, "boundary_s": t_prep, "cores": 1, "kind": "port",
+            "sample": "one V-cycle of the FP64 oracle on the same tree and source"}
+
+
 def load_traffic(workload_key):
     p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload_key)
     if os.path.exists(p):
@@ -343,6 +390,11 @@ def main():
         log("CPU baseline (FP64 oracle)")
         cpu = cpu_baseline(n)
         log("  %.3e cell-updates/s on %d threads" % (cpu["value"], cpu["cores"]))
+        if extra is not None and "poisson_binary_tree" in extra:
+            log("CPU oracle: one Poisson V-cycle on the same octree")
+            pc = cpu_poisson_baseline()
+            extra["poisson_binary_tree"]["cpu_baseline"] = pc
+            log("  %.2f s per V-cycle (1 thread)" % pc["vcycle_s"])
 
     head = results.get("mixed") or results.get("fp64")
     head_name = "mixed" if "mixed" in results else "fp64"
