@@ -115,6 +115,9 @@ SIGNATURES = {
     "amrh_tree_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, IP]),
     "amrh_tree_checkpoint_read": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(C.c_void_p),
                                             IP, DP, DP]),
+    "amrh_tree_output_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "amrh_tree_output_mesh": (C.c_int, [C.c_void_p, DP, DP, DP, IP, DP]),
+    "amrh_tree_output_vtk": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     "amrh_poisson_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, IP, IP, C.c_void_p,
                                       C.POINTER(C.c_void_p)]),
     "amrh_poisson_destroy": (None, [C.c_void_p]),
@@ -370,6 +373,19 @@ class Batch:
         B.G = cfg.n + 2 * cfg.g
         B.nb = nb.value
         return B, t.value, a.value
+
+    def tree_output_mesh(self):
+        """GridNode::output's mesh: (x, y, z, nodelist (nzones, 8), values (nf, nzones))"""
+        npts, nz = C.c_long(), C.c_long()
+        _ck(lib().amrh_tree_output_counts(self.ctx, C.byref(npts), C.byref(nz)))
+        x, y, z = (np.empty(npts.value) for _ in range(3))
+        nl = np.empty((nz.value, 8), np.int32)
+        v = np.empty((self.cfg.nf, nz.value))
+        _ck(lib().amrh_tree_output_mesh(self.ctx, _dp(x), _dp(y), _dp(z), nl.ctypes.data_as(IP), _dp(v)))
+        return x, y, z, nl, v
+
+    def tree_output_vtk(self, path, time=0.0):
+        _ck(lib().amrh_tree_output_vtk(self.ctx, path.encode(), float(time)))
 
     def tree_step(self, dt, rk=3):
         """HydroDriver::step on the octree (euler/hydro_driver.cpp:171-194, regrid excluded)."""

@@ -8,6 +8,7 @@
 // on reading (the reference never reads them, grid/grid_node.cpp:59-112).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -39,9 +40,194 @@ struct Reader {
   }
 };
 
+// GridNode::output's zones of one node (grid/grid_node.cpp:1326-1346): interior cells in k, j, i
+// order, skipping those covered by a child; a zone's rank among its node's zones in closed form.
+template <int N, int G>
+__global__ __launch_bounds__(256) void zone_gather_kernel(const double* U, int nf, const int* nodes,
+                                                          const long* zone0, const unsigned char* covered,
+                                                          long nzones, double* out) {
+  constexpr long R = (long)N * N * N + 6L * G * N * N;
+  constexpr int H = N / 2;
+  const int b = nodes[blockIdx.x];
+  const unsigned cov = covered[b];
+  auto open = [&](int cx, int cy, int cz) { return !((cov >> (cx | (cy << 1) | (cz << 2))) & 1u); };
+  const double* Ub = U + (long)b * nf * R;
+  for (int e = threadIdx.x; e < N * N * N; e += blockDim.x) {
+    const int i = e % N, j = (e / N) % N, k = e / (N * N);
+    const int cx = i / H, cy = j / H, cz = k / H;
+    if (!open(cx, cy, cz)) continue;
+    long r = 0;
+    for (int z = 0; z < cz; z++)  // whole planes below
+      r += (long)H * H * H * ((int)open(0, 0, z) + open(1, 0, z) + open(0, 1, z) + open(1, 1, z));
+    r += (long)(k - cz * H) * H * H * ((int)open(0, 0, cz) + open(1, 0, cz) + open(0, 1, cz) + open(1, 1, cz));
+    for (int y = 0; y < cy; y++) r += (long)H * H * ((int)open(0, y, cz) + open(1, y, cz));
+    r += (long)(j - cy * H) * H * ((int)open(0, cy, cz) + open(1, cy, cz));
+    if (cx == 1 && open(0, cy, cz)) r += H;
+    r += i - cx * H;
+    const long z = zone0[blockIdx.x] + r;
+    for (int l = 0; l < nf; l++) out[(long)l * nzones + z] = Ub[l * R + e];
+  }
+}
+
+// the nodes in depth-first pre-order (GridNode::output's recursion, grid/grid_node.cpp:1348-1352)
+std::vector<int> preorder(const TreeState* t) {
+  std::vector<int> out, stack = {0};
+  while (!stack.empty()) {
+    const int b = stack.back();
+    stack.pop_back();
+    out.push_back(b);
+    for (int c = 7; c >= 0; c--)
+      if (t->child[b][c] >= 0) stack.push_back(t->child[b][c]);
+  }
+  return out;
+}
+
+int open_octants(const TreeState* t, int b) {
+  int o = 0;
+  for (int c = 0; c < 8; c++) o += t->child[b][c] < 0;
+  return o;
+}
+
+void put_be(FILE* fp, const void* v, int bytes, bool* ok) {  // legacy VTK binary is big-endian
+  unsigned char buf[8];
+  const unsigned char* p = (const unsigned char*)v;
+  for (int q = 0; q < bytes; q++) buf[q] = p[bytes - 1 - q];
+  *ok = *ok && fwrite(buf, 1, bytes, fp) == (size_t)bytes;
+}
+
 }  // namespace
 
 extern "C" {
+
+int amrh_tree_output_counts(amrh_ctx* c, long* npoints, long* nzones) {
+  if (!c || !npoints || !nzones) return fail(AMRH_EINVAL, "null argument");
+  if (!c->tree) return fail(AMRH_ESTATE, "output needs an octree: call amrh_tree_set");
+  const long G1 = c->cfg.n + 2 * c->cfg.g + 1, H = c->cfg.n / 2;
+  long z = 0;
+  for (int b = 0; b < c->nb; b++) z += open_octants(c->tree, b) * H * H * H;
+  *npoints = (long)c->nb * G1 * G1 * G1;  // GRID_NNODES per node (grid/grid.h:11)
+  *nzones = z;
+  return AMRH_OK;
+}
+
+int amrh_tree_output_mesh(amrh_ctx* c, double* x, double* y, double* z, int* nodelist, double* values) {
+  if (!c) return fail(AMRH_EINVAL, "null argument");
+  long npts, nz;
+  if (int rc = amrh_tree_output_counts(c, &npts, &nz)) return rc;
+  if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
+  if (c->cfg.g != 2 || (c->cfg.n != 8 && c->cfg.n != 16)) return fail(AMRH_EINVAL, "n = 8 or 16 with g = 2 only");
+  const TreeState* t = c->tree;
+  const int n = c->cfg.n, g = c->cfg.g, GNX = n + 2 * g, G1 = GNX + 1, H = n / 2;
+  const std::vector<int> order = preorder(t);
+  const double h0 = t->h[0], gd = c->cfg.grid_dim;
+  long pi = 0, ni = 0;
+  std::vector<long> zone0(order.size());
+  long zacc = 0;
+  for (size_t q = 0; q < order.size(); q++) {
+    const int b = order[q];
+    const double h = t->h[b];
+    zone0[q] = zacc;
+    zacc += open_octants(t, b) * (long)H * H * H;
+    // Grid::xf / yf / zf (grid/grid.cpp:510-526), origin 0
+    if (x && y && z) {
+      for (int k = 0; k < G1; k++)
+        for (int j = 0; j < G1; j++)
+          for (int i = 0; i < G1; i++) {
+            const long p = pi + ((long)k * G1 + j) * G1 + i;
+            x[p] = double(t->off[b][0] + i) * h - gd - double(g) * h0 - 0.0;
+            y[p] = double(t->off[b][1] + j) * h - gd - double(g) * h0 - 0.0;
+            z[p] = c->cfg.z_reflect ? double(t->off[b][2] + k) * h - double(g) * h0 - 0.0
+                                    : double(t->off[b][2] + k) * h - gd - double(g) * h0;
+          }
+    }
+    if (nodelist) {
+      for (int k = g; k < GNX - g; k++)
+        for (int j = g; j < GNX - g; j++)
+          for (int i = g; i < GNX - g; i++) {
+            const int cidx = ((2 * i) / GNX) | (((2 * j) / GNX) << 1) | (((2 * k) / GNX) << 2);
+            if (t->child[b][cidx] >= 0) continue;  // GridNode::zone_is_refined
+            auto at = [&](int ii, int jj, int kk) { return (int)(ii + G1 * (jj + G1 * kk) + pi); };
+            int* o = nodelist + 8 * ni;
+            o[0] = at(i, j, k), o[1] = at(i + 1, j, k), o[2] = at(i + 1, j + 1, k), o[3] = at(i, j + 1, k);
+            o[4] = at(i, j, k + 1), o[5] = at(i + 1, j, k + 1), o[6] = at(i + 1, j + 1, k + 1), o[7] = at(i, j + 1, k + 1);
+            ni++;
+          }
+    }
+    pi += (long)G1 * G1 * G1;
+  }
+  if (!values) return AMRH_OK;
+  int* nodes_dev = nullptr;
+  long* z0_dev = nullptr;
+  double* out_dev = nullptr;
+  hipError_t e = hipMalloc(&nodes_dev, sizeof(int) * order.size());
+  if (e == hipSuccess) e = hipMalloc(&z0_dev, sizeof(long) * order.size());
+  if (e == hipSuccess) e = hipMalloc(&out_dev, sizeof(double) * std::max<long>(1, nz * c->cfg.nf));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(nodes_dev, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(z0_dev, zone0.data(), sizeof(long) * order.size(), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    if (n == 8)
+      zone_gather_kernel<8, 2><<<(unsigned)order.size(), 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, nodes_dev,
+                                                                              z0_dev, t->covered_dev, nz, out_dev);
+    else
+      zone_gather_kernel<16, 2><<<(unsigned)order.size(), 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, nodes_dev,
+                                                                               z0_dev, t->covered_dev, nz, out_dev);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(values, out_dev, sizeof(double) * nz * c->cfg.nf, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)nodes_dev, (void*)z0_dev, (void*)out_dev})
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) return fail(AMRH_EHIP, "output gather: %s", hipGetErrorString(e));
+  return AMRH_OK;
+}
+
+int amrh_tree_output_vtk(amrh_ctx* c, const char* path, double time) {
+  if (!c || !path) return fail(AMRH_EINVAL, "null argument");
+  long npts, nz;
+  if (int rc = amrh_tree_output_counts(c, &npts, &nz)) return rc;
+  if (nz > 0x7fffffffL / 9 || npts > 0x7fffffffL) return fail(AMRH_EINVAL, "mesh too large for one file");
+  const int nf = c->cfg.nf;
+  std::vector<double> x(npts), y(npts), z(npts), v((size_t)nz * nf);
+  std::vector<int> nl((size_t)nz * 8);
+  if (int rc = amrh_tree_output_mesh(c, x.data(), y.data(), z.data(), nl.data(), v.data())) return rc;
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return fail(AMRH_EINVAL, "cannot open %s for writing", path);
+  bool ok = fprintf(fp, "# vtk DataFile Version 3.0\namrhydro time %.17g\nBINARY\nDATASET UNSTRUCTURED_GRID\n", time) > 0;
+  ok = ok && fprintf(fp, "POINTS %ld double\n", npts) > 0;
+  for (long p = 0; p < npts && ok; p++) {
+    put_be(fp, &x[p], 8, &ok);
+    put_be(fp, &y[p], 8, &ok);
+    put_be(fp, &z[p], 8, &ok);
+  }
+  ok = ok && fprintf(fp, "\nCELLS %ld %ld\n", nz, nz * 9) > 0;
+  const int eight = 8, hex = 12;  // VTK_HEXAHEDRON: the reference's corner order (grid_node.cpp:1333-1340)
+  for (long q = 0; q < nz && ok; q++) {
+    put_be(fp, &eight, 4, &ok);
+    for (int m = 0; m < 8; m++) put_be(fp, &nl[8 * q + m], 4, &ok);
+  }
+  ok = ok && fprintf(fp, "\nCELL_TYPES %ld\n", nz) > 0;
+  for (long q = 0; q < nz && ok; q++) put_be(fp, &hex, 4, &ok);
+  ok = ok && fprintf(fp, "\nCELL_DATA %ld\n", nz) > 0;
+  // State::field_name (euler/state.cpp:24-42); fields beyond the ideal gas's six are numbered
+  static const char* names[6] = {"d", "sx", "sy", "sz", "et", "tau"};
+  for (int l = 0; l < nf && ok; l++) {
+    char nm[16];
+    if (c->cfg.physics == AMRH_PHYS_EULER && l < 6)
+      snprintf(nm, sizeof nm, "%s", names[l]);
+    else
+      snprintf(nm, sizeof nm, "f%d", l);
+    ok = ok && fprintf(fp, "SCALARS %s double 1\nLOOKUP_TABLE default\n", nm) > 0;
+    for (long q = 0; q < nz && ok; q++) put_be(fp, &v[(size_t)l * nz + q], 8, &ok);
+    ok = ok && fprintf(fp, "\n") > 0;
+  }
+  ok = fclose(fp) == 0 && ok;
+  if (!ok) return fail(AMRH_EHIP, "writing %s failed", path);
+  return AMRH_OK;
+}
+
 
 int amrh_tree_checkpoint_write(amrh_ctx* c, const char* path, double time, double refine_alpha, const int* ages) {
   if (!c || !path) return fail(AMRH_EINVAL, "null argument");

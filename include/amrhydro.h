@@ -209,6 +209,19 @@ int amrh_tree_checkpoint_write(amrh_ctx* ctx, const char* path, double time, dou
  * are returned. */
 int amrh_tree_checkpoint_read(const char* path, const amrh_config* cfg, amrh_ctx** out, int* nblocks,
                               double* time, double* refine_alpha);
+/* The mesh GridNode::output hands to Silo (grid/grid_node.cpp:1315-1356, :1422-1487): nodes in
+ * depth-first order; per node the (GNX+1)^3 cell-corner points xf(i), yf(j), zf(k), x fastest; one
+ * hexahedral zone per interior cell not covered by a child, in k, j, i order, with its 8 corner
+ * indices in the reference's order; per zone the state's fields.  Counts first: npoints =
+ * nodes * (GNX+1)^3, nzones = uncovered interior cells. */
+int amrh_tree_output_counts(amrh_ctx* ctx, long* npoints, long* nzones);
+/* x, y, z: npoints doubles each; nodelist: 8 * nzones ints; values: nf * nzones doubles, field
+ * major.  Any pointer may be NULL to skip that part.  Synchronous. */
+int amrh_tree_output_mesh(amrh_ctx* ctx, double* x, double* y, double* z, int* nodelist, double* values);
+/* The same mesh and zone fields as a legacy binary VTK unstructured grid (readable where the
+ * reference's Silo files are read); the fields are named as State::field_name
+ * (euler/state.cpp:24-42).  Synchronous. */
+int amrh_tree_output_vtk(amrh_ctx* ctx, const char* path, double time);
 
 /* ---- diagnostics (device reductions) ---------------------------------------------------------
  * GridNode::state_sum (grid/grid_node.cpp:1489-1527) over the uncovered interior cells of every
