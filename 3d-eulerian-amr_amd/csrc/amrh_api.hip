@@ -318,7 +318,7 @@ hipError_t amrh::ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int
     r.dt_dev = a.dt_dev;
     r.dt = a.dt;
     r.beta = a.beta;
-    hipError_t e = launch_rot_substep(c->cfg.g, c->cfg.recon, c->cfg.eos, r, grid, c->stream);
+    hipError_t e = launch_rot_substep(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, grid, c->stream);
     if (e == hipSuccess && a.outflow) e = ctx_outflow_ghosts(c, a.Uout);
     return e;
   }
@@ -335,7 +335,7 @@ hipError_t amrh::ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int g
     r.Uin = m.U;
     r.block_eig = m.block_eig;
     r.gmax = m.gmax;
-    return launch_rot_maxeig(c->cfg.g, c->cfg.recon, c->cfg.eos, r, grid, c->stream);
+    return launch_rot_maxeig(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, grid, c->stream);
   }
   if (c->cfg.recon == AMRH_RECON_MINMOD) return launch_maxeig(c->cfg.n, m, mixed, grid, c->stream);
   return launch_block_maxeig(c->cfg.n, c->cfg.g, c->cfg.recon, m, mixed, grid, c->stream);

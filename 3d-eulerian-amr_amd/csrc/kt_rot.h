@@ -24,8 +24,8 @@ struct RotArgs {
   unsigned long long* gmax;  // CFL scan: max over blocks of eig/h as ordered bits (nullptr = skip)
 };
 
-hipError_t launch_rot_substep(int g, int recon, int eos, const RotArgs& a, int grid, hipStream_t s);
-hipError_t launch_rot_maxeig(int g, int recon, int eos, const RotArgs& a, int grid, hipStream_t s);
+hipError_t launch_rot_substep(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
+hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
 hipError_t launch_rot_ghosts(int g, double* U, const double* x0, const double* h, int nblocks, int z_reflect,
                              const rot::RotParams& c, hipStream_t s);
 

@@ -1,5 +1,6 @@
 // Rotating-frame binary-star substep (the reference's BinaryDriver::sub_step with a given
-// gravitational potential), its CFL scan and its outflow ghost fill, on 8^3 blocks, FP64.
+// gravitational potential), its CFL scan and its outflow ghost fill, on 8^3 blocks; FP64, or the
+// mixed form with FP32 faces.
 //
 // One workgroup of N^3 = 512 threads per block, a thread per interior cell.  For each direction
 // in x, y, z order the primitives of the direction's pencils (interior + its two face ghost slabs)
@@ -12,7 +13,8 @@
 // zeroes D[pot] and D[T], adds the geometric source and forms U = (U + D dt) beta + U0 (1 - beta)
 // (grid/grid.cpp:406-420), floors it with the Newton solve for T (binary/state_ztwd.cpp:482-554) and
 // resets pot = rho (phi - R^2 Omega0^2 / 2) (binary/binary.cpp:825-849).
-// LDS: 13 x 896 staged primitives + 13 x 576 face fluxes + 576 face velocities, FP64: 154 KB.
+// LDS: 13 x 896 staged primitives + 13 x 576 face fluxes + 576 face velocities: 154 KB in FP64,
+// 77 KB in the mixed form.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -94,14 +96,18 @@ __device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int 
   }
 }
 
-template <int G, int RECON, int EOS, int MODE>
+// MIXED: the staged primitives, the reconstruction, the face states' EOS and fluxes and the KT
+// combination in FP32 (rot::sp); the flux differences, sources, RK update and floor in FP64.
+template <int G, int RECON, int EOS, int MODE, bool MIXED>
 __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
+  using T = typename std::conditional<MIXED, float, double>::type;
   constexpr int RS = N3 + 2 * G * N2;  // staged elements per field
   constexpr int R = N3 + 6 * G * N2;   // record elements per field
-  __shared__ double P[NF][RS];
-  __shared__ double Fb[MODE == 0 ? NF : 1][MODE == 0 ? NFACE : 1];
-  __shared__ double Vf[MODE == 0 ? NFACE : 1];
+  __shared__ T P[NF][RS];
+  __shared__ T Fb[MODE == 0 ? NF : 1][MODE == 0 ? NFACE : 1];
+  __shared__ T Vf[MODE == 0 ? NFACE : 1];
   __shared__ double red[N3 / 64];
+  const sp::RotF cf = sp::make(a.c);
 
   const int tid = threadIdx.x;
   const RotParams& c = a.c;
@@ -138,12 +144,65 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         cell_pos(x0, h, i, j, k, X);
         to_prim(c, u, X);
 #pragma unroll
-        for (int l = 0; l < NF; l++) P[l][r] = u[l];
+        for (int l = 0; l < NF; l++) P[l][r] = (T)u[l];
       }
       __syncthreads();
 #pragma unroll 1
       for (int q = tid; q < NFACE; q += N3) {
         const int f = q % (N + 1), t = q / (N + 1), ta = t % N, tb = t / N;
+        if constexpr (MIXED) {
+          float ql[NF], qr[NF];
+          if constexpr (RECON == AMRH_RECON_PPM) {
+            int e6[6];
+#pragma unroll
+            for (int m = 0; m < 6; m++) e6[m] = pencil_region<G>(d, f - 3 + m, ta, tb);
+#pragma unroll
+            for (int l = 0; l < NF; l++) {
+              float v[6];
+#pragma unroll
+              for (int m = 0; m < 6; m++) v[m] = (float)P[l][e6[m]];
+              f32::ppm_face(v, ql[l], qr[l], l == POT, l == POT);
+            }
+          } else {
+            int e4[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) e4[m] = pencil_region<G>(d, f - 2 + m, ta, tb);
+#pragma unroll
+            for (int l = 0; l < NF; l++) {
+              float v[4];
+#pragma unroll
+              for (int m = 0; m < 4; m++) v[m] = (float)P[l][e4[m]];
+              f32::minmod_face(v, ql[l], qr[l], (float)c.theta);
+            }
+          }
+          float X[3];
+          {
+            const int ii = d == 0 ? f : ta;
+            const int jj = d == 0 ? ta : (d == 1 ? f : tb);
+            const int kk = d == 2 ? f : tb;
+            double Xd[3];
+            cell_pos(x0, h, ii, jj, kk, Xd);
+            Xd[d] = x0[d] + (double)f * h;
+            X[0] = (float)Xd[0], X[1] = (float)Xd[1], X[2] = (float)Xd[2];
+          }
+          sp::to_con(cf, ql, X);
+          sp::to_con(cf, qr, X);
+          float fl[NF], fr[NF];
+          const float el = sp::flux<EOS, d>(H, cf, ql, X, fl);
+          const float er = sp::flux<EOS, d>(H, cf, qr, X, fr);
+          const float amax = fmaxf(el, er);
+          if constexpr (MODE == 0) {
+            float uf[NF];
+#pragma unroll
+            for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5f;
+            Vf[q] = (T)sp::vel<d>(cf, uf, X, sp::Rc(X));
+#pragma unroll
+            for (int l = 0; l < NF; l++) Fb[l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5f);
+          } else {
+            emax = rmax(emax, (double)amax);
+          }
+          continue;
+        }
         double ql[NF], qr[NF];
         if constexpr (RECON == AMRH_RECON_PPM) {
           int e6[6];
@@ -153,7 +212,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           for (int l = 0; l < NF; l++) {
             double v[6];
 #pragma unroll
-            for (int m = 0; m < 6; m++) v[m] = P[l][e6[m]];
+            for (int m = 0; m < 6; m++) v[m] = (double)P[l][e6[m]];
             // the potential is low-order and smooth (binary/state_ztwd.cpp:50-56)
             x64::ppm_face(v, ql[l], qr[l], l == POT, l == POT);
           }
@@ -165,7 +224,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           for (int l = 0; l < NF; l++) {
             double v[4];
 #pragma unroll
-            for (int m = 0; m < 4; m++) v[m] = P[l][e4[m]];
+            for (int m = 0; m < 4; m++) v[m] = (double)P[l][e4[m]];
             x64::minmod_face(v, ql[l], qr[l], c.theta);
           }
         }
@@ -184,13 +243,13 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           double uf[NF];
 #pragma unroll
           for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5;
-          Vf[q] = d == 0 ? vx(c, uf, X) : (d == 1 ? vy(c, uf, X) : vz(uf));
+          Vf[q] = (T)(d == 0 ? vx(c, uf, X) : (d == 1 ? vy(c, uf, X) : vz(uf)));
           double fl[NF], fr[NF];
           const double el = flux<EOS, d>(H, c, ql, X, fl);
           const double er = flux<EOS, d>(H, c, qr, X, fr);
           const double amax = rmax(el, er);
 #pragma unroll
-          for (int l = 0; l < NF; l++) Fb[l][q] = ((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5;
+          for (int l = 0; l < NF; l++) Fb[l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5);
         } else {
           double F[NF];
           const double el = flux<EOS, d>(H, c, ql, X, F);
@@ -205,9 +264,10 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         const int tb = d == 2 ? cj : ck;
         const int q0 = p + (N + 1) * (ta + N * tb);
 #pragma unroll
-        for (int l = 0; l < NF; l++) D[l] -= (Fb[l][q0 + 1] - Fb[l][q0]) * hinv;
-        vlo[d] = Vf[q0];
-        vhi[d] = Vf[q0 + 1];
+        // (with FP32 fluxes the FP64 difference is exact: the divergence still telescopes)
+        for (int l = 0; l < NF; l++) D[l] -= ((double)Fb[l][q0 + 1] - (double)Fb[l][q0]) * hinv;
+        vlo[d] = (double)Vf[q0];
+        vhi[d] = (double)Vf[q0 + 1];
         __syncthreads();  // P, Fb and Vf are restaged for the next direction
       }
     };
@@ -357,33 +417,33 @@ __global__ __launch_bounds__(256) void rot_ghost_kernel(double* U, const double*
   for (int l = 0; l < NF; l++) Ub[(long)l * R + e] = u[l];
 }
 
-template <int G, int RECON, int EOS, int MODE>
+template <int G, int RECON, int EOS, int MODE, bool MIXED>
 hipError_t launch_one(const RotArgs& a, int grid, hipStream_t s) {
   if (a.nblocks <= 0) return hipSuccess;
   const int g = grid > 0 && grid < a.nblocks ? grid : a.nblocks;
-  kt_rot_kernel<G, RECON, EOS, MODE><<<g, N3, 0, s>>>(a);
+  kt_rot_kernel<G, RECON, EOS, MODE, MIXED><<<g, N3, 0, s>>>(a);
   return hipGetLastError();
 }
 
-template <int MODE>
+template <int MODE, bool MIXED>
 hipError_t dispatch(int g, int recon, int eos, const RotArgs& a, int grid, hipStream_t s) {
   if (g == 3 && recon == AMRH_RECON_PPM)
-    return eos == EOS_HELMHOLTZ ? launch_one<3, AMRH_RECON_PPM, EOS_HELMHOLTZ, MODE>(a, grid, s)
-                                : launch_one<3, AMRH_RECON_PPM, EOS_IDEAL, MODE>(a, grid, s);
+    return eos == EOS_HELMHOLTZ ? launch_one<3, AMRH_RECON_PPM, EOS_HELMHOLTZ, MODE, MIXED>(a, grid, s)
+                                : launch_one<3, AMRH_RECON_PPM, EOS_IDEAL, MODE, MIXED>(a, grid, s);
   if (g == 2 && recon == AMRH_RECON_MINMOD)
-    return eos == EOS_HELMHOLTZ ? launch_one<2, AMRH_RECON_MINMOD, EOS_HELMHOLTZ, MODE>(a, grid, s)
-                                : launch_one<2, AMRH_RECON_MINMOD, EOS_IDEAL, MODE>(a, grid, s);
+    return eos == EOS_HELMHOLTZ ? launch_one<2, AMRH_RECON_MINMOD, EOS_HELMHOLTZ, MODE, MIXED>(a, grid, s)
+                                : launch_one<2, AMRH_RECON_MINMOD, EOS_IDEAL, MODE, MIXED>(a, grid, s);
   return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-hipError_t launch_rot_substep(int g, int recon, int eos, const RotArgs& a, int grid, hipStream_t s) {
-  return dispatch<0>(g, recon, eos, a, grid, s);
+hipError_t launch_rot_substep(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s) {
+  return mixed ? dispatch<0, true>(g, recon, eos, a, grid, s) : dispatch<0, false>(g, recon, eos, a, grid, s);
 }
 
-hipError_t launch_rot_maxeig(int g, int recon, int eos, const RotArgs& a, int grid, hipStream_t s) {
-  return dispatch<1>(g, recon, eos, a, grid, s);
+hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s) {
+  return mixed ? dispatch<1, true>(g, recon, eos, a, grid, s) : dispatch<1, false>(g, recon, eos, a, grid, s);
 }
 
 hipError_t launch_rot_ghosts(int g, double* U, const double* x0, const double* h, int nblocks, int z_reflect,

@@ -269,5 +269,106 @@ __device__ __forceinline__ void enforce_outflow(const RotParams& c, double* u, c
   u[ET] += 0.5 * (u[SR] * u[SR] + t * t + u[SZ] * u[SZ]) / u[D0];
 }
 
+// ------------------------------------------------------------------------------------------
+// The mixed form's face states: reconstructed primitives back to conserved, EOS pressure and
+// sound speed, and the directional flux in FP32 (the Helmholtz EOS itself runs in FP64 on the
+// FP32 face state).  Same formulas as above; not bit-matched to the reference.
+// ------------------------------------------------------------------------------------------
+namespace sp {
+
+struct RotF {
+  float Omega, Omega0, gamma, gm1inv, rho_floor;
+  float abar_inv[NSPEC];
+};
+
+__host__ __device__ inline RotF make(const RotParams& c) {
+  RotF f;
+  f.Omega = (float)c.Omega;
+  f.Omega0 = (float)c.Omega0;
+  f.gamma = (float)c.gamma;
+  f.gm1inv = (float)(1.0 / (c.gamma - 1.0));
+  f.rho_floor = (float)c.rho_floor;
+  for (int i = 0; i < NSPEC; i++) f.abar_inv[i] = (float)(1.0 / c.abar[i]);
+  return f;
+}
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float Rc(const float* X) { return __builtin_amdgcn_sqrtf(X[0] * X[0] + X[1] * X[1]); }
+__device__ __forceinline__ float st(const RotF& c, const float* u, float R) { return u[LZ] * rcp(R) - u[D0] * R * c.Omega; }
+
+__device__ __forceinline__ void to_con(const RotF& c, float* u, const float* X) {
+#pragma clang fp contract(fast)
+  const float R = Rc(X);
+  u[SZ] *= u[D0];
+  u[POT] *= u[D0];
+  u[SR] *= u[D0];
+  u[LZ] = (u[LZ] + c.Omega * R) * (u[D0] * R);
+  const float t = st(c, u, R);
+  u[ET] += 0.5f * (u[SR] * u[SR] + t * t + u[SZ] * u[SZ]) * rcp(u[D0]);
+  const float s1 = rcp(u[SPEC] + u[SPEC + 1] + u[SPEC + 2]) * u[D0];
+#pragma unroll
+  for (int i = 0; i < NSPEC; i++) u[SPEC + i] *= s1;
+  const float s2 = rcp(u[DDON] + u[DACC]) * u[D0];
+  u[DDON] *= s2;
+  u[DACC] *= s2;
+}
+
+// velocity along d of a conserved state
+template <int D>
+__device__ __forceinline__ float vel(const RotF& c, const float* u, const float* X, float R) {
+#pragma clang fp contract(fast)
+  const float ir = rcp(R * u[D0]);
+  if constexpr (D == 0) return (X[0] * u[SR] - X[1] * st(c, u, R)) * ir;
+  if constexpr (D == 1) return (X[1] * u[SR] + X[0] * st(c, u, R)) * ir;
+  return u[SZ] * rcp(u[D0]);
+}
+
+template <int EOS>
+__device__ __forceinline__ void p_cs(const helm::HelmDev& H, const RotF& c, const float* u, float& p, float& cs) {
+#pragma clang fp contract(fast)
+  const float abar = u[D0] * rcp(u[SPEC] * c.abar_inv[0] + u[SPEC + 1] * c.abar_inv[1] + u[SPEC + 2] * c.abar_inv[2]);
+  const float zbar = 0.5f * abar;
+  if constexpr (EOS == EOS_HELMHOLTZ) {
+    helm::Eos q{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    q.rho = u[D0];
+    q.T = u[TT];
+    q.abar = abar;
+    q.zbar = zbar;
+    helm::eos(H, q);
+    p = (float)q.p;
+    cs = (float)q.cs;
+  } else {
+    const float T = u[TT] > (float)TMIN_IDEAL ? u[TT] : (float)TMIN_IDEAL;
+    p = u[D0] * T * ((1.0f + zbar) * rcp(abar));
+    cs = __builtin_amdgcn_sqrtf(c.gamma * p * rcp(u[D0]));
+  }
+}
+
+// flux of a conserved face state along D; returns |v_d| + c_s
+template <int EOS, int D>
+__device__ __forceinline__ float flux(const helm::HelmDev& H, const RotF& c, const float* u, const float* X, float* F) {
+#pragma clang fp contract(fast)
+  float p, cs;
+  p_cs<EOS>(H, c, u, p, cs);
+  const float R = Rc(X);
+  const float v = vel<D>(c, u, X, R);
+#pragma unroll
+  for (int i = 0; i < NF; i++) F[i] = v * u[i];
+  if constexpr (D == 0) {
+    F[SR] += X[0] * p * rcp(R);
+    F[LZ] -= X[1] * p;
+    F[ET] += (v - X[1] * (c.Omega - c.Omega0)) * p;
+  } else if constexpr (D == 1) {
+    F[SR] += X[1] * p * rcp(R);
+    F[LZ] += X[0] * p;
+    F[ET] += (v + X[0] * (c.Omega - c.Omega0)) * p;
+  } else {
+    F[SZ] += p;
+    F[ET] += v * p;
+  }
+  return fabsf(v) + cs;
+}
+
+}  // namespace sp
 }  // namespace rot
 }  // namespace amrh

@@ -106,7 +106,8 @@ def time_steps(B, torch, cells, steps, warmup):
 def extra_configs(amrhydro, torch, steps, warmup):
     """The other workloads of the hot path, one line each (FP64 and, where built, mixed): the
     uneven multilevel leaf batch (59,518 leaves of 8^3), PPM on 8^3 blocks (g = 3), the rotating
-    binary (20,000 leaves, PPM) with the ideal-gas and the Helmholtz EOS, and the 73-node octree
+    binary (20,000 leaves, PPM) with the ideal-gas and the Helmholtz EOS (mixed and FP64), the
+    self-gravity V-cycles on the binary's octree, and the 73-node octree
     of the Sedov blast (ghost exchange, reflux and restriction included, one captured graph per
     step)."""
     import workloads
@@ -146,7 +147,6 @@ def extra_configs(amrhydro, torch, steps, warmup):
     for eos, tag in ((amrhydro.AMRH_EOS_IDEAL, "ideal"), (amrhydro.AMRH_EOS_HELMHOLTZ, "helmholtz")):
         log("rotating binary, %s EOS (20,000 leaves of 8^3, PPM, 13 fields)" % tag)
         Bn = workloads.binary_batch(eos="ideal" if eos == amrhydro.AMRH_EOS_IDEAL else "helm")
-        cfg = amrhydro.rotating_config(eos=eos, omega=Bn["omega"], boundary=OUT)
         if eos == amrhydro.AMRH_EOS_HELMHOLTZ:
             eos_tab = amrhydro.HelmholtzEos(workloads.helmholtz_table())
             eos_tab.set_units(*workloads.HELM_UNITS)
@@ -156,7 +156,9 @@ def extra_configs(amrhydro, torch, steps, warmup):
             B.set_potential(Bn["phi"])
             if eos == amrhydro.AMRH_EOS_HELMHOLTZ:
                 B.attach_eos(eos_tab)
-        out["rotating_%s_fp64" % tag] = flat("rotating %s fp64" % tag, cfg, Bn["U"], Bn["h"], prep)
+        for prec, ptag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+            cfg = amrhydro.rotating_config(eos=eos, omega=Bn["omega"], boundary=OUT, precision=prec)
+            out["rotating_%s_%s" % (tag, ptag)] = flat("rotating %s %s" % (tag, ptag), cfg, Bn["U"], Bn["h"], prep)
         del Bn
     if eos_tab is not None:
         eos_tab.close()
