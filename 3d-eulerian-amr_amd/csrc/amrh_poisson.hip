@@ -847,15 +847,14 @@ int relax_loop(amrh_poisson* p, int l, int sweep, double limit) {
   export_kernel<<<p->lv_count[l], NTHREADS, 0, p->stream>>>(p->args(p->lv_start[l]));
   PCHK(hipGetLastError());
   int& last = p->steps_last[sweep * (p->maxlevel + 1) + l];
-  int k = 0;
+  int k = 0, chunk = std::max(2, last);  // the previous count: a loop's length varies little
   for (;;) {
-    const int chunk = std::max(2, last + 1);
     for (int q = 0; q < chunk; q++) PCHK(launch_step(p, l, k++, limit));
     PCHK(hipMemcpyAsync(p->st_host, p->st_dev, sizeof(LoopState), hipMemcpyDeviceToHost, p->stream));
     PCHK(hipStreamSynchronize(p->stream));
     if (p->st_host->done) break;
     if (k > 200000) return fail(AMRH_ESTATE, "poisson: relaxation at level %d does not settle", l);
-    last = 2 * last + 2;  // grow the next chunk
+    chunk = std::max(2, k / 4);  // a few more, growing if the loop keeps going
   }
   last = p->st_host->iters;
   return AMRH_OK;
