@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 import orc
@@ -56,4 +57,51 @@ def test_create_rejects_unsupported_schemes(n, g, recon, msg):
     ctx = C.c_void_p()
     rc = amrhydro.lib().amrh_create(C.byref(cfg), 4, C.byref(ctx))
     assert rc == 1 and not ctx
+    assert msg in amrhydro.lib().amrh_last_error().decode()
+
+
+def _amrhydro():
+    import sys
+    sys.path.insert(0, PKG)
+    import amrhydro
+    return amrhydro
+
+
+@pytest.mark.parametrize("nblocks", [0, -3])
+def test_create_rejects_empty_batches(nblocks):
+    amrhydro = _amrhydro()
+    cfg = amrhydro.euler_config(n=8)
+    ctx = C.c_void_p()
+    assert amrhydro.lib().amrh_create(C.byref(cfg), nblocks, C.byref(ctx)) == 1 and not ctx
+    assert "nblocks must be positive" in amrhydro.lib().amrh_last_error().decode()
+
+
+@pytest.mark.parametrize("parent,child,msg", [
+    ([0, 0], [0, 0], "node 0 must be the root"),
+    ([-1, 1], [0, 0], "bad parent"),           # a node's parent after it
+    ([-1, 0, 0], [0, 3, 3], "bad parent"),     # two children in one slot
+    ([-1, 0], [0, 8], "bad parent"),           # child index out of range
+])
+def test_poisson_rejects_bad_trees(parent, child, msg):
+    """The tree is checked before anything is allocated on the device."""
+    amrhydro = _amrhydro()
+    p = np.array(parent, np.int32)
+    c = np.array(child, np.int32)
+    h = C.c_void_p()
+    IP = C.POINTER(C.c_int)
+    rc = amrhydro.lib().amrh_poisson_create(8, 2, 1.0, 0, len(p), p.ctypes.data_as(IP), c.ctypes.data_as(IP),
+                                            None, C.byref(h))
+    assert rc == 1 and not h
+    assert msg in amrhydro.lib().amrh_last_error().decode()
+
+
+@pytest.mark.parametrize("n,bw,msg", [(10, 2, "n = 8, 12 or 16"), (8, 0, "ghost width")])
+def test_poisson_rejects_bad_shapes(n, bw, msg):
+    amrhydro = _amrhydro()
+    p = np.array([-1], np.int32)
+    h = C.c_void_p()
+    IP = C.POINTER(C.c_int)
+    rc = amrhydro.lib().amrh_poisson_create(n, bw, 1.0, 0, 1, p.ctypes.data_as(IP), p.ctypes.data_as(IP), None,
+                                            C.byref(h))
+    assert rc == 1 and not h
     assert msg in amrhydro.lib().amrh_last_error().decode()

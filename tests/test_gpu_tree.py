@@ -90,3 +90,15 @@ def test_partial_tree_reflux_matches_oracle():
 def test_partial_tree_z_reflect_matches_oracle():
     cfg = orc.euler_cfg(n=8, g=2, grid_dim=0.5, z_reflect=1)
     run_and_compare(cfg, tc.partial_spec(), tc.blob_ic, 3, "zreflect")
+
+
+def test_root_only_tree_matches_oracle():
+    """A tree of one node: every face is physical, nothing to reflux or restrict."""
+    cfg = orc.euler_cfg(n=8, g=2, grid_dim=0.5, z_reflect=0)
+    run_and_compare(cfg, [], tc.blob_ic, 6, "root")
+
+
+def test_n16_tree_matches_oracle():
+    """16^3 blocks on the octree: one refined level, coarse/fine faces at the root's children."""
+    cfg = orc.euler_cfg(n=16, g=2, grid_dim=0.5, z_reflect=0)
+    run_and_compare(cfg, [(0, c) for c in range(8)] + [(1, c) for c in range(8)], tc.blob_ic, 3, "n16")
