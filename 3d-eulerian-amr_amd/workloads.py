@@ -179,6 +179,72 @@ def lane_emden(n=1.5, dxi=1e-3):
     raise RuntimeError("no surface")
 
 
+def _binary_model(q=0.7, r_donor=0.2):
+    """The two n = 1.5 polytropes of binary_batch: Lane-Emden profile, radii, central densities,
+    masses, centres, the polytropic K, the Keplerian Omega and the background."""
+    xi, th, xi1, om = lane_emden(1.5)
+    a = 2.5 * r_donor
+    r_acc = r_donor * q ** (1.0 / 3.0)
+    rc_d, rc_a = 1.0, 1.0 / (q * q)
+    m_d = 4.0 * np.pi * rc_d * r_donor ** 3 * om / xi1 ** 3
+    m_a = 4.0 * np.pi * rc_a * r_acc ** 3 * om / xi1 ** 3
+    K = 4.0 * np.pi * r_donor ** 2 * rc_d ** (1.0 - 1.0 / 1.5) / (2.5 * xi1 ** 2)
+    omega = np.sqrt((m_a + m_d) / a ** 3)
+    xa, xd = -a * m_d / (m_a + m_d), a * m_a / (m_a + m_d)
+    rho_bg, p_bg = 1e-4 * rc_d, 1e-3 * K * rc_d ** (5.0 / 3.0)
+    return dict(xi=xi, th=th, xi1=xi1, r_acc=r_acc, r_donor=r_donor, rc_a=rc_a, rc_d=rc_d, m_a=m_a, m_d=m_d,
+                K=K, omega=omega, xa=xa, xd=xd, rho_bg=rho_bg, p_bg=p_bg)
+
+
+def _binary_fields(M, X, Y, Z, eos):
+    """The 13 conserved fields of the binary model at cell centres (X, Y, Z), at rest in the frame."""
+    def star(r, R, rc):
+        x = np.clip(r / R, 0.0, 1.0) * M["xi1"]
+        t = np.interp(x, M["xi"], M["th"])
+        return np.where(r < R, rc * np.maximum(t, 0.0) ** 1.5, 0.0)
+
+    xa, xd, omega = M["xa"], M["xd"], M["omega"]
+    ra = np.sqrt((X - xa) ** 2 + Y ** 2 + Z ** 2)
+    rd = np.sqrt((X - xd) ** 2 + Y ** 2 + Z ** 2)
+    rho_a, rho_d = star(ra, M["r_acc"], M["rc_a"]), star(rd, M["r_donor"], M["rc_d"])
+    rho = rho_a + rho_d + M["rho_bg"]
+    p = M["K"] * (rho_a + rho_d) ** (5.0 / 3.0) + M["p_bg"]
+    w_a = rho_a / rho
+    R2 = X * X + Y * Y
+    abar = 1.0 / ((1.0 - w_a) / ROT_ABAR[0] + w_a / ROT_ABAR[1])
+    zbar = abar / 2.0
+    eps = 1.5 * p
+    if eos == "ideal":
+        T = p * abar / (rho * (1.0 + zbar))
+    else:
+        cm, gr, sec, _ = HELM_UNITS
+        p_cgs = p * (gr / (cm * sec * sec))
+        rho_cgs = rho * (gr / cm ** 3)
+        T = p_cgs / (HELM_AVO * HELM_KERG * rho_cgs * (1.0 / abar + zbar / abar))
+    u = np.empty((13,) + np.broadcast(X, Y, Z).shape)
+    u[0] = rho
+    u[1] = rho * w_a
+    u[2] = rho * (1.0 - w_a)
+    u[3] = 0.0
+    u[4] = rho * R2 * omega
+    u[5] = 0.0
+    u[6] = rho * (_binary_phi(M, X, Y, Z) - 0.5 * R2 * omega * omega)
+    u[7] = eps
+    u[8] = eps
+    u[9] = T
+    u[10] = rho * (1.0 - w_a)
+    u[11] = rho * w_a
+    u[12] = 0.0
+    return u
+
+
+def _binary_phi(M, X, Y, Z):
+    """the two stars' Plummer-softened point-mass potential (softening R/2)"""
+    ra = np.sqrt((X - M["xa"]) ** 2 + Y ** 2 + Z ** 2)
+    rd = np.sqrt((X - M["xd"]) ** 2 + Y ** 2 + Z ** 2)
+    return -M["m_a"] / np.sqrt(ra ** 2 + (0.5 * M["r_acc"]) ** 2) - M["m_d"] / np.sqrt(rd ** 2 + (0.5 * M["r_donor"]) ** 2)
+
+
 def binary_batch(counts=BINARY_COUNTS, n=8, g=3, eos="ideal", q=0.7, r_donor=0.2, seed=SEED):
     """Two n = 1.5 polytropes (Lane-Emden, one K) with mass ratio q = M_donor/M_accretor, separation
     2.5 R_donor, centre of mass at the origin, at rest in the frame rotating at the Keplerian
@@ -190,22 +256,7 @@ def binary_batch(counts=BINARY_COUNTS, n=8, g=3, eos="ideal", q=0.7, r_donor=0.2
     lev, org = multilevel_layout(counts, seed)
     nb = len(lev)
     G = n + 2 * g
-    xi, th, xi1, om = lane_emden(1.5)
-    a = 2.5 * r_donor
-    r_acc = r_donor * q ** (1.0 / 3.0)
-    rc_d, rc_a = 1.0, 1.0 / (q * q)
-    m_d = 4.0 * np.pi * rc_d * r_donor ** 3 * om / xi1 ** 3
-    m_a = 4.0 * np.pi * rc_a * r_acc ** 3 * om / xi1 ** 3
-    K = 4.0 * np.pi * r_donor ** 2 * rc_d ** (1.0 - 1.0 / 1.5) / (2.5 * xi1 ** 2)
-    omega = np.sqrt((m_a + m_d) / a ** 3)
-    xa, xd = -a * m_d / (m_a + m_d), a * m_a / (m_a + m_d)
-    rho_bg, p_bg = 1e-4 * rc_d, 1e-3 * K * rc_d ** (5.0 / 3.0)
-
-    def star(r, R, rc):
-        x = np.clip(r / R, 0.0, 1.0) * xi1
-        t = np.interp(x, xi, th)
-        return np.where(r < R, rc * np.maximum(t, 0.0) ** 1.5, 0.0)
-
+    M = _binary_model(q, r_donor)
     h = 2.0 / (n * 2.0 ** (lev + 1))
     U = np.empty((nb, 13, G, G, G))
     P = n + 2
@@ -221,91 +272,35 @@ def binary_batch(counts=BINARY_COUNTS, n=8, g=3, eos="ideal", q=0.7, r_donor=0.2
             return x[:, None, None, :], y[:, None, :, None], z[:, :, None, None]
 
         X, Y, Z = grid(np.arange(G) - g + 0.5)
-        ra = np.sqrt((X - xa) ** 2 + Y ** 2 + Z ** 2)
-        rd = np.sqrt((X - xd) ** 2 + Y ** 2 + Z ** 2)
-        rho_a, rho_d = star(ra, r_acc, rc_a), star(rd, r_donor, rc_d)
-        rho = rho_a + rho_d + rho_bg
-        p = K * (rho_a + rho_d) ** (5.0 / 3.0) + p_bg
-        w_a = rho_a / rho
-        R2 = X * X + Y * Y
-        pot_phi = -m_a / np.sqrt(ra ** 2 + (0.5 * r_acc) ** 2) - m_d / np.sqrt(rd ** 2 + (0.5 * r_donor) ** 2)
-        abar = 1.0 / ((1.0 - w_a) / ROT_ABAR[0] + w_a / ROT_ABAR[1])
-        zbar = abar / 2.0
-        eps = 1.5 * p
-        if eos == "ideal":
-            T = p * abar / (rho * (1.0 + zbar))
-        else:
-            cm, gr, sec, _ = HELM_UNITS
-            p_cgs = p * (gr / (cm * sec * sec))
-            rho_cgs = rho * (gr / cm ** 3)
-            T = p_cgs / (HELM_AVO * HELM_KERG * rho_cgs * (1.0 / abar + zbar / abar))
-        u = U[s:e]
-        u[:, 0] = rho
-        u[:, 1] = rho * w_a
-        u[:, 2] = rho * (1.0 - w_a)
-        u[:, 3] = 0.0
-        u[:, 4] = rho * R2 * omega
-        u[:, 5] = 0.0
-        u[:, 6] = rho * (pot_phi - 0.5 * R2 * omega * omega)
-        u[:, 7] = eps
-        u[:, 8] = eps
-        u[:, 9] = T
-        u[:, 10] = rho * (1.0 - w_a)
-        u[:, 11] = rho * w_a
-        u[:, 12] = 0.0
-        Xp, Yp, Zp = grid(np.arange(P) - 1 + 0.5)
-        ra = np.sqrt((Xp - xa) ** 2 + Yp ** 2 + Zp ** 2)
-        rd = np.sqrt((Xp - xd) ** 2 + Yp ** 2 + Zp ** 2)
-        phi[s:e] = -m_a / np.sqrt(ra ** 2 + (0.5 * r_acc) ** 2) - m_d / np.sqrt(rd ** 2 + (0.5 * r_donor) ** 2)
+        U[s:e] = np.moveaxis(_binary_fields(M, X, Y, Z, eos), 0, 1)
+        phi[s:e] = _binary_phi(M, *grid(np.arange(P) - 1 + 0.5))
     x0 = np.ascontiguousarray(org)
-    return dict(U=U, phi=phi, x0=x0, h=h, level=lev, omega=omega)
+    return dict(U=U, phi=phi, x0=x0, h=h, level=lev, omega=M["omega"])
 
 
-# ---- Sedov blast on a fully refined octree -----------------------------------------------------
-def sedov_tree(maxlevel=2, n=8, g=2, grid_dim=0.5):
-    """Fully refined octree (1 + 8 + 64 nodes for maxlevel 2, breadth first: every parent before
-    its children, child c with x, y, z bits c & 1, c >> 1 & 1, c >> 2 & 1) over the cube
-    [-grid_dim, grid_dim]^3: ambient rho = 1, p = 1e-5 at rest; energy 1.0 deposited uniformly in
-    the finest-level cells whose centres lie within 2 finest cells of the origin.  The node at
-    level l with integer location L has padded cell p centred at -grid_dim + (L n + p - g + 1/2) h_l
-    (grid/grid.cpp:500-526).  Every padded cell holds the analytic field; the library's ghost fill
-    and restriction make them consistent.  Returns (U (nodes, 6, G, G, G), parent, child)."""
-    parent, child, level, loc = [-1], [-1], [0], [(0, 0, 0)]
-    frontier = [0]
-    for _ in range(maxlevel):
-        new = []
-        for p in frontier:
-            for c in range(8):
-                parent.append(p)
-                child.append(c)
-                level.append(level[p] + 1)
-                loc.append(tuple(2 * loc[p][d] + ((c >> d) & 1) for d in range(3)))
-                new.append(len(parent) - 1)
-        frontier = new
-    G = n + 2 * g
-    h0 = 2.0 * grid_dim / n
-    hf = h0 / 2 ** maxlevel
-    r = 2.0 * hf
-    m = int(round(2.0 * grid_dim / hf))
-    c1 = -grid_dim + (np.arange(m) + 0.5) * hf
-    Zc, Yc, Xc = np.meshgrid(c1, c1, c1, indexing="ij")
-    ndep = int((Xc ** 2 + Yc ** 2 + Zc ** 2 <= r * r).sum())
-    edep = 1.0 / (ndep * hf ** 3)
-    U = np.empty((len(parent), 6, G, G, G))
+def binary_tree_state(level, loc, n=8, g=3, eos="ideal"):
+    """binary_batch's model on the nodes of an octree over [-1, 1]^3 (binary_tree): U (nodes, 13,
+    G, G, G) at the padded cell centres, the point-mass phi on (n+2)^3, the low interior corner x0
+    and the cell width h of every node, and Omega."""
+    M = _binary_model()
+    G, P = n + 2 * g, n + 2
+    nn = len(level)
+    U = np.empty((nn, 13, G, G, G))
+    phi = np.empty((nn, P, P, P))
+    x0 = np.empty((nn, 3))
+    h = np.empty(nn)
     for b, (l, L) in enumerate(zip(level, loc)):
-        h = h0 / 2 ** l
-        x, y, z = (-grid_dim + (L[d] * n + np.arange(G) - g + 0.5) * h for d in range(3))
-        Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
-        E = np.full(X.shape, 1e-5 / (GAMMA - 1.0))
-        E = np.where(X * X + Y * Y + Z * Z <= r * r, E + edep, E)
-        U[b, 0] = 1.0
-        U[b, 1:4] = 0.0
-        U[b, 4] = E
-        U[b, 5] = E ** (1.0 / GAMMA)
-    return U, np.array(parent, np.int32), np.array(child, np.int32)
+        h[b] = 2.0 / n / 2 ** l
+        x0[b] = [-1.0 + L[d] * n * h[b] for d in range(3)]
+        ax = [x0[b, d] + (np.arange(G) - g + 0.5) * h[b] for d in range(3)]
+        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        U[b] = _binary_fields(M, X, Y, Z, eos)
+        ax = [x0[b, d] + (np.arange(P) - 1 + 0.5) * h[b] for d in range(3)]
+        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        phi[b] = _binary_phi(M, X, Y, Z)
+    return dict(U=U, phi=phi, x0=x0, h=h, omega=M["omega"])
 
 
-# ---- self-gravity: the binary's density on a properly nested octree ---------------------------
 def _binary_stars(q=0.7, r_donor=0.2):
     """centres, radii and central densities of the two n = 1.5 polytropes of binary_batch"""
     xi, th, xi1, om = lane_emden(1.5)

@@ -105,6 +105,29 @@ int orc_rot_batch_max_eig(const orc_rot_cfg* c, int nb, const double* U, const d
 int orc_rot_batch_outflow_ghosts(const orc_rot_cfg* c, int nb, double* U, const double* x0, const double* h);
 int orc_rot_state(const orc_rot_cfg* c, int op, double* u, const double* X, double* F);
 
+/* The rotating-frame state on the octree: nodes as (parent, child index) arrays, parent first, node
+ * 0 the root; per node U (13, G, G, G), phi (n+2)^3 and the Poisson face forces fx, fy, fz
+ * (3 x (n+2)^3, face i between cells i-1 and i), read instead of phi's differences when
+ * use_forces is set. */
+typedef struct orc_rot_tree orc_rot_tree;
+orc_rot_tree* orc_rot_tree_create(const orc_rot_cfg* c, double grid_dim, int nn, const int* parent,
+                                  const int* cidx);
+void orc_rot_tree_destroy(orc_rot_tree* t);
+double* orc_rot_tree_U(orc_rot_tree* t, int node);
+double* orc_rot_tree_phi(orc_rot_tree* t, int node);
+double* orc_rot_tree_force(orc_rot_tree* t, int node);
+const double* orc_rot_tree_F(const orc_rot_tree* t, int node, int d);
+int orc_rot_tree_node_x0(const orc_rot_tree* t, int node, double* x0, double* h);
+void orc_rot_tree_use_forces(orc_rot_tree* t, int on);
+int orc_rot_tree_fill_ghosts(orc_rot_tree* t);
+int orc_rot_tree_store(orc_rot_tree* t);
+int orc_rot_tree_substep(orc_rot_tree* t, double dt, double beta);
+int orc_rot_tree_pot_to_grid(orc_rot_tree* t);
+int orc_rot_tree_inject(orc_rot_tree* t);
+double orc_rot_tree_max_dt(orc_rot_tree* t);
+int orc_rot_tree_flow_off(const orc_rot_tree* t, double* fo);
+int orc_rot_tree_state_sum(const orc_rot_tree* t, double* sum);
+
 /* ---- self-gravity: the multigrid Poisson solver on the octree (poisson/), poisson.cpp ------
  * Nodes are given as (parent, child index) arrays, parent before child, node 0 the root.  Every
  * per-node array is (n+2)^3 doubles, x fastest, one ghost layer (Poisson's PNX = GNX - 2 BW + 2).

@@ -19,6 +19,7 @@
 // x0 + (p - g) h and the cell centre half a cell above -- the reference's
 // (offset + p) h - GRID_DIM - bw h0 - origin (grid/grid.cpp:500-526) for dyadic blocks.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -41,12 +42,12 @@ constexpr double TMIN_IDEAL = 1.0e-20;  // the ideal gas is unit-free: its T flo
 struct RC {
   int n, g, G, recon, zref, eos;
   long G3;
-  double theta, rho_floor, Omega, Omega0, fR, ftheta, gamma;
+  double theta, rho_floor, Omega, Omega0, fR, ftheta, gamma, cfl;
   double abar[NSPEC];
   explicit RC(const orc_rot_cfg* c)
       : n(c->n), g(c->g), G(c->n + 2 * c->g), recon(c->recon), zref(c->z_reflect), eos(c->eos),
         G3((long)G * G * G), theta(c->theta), rho_floor(c->rho_floor), Omega(c->Omega), Omega0(c->Omega0),
-        fR(c->fR), ftheta(c->ftheta), gamma(c->gamma) {
+        fR(c->fR), ftheta(c->ftheta), gamma(c->gamma), cfl(c->cfl) {
     for (int i = 0; i < NSPEC; i++) abar[i] = c->abar[i];
   }
   long at(int i, int j, int k) const { return ((long)k * G + j) * G + i; }
@@ -355,21 +356,26 @@ inline double phi_at(const RC& c, const double* phi, int i, int j, int k) {
   return phi[((long)(k + 1) * P + (j + 1)) * P + (i + 1)];
 }
 
-int block_substep(const RC& c, double* U, const double* U0, const double* phi, const double* x0, double h,
-                  double dt, double beta) {
-  int err = 0;
-  std::vector<double> F((size_t)NFR * c.G3), D((size_t)NFR * c.G3, 0.0), vf((size_t)3 * c.G3, 0.0);
+// Grid::sum_{x,y,z}_difs (grid/grid.cpp:304-344) for direction d from the face fluxes F
+void block_sum_difs(const RC& c, const double* F, double* D, int d, double h) {
   const double hinv = 1.0 / h;
-  for (int d = 0; d < 3; d++) {
-    err |= block_flux(c, U, x0, h, d, F.data(), vf.data() + (size_t)d * c.G3, nullptr);
-    const long step = d == 0 ? 1 : (d == 1 ? c.G : (long)c.G * c.G);
-    for (int k = c.g; k < c.G - c.g; k++)
-      for (int j = c.g; j < c.G - c.g; j++)
-        for (int i = c.g; i < c.G - c.g; i++) {
-          const long e = c.at(i, j, k);
-          for (int l = 0; l < NFR; l++) D[l * c.G3 + e] -= (F[l * c.G3 + e + step] - F[l * c.G3 + e]) * hinv;
-        }
-  }
+  const long step = d == 0 ? 1 : (d == 1 ? c.G : (long)c.G * c.G);
+  for (int k = c.g; k < c.G - c.g; k++)
+    for (int j = c.g; j < c.G - c.g; j++)
+      for (int i = c.g; i < c.G - c.g; i++) {
+        const long e = c.at(i, j, k);
+        for (int l = 0; l < NFR; l++) D[l * c.G3 + e] -= (F[l * c.G3 + e + step] - F[l * c.G3 + e]) * hinv;
+      }
+}
+
+// Binary::add_difs (binary/binary.cpp:205-254) and Grid::add_difs (grid/grid.cpp:406-420) with
+// the floor, then Binary::pot_to_grid (binary/binary.cpp:825-849).  The cell force is the mean
+// of its two face forces: from the given Poisson face forces (force[d], (n+2)^3 each, face i
+// between cells i-1 and i), or Poisson::compute_forces of phi (poisson/poisson.cpp:223-247).
+int block_update(const RC& c, double* U, const double* U0, const double* D, const double* vf, const double* phi,
+                 const double* const* force, const double* x0, double h, double dt, double beta) {
+  int err = 0;
+  const int P = c.n + 2;
   for (int k = c.g; k < c.G - c.g; k++)
     for (int j = c.g; j < c.G - c.g; j++)
       for (int i = c.g; i < c.G - c.g; i++) {
@@ -380,11 +386,15 @@ int block_substep(const RC& c, double* U, const double* U0, const double* phi, c
           s[l] = U[l * c.G3 + e];
           Dl[l] = D[l * c.G3 + e];
         }
-        // Binary::add_difs (binary/binary.cpp:205-254) with the forces of Poisson::compute_forces
         const int ii = i - c.g, jj = j - c.g, kk = k - c.g;
         const double pc = phi_at(c, phi, ii, jj, kk);
         double gf[3];
-        {
+        if (force) {
+          auto fat = [&](int d, int a, int b2, int c2) { return force[d][((long)c2 * P + b2) * P + a]; };
+          gf[0] = 0.5 * (fat(0, ii + 1, jj + 1, kk + 1) + fat(0, ii + 2, jj + 1, kk + 1));
+          gf[1] = 0.5 * (fat(1, ii + 1, jj + 1, kk + 1) + fat(1, ii + 1, jj + 2, kk + 1));
+          gf[2] = 0.5 * (fat(2, ii + 1, jj + 1, kk + 1) + fat(2, ii + 1, jj + 1, kk + 2));
+        } else {
           const double fxl = -(pc - phi_at(c, phi, ii - 1, jj, kk)) / h;
           const double fxh = -(phi_at(c, phi, ii + 1, jj, kk) - pc) / h;
           const double fyl = -(pc - phi_at(c, phi, ii, jj - 1, kk)) / h;
@@ -406,9 +416,9 @@ int block_substep(const RC& c, double* U, const double* U0, const double* phi, c
         sv[ETR] += s[D0] * (dvx(c, X) * gf[0] + dvy(c, X) * gf[1]);
         for (int l = 0; l < NFR; l++) Dl[l] += sv[l];
         // -p div u from the saved face states (binary.cpp:231-239)
-        const double* vX = vf.data();
-        const double* vY = vf.data() + c.G3;
-        const double* vZ = vf.data() + 2 * c.G3;
+        const double* vX = vf;
+        const double* vY = vf + c.G3;
+        const double* vZ = vf + 2 * c.G3;
         double du = 0.0;
         du += vX[c.at(i + 1, j, k)];
         du += vY[c.at(i, j + 1, k)];
@@ -447,11 +457,23 @@ int block_substep(const RC& c, double* U, const double* U0, const double* phi, c
   return err;
 }
 
+int block_substep(const RC& c, double* U, const double* U0, const double* phi, const double* x0, double h,
+                  double dt, double beta) {
+  int err = 0;
+  std::vector<double> F((size_t)NFR * c.G3), D((size_t)NFR * c.G3, 0.0), vf((size_t)3 * c.G3, 0.0);
+  for (int d = 0; d < 3; d++) {
+    err |= block_flux(c, U, x0, h, d, F.data(), vf.data() + (size_t)d * c.G3, nullptr);
+    block_sum_difs(c, F.data(), D.data(), d, h);
+  }
+  err |= block_update(c, U, U0, D.data(), vf.data(), phi, nullptr, x0, h, dt, beta);
+  return err;
+}
+
 // GridPhysBound (grid/grid_phys_bound.cpp:27-54, :60-169): a ghost copies the boundary cell next
 // to it and applies enforce_outflow at the boundary face; the lower z face mirrors with z_reflect
-void block_outflow_ghosts(const RC& c, double* U, const double* x0, double h) {
+void face_outflow_ghosts(const RC& c, double* U, const double* x0, double h, int nf) {
   const int G = c.G, g = c.g;
-  for (int nf = 0; nf < 6; nf++) {
+  {
     const int d = nf >> 1, up = nf & 1;
     for (int b = g; b < G - g; b++)
       for (int a = g; a < G - g; a++)
@@ -478,9 +500,434 @@ void block_outflow_ghosts(const RC& c, double* U, const double* x0, double h) {
   }
 }
 
+void block_outflow_ghosts(const RC& c, double* U, const double* x0, double h) {
+  for (int nf = 0; nf < 6; nf++) face_outflow_ghosts(c, U, x0, h, nf);
+}
+
+}  // namespace
+
+// ---- the rotating-frame state on the octree (BinaryDriver::sub_step, binary_driver.cpp:346-382) ----
+// GridNode::enforce_boundaries (grid/grid_node.cpp:59-101) with GridAMRBound
+// (grid/grid_amr_bound.cpp:27-78) and GridPhysBound (grid/grid_phys_bound.cpp:27-169);
+// compute_*_flux children first, adjust_*_flux (grid/grid_node.cpp:1076-1274), sum_*_difs with
+// sum_*_flow_off at the root (grid/grid_node.cpp:515-551), Binary::add_difs + Grid::add_difs
+// (binary/binary.cpp:205-254, grid/grid_node.cpp:337-357) with the Poisson face forces, and
+// inject_from_children with the rotating floor at the parent cell (grid/grid_node.cpp:1035-1074).
+struct orc_rot_tree {
+  struct RNode {
+    int level = 0, parent = -1, cidx = -1, child[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    int loc[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+    double h = 0.0, x0[3] = {0, 0, 0};
+    std::vector<double> U, U0, F, vf, D, phi, force;  // F: the face fluxes of x, y, z
+    double DFO[NFR] = {0}, FO[NFR] = {0}, FO0[NFR] = {0};
+  };
+  RC c;
+  double h0 = 0.0;
+  bool use_forces = false;
+  std::vector<RNode> nodes;
+  explicit orc_rot_tree(const orc_rot_cfg* cfg) : c(cfg) {}
+  int sibling(int b, int face) const {
+    const RNode& nd = nodes[b];
+    const int d = face >> 1;
+    int l[3] = {nd.loc[0], nd.loc[1], nd.loc[2]};
+    l[d] += (face & 1) ? 1 : -1;
+    if (l[d] < 0 || l[d] >= (1 << nd.level)) return -1;
+    for (int q = 0; q < (int)nodes.size(); q++)
+      if (nodes[q].level == nd.level && nodes[q].loc[0] == l[0] && nodes[q].loc[1] == l[1] && nodes[q].loc[2] == l[2])
+        return q;
+    return -1;
+  }
+  bool phys(int b, int face) const {  // oct_node/oct_node.cpp:86-110
+    const RNode& nd = nodes[b];
+    const int d = face >> 1;
+    return (face & 1) ? nd.loc[d] == (1 << nd.level) - 1 : nd.loc[d] == 0;
+  }
+  void postorder(int i, std::vector<int>& out) const {
+    for (int q = 0; q < 8; q++)
+      if (nodes[i].child[q] >= 0) postorder(nodes[i].child[q], out);
+    out.push_back(i);
+  }
+  void preorder(int i, std::vector<int>& out) const {
+    out.push_back(i);
+    for (int q = 0; q < 8; q++)
+      if (nodes[i].child[q] >= 0) preorder(nodes[i].child[q], out);
+  }
+  std::vector<int> level_order() const {
+    std::vector<int> ord;
+    int maxl = 0;
+    for (auto& nd : nodes) maxl = nd.level > maxl ? nd.level : maxl;
+    for (int l = 0; l <= maxl; l++)
+      for (int i = 0; i < (int)nodes.size(); i++)
+        if (nodes[i].level == l) ord.push_back(i);
+    return ord;
+  }
+};
+
+namespace {
+
+using RT = orc_rot_tree;
+
+// GridAMRBound::operator() for ghost (i, j, k) of node nd across a face of direction d: the
+// parent's cell with a quarter of the minmod slope along d, sign by the cell's parity
+void rot_amr_ghost(const RT* t, const RT::RNode& nd, int d, int i, int j, int k, double* u) {
+  const RC& c = t->c;
+  const RT::RNode& P = t->nodes[nd.parent];
+  const int idx[3] = {i, j, k};
+  int i1[3], par[3];
+  for (int e = 0; e < 3; e++) {
+    const int f = nd.off[e] + idx[e] - 2 * P.off[e];
+    par[e] = f % 2;
+    i1[e] = f / 2;
+  }
+  int p1[3] = {i1[0], i1[1], i1[2]}, m1[3] = {i1[0], i1[1], i1[2]};
+  p1[d] += 1;
+  m1[d] -= 1;
+  const long e0 = c.at(i1[0], i1[1], i1[2]), ep = c.at(p1[0], p1[1], p1[2]), em = c.at(m1[0], m1[1], m1[2]);
+  for (int l = 0; l < NFR; l++) {
+    const double uc = P.U[l * c.G3 + e0];
+    double m = minmod2(P.U[l * c.G3 + ep] - uc, uc - P.U[l * c.G3 + em]);
+    if (par[d] == 1) m = 0.0 - m;
+    u[l] = uc - m * 0.25;
+  }
+}
+
+void rot_node_ghosts(RT* t, int ni) {
+  const RC& c = t->c;
+  RT::RNode& nd = t->nodes[ni];
+  for (int face = 0; face < 6; face++) {
+    const int d = face >> 1;
+    const bool upper = face & 1;
+    const int s = t->sibling(ni, face);
+    if (s < 0 && t->phys(ni, face)) {
+      face_outflow_ghosts(c, nd.U.data(), nd.x0, nd.h, face);
+      continue;
+    }
+    for (int b = c.g; b < c.G - c.g; b++)
+      for (int a = c.g; a < c.G - c.g; a++)
+        for (int tt = 0; tt < c.g; tt++) {
+          const int gp = upper ? (c.G - c.g + tt) : tt;
+          const long eg = pencil_cell(c, d, a, b, gp);
+          double u[NFR];
+          if (s >= 0) {
+            // the sibling's interior cell at the same distance from the shared face
+            const int sp = upper ? (gp - c.G + 2 * c.g) : (c.G - 2 * c.g + gp);
+            const RT::RNode& S = t->nodes[s];
+            const long es = pencil_cell(c, d, a, b, sp);
+            for (int l = 0; l < NFR; l++) u[l] = S.U[l * c.G3 + es];
+          } else {
+            int ijk[3];
+            if (d == 0) ijk[0] = gp, ijk[1] = a, ijk[2] = b;
+            else if (d == 1) ijk[0] = a, ijk[1] = gp, ijk[2] = b;
+            else ijk[0] = a, ijk[1] = b, ijk[2] = gp;
+            rot_amr_ghost(t, nd, d, ijk[0], ijk[1], ijk[2], u);
+          }
+          for (int l = 0; l < NFR; l++) nd.U[l * c.G3 + eg] = u[l];
+        }
+  }
+}
+
+// GridNode::adjust_{x,y,z}_flux for one node: where exactly one side of a face plane (the node's
+// low face, its middle plane, its high face) is refined, the node's face fluxes there become the
+// means of the 2x2 fine face fluxes
+void rot_node_adjust_flux(RT* t, int ni, int d) {
+  const RC& c = t->c;
+  RT::RNode& nd = t->nodes[ni];
+  auto child_of = [&](int node, int cx, int cy, int cz) -> int {
+    if (node < 0) return -1;
+    return t->nodes[node].child[cx | (cy << 1) | (cz << 2)];
+  };
+  const int sL = t->sibling(ni, 2 * d), sU = t->sibling(ni, 2 * d + 1);
+  for (int l = 0; l < 3; l++)
+    for (int ck = 0; ck < 2; ck++)
+      for (int cj = 0; cj < 2; cj++) {
+        auto ci = [&](int node, int nbit) {
+          if (d == 0) return child_of(node, nbit, cj, ck);
+          if (d == 1) return child_of(node, cj, nbit, ck);
+          return child_of(node, cj, ck, nbit);
+        };
+        int cl, cr, i0;
+        if (l == 0) { cr = ci(ni, 0); cl = ci(sL, 1); i0 = c.g; }
+        else if (l == 1) { cl = ci(ni, 0); cr = ci(ni, 1); i0 = c.G / 2; }
+        else { cr = ci(sU, 0); cl = ci(ni, 1); i0 = c.G - c.g; }
+        int ch, i;
+        if (cr < 0 && cl >= 0) { ch = cl; i = c.G - c.g; }
+        else if (cl < 0 && cr >= 0) { ch = cr; i = c.g; }
+        else continue;
+        const RT::RNode& C = t->nodes[ch];
+        for (int k = c.g; k <= c.G - c.g - 1; k += 2)
+          for (int j = c.g; j <= c.G - c.g - 1; j += 2) {
+            const int k0 = (c.g + k) / 2 + ck * (c.G / 2 - c.g);
+            const int j0 = (c.g + j) / 2 + cj * (c.G / 2 - c.g);
+            for (int f = 0; f < NFR; f++) {
+              const double* CF = C.F.data() + ((size_t)d * NFR + f) * c.G3;
+              double v = CF[pencil_cell(c, d, j, k, i)];
+              v += CF[pencil_cell(c, d, j + 1, k, i)];
+              v += CF[pencil_cell(c, d, j, k + 1, i)];
+              v += CF[pencil_cell(c, d, j + 1, k + 1, i)];
+              v *= 0.25;
+              nd.F[((size_t)d * NFR + f) * c.G3 + pencil_cell(c, d, j0, k0, i0)] = v;
+            }
+          }
+      }
+}
+
+// Grid::sum_{x,y,z}_flow_off (grid/grid.cpp:350-404)
+void rot_flow_off(RT* t, RT::RNode& nd, int d) {
+  const RC& c = t->c;
+  for (int k = c.g; k < c.G - c.g; k++)
+    for (int j = c.g; j < c.G - c.g; j++)
+      for (int f = 0; f < NFR; f++) {
+        const double* F = nd.F.data() + ((size_t)d * NFR + f) * c.G3;
+        double tmp;
+        if (d == 2 && c.zref)
+          tmp = F[pencil_cell(c, d, j, k, c.G - c.g)] * nd.h * nd.h;
+        else
+          tmp = (F[pencil_cell(c, d, j, k, c.G - c.g)] - F[pencil_cell(c, d, j, k, c.g)]) * nd.h * nd.h;
+        nd.DFO[f] += tmp;
+      }
+}
+
+// GridNode::inject_from_children for one (parent, child) pair, floored at the parent cell
+int rot_inject_child(RT* t, int pi, int q) {
+  const RC& c = t->c;
+  RT::RNode& P = t->nodes[pi];
+  const RT::RNode& C = t->nodes[P.child[q]];
+  const int cx = q & 1, cy = (q >> 1) & 1, cz = (q >> 2) & 1;
+  int err = 0;
+  for (int k = c.g; k <= c.G - c.g - 1; k += 2)
+    for (int j = c.g; j <= c.G - c.g - 1; j += 2)
+      for (int i = c.g; i < c.G - c.g; i += 2) {
+        const int k0 = (c.g + k) / 2 + cz * (c.G / 2 - c.g);
+        const int j0 = (c.g + j) / 2 + cy * (c.G / 2 - c.g);
+        const int i0 = (c.g + i) / 2 + cx * (c.G / 2 - c.g);
+        double u[NFR];
+        for (int l = 0; l < NFR; l++) {
+          const double* f = C.U.data() + l * c.G3;
+          double s = 0.0;
+          s += f[c.at(i + 0, j + 0, k + 0)];
+          s += f[c.at(i + 1, j + 0, k + 0)];
+          s += f[c.at(i + 0, j + 1, k + 0)];
+          s += f[c.at(i + 1, j + 1, k + 0)];
+          s += f[c.at(i + 0, j + 0, k + 1)];
+          s += f[c.at(i + 1, j + 0, k + 1)];
+          s += f[c.at(i + 0, j + 1, k + 1)];
+          s += f[c.at(i + 1, j + 1, k + 1)];
+          s *= 0.125;
+          u[l] = s;
+        }
+        err |= floor_state(c, u, pos(c, P.x0, P.h, i0, j0, k0, -1));
+        for (int l = 0; l < NFR; l++) P.U[l * c.G3 + c.at(i0, j0, k0)] = u[l];
+      }
+  return err;
+}
+
 }  // namespace
 
 extern "C" {
+
+orc_rot_tree* orc_rot_tree_create(const orc_rot_cfg* cfg, double grid_dim, int nn, const int* parent,
+                                  const int* cidx) {
+  RC c(cfg);
+  if (c.G > MAXG || nn < 1 || parent[0] != -1) return nullptr;
+  orc_rot_tree* t = new orc_rot_tree(cfg);
+  t->h0 = 2.0 * grid_dim / double(c.n);  // grid/grid.cpp:70
+  const int P = c.n + 2;
+  t->nodes.resize(nn);
+  for (int b = 0; b < nn; b++) {
+    RT::RNode& nd = t->nodes[b];
+    if (b > 0) {
+      const int p = parent[b], q = cidx[b];
+      if (p < 0 || p >= b || q < 0 || q > 7 || t->nodes[p].child[q] >= 0) {
+        delete t;
+        return nullptr;
+      }
+      RT::RNode& Pn = t->nodes[p];
+      Pn.child[q] = b;
+      nd.parent = p;
+      nd.cidx = q;
+      nd.level = Pn.level + 1;
+      for (int d = 0; d < 3; d++) {
+        const int cv = (q >> d) & 1;
+        nd.loc[d] = 2 * Pn.loc[d] + cv;                       // oct_node/oct_node.cpp:177
+        nd.off[d] = 2 * Pn.off[d] + c.g + cv * (c.G - 2 * c.g);  // grid/grid_node.cpp:276
+      }
+      nd.h = Pn.h * 0.5;
+    } else {
+      nd.h = t->h0;
+    }
+    // low interior face, Grid::xf(bw) (grid/grid.cpp:512-526)
+    for (int d = 0; d < 3; d++) {
+      const double shift = (d == 2 && c.zref) ? 0.0 : grid_dim;
+      nd.x0[d] = double(nd.off[d] + c.g) * nd.h - shift - c.g * t->h0;
+    }
+    nd.U.assign((size_t)NFR * c.G3, 0.0);
+    nd.U0 = nd.U;
+    nd.D = nd.U;
+    nd.F.assign((size_t)3 * NFR * c.G3, 0.0);
+    nd.vf.assign((size_t)3 * c.G3, 0.0);
+    nd.phi.assign((size_t)P * P * P, 0.0);
+    nd.force.assign((size_t)3 * P * P * P, 0.0);
+  }
+  return t;
+}
+
+void orc_rot_tree_destroy(orc_rot_tree* t) { delete t; }
+
+double* orc_rot_tree_U(orc_rot_tree* t, int node) { return t->nodes[node].U.data(); }
+double* orc_rot_tree_phi(orc_rot_tree* t, int node) { return t->nodes[node].phi.data(); }
+double* orc_rot_tree_force(orc_rot_tree* t, int node) { return t->nodes[node].force.data(); }
+const double* orc_rot_tree_F(const orc_rot_tree* t, int node, int d) {
+  return t->nodes[node].F.data() + (size_t)d * NFR * t->c.G3;
+}
+
+int orc_rot_tree_node_x0(const orc_rot_tree* t, int node, double* x0, double* h) {
+  if (node < 0 || node >= (int)t->nodes.size()) return 1;
+  for (int d = 0; d < 3; d++) x0[d] = t->nodes[node].x0[d];
+  *h = t->nodes[node].h;
+  return 0;
+}
+
+void orc_rot_tree_use_forces(orc_rot_tree* t, int on) { t->use_forces = on != 0; }
+
+int orc_rot_tree_fill_ghosts(orc_rot_tree* t) {
+  // level order: a node reads same-level interiors, its own interior and its parent's cells with
+  // the parent's face ghosts, which the reference's pre-order recursion has filled
+  for (int ni : t->level_order()) rot_node_ghosts(t, ni);
+  return 0;
+}
+
+int orc_rot_tree_store(orc_rot_tree* t) {
+  for (auto& nd : t->nodes) {
+    nd.U0 = nd.U;
+    for (int l = 0; l < NFR; l++) nd.FO0[l] = nd.FO[l];
+  }
+  return 0;
+}
+
+// enforce_boundaries, the fluxes with reflux, the flux divergence, Binary::add_difs (sources, RK
+// combination, floor) and the potential energy from each node's phi; no restriction (the driver
+// solves for phi and restricts afterwards).  Returns the OR of the floor's EOS flags.
+int orc_rot_tree_substep(orc_rot_tree* t, double dt, double beta) {
+  const RC& c = t->c;
+  orc_rot_tree_fill_ghosts(t);
+  const int nn = (int)t->nodes.size();
+  for (auto& nd : t->nodes) {
+    std::fill(nd.D.begin(), nd.D.end(), 0.0);
+    for (int l = 0; l < NFR; l++) nd.DFO[l] = 0.0;
+  }
+  std::vector<int> po;
+  t->postorder(0, po);
+  int err = 0;
+  for (int d = 0; d < 3; d++) {
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+    for (int ni = 0; ni < nn; ni++) {
+      RT::RNode& nd = t->nodes[ni];
+      err |= block_flux(c, nd.U.data(), nd.x0, nd.h, d, nd.F.data() + (size_t)d * NFR * c.G3,
+                        nd.vf.data() + (size_t)d * c.G3, nullptr);
+    }
+    for (int ni : po) rot_node_adjust_flux(t, ni, d);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ni = 0; ni < nn; ni++)
+      block_sum_difs(c, t->nodes[ni].F.data() + (size_t)d * NFR * c.G3, t->nodes[ni].D.data(), d, t->nodes[ni].h);
+    rot_flow_off(t, t->nodes[0], d);
+  }
+  const int P = c.n + 2;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+  for (int ni = 0; ni < nn; ni++) {
+    RT::RNode& nd = t->nodes[ni];
+    const double* fp[3] = {nd.force.data(), nd.force.data() + (size_t)P * P * P, nd.force.data() + (size_t)2 * P * P * P};
+    err |= block_update(c, nd.U.data(), nd.U0.data(), nd.D.data(), nd.vf.data(), nd.phi.data(),
+                        t->use_forces ? fp : nullptr, nd.x0, nd.h, dt, beta);
+  }
+  RT::RNode& r = t->nodes[0];
+  for (int l = 0; l < NFR; l++) r.FO[l] = (r.FO[l] + r.DFO[l] * dt) * beta + r.FO0[l] * (1.0 - beta);
+  return err;
+}
+
+// Binary::pot_to_grid (binary/binary.cpp:825-849) from each node's phi
+int orc_rot_tree_pot_to_grid(orc_rot_tree* t) {
+  const RC& c = t->c;
+  for (auto& nd : t->nodes)
+    for (int k = c.g; k < c.G - c.g; k++)
+      for (int j = c.g; j < c.G - c.g; j++)
+        for (int i = c.g; i < c.G - c.g; i++) {
+          const X3 X = pos(c, nd.x0, nd.h, i, j, k, -1);
+          const long e = c.at(i, j, k);
+          const double d = nd.U[D0 * c.G3 + e];
+          const double R2 = X.x[0] * X.x[0] + X.x[1] * X.x[1];
+          nd.U[POT * c.G3 + e] = d * (phi_at(c, nd.phi.data(), i - c.g, j - c.g, k - c.g) - 0.5 * R2 * c.Omega0 * c.Omega0);
+        }
+  return 0;
+}
+
+int orc_rot_tree_inject(orc_rot_tree* t) {
+  std::vector<int> po;
+  t->postorder(0, po);
+  int err = 0;
+  for (int ni : po)
+    for (int q = 0; q < 8; q++)
+      if (t->nodes[ni].child[q] >= 0) err |= rot_inject_child(t, ni, q);
+  return err;
+}
+
+// GridNode::max_dt (grid/grid_node.cpp:163-174) over Grid::max_dt (grid/grid.cpp:422-483)
+static double rot_max_dt_rec(const orc_rot_tree* t, const std::vector<double>& eig, int ni) {
+  const auto& nd = t->nodes[ni];
+  double dtinv = 0.0;
+  for (int q = 0; q < 8; q++)
+    if (nd.child[q] >= 0) {
+      const double cdtinv = 1.0 / rot_max_dt_rec(t, eig, nd.child[q]);
+      dtinv = rmax(cdtinv, dtinv);
+    }
+  double own_inv = eig[ni] / nd.h;
+  if (own_inv == 0.0) own_inv = 1.0e-10;
+  const double own = (1.0 / own_inv) * t->c.cfl;
+  dtinv = rmax(1.0 / own, dtinv);
+  return 1.0 / dtinv;
+}
+
+double orc_rot_tree_max_dt(orc_rot_tree* t) {
+  const int nn = (int)t->nodes.size();
+  std::vector<double> eig(nn, 0.0);
+  orc_rot_tree_fill_ghosts(t);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ni = 0; ni < nn; ni++) {
+    double m = 0.0;
+    for (int d = 0; d < 3; d++)
+      block_flux(t->c, t->nodes[ni].U.data(), t->nodes[ni].x0, t->nodes[ni].h, d, nullptr, nullptr, &m);
+    eig[ni] = m;
+  }
+  return rot_max_dt_rec(t, eig, 0);
+}
+
+int orc_rot_tree_flow_off(const orc_rot_tree* t, double* fo) {
+  for (int l = 0; l < NFR; l++) fo[l] = t->nodes[0].FO[l];
+  return 0;
+}
+
+// State sum over the uncovered cells, weighted by h^3 (grid/grid_node.cpp:1489-1523)
+int orc_rot_tree_state_sum(const orc_rot_tree* t, double* sum) {
+  const RC& c = t->c;
+  for (int l = 0; l < NFR; l++) sum[l] = 0.0;
+  std::vector<int> pre;
+  t->preorder(0, pre);
+  for (int ni : pre) {
+    const auto& nd = t->nodes[ni];
+    const double h3 = std::pow(nd.h, 3);
+    for (int l = 0; l < NFR; l++) {
+      double a0 = 0.0;
+      for (int k = c.g; k < c.G - c.g; k++)
+        for (int j = c.g; j < c.G - c.g; j++)
+          for (int i = c.g; i < c.G - c.g; i++) {
+            const int q = (2 * i / c.G) | ((2 * j / c.G) << 1) | ((2 * k / c.G) << 2);
+            if (nd.child[q] >= 0) continue;
+            a0 += nd.U[l * c.G3 + c.at(i, j, k)] * h3;
+          }
+      sum[l] += a0;
+    }
+  }
+  return 0;
+}
 
 int orc_rot_batch_substep(const orc_rot_cfg* cfg, int nb, double* U, const double* U0, const double* phi,
                           const double* x0, const double* h, double dt, double beta, int* flags) {

@@ -322,6 +322,120 @@ def rot_state(cfg, op, u, X, F=None):
     return u, F, err
 
 
+class RotTree:
+    """The oracle's rotating-frame state on an octree given as (parent, child index) arrays.
+    U(node) is (13, G, G, G) [field, k, j, i]; phi(node) and the Poisson face forces
+    force(node) (3, n+2, n+2, n+2) are views into the oracle's arrays."""
+
+    def __init__(self, cfg, parent, child, grid_dim=1.0):
+        L = _rot_lib()
+        if not getattr(L, "_rot_tree_ready", False):
+            vp = C.c_void_p
+            L.orc_rot_tree_create.argtypes = [C.POINTER(OrcRotCfg), C.c_double, C.c_int, IP, IP]
+            L.orc_rot_tree_create.restype = vp
+            L.orc_rot_tree_destroy.argtypes = [vp]
+            L.orc_rot_tree_destroy.restype = None
+            for f in ("U", "phi", "force", "F"):
+                getattr(L, "orc_rot_tree_" + f).argtypes = [vp, C.c_int]
+                getattr(L, "orc_rot_tree_" + f).restype = DP
+            L.orc_rot_tree_F.argtypes = [vp, C.c_int, C.c_int]
+            L.orc_rot_tree_node_x0.argtypes = [vp, C.c_int, DP, DP]
+            L.orc_rot_tree_use_forces.argtypes = [vp, C.c_int]
+            L.orc_rot_tree_use_forces.restype = None
+            for f in ("fill_ghosts", "store", "pot_to_grid", "inject"):
+                getattr(L, "orc_rot_tree_" + f).argtypes = [vp]
+            L.orc_rot_tree_substep.argtypes = [vp, C.c_double, C.c_double]
+            L.orc_rot_tree_max_dt.argtypes = [vp]
+            L.orc_rot_tree_max_dt.restype = C.c_double
+            L.orc_rot_tree_flow_off.argtypes = [vp, DP]
+            L.orc_rot_tree_state_sum.argtypes = [vp, DP]
+            L._rot_tree_ready = True
+        self.L = L
+        self.cfg = cfg
+        self.n, self.g = cfg.n, cfg.g
+        self.G, self.P = cfg.n + 2 * cfg.g, cfg.n + 2
+        self.parent = np.ascontiguousarray(parent, np.int32)
+        self.child = np.ascontiguousarray(child, np.int32)
+        self.nn = len(self.parent)
+        self.t = L.orc_rot_tree_create(C.byref(cfg), grid_dim, self.nn, self.parent.ctypes.data_as(IP),
+                                       self.child.ctypes.data_as(IP))
+        if not self.t:
+            raise ValueError("bad rotating tree")
+
+    def __del__(self):
+        if getattr(self, "t", None):
+            self.L.orc_rot_tree_destroy(self.t)
+            self.t = None
+
+    def U(self, node):
+        G = self.G
+        return np.ctypeslib.as_array(self.L.orc_rot_tree_U(self.t, node), shape=(13, G, G, G))
+
+    def F(self, node, d):
+        """the face fluxes of direction d of the last substep; face p lies below cell p"""
+        G = self.G
+        return np.ctypeslib.as_array(self.L.orc_rot_tree_F(self.t, node, d), shape=(13, G, G, G))
+
+    def phi(self, node):
+        P = self.P
+        return np.ctypeslib.as_array(self.L.orc_rot_tree_phi(self.t, node), shape=(P, P, P))
+
+    def force(self, node):
+        P = self.P
+        return np.ctypeslib.as_array(self.L.orc_rot_tree_force(self.t, node), shape=(3, P, P, P))
+
+    def state(self):
+        return np.stack([self.U(b).copy() for b in range(self.nn)])
+
+    def set_state(self, U):
+        for b in range(self.nn):
+            self.U(b)[:] = U[b]
+
+    def set_phi(self, phi):
+        for b in range(self.nn):
+            self.phi(b)[:] = phi[b]
+
+    def set_forces(self, fx, fy, fz):
+        for b in range(self.nn):
+            self.force(b)[0], self.force(b)[1], self.force(b)[2] = fx[b], fy[b], fz[b]
+
+    def x0(self, node):
+        x0, h = np.zeros(3), C.c_double(0.0)
+        self.L.orc_rot_tree_node_x0(self.t, node, dptr(x0), C.byref(h))
+        return x0, h.value
+
+    def use_forces(self, on):
+        self.L.orc_rot_tree_use_forces(self.t, int(on))
+
+    def fill_ghosts(self):
+        self.L.orc_rot_tree_fill_ghosts(self.t)
+
+    def store(self):
+        self.L.orc_rot_tree_store(self.t)
+
+    def substep(self, dt, beta):
+        return self.L.orc_rot_tree_substep(self.t, dt, beta)
+
+    def pot_to_grid(self):
+        self.L.orc_rot_tree_pot_to_grid(self.t)
+
+    def inject(self):
+        return self.L.orc_rot_tree_inject(self.t)
+
+    def max_dt(self):
+        return self.L.orc_rot_tree_max_dt(self.t)
+
+    def flow_off(self):
+        fo = np.zeros(13)
+        self.L.orc_rot_tree_flow_off(self.t, dptr(fo))
+        return fo
+
+    def state_sum(self):
+        s = np.zeros(13)
+        self.L.orc_rot_tree_state_sum(self.t, dptr(s))
+        return s
+
+
 # ---- self-gravity: the multigrid Poisson solver (poisson/) --------------------------------------
 _poisson_ready = False
 
