@@ -107,6 +107,9 @@ SIGNATURES = {
     "amrh_set_block_origin": (C.c_int, [C.c_void_p, DP]),
     "amrh_set_potential": (C.c_int, [C.c_void_p, DP]),
     "amrh_set_potential_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "amrh_set_forces": (C.c_int, [C.c_void_p, DP]),
+    "amrh_set_forces_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "amrh_pot_to_grid": (C.c_int, [C.c_void_p]),
     "amrh_state_sum": (C.c_int, [C.c_void_p, DP]),
     "amrh_state_max": (C.c_int, [C.c_void_p, DP]),
     "amrh_state_min": (C.c_int, [C.c_void_p, DP]),
@@ -124,6 +127,7 @@ SIGNATURES = {
     "amrh_poisson_upload": (C.c_int, [C.c_void_p, C.c_int, DP]),
     "amrh_poisson_download": (C.c_int, [C.c_void_p, C.c_int, DP]),
     "amrh_poisson_phi_device": (C.c_void_p, [C.c_void_p]),
+    "amrh_poisson_forces_device": (C.c_void_p, [C.c_void_p]),
     "amrh_poisson_source_from_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
     "amrh_poisson_prepare": (C.c_int, [C.c_void_p]),
     "amrh_poisson_boundary_tables": (C.c_int, [C.c_void_p, DP, C.c_int]),
@@ -309,6 +313,23 @@ class Batch:
         """phi already on the device, e.g. Poisson.phi_device()"""
         _ck(lib().amrh_set_potential_device(self.ctx, C.c_void_p(ptr)))
 
+    def set_forces(self, f):
+        """Poisson face forces (3, nb, n+2, n+2, n+2) in place of phi's differences; None: phi's"""
+        if f is None:
+            _ck(lib().amrh_set_forces(self.ctx, None))
+            return
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        P = self.cfg.n + 2
+        assert f.shape == (3, self.nb, P, P, P), f.shape
+        _ck(lib().amrh_set_forces(self.ctx, _dp(f)))
+
+    def set_forces_device(self, ptr):
+        """face forces already on the device, e.g. Poisson.forces_device(); 0/None: phi's"""
+        _ck(lib().amrh_set_forces_device(self.ctx, C.c_void_p(ptr) if ptr else None))
+
+    def pot_to_grid(self):
+        _ck(lib().amrh_pot_to_grid(self.ctx))
+
     def attach_eos(self, eos):
         self._eos = eos  # the table must outlive the context
         _ck(lib().amrh_attach_eos(self.ctx, eos.h))
@@ -486,6 +507,9 @@ class Poisson:
 
     def phi_device(self):
         return lib().amrh_poisson_phi_device(self.h)
+
+    def forces_device(self):
+        return lib().amrh_poisson_forces_device(self.h)
 
     def source_from(self, batch, G=1.0):
         _ck(lib().amrh_poisson_source_from_state(self.h, batch.ctx, float(G)))

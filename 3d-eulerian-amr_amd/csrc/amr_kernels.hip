@@ -272,27 +272,27 @@ __global__ __launch_bounds__(256) void restrict_kernel(double* U, const int2* pa
   for (int l = 0; l < NFE; l++) Up[l * R + e] = u[l];
 }
 
-template <int N>
+template <int N, int NFL>
 __global__ void ledger_kernel(const double* ledger, double h, const double* dt_dev, double dt, double beta,
                               int z_reflect, double* fo /* [FO | FO0] */) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (dt_dev) dt = *dt_dev;
-  double dfo[NFE];
-  for (int l = 0; l < NFE; l++) dfo[l] = 0.0;
+  double dfo[NFL];
+  for (int l = 0; l < NFL; l++) dfo[l] = 0.0;
   for (int d = 0; d < 3; d++)
     for (int b = 0; b < N; b++)
       for (int a = 0; a < N; a++)
-        for (int l = 0; l < NFE; l++) {
-          const double lo = ledger[(((d * 2 + 0) * N + b) * N + a) * NFE + l];
-          const double up = ledger[(((d * 2 + 1) * N + b) * N + a) * NFE + l];
+        for (int l = 0; l < NFL; l++) {
+          const double lo = ledger[(((d * 2 + 0) * N + b) * N + a) * NFL + l];
+          const double up = ledger[(((d * 2 + 1) * N + b) * N + a) * NFL + l];
           const double tmp = (d == 2 && z_reflect) ? up * h * h : (up - lo) * h * h;
           dfo[l] += tmp;
         }
-  for (int l = 0; l < NFE; l++) fo[l] = (fo[l] + dfo[l] * dt) * beta + fo[NFE + l] * (1.0 - beta);
+  for (int l = 0; l < NFL; l++) fo[l] = (fo[l] + dfo[l] * dt) * beta + fo[NFL + l] * (1.0 - beta);
 }
 
-__global__ void ledger_store_kernel(double* fo) {
-  if (threadIdx.x < NFE) fo[NFE + threadIdx.x] = fo[threadIdx.x];
+__global__ void ledger_store_kernel(double* fo, int nf) {
+  if (threadIdx.x < nf) fo[nf + threadIdx.x] = fo[threadIdx.x];
 }
 
 }  // namespace
@@ -343,19 +343,21 @@ hipError_t launch_restrict(int n, double* U, const int2* pairs, int count, const
   return hipGetLastError();
 }
 
-hipError_t launch_ledger(int n, const double* ledger, double h, const double* dt_dev, double dt, double beta,
+hipError_t launch_ledger(int n, int nf, const double* ledger, double h, const double* dt_dev, double dt, double beta,
                          int z_reflect, double* fo, hipStream_t s) {
-  if (n == 8)
-    ledger_kernel<8><<<1, 64, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
-  else if (n == 16)
-    ledger_kernel<16><<<1, 64, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
+  if (n == 8 && nf == NFE)
+    ledger_kernel<8, NFE><<<1, 64, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
+  else if (n == 16 && nf == NFE)
+    ledger_kernel<16, NFE><<<1, 64, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
+  else if (n == 8 && nf == 13)
+    ledger_kernel<8, 13><<<1, 64, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_ledger_store(double* fo, hipStream_t s) {
-  ledger_store_kernel<<<1, 64, 0, s>>>(fo);
+hipError_t launch_ledger_store(double* fo, int nf, hipStream_t s) {
+  ledger_store_kernel<<<1, 64, 0, s>>>(fo, nf);
   return hipGetLastError();
 }
 

@@ -136,6 +136,7 @@ void amrh_destroy(amrh_ctx* c) {
   if (c->x0_dev) (void)hipFree(c->x0_dev);
   if (c->diag_part) (void)hipFree(c->diag_part);
   if (c->phi_dev) (void)hipFree(c->phi_dev);
+  if (c->force_own) (void)hipFree(c->force_own);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (c->tree) {
     c->tree->free_dev();
@@ -212,7 +213,7 @@ int amrh_store(amrh_ctx* c) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
   c->u0buf = c->cur;
-  if (c->tree) HIPCHK(launch_ledger_store(c->tree->fo_dev, c->stream));  // FO0 = FO
+  if (c->tree) HIPCHK(launch_ledger_store(c->tree->fo_dev, c->cfg.nf, c->stream));  // FO0 = FO
   return AMRH_OK;
 }
 
@@ -256,6 +257,37 @@ int amrh_set_potential_device(amrh_ctx* c, const void* phi) {
   return AMRH_OK;
 }
 
+int amrh_set_forces(amrh_ctx* c, const double* f) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "the forces belong to ROTATING physics");
+  if (!f) {
+    c->force_dev = nullptr;
+    return AMRH_OK;
+  }
+  if (!c->force_own) HIPCHK(hipMalloc(&c->force_own, 3 * phi_bytes(c)));
+  HIPCHK(hipMemcpyAsync(c->force_own, f, 3 * phi_bytes(c), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->force_dev = c->force_own;
+  return AMRH_OK;
+}
+
+int amrh_set_forces_device(amrh_ctx* c, const void* f) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "the forces belong to ROTATING physics");
+  c->force_dev = (const double*)f;
+  return AMRH_OK;
+}
+
+int amrh_pot_to_grid(amrh_ctx* c) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "pot belongs to ROTATING physics");
+  if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
+  if (int rc = ctx_ready(c)) return rc;
+  HIPCHK(launch_rot_pot_to_grid(c->cfg.g, c->buf[c->cur], c->phi_dev, c->x0_dev, c->h_dev, c->nb,
+                                ctx_rot_args(c).c, c->stream));
+  return AMRH_OK;
+}
+
 int amrh_attach_eos(amrh_ctx* c, amrh_eos* eos) {
   if (!c || !eos) return fail(AMRH_EINVAL, "null argument");
   if (c->cfg.physics != AMRH_PHYS_ROTATING || c->cfg.eos != AMRH_EOS_HELMHOLTZ)
@@ -280,9 +312,13 @@ rot::RotParams rot_params(const amrh_config& g) {
   return r;
 }
 
-RotArgs rot_args(const amrh_ctx* c) {
+}  // namespace
+
+RotArgs amrh::ctx_rot_args(const amrh_ctx* c) {
   RotArgs r{};
   r.phi = c->phi_dev;
+  r.force = c->force_dev;
+  r.nforce = c->nb;
   r.x0 = c->x0_dev;
   r.h = c->h_dev;
   r.nblocks = c->nb;
@@ -292,6 +328,9 @@ RotArgs rot_args(const amrh_ctx* c) {
   r.H.err = c->err_dev;
   return r;
 }
+
+namespace {
+RotArgs rot_args(const amrh_ctx* c) { return amrh::ctx_rot_args(c); }
 }  // namespace
 
 int amrh::ctx_ready(const amrh_ctx* c) {

@@ -10,6 +10,7 @@
 #include "amrhydro.h"
 #include "euler_phys.h"
 #include "kt_kernels.h"
+#include "kt_rot.h"
 
 namespace amrh {
 
@@ -30,6 +31,7 @@ struct TreeState {
   int* rel_dev = nullptr;
   int* level_nodes_dev = nullptr;
   int* slot_dev = nullptr;
+  int* exp_dev = nullptr;  // [node][face][2]: the override slots its boundary-face means go to, or -1
   double* ovr_dev = nullptr;
   ExportItem* items_dev = nullptr;
   int2* pairs_dev = nullptr;
@@ -60,6 +62,8 @@ struct amrh_ctx {
   // AMRH_PHYS_ROTATING
   double* x0_dev = nullptr;    // [nb][3] block origins
   double* phi_dev = nullptr;   // [nb][(n+2)^3] potential
+  const double* force_dev = nullptr;  // Poisson face forces fx | fy | fz, [3][nb][(n+2)^3] (nullptr: phi's)
+  double* force_own = nullptr;        // the context's copy of host-given forces
   const amrh_eos* eos = nullptr;
   bool have_x0 = false, have_phi = false;
   double* diag_part = nullptr;  // state_sum/max/min partials (allocated on first use)
@@ -84,6 +88,8 @@ hipError_t ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid);
 int ctx_ready(const amrh_ctx* c);
 // ghost fill of isolated outflow blocks for the context's physics
 hipError_t ctx_outflow_ghosts(const amrh_ctx* c, double* U);
+// the rotating kernels' arguments common to every launch: potential, forces, positions, constants
+RotArgs ctx_rot_args(const amrh_ctx* c);
 
 }  // namespace amrh
 

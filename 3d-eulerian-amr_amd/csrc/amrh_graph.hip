@@ -69,6 +69,9 @@ int step_once(amrh_ctx* c, int rk, double* dt_dev) {
     const double beta = kBetas[rk - 1][s];
     const int rc = c->tree ? amrh_tree_substep_dev(c, dt_dev, beta) : amrh_substep_dev(c, dt_dev, beta);
     if (rc) return rc;
+    // the rotating state's substep leaves the restriction to the driver, after the potential
+    if (c->tree && c->cfg.physics == AMRH_PHYS_ROTATING)
+      if (int rc2 = amrh_tree_inject(c)) return rc2;
   }
   return AMRH_OK;
 }

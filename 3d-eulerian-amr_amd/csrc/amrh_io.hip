@@ -115,7 +115,8 @@ int amrh_tree_output_mesh(amrh_ctx* c, double* x, double* y, double* z, int* nod
   long npts, nz;
   if (int rc = amrh_tree_output_counts(c, &npts, &nz)) return rc;
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
-  if (c->cfg.g != 2 || (c->cfg.n != 8 && c->cfg.n != 16)) return fail(AMRH_EINVAL, "n = 8 or 16 with g = 2 only");
+  if (!((c->cfg.g == 2 && (c->cfg.n == 8 || c->cfg.n == 16)) || (c->cfg.g == 3 && c->cfg.n == 8)))
+    return fail(AMRH_EINVAL, "n = 8 or 16 with g = 2, or n = 8 with g = 3");
   const TreeState* t = c->tree;
   const int n = c->cfg.n, g = c->cfg.g, GNX = n + 2 * g, G1 = GNX + 1, H = n / 2;
   const std::vector<int> order = preorder(t);
@@ -167,7 +168,10 @@ int amrh_tree_output_mesh(amrh_ctx* c, double* x, double* y, double* z, int* nod
   if (e == hipSuccess)
     e = hipMemcpyAsync(z0_dev, zone0.data(), sizeof(long) * order.size(), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
-    if (n == 8)
+    if (n == 8 && g == 3)
+      zone_gather_kernel<8, 3><<<(unsigned)order.size(), 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, nodes_dev,
+                                                                              z0_dev, t->covered_dev, nz, out_dev);
+    else if (n == 8)
       zone_gather_kernel<8, 2><<<(unsigned)order.size(), 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, nodes_dev,
                                                                               z0_dev, t->covered_dev, nz, out_dev);
     else
@@ -211,12 +215,16 @@ int amrh_tree_output_vtk(amrh_ctx* c, const char* path, double time) {
   ok = ok && fprintf(fp, "\nCELL_TYPES %ld\n", nz) > 0;
   for (long q = 0; q < nz && ok; q++) put_be(fp, &hex, 4, &ok);
   ok = ok && fprintf(fp, "\nCELL_DATA %ld\n", nz) > 0;
-  // State::field_name (euler/state.cpp:24-42); fields beyond the ideal gas's six are numbered
+  // State::field_name (euler/state.cpp:24-42, binary/state_ztwd.cpp:58-89)
   static const char* names[6] = {"d", "sx", "sy", "sz", "et", "tau"};
+  static const char* rot_names[13] = {"d_tot", "d_acc", "d_don", "sr", "lz", "sz", "pot",
+                                      "et",    "tau",   "T",     "s1", "s2", "s3"};
   for (int l = 0; l < nf && ok; l++) {
     char nm[16];
     if (c->cfg.physics == AMRH_PHYS_EULER && l < 6)
       snprintf(nm, sizeof nm, "%s", names[l]);
+    else if (c->cfg.physics == AMRH_PHYS_ROTATING && l < 13)
+      snprintf(nm, sizeof nm, "%s", rot_names[l]);
     else
       snprintf(nm, sizeof nm, "f%d", l);
     ok = ok && fprintf(fp, "SCALARS %s double 1\nLOOKUP_TABLE default\n", nm) > 0;

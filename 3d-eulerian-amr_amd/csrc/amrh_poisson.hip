@@ -1091,6 +1091,7 @@ int amrh_poisson_download(amrh_poisson* p, int field, double* host) {
 }
 
 const void* amrh_poisson_phi_device(const amrh_poisson* p) { return p ? p->fields + (long)PHI * p->nn * p->P3 : nullptr; }
+const void* amrh_poisson_forces_device(const amrh_poisson* p) { return p ? p->fields + (long)FX * p->nn * p->P3 : nullptr; }
 
 int amrh_poisson_source_from_state(amrh_poisson* p, const amrh_ctx* c, double G) {
   if (!p || !c) return fail(AMRH_EINVAL, "null argument");

@@ -108,8 +108,10 @@ int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index) {
   if (!src->have_u) return fail(AMRH_ESTATE, "the source holds no state");
   if (dst->cfg.n != src->cfg.n || dst->cfg.g != src->cfg.g || dst->cfg.nf != src->cfg.nf)
     return fail(AMRH_EINVAL, "block shapes differ");
+  if (dst->cfg.physics != AMRH_PHYS_EULER || src->cfg.physics != AMRH_PHYS_EULER)
+    return fail(AMRH_EINVAL, "re-packing prolongs the ideal-gas state (GridNode::inject_from_parent) only");
   const TreeState* t = dst->tree;
-  const int nb = dst->nb, n = dst->cfg.n;
+  const int nb = dst->nb, n = dst->cfg.n, nf = dst->cfg.nf;
   std::vector<int> idx(src_index, src_index + nb);
   // A regrid adds at most one level below a surviving node (create_child clears the new node's
   // refine flags, grid/grid_node.cpp:295-297), and the new nodes read their parents' ghost cells as
@@ -135,7 +137,7 @@ int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index) {
     gather_kernel<<<dim3(8, nb < 65535 ? nb : 65535), 256, 0, dst->stream>>>(dst->buf[dst->cur], src->buf[src->cur], idx_dev, nb, be);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(t->fo_dev, src->tree->fo_dev, sizeof(double) * 12, hipMemcpyDeviceToDevice,
+  if (e == hipSuccess) e = hipMemcpyAsync(t->fo_dev, src->tree->fo_dev, sizeof(double) * 2 * nf, hipMemcpyDeviceToDevice,
                                           dst->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(dst->stream);
   (void)hipFree(idx_dev);

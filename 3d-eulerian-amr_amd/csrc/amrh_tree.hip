@@ -19,9 +19,10 @@ using namespace amrh;
 
 void TreeState::free_dev() {
   for (void* p : {(void*)ghost_dev, (void*)rel_dev, (void*)level_nodes_dev, (void*)slot_dev, (void*)ovr_dev,
-                  (void*)items_dev, (void*)pairs_dev, (void*)ledger_dev, (void*)fo_dev, (void*)covered_dev})
+                  (void*)items_dev, (void*)pairs_dev, (void*)ledger_dev, (void*)fo_dev, (void*)covered_dev,
+                  (void*)exp_dev})
     if (p) (void)hipFree(p);
-  ghost_dev = rel_dev = level_nodes_dev = slot_dev = nullptr;
+  ghost_dev = rel_dev = level_nodes_dev = slot_dev = exp_dev = nullptr;
   ovr_dev = ledger_dev = fo_dev = nullptr;
   covered_dev = nullptr;
   items_dev = nullptr;
@@ -40,7 +41,7 @@ hipError_t upload_vec(T** dst, const std::vector<T>& v) {
 }
 
 int build(amrh_ctx* c, const int* parent, const int* child) {
-  const int n = c->cfg.n, g = c->cfg.g, G = n + 2 * g;
+  const int n = c->cfg.n, g = c->cfg.g, G = n + 2 * g, nf = c->cfg.nf;
   TreeState* t = new TreeState;
   t->nn = c->nb;
   t->parent.assign(t->nn, -1);
@@ -154,6 +155,22 @@ int build(amrh_ctx* c, const int* parent, const int* child) {
             by_level[t->level[fine]].push_back(ExportItem{fine, d, side, s});
           }
     }
+  // per node and face: the slots its boundary-face means go to -- the parent's and, across a
+  // parent's face, the parent's neighbour's (both adjust their face there)
+  std::vector<int> exp_slot(t->nn * 12, -1);
+  for (int l = 0; l <= t->maxlevel; l++)
+    for (const ExportItem& it : by_level[l]) {
+      int* e2 = &exp_slot[(it.fine * 6 + 2 * it.dir + it.side) * 2];
+      if (e2[0] < 0) {
+        e2[0] = it.slot;
+      } else if (e2[1] < 0) {
+        e2[1] = it.slot;
+      } else {
+        delete t;
+        return fail(AMRH_EINVAL, "node %d face %d: more than two coarse faces take its fluxes", it.fine,
+                    2 * it.dir + it.side);
+      }
+    }
   std::vector<ExportItem> items;
   t->item_start.assign(t->maxlevel + 1, 0);
   t->item_count.assign(t->maxlevel + 1, 0);
@@ -179,6 +196,7 @@ int build(amrh_ctx* c, const int* parent, const int* child) {
   if (e == hipSuccess) e = upload_vec(&t->level_nodes_dev, t->level_nodes);
   if (e == hipSuccess) e = upload_vec(&t->slot_dev, slot);
   if (e == hipSuccess) e = upload_vec(&t->items_dev, items);
+  if (e == hipSuccess) e = upload_vec(&t->exp_dev, exp_slot);
   if (e == hipSuccess) e = upload_vec(&t->pairs_dev, pairs);
   std::vector<unsigned char> covered(t->nn, 0);
   for (int b = 0; b < t->nn; b++)
@@ -186,11 +204,11 @@ int build(amrh_ctx* c, const int* parent, const int* child) {
       if (t->child[b][ci] >= 0) covered[b] |= (unsigned char)(1u << ci);
   if (e == hipSuccess) e = upload_vec(&t->covered_dev, covered);
   const long half = n / 2;
-  if (e == hipSuccess) e = hipMalloc(&t->ovr_dev, sizeof(double) * std::max<long>(1, t->nslots) * half * half * 6);
-  if (e == hipSuccess) e = hipMalloc(&t->ledger_dev, sizeof(double) * 3 * 2 * n * n * 6);
-  if (e == hipSuccess) e = hipMalloc(&t->fo_dev, sizeof(double) * 12);
-  if (e == hipSuccess) e = hipMemset(t->fo_dev, 0, sizeof(double) * 12);
-  if (e == hipSuccess) e = hipMemset(t->ledger_dev, 0, sizeof(double) * 3 * 2 * n * n * 6);
+  if (e == hipSuccess) e = hipMalloc(&t->ovr_dev, sizeof(double) * std::max<long>(1, t->nslots) * half * half * nf);
+  if (e == hipSuccess) e = hipMalloc(&t->ledger_dev, sizeof(double) * 3 * 2 * n * n * nf);
+  if (e == hipSuccess) e = hipMalloc(&t->fo_dev, sizeof(double) * 2 * nf);
+  if (e == hipSuccess) e = hipMemset(t->fo_dev, 0, sizeof(double) * 2 * nf);
+  if (e == hipSuccess) e = hipMemset(t->ledger_dev, 0, sizeof(double) * 3 * 2 * n * n * nf);
   if (e != hipSuccess) {
     t->free_dev();
     delete t;
@@ -206,14 +224,30 @@ int build(amrh_ctx* c, const int* parent, const int* child) {
 
 int fill_ghosts(amrh_ctx* c) {
   TreeState* t = c->tree;
-  for (int l = 0; l <= t->maxlevel; l++)
-    HIPCHK(launch_tree_ghosts(c->cfg.n, c->buf[c->cur], t->level_nodes_dev + t->level_start[l], t->level_count[l],
-                              t->ghost_dev, t->rel_dev, c->cfg.z_reflect, c->err_dev, c->stream));
+  const bool rot = c->cfg.physics == AMRH_PHYS_ROTATING;
+  const rot::RotParams rp = rot ? ctx_rot_args(c).c : rot::RotParams{};
+  for (int l = 0; l <= t->maxlevel; l++) {
+    const int* nodes = t->level_nodes_dev + t->level_start[l];
+    if (rot)
+      HIPCHK(launch_rot_tree_ghosts(c->cfg.g, c->buf[c->cur], nodes, t->level_count[l], t->ghost_dev, t->rel_dev,
+                                    c->x0_dev, c->h_dev, c->cfg.z_reflect, rp, c->err_dev, c->stream));
+    else
+      HIPCHK(launch_tree_ghosts(c->cfg.n, c->buf[c->cur], nodes, t->level_count[l], t->ghost_dev, t->rel_dev,
+                                c->cfg.z_reflect, c->err_dev, c->stream));
+  }
   return AMRH_OK;
 }
 
 int inject(amrh_ctx* c) {
   TreeState* t = c->tree;
+  if (c->cfg.physics == AMRH_PHYS_ROTATING) {
+    if (c->cfg.eos == AMRH_EOS_HELMHOLTZ && !c->eos) return fail(AMRH_ESTATE, "eos = HELMHOLTZ needs amrh_attach_eos");
+    const RotArgs a = ctx_rot_args(c);
+    for (int l = t->maxlevel - 1; l >= 0; l--)
+      HIPCHK(launch_rot_restrict(c->cfg.g, c->cfg.eos, c->buf[c->cur], t->pairs_dev + t->pair_start[l],
+                                 t->pair_count[l], c->x0_dev, c->h_dev, a.c, a.H, c->stream));
+    return AMRH_OK;
+  }
   const PhysParams P = params(c->cfg);
   for (int l = t->maxlevel - 1; l >= 0; l--)
     HIPCHK(launch_restrict(c->cfg.n, c->buf[c->cur], t->pairs_dev + t->pair_start[l], t->pair_count[l], P,
@@ -221,8 +255,43 @@ int inject(amrh_ctx* c) {
   return AMRH_OK;
 }
 
+// BinaryDriver::sub_step's hydro part (binary/binary_driver.cpp:346-372) on the octree: ghost fill,
+// then the rotating substep level by level, finest first, each level applying the reflux means its
+// finer neighbours exported and exporting its own boundary faces' means to its coarser neighbours;
+// the root's ledger; no restriction (the driver restricts after the potential is in place)
+int rot_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
+  TreeState* t = c->tree;
+  if (int rc = ctx_ready(c)) return rc;
+  if (beta != 1.0 && c->u0buf < 0) return fail(AMRH_ESTATE, "beta != 1 needs amrh_store() first");
+  if (int rc = fill_ghosts(c)) return rc;
+  RotArgs a = ctx_rot_args(c);
+  a.Uin = c->buf[c->cur];
+  int out = c->cur;
+  if (c->u0buf == c->cur) out = 1 - c->cur;  // never overwrite the RK base
+  a.Uout = c->buf[out];
+  a.U0 = c->u0buf >= 0 ? c->buf[c->u0buf] : c->buf[c->cur];
+  a.dt_dev = dt_dev;
+  a.dt = dt;
+  a.beta = beta;
+  a.slot = t->slot_dev;
+  a.ovr = t->ovr_dev;
+  a.exp_slot = t->exp_dev;
+  a.ledger = t->ledger_dev;
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  for (int l = t->maxlevel; l >= 0; l--) {
+    a.nodes = t->level_nodes_dev + t->level_start[l];
+    a.nblocks = t->level_count[l];
+    HIPCHK(launch_rot_substep(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, a, 0, c->stream));
+  }
+  c->cur = out;
+  HIPCHK(launch_ledger(c->cfg.n, c->cfg.nf, t->ledger_dev, t->h[0], dt_dev, dt, beta, c->cfg.z_reflect, t->fo_dev,
+                       c->stream));
+  return AMRH_OK;
+}
+
 int tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
   TreeState* t = c->tree;
+  if (c->cfg.physics == AMRH_PHYS_ROTATING) return rot_tree_substep(c, dt, beta, dt_dev);
   if (int rc = fill_ghosts(c)) return rc;
   const PhysParams P = params(c->cfg);
   const bool mixed = c->cfg.precision == AMRH_MIXED;
@@ -230,7 +299,7 @@ int tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
     HIPCHK(launch_reflux_export(c->cfg.n, mixed, c->buf[c->cur], t->items_dev + t->item_start[l],
                                 t->item_count[l], t->slot_dev, t->ovr_dev, P, c->stream));
   if (int rc = substep_core(c, dt, beta, dt_dev, true)) return rc;
-  HIPCHK(launch_ledger(c->cfg.n, t->ledger_dev, t->h[0], dt_dev, dt, beta, c->cfg.z_reflect, t->fo_dev,
+  HIPCHK(launch_ledger(c->cfg.n, c->cfg.nf, t->ledger_dev, t->h[0], dt_dev, dt, beta, c->cfg.z_reflect, t->fo_dev,
                        c->stream));
   return inject(c);
 }
@@ -261,8 +330,10 @@ extern "C" {
 
 int amrh_tree_set(amrh_ctx* c, const int* parent, const int* child) {
   if (!c || !parent || !child) return fail(AMRH_EINVAL, "null argument");
-  if (c->cfg.recon != AMRH_RECON_MINMOD || c->cfg.physics != AMRH_PHYS_EULER)
-    return fail(AMRH_EINVAL, "the octree coupling runs the ideal-gas minmod scheme (g = 2) only");
+  const bool euler_ok = c->cfg.physics == AMRH_PHYS_EULER && c->cfg.recon == AMRH_RECON_MINMOD;
+  const bool rot_ok = c->cfg.physics == AMRH_PHYS_ROTATING;  // n = 8, minmod g = 2 or PPM g = 3
+  if (!euler_ok && !rot_ok)
+    return fail(AMRH_EINVAL, "the octree coupling runs the ideal-gas minmod scheme or the rotating-frame state");
   if (int rc = build(c, parent, child)) return rc;
   HIPCHK(hipMemcpy(c->h_dev, c->tree->h.data(), sizeof(double) * c->nb, hipMemcpyHostToDevice));
   c->have_h = true;
@@ -319,7 +390,7 @@ int amrh_tree_max_dt(amrh_ctx* c, double* dt) {
 int amrh_tree_flow_off(amrh_ctx* c, double* fo) {
   if (!c || !fo) return fail(AMRH_EINVAL, "null argument");
   if (!c->tree) return fail(AMRH_ESTATE, "tree not set");
-  HIPCHK(hipMemcpyAsync(fo, c->tree->fo_dev, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(fo, c->tree->fo_dev, sizeof(double) * c->cfg.nf, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return AMRH_OK;
 }

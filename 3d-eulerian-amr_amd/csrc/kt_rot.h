@@ -22,11 +22,31 @@ struct RotArgs {
   helm::HelmDev H;        // the Helmholtz table (err points at the context's flags)
   double* block_eig;      // CFL scan: per-block max eigenvalue (nullptr = skip)
   unsigned long long* gmax;  // CFL scan: max over blocks of eig/h as ordered bits (nullptr = skip)
+  // Poisson face forces (nullptr: phi's differences): fx of every block, then fy, then fz, each
+  // block (n+2)^3 with face i between cells i-1 and i; nforce = blocks per component
+  const double* force;
+  long nforce;
+  // octree (nodes != nullptr): the launch's blocks are nodes[0 .. nblocks)
+  const int* nodes;
+  const int* slot;      // reflux override slot per [node][dir][plane 0, n/2, n][quadrant] or -1
+  double* ovr;          // override face fluxes [slot][n/2][n/2][13]: read, and written by exports
+  const int* exp_slot;  // [node][face][2]: the slots this node's boundary-face 2x2 means go to, or -1
+  double* ledger;       // the root's boundary face fluxes [dir][low, high][n][n][13] (nullptr: none)
 };
 
 hipError_t launch_rot_substep(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
 hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
 hipError_t launch_rot_ghosts(int g, double* U, const double* x0, const double* h, int nblocks, int z_reflect,
                              const rot::RotParams& c, hipStream_t s);
+// the octree's ghost fill of one level (ghost/rel: the tables of amr_kernels.h)
+hipError_t launch_rot_tree_ghosts(int g, double* U, const int* nodes, int count, const int* ghost, const int* rel,
+                                  const double* x0, const double* h, int z_reflect, const rot::RotParams& c, int* err,
+                                  hipStream_t s);
+// inject_from_children for one parent level: pairs (parent, child << 24 | child node)
+hipError_t launch_rot_restrict(int g, int eos, double* U, const int2* pairs, int count, const double* x0,
+                               const double* h, const rot::RotParams& c, const helm::HelmDev& H, hipStream_t s);
+// Binary::pot_to_grid on every block: pot = rho (phi - R^2 Omega0^2 / 2)
+hipError_t launch_rot_pot_to_grid(int g, double* U, const double* phi, const double* x0, const double* h,
+                                  int nblocks, const rot::RotParams& c, hipStream_t s);
 
 }  // namespace amrh

@@ -19,7 +19,9 @@
 
 #include <type_traits>
 
+#include "amr_kernels.h"
 #include "amrh_internal.h"
+#include "amrh_layout.h"
 #include "amrhydro.h"
 #include "kt_rot.h"
 #include "recon.h"
@@ -98,7 +100,8 @@ __device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int 
 
 // MIXED: the staged primitives, the reconstruction, the face states' EOS and fluxes and the KT
 // combination in FP32 (rot::sp); the flux differences, sources, RK update and floor in FP64.
-template <int G, int RECON, int EOS, int MODE, bool MIXED>
+// TREE: the octree's reflux overrides, exports and the root's ledger (a.slot, a.exp_slot, a.ledger)
+template <int G, int RECON, int EOS, int MODE, bool MIXED, bool TREE>
 __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   using T = typename std::conditional<MIXED, float, double>::type;
   constexpr int RS = N3 + 2 * G * N2;  // staged elements per field
@@ -118,7 +121,8 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   const bool has_u0 = MODE == 0 && beta != 1.0;
   const int ci = tid % N, cj = (tid / N) % N, ck = tid / N2;
 
-  for (int b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+  for (int bi = blockIdx.x; bi < a.nblocks; bi += gridDim.x) {
+    const int b = a.nodes ? a.nodes[bi] : bi;
     const double* Ub = a.Uin + (long)b * NF * R;
     const double h = a.h[b];
     const double x0[3] = {a.x0[3 * b], a.x0[3 * b + 1], a.x0[3 * b + 2]};
@@ -259,6 +263,54 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       }
       __syncthreads();
       if constexpr (MODE == 0) {
+        if constexpr (TREE) {
+          // adjust_*_flux (grid/grid_node.cpp:1076-1274): faces of the node's low, middle and high
+          // planes next to a finer region take the means its fine faces exported
+#pragma unroll 1
+          for (int q2 = tid; q2 < 3 * N2; q2 += N3) {
+            const int pl = q2 / N2, ta = q2 % N, tb = (q2 / N) % N;
+            const int qa = ta >= N / 2, qb = tb >= N / 2;
+            const int sl = a.slot[((b * 3 + d) * 3 + pl) * 4 + qa + 2 * qb];
+            if (sl < 0) continue;
+            const int f = pl == 0 ? 0 : (pl == 1 ? N / 2 : N);
+            const double* o = a.ovr + (((long)sl * (N / 2) + (tb - qb * (N / 2))) * (N / 2) + (ta - qa * (N / 2))) * NF;
+            const int q = f + (N + 1) * (ta + N * tb);
+#pragma unroll
+            for (int l = 0; l < NF; l++) Fb[l][q] = (T)o[l];
+          }
+          __syncthreads();
+          // this node's boundary faces, 2x2 means, to the coarse node's slot
+          if (tid < 4 * (N / 2) * (N / 2)) {
+            const int w = tid / ((N / 2) * (N / 2)), r = tid % ((N / 2) * (N / 2));
+            const int side = w >> 1;
+            const int sl = a.exp_slot[(b * 6 + 2 * d + side) * 2 + (w & 1)];
+            if (sl >= 0) {
+              const int al = r % (N / 2), bl = r / (N / 2);
+              const int q = (side ? N : 0) + (N + 1) * (2 * al + N * 2 * bl);
+              double* o = a.ovr + (((long)sl * (N / 2) + bl) * (N / 2) + al) * NF;
+#pragma unroll
+              for (int l = 0; l < NF; l++) {
+                double v = (double)Fb[l][q];
+                v += (double)Fb[l][q + (N + 1)];
+                v += (double)Fb[l][q + (N + 1) * N];
+                v += (double)Fb[l][q + (N + 1) * (N + 1)];
+                v *= 0.25;
+                o[l] = v;
+              }
+            }
+          }
+          // the root's boundary faces for the flow-off ledger (grid/grid.cpp:350-404)
+          if (b == 0 && a.ledger) {
+#pragma unroll 1
+            for (int q2 = tid; q2 < 2 * N2; q2 += N3) {
+              const int side = q2 / N2, ta = q2 % N, tb = (q2 / N) % N;
+              const int q = (side ? N : 0) + (N + 1) * (ta + N * tb);
+              double* o = a.ledger + ((((long)d * 2 + side) * N + tb) * N + ta) * NF;
+#pragma unroll
+              for (int l = 0; l < NF; l++) o[l] = (double)Fb[l][q];
+            }
+          }
+        }
         const int p = d == 0 ? ci : (d == 1 ? cj : ck);
         const int ta = d == 0 ? cj : ci;
         const int tb = d == 2 ? cj : ck;
@@ -288,7 +340,17 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       auto phi_at = [&](int i, int j, int k) { return ph[((k + 1) * PP + (j + 1)) * PP + (i + 1)]; };
       const double pc = phi_at(ci, cj, ck);
       double gf[3];
-      {
+      if (a.force) {
+        // Binary::add_difs (binary/binary.cpp:225-227) on the Poisson face forces
+        const long P3 = (long)PP * PP * PP;
+        const double* fx = a.force + (long)b * P3;
+        const double* fy = fx + a.nforce * P3;
+        const double* fz = fy + a.nforce * P3;
+        auto at = [&](int i, int j, int k) { return ((long)k * PP + j) * PP + i; };
+        gf[0] = 0.5 * (fx[at(ci + 1, cj + 1, ck + 1)] + fx[at(ci + 2, cj + 1, ck + 1)]);
+        gf[1] = 0.5 * (fy[at(ci + 1, cj + 1, ck + 1)] + fy[at(ci + 1, cj + 2, ck + 1)]);
+        gf[2] = 0.5 * (fz[at(ci + 1, cj + 1, ck + 1)] + fz[at(ci + 1, cj + 1, ck + 2)]);
+      } else {
         const double fxl = -(pc - phi_at(ci - 1, cj, ck)) / h;
         const double fxh = -(phi_at(ci + 1, cj, ck) - pc) / h;
         const double fyl = -(pc - phi_at(ci, cj - 1, ck)) / h;
@@ -417,11 +479,150 @@ __global__ __launch_bounds__(256) void rot_ghost_kernel(double* U, const double*
   for (int l = 0; l < NF; l++) Ub[(long)l * R + e] = u[l];
 }
 
+// GridNode::enforce_boundaries (grid/grid_node.cpp:59-112) for one level of the octree: a face
+// slab is copied from the same-level neighbour's interior, interpolated from the parent
+// (GridAMRBound, grid/grid_amr_bound.cpp:27-78: the parent cell minus a quarter of the minmod slope
+// along the face normal, the sign by the fine cell's parity) or filled by the physical boundary
+// as for isolated blocks
+template <int G>
+__global__ __launch_bounds__(256) void rot_tree_ghost_kernel(double* U, const int* nodes, int count, const int* ghost,
+                                                             const int* rel, const double* x0s, const double* hs,
+                                                             int z_reflect, RotParams c, int* err) {
+  constexpr int S = G * N2, R = N3 + 6 * S;
+  constexpr Layout L{N, G, NF};
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)count * 6 * S) return;
+  const int b = nodes[gid / (6 * S)];
+  const int e = N3 + (int)(gid % (6 * S));
+  int q[3];
+  elem_ijk<G>(e, q[0], q[1], q[2]);
+  const int nf = (e - N3) / S;
+  const int d = nf >> 1, up = nf & 1;
+  const int type = ghost[(b * 6 + nf) * 2], src = ghost[(b * 6 + nf) * 2 + 1];
+  double* Ub = U + (long)b * NF * R;
+  double u[NF];
+  if (type == GHOST_SIB) {
+    int s3[3] = {q[0], q[1], q[2]};
+    s3[d] = up ? q[d] - N : q[d] + N;  // the neighbour's cell at the same distance from the face
+    const double* Us = U + (long)src * NF * R;
+    const long es = ((long)s3[2] * N + s3[1]) * N + s3[0];
+#pragma unroll
+    for (int l = 0; l < NF; l++) u[l] = Us[(long)l * R + es];
+  } else if (type == GHOST_PHYS) {
+    int s3[3] = {q[0], q[1], q[2]};
+    const bool mirror = z_reflect && nf == 4;
+    s3[d] = mirror ? -1 - q[d] : (up ? N - 1 : 0);
+    const long es = ((long)s3[2] * N + s3[1]) * N + s3[0];
+#pragma unroll
+    for (int l = 0; l < NF; l++) u[l] = Ub[(long)l * R + es];
+    if (mirror) {
+      u[rot::SZ] = -u[rot::SZ];
+    } else {
+      const double h = hs[b];
+      double X[3];
+      cell_pos(x0s + 3L * b, h, s3[0], s3[1], s3[2], X);
+      X[d] = x0s[3L * b + d] + (double)(up ? N : 0) * h;
+      enforce_outflow(c, u, X, nf);
+    }
+  } else {
+    const double* Up = U + (long)src * NF * R;
+    int i1[3], par = 0;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      const int f = rel[b * 3 + t] + q[t] + G;  // parent-padded index of the fine cell, doubled
+      i1[t] = f >> 1;
+      if (t == d) par = f & 1;
+    }
+    int ip[3] = {i1[0], i1[1], i1[2]}, im[3] = {i1[0], i1[1], i1[2]};
+    ip[d] += 1;
+    im[d] -= 1;
+    const long e0 = L.from_padded(i1[0], i1[1], i1[2]);
+    const long ep = L.from_padded(ip[0], ip[1], ip[2]);
+    const long em = L.from_padded(im[0], im[1], im[2]);
+    if (e0 < 0 || ep < 0 || em < 0) {
+      atomicOr(err, 2);
+      return;
+    }
+#pragma unroll
+    for (int l = 0; l < NF; l++) {
+      const double uc = Up[(long)l * R + e0];
+      double m = x64::minmod2(Up[(long)l * R + ep] - uc, uc - Up[(long)l * R + em]);
+      if (par) m = 0.0 - m;
+      u[l] = uc - m * 0.25;
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NF; l++) Ub[(long)l * R + e] = u[l];
+}
+
+// GridNode::inject_from_children (grid/grid_node.cpp:1035-1074): a parent cell is the mean of its
+// 8 children's cells, floored at the parent cell's position (binary/state_ztwd.cpp:482-554)
+template <int G, int EOS>
+__global__ __launch_bounds__(256) void rot_restrict_kernel(double* U, const int2* pairs, int count, const double* x0s,
+                                                           const double* hs, RotParams c, helm::HelmDev H) {
+  constexpr int R = N3 + 6 * G * N2;
+  constexpr int HN = N / 2;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)count * HN * HN * HN) return;
+  const int2 pc = pairs[gid / (HN * HN * HN)];
+  const int r = (int)(gid % (HN * HN * HN));
+  const int parent = pc.x, ch = (pc.y >> 24) & 7, child = pc.y & 0xffffff;
+  const int ii = r % HN, jj = (r / HN) % HN, kk = r / (HN * HN);
+  const int i = 2 * ii, j = 2 * jj, k = 2 * kk;
+  const double* Uc = U + (long)child * NF * R;
+  auto at = [](int x, int y, int z) { return ((long)z * N + y) * N + x; };
+  double u[NF];
+#pragma unroll
+  for (int l = 0; l < NF; l++) {
+    const double* f = Uc + (long)l * R;
+    double s = 0.0;
+    s += f[at(i + 0, j + 0, k + 0)];
+    s += f[at(i + 1, j + 0, k + 0)];
+    s += f[at(i + 0, j + 1, k + 0)];
+    s += f[at(i + 1, j + 1, k + 0)];
+    s += f[at(i + 0, j + 0, k + 1)];
+    s += f[at(i + 1, j + 0, k + 1)];
+    s += f[at(i + 0, j + 1, k + 1)];
+    s += f[at(i + 1, j + 1, k + 1)];
+    s *= 0.125;
+    u[l] = s;
+  }
+  const int pi = (ch & 1) * HN + ii, pj = ((ch >> 1) & 1) * HN + jj, pk = ((ch >> 2) & 1) * HN + kk;
+  double X[3];
+  cell_pos(x0s + 3L * parent, hs[parent], pi, pj, pk, X);
+  floor_state<EOS>(H, c, u, X);
+  double* Up = U + (long)parent * NF * R;
+#pragma unroll
+  for (int l = 0; l < NF; l++) Up[(long)l * R + at(pi, pj, pk)] = u[l];
+}
+
+// Binary::pot_to_grid (binary/binary.cpp:825-849)
+template <int G>
+__global__ __launch_bounds__(256) void rot_pot_kernel(double* U, const double* phi, const double* x0s,
+                                                      const double* hs, int nblocks, RotParams c) {
+  constexpr int R = N3 + 6 * G * N2;
+  constexpr int PP = N + 2;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)nblocks * N3) return;
+  const int b = (int)(gid / N3), e = (int)(gid % N3);
+  const int i = e % N, j = (e / N) % N, k = e / N2;
+  double X[3];
+  cell_pos(x0s + 3L * b, hs[b], i, j, k, X);
+  double* Ub = U + (long)b * NF * R;
+  const double d = Ub[(long)D0 * R + e];
+  const double R2 = X[0] * X[0] + X[1] * X[1];
+  const double ph = phi[(long)b * PP * PP * PP + ((k + 1) * PP + (j + 1)) * PP + (i + 1)];
+  Ub[(long)POT * R + e] = d * (ph - 0.5 * R2 * c.Omega0 * c.Omega0);
+}
+
 template <int G, int RECON, int EOS, int MODE, bool MIXED>
 hipError_t launch_one(const RotArgs& a, int grid, hipStream_t s) {
   if (a.nblocks <= 0) return hipSuccess;
   const int g = grid > 0 && grid < a.nblocks ? grid : a.nblocks;
-  kt_rot_kernel<G, RECON, EOS, MODE, MIXED><<<g, N3, 0, s>>>(a);
+  if (MODE == 0 && a.slot)
+    kt_rot_kernel<G, RECON, EOS, MODE, MIXED, MODE == 0><<<g, N3, 0, s>>>(a);
+  else
+    kt_rot_kernel<G, RECON, EOS, MODE, MIXED, false><<<g, N3, 0, s>>>(a);
   return hipGetLastError();
 }
 
@@ -444,6 +645,54 @@ hipError_t launch_rot_substep(int g, int recon, int eos, bool mixed, const RotAr
 
 hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s) {
   return mixed ? dispatch<1, true>(g, recon, eos, a, grid, s) : dispatch<1, false>(g, recon, eos, a, grid, s);
+}
+
+hipError_t launch_rot_tree_ghosts(int g, double* U, const int* nodes, int count, const int* ghost, const int* rel,
+                                  const double* x0, const double* h, int z_reflect, const RotParams& c, int* err,
+                                  hipStream_t s) {
+  const long total = (long)count * 6 * g * N2;
+  if (total == 0) return hipSuccess;
+  const unsigned grid = (unsigned)((total + 255) / 256);
+  if (g == 3)
+    rot_tree_ghost_kernel<3><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, x0, h, z_reflect, c, err);
+  else if (g == 2)
+    rot_tree_ghost_kernel<2><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, x0, h, z_reflect, c, err);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_rot_restrict(int g, int eos, double* U, const int2* pairs, int count, const double* x0,
+                               const double* h, const RotParams& c, const helm::HelmDev& H, hipStream_t s) {
+  const long total = (long)count * (N / 2) * (N / 2) * (N / 2);
+  if (total == 0) return hipSuccess;
+  const unsigned grid = (unsigned)((total + 255) / 256);
+  const bool helm = eos == EOS_HELMHOLTZ;
+  if (g == 3 && helm)
+    rot_restrict_kernel<3, EOS_HELMHOLTZ><<<grid, 256, 0, s>>>(U, pairs, count, x0, h, c, H);
+  else if (g == 3)
+    rot_restrict_kernel<3, EOS_IDEAL><<<grid, 256, 0, s>>>(U, pairs, count, x0, h, c, H);
+  else if (g == 2 && helm)
+    rot_restrict_kernel<2, EOS_HELMHOLTZ><<<grid, 256, 0, s>>>(U, pairs, count, x0, h, c, H);
+  else if (g == 2)
+    rot_restrict_kernel<2, EOS_IDEAL><<<grid, 256, 0, s>>>(U, pairs, count, x0, h, c, H);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_rot_pot_to_grid(int g, double* U, const double* phi, const double* x0, const double* h,
+                                  int nblocks, const RotParams& c, hipStream_t s) {
+  const long total = (long)nblocks * N3;
+  if (total == 0) return hipSuccess;
+  const unsigned grid = (unsigned)((total + 255) / 256);
+  if (g == 3)
+    rot_pot_kernel<3><<<grid, 256, 0, s>>>(U, phi, x0, h, nblocks, c);
+  else if (g == 2)
+    rot_pot_kernel<2><<<grid, 256, 0, s>>>(U, phi, x0, h, nblocks, c);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 hipError_t launch_rot_ghosts(int g, double* U, const double* x0, const double* h, int nblocks, int z_reflect,

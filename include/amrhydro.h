@@ -166,7 +166,9 @@ int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_gr
  * (h0 = 2*grid_dim/n, halved per level; grid/grid.cpp:70, grid/grid_node.cpp:279) and the ghost,
  * reflux and restriction tables are derived here.  Replaces OctNode::create_child's wiring
  * (oct_node/oct_node.cpp:172-240) and GridNode::create_child's boundary objects
- * (grid/grid_node.cpp:271-304) for a tree built on the host. */
+ * (grid/grid_node.cpp:271-304) for a tree built on the host.  Runs the ideal gas with minmod
+ * (g = 2) and the rotating-frame state (n = 8; minmod g = 2 or PPM g = 3), whose node origins
+ * (Grid::xf, grid/grid.cpp:512-526) follow from the tree. */
 int amrh_tree_set(amrh_ctx* ctx, const int* parent, const int* child);
 /* GridNode::enforce_boundaries (grid/grid_node.cpp:59-112), level by level: same-level copy,
  * coarse->fine interpolation (grid/grid_amr_bound.cpp:27-78), physical boundary
@@ -174,7 +176,13 @@ int amrh_tree_set(amrh_ctx* ctx, const int* parent, const int* child);
 int amrh_tree_fill_ghosts(amrh_ctx* ctx);
 /* HydroDriver::sub_step + inject_from_children (euler/hydro_driver.cpp:171-214): ghost fill,
  * fluxes with reflux (grid/grid_node.cpp:1076-1274), divergence, add_difs on every node, the root's
- * flow-off ledger (grid/grid.cpp:350-404, 419), then restriction (grid/grid_node.cpp:1035-1074). */
+ * flow-off ledger (grid/grid.cpp:350-404, 419), then restriction (grid/grid_node.cpp:1035-1074).
+ * The rotating-frame state runs BinaryDriver::sub_step's hydro part (binary/binary_driver.cpp:
+ * 346-372): the same steps with Binary::add_difs's sources (binary/binary.cpp:205-254) from the
+ * potential or the Poisson face forces and pot reset from the potential, level by level finest
+ * first, and stops before the restriction: the driver solves for phi, calls amrh_pot_to_grid and
+ * then amrh_tree_inject (BinaryDriver::solve, binary/binary_driver.cpp:296-344), or calls
+ * amrh_tree_inject at once for a fixed potential. */
 int amrh_tree_substep(amrh_ctx* ctx, double dt, double beta);
 int amrh_tree_substep_dev(amrh_ctx* ctx, const double* dt_dev, double beta);
 /* GridNode::inject_from_children (grid/grid_node.cpp:1035-1074). */
@@ -246,6 +254,18 @@ int amrh_set_block_origin(amrh_ctx* ctx, const double* x0);
  * synchronous; the _device form takes a device pointer in the same layout. */
 int amrh_set_potential(amrh_ctx* ctx, const double* phi);
 int amrh_set_potential_device(amrh_ctx* ctx, const void* phi_dev);
+/* The gravitational force of Binary::add_difs (binary/binary.cpp:225-227) from Poisson's face
+ * forces instead of phi's differences: fx of every block, then fy, then fz (3 * nblocks * (n+2)^3
+ * doubles), each in phi's layout, fx(i) on the face between cells i-1 and i; the cell's force is
+ * the mean of its two faces'.  Where a fine node meets a coarse one these carry the coarse-fine
+ * averaging of Poisson::adjust_*_force (poisson/poisson.cpp:474-635).  Host array (copied,
+ * synchronous) or a device pointer kept by reference (amrh_poisson_forces_device); NULL returns
+ * to phi's differences. */
+int amrh_set_forces(amrh_ctx* ctx, const double* f);
+int amrh_set_forces_device(amrh_ctx* ctx, const void* f_dev);
+/* Binary::pot_to_grid (binary/binary.cpp:825-849): pot = rho (phi - R^2 omega0^2 / 2) on every
+ * block's interior from the current potential.  Asynchronous. */
+int amrh_pot_to_grid(amrh_ctx* ctx);
 
 /* ---- Helmholtz EOS (helmholtz/helmeos.cpp) -------------------------------------------------
  * The electron-positron free-energy table and the EOS built on it, on the device, FP64, in the
@@ -310,6 +330,9 @@ int amrh_poisson_upload(amrh_poisson* p, int field, const double* host);
 int amrh_poisson_download(amrh_poisson* p, int field, double* host);
 /* Device pointer to phi (nnodes * (n+2)^3 doubles), valid while p lives. */
 const void* amrh_poisson_phi_device(const amrh_poisson* p);
+/* Device pointer to the face forces fx, fy, fz (3 * nnodes * (n+2)^3 doubles, amrh_set_forces's
+ * layout), valid while p lives. */
+const void* amrh_poisson_forces_device(const amrh_poisson* p);
 /* Poisson::set_source (poisson/poisson.cpp:956-972) from a hydro batch whose blocks are the same
  * nodes (same n): S = 4 pi G rho on the interior (State::poisson_source,
  * binary/state_ztwd.cpp:219-222).  Synchronous. */

@@ -1,0 +1,188 @@
+"""The rotating-frame state on the octree on the GPU (amrh_tree_* with AMRH_PHYS_ROTATING) against
+the oracle's restatement of BinaryDriver::sub_step (binary/binary_driver.cpp:346-382) through the
+C ABI: ghost fill (sibling copy, coarse->fine interpolation, outflow and z mirror) bit for bit;
+substeps with reflux, sources, floor and the potential energy, then restriction floored at the
+parent cell, to <= 1e-12 relative L1 per field (pow ulps as on flat batches); the root's flow-off
+ledger; the Poisson face forces handed over by device pointer; whole RK3 steps as direct launches
+and as captured graphs; the mixed form's error against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import orc
+from gpu_util import amrhydro, say
+from test_gpu_rotating import helm_eos, rel_l1_rot
+from test_oracle_rot_tree import TREES, binary_tree_case, levels
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+TOL = 1e-12
+NF = 13
+
+
+def device_batch(T, cfg, precision=amrhydro.AMRH_FP64):
+    parent, child = T.parent, T.child
+    d = amrhydro.rotating_config(eos=cfg.eos, recon=cfg.recon, omega=cfg.Omega, omega0=cfg.Omega0,
+                                 z_reflect=cfg.z_reflect, precision=precision)
+    B = amrhydro.Batch(d, T.nn, 1.0)
+    B.tree_set(parent, child)
+    B.upload(T.state())
+    B.set_potential(np.stack([T.phi(b).copy() for b in range(T.nn)]))
+    if cfg.eos == 1:
+        B.attach_eos(helm_eos())
+    return B
+
+
+def slab_mask(g, G):
+    m = np.zeros((G, G, G), bool)
+    inner = slice(g, G - g)
+    m[:, inner, inner] = True
+    m[inner, :, inner] = True
+    m[inner, inner, :] = True
+    return m
+
+
+@pytest.mark.parametrize("name", list(TREES))
+@pytest.mark.parametrize("recon", [1, 0])
+@pytest.mark.parametrize("z_reflect", [0, 1])
+def test_rot_tree_ghosts_match_oracle(name, recon, z_reflect):
+    T, cfg = binary_tree_case(name, recon=recon, z_reflect=z_reflect, seed=21)
+    say("rot tree ghosts %s recon=%d zr=%d: %d nodes" % (name, recon, z_reflect, T.nn))
+    B = device_batch(T, cfg)
+    T.fill_ghosts()
+    B.tree_fill_ghosts()
+    g, G = cfg.g, cfg.n + 2 * cfg.g
+    m = slab_mask(g, G)
+    np.testing.assert_array_equal(B.download()[:, :, m], T.state()[:, :, m])
+    assert B.error_flags() == 0
+    B.close()
+
+
+@pytest.mark.parametrize("name,recon,z_reflect,eos", [("partial", 1, 0, 0), ("mid", 1, 1, 0), ("partial", 0, 0, 0),
+                                                      ("mid", 0, 1, 0), ("partial", 1, 0, 1)])
+def test_rot_tree_rk3_matches_oracle(name, recon, z_reflect, eos):
+    if eos == 1:
+        helm_eos()  # loads the oracle's table too
+    T, cfg = binary_tree_case(name, recon=recon, z_reflect=z_reflect, seed=22, eos=eos)
+    B = device_batch(T, cfg)
+    g = cfg.g
+    say("rot tree rk3 %s recon=%d zr=%d eos=%d: %d nodes" % (name, recon, z_reflect, eos, T.nn))
+    for step in range(2):
+        T.fill_ghosts()
+        dt = 0.5 * T.max_dt()
+        B.tree_fill_ghosts()
+        dt_dev = B.tree_max_dt()
+        np.testing.assert_allclose(dt_dev, 2 * dt, rtol=1e-14)
+        T.store()
+        B.store()
+        for beta in (1.0, 0.25, 2.0 / 3.0):
+            assert T.substep(dt, beta) == 0
+            assert T.inject() == 0
+            B.tree_substep(dt, beta)
+            B.tree_inject()
+            out, ref = B.download(), T.state()
+            e = rel_l1_rot(out, ref, g)
+            I = (slice(None), slice(None), slice(g, -g), slice(g, -g), slice(g, -g))
+            same = np.mean(out[I] == ref[I])
+            say("  step %d beta=%.4f rel-L1 max %.1e (field %d)  bit-identical %.6f" % (step, beta, e.max(),
+                                                                                      int(np.argmax(e)), same))
+            assert np.all(e <= TOL), e
+            fo, fo_ref = B.tree_flow_off(), T.flow_off()
+            np.testing.assert_allclose(fo, fo_ref, rtol=1e-11, atol=1e-14 * np.abs(fo_ref).max())
+    assert B.error_flags() == 0
+    B.close()
+
+
+def test_rot_tree_forces_and_phi_from_the_device_poisson_solver():
+    """BinaryDriver's coupling: S = 4 pi G rho from the tree state, the V-cycles, then the hydro
+    substep reading the solver's face forces and phi by device pointer, pot_to_grid, restriction"""
+    T, cfg = binary_tree_case("one", seed=23)
+    B = device_batch(T, cfg)
+    P = amrhydro.Poisson(8, T.parent, T.child, grid_dim=1.0, bw=cfg.g)
+    P.source_from(B, 1.0)
+    it, err = P.solve(1e-8)
+    phi = P.download("phi")
+    f = np.stack([P.download(k) for k in ("fx", "fy", "fz")])
+    say("rot tree + poisson: %d V-cycles, error %.1e" % (it, err))
+    B.set_potential_device(P.phi_device())
+    B.set_forces_device(P.forces_device())
+    T.set_phi(phi)
+    T.set_forces(f[0], f[1], f[2])
+    T.use_forces(True)
+    dt = 1e-4
+    T.store()
+    B.store()
+    T.substep(dt, 1.0)
+    T.pot_to_grid()
+    T.inject()
+    B.tree_substep(dt, 1.0)
+    B.pot_to_grid()
+    B.tree_inject()
+    e = rel_l1_rot(B.download(), T.state(), cfg.g)
+    assert np.all(e <= TOL), e
+    # the forces differ from phi's differences at coarse-fine faces: the two paths part there
+    B.set_forces_device(None)
+    B.close()
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_rot_tree_whole_steps_conserve(use_graph):
+    """amrh_run_steps on the octree (fill, device dt, store, 3 substeps each followed by the
+    restriction), direct and as graphs: with Omega = phi = 0 the leaf sums plus the ledger keep rho,
+    l_z, s_z and E to roundoff"""
+    T, cfg = binary_tree_case("partial", quiet=True, seed=24)
+    B = device_batch(T, cfg)
+    B.tree_fill_ghosts()
+    B.tree_inject()
+    s0 = B.state_sum()[0] + B.tree_flow_off()
+    B.tree_fill_ghosts()
+    # the first step's dt, as HydroDriver's first step (euler/hydro_driver.cpp:108-111); later
+    # steps grow it by at most 1.25 up to the CFL limit
+    dt = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
+    B.run_steps(3, dt.data_ptr(), rk=3, use_graph=bool(use_graph))
+    assert dt.item() > 0
+    s1 = B.state_sum()[0] + B.tree_flow_off()
+    scale = np.abs(B.state_sum()[0])
+    for f in (0, 4, 5, 7):
+        assert abs(s1[f] - s0[f]) <= 1e-13 * max(scale[f], scale[0]), (f, s0[f], s1[f])
+    say("rot tree run_steps graph=%d: dt %.3e, drift rho %.1e E %.1e" % (use_graph, dt.item(), abs(s1[0] - s0[0]),
+                                                                          abs(s1[7] - s0[7])))
+    B.close()
+
+
+def test_rot_tree_graph_steps_equal_direct_steps():
+    outs = []
+    for use_graph in (0, 1):
+        T, cfg = binary_tree_case("partial", seed=25)
+        B = device_batch(T, cfg)
+        B.tree_fill_ghosts()
+        dt = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
+        B.run_steps(2, dt.data_ptr(), rk=3, use_graph=bool(use_graph))
+        assert dt.item() > 0
+        outs.append((B.download(), B.tree_flow_off()))
+        B.close()
+    g = 3
+    I = (slice(None), slice(None), slice(g, -g), slice(g, -g), slice(g, -g))
+    np.testing.assert_array_equal(outs[0][0][I], outs[1][0][I])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_rot_tree_mixed_stays_near_fp64_oracle():
+    T, cfg = binary_tree_case("partial", seed=26)
+    B = device_batch(T, cfg, precision=amrhydro.AMRH_MIXED)
+    g = cfg.g
+    for step in range(5):
+        T.fill_ghosts()
+        dt = T.max_dt()
+        T.store()
+        B.store()
+        for beta in (1.0, 0.25, 2.0 / 3.0):
+            T.substep(dt, beta)
+            T.inject()
+            B.tree_substep(dt, beta)
+            B.tree_inject()
+    e = rel_l1_rot(B.download(), T.state(), g)
+    say("rot tree mixed after 5 RK3 steps: rel-L1 rho %.2e, max over fields %.2e" % (e[0], e.max()))
+    assert e[0] <= 1e-6 and e.max() <= 1e-5
+    B.close()

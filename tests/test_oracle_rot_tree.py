@@ -40,7 +40,8 @@ def partial_spec():
     return full(1) + [(1, c) for c in range(8)] + [(9 + 7, c) for c in range(8)]
 
 
-TREES = {"mid": tree(mid_spec()), "partial": tree(partial_spec())}
+# "one": a properly nested tree (every refined node's 26 neighbours exist), as the Poisson solver needs
+TREES = {"mid": tree(mid_spec()), "partial": tree(partial_spec()), "one": tree(full(1) + [(3, c) for c in range(8)])}
 
 
 def levels(parent, child):
@@ -58,15 +59,15 @@ def centres(x0, h, G, g):
     return X, Y, Z
 
 
-def binary_tree_case(name, recon=1, omega=None, z_reflect=0, seed=7, amp=0.3, quiet=False):
+def binary_tree_case(name, recon=1, omega=None, z_reflect=0, seed=7, amp=0.3, quiet=False, eos=0):
     """the binary model on tree `name` with random rotating-frame velocities; quiet: Omega = 0 and
-    phi = pot = 0 (no sources in rho, l_z, s_z, E)"""
+    phi = pot = 0 (no sources in rho, l_z, s_z, E); eos 1: the Helmholtz EOS (its table loaded)"""
     parent, child = TREES[name]
     lv, loc = levels(parent, child)
     g = 3 if recon == 1 else 2
-    S = workloads.binary_tree_state(lv, loc, n=8, g=g)
+    S = workloads.binary_tree_state(lv, loc, n=8, g=g, eos="helm" if eos == 1 else "ideal")
     om = 0.0 if quiet else (S["omega"] if omega is None else omega)
-    cfg = orc.rot_cfg(recon=recon, omega=om, z_reflect=z_reflect)
+    cfg = orc.rot_cfg(recon=recon, omega=om, z_reflect=z_reflect, eos=eos)
     T = orc.RotTree(cfg, parent, child)
     U, phi = S["U"], S["phi"]
     G = 8 + 2 * g
