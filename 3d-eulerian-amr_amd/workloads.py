@@ -301,6 +301,50 @@ def binary_tree_state(level, loc, n=8, g=3, eos="ideal"):
     return dict(U=U, phi=phi, x0=x0, h=h, omega=M["omega"])
 
 
+def sedov_tree(maxlevel=2, n=8, g=2, grid_dim=0.5):
+    """Fully refined octree (1 + 8 + 64 nodes for maxlevel 2, breadth first: every parent before
+    its children, child c with x, y, z bits c & 1, c >> 1 & 1, c >> 2 & 1) over the cube
+    [-grid_dim, grid_dim]^3: ambient rho = 1, p = 1e-5 at rest; energy 1.0 deposited uniformly in
+    the finest-level cells whose centres lie within 2 finest cells of the origin.  The node at
+    level l with integer location L has padded cell p centred at -grid_dim + (L n + p - g + 1/2) h_l
+    (grid/grid.cpp:500-526).  Every padded cell holds the analytic field; the library's ghost fill
+    and restriction make them consistent.  Returns (U (nodes, 6, G, G, G), parent, child)."""
+    parent, child, level, loc = [-1], [-1], [0], [(0, 0, 0)]
+    frontier = [0]
+    for _ in range(maxlevel):
+        new = []
+        for p in frontier:
+            for c in range(8):
+                parent.append(p)
+                child.append(c)
+                level.append(level[p] + 1)
+                loc.append(tuple(2 * loc[p][d] + ((c >> d) & 1) for d in range(3)))
+                new.append(len(parent) - 1)
+        frontier = new
+    G = n + 2 * g
+    h0 = 2.0 * grid_dim / n
+    hf = h0 / 2 ** maxlevel
+    r = 2.0 * hf
+    m = int(round(2.0 * grid_dim / hf))
+    c1 = -grid_dim + (np.arange(m) + 0.5) * hf
+    Zc, Yc, Xc = np.meshgrid(c1, c1, c1, indexing="ij")
+    ndep = int((Xc ** 2 + Yc ** 2 + Zc ** 2 <= r * r).sum())
+    edep = 1.0 / (ndep * hf ** 3)
+    U = np.empty((len(parent), 6, G, G, G))
+    for b, (l, L) in enumerate(zip(level, loc)):
+        h = h0 / 2 ** l
+        x, y, z = (-grid_dim + (L[d] * n + np.arange(G) - g + 0.5) * h for d in range(3))
+        Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+        E = np.full(X.shape, 1e-5 / (GAMMA - 1.0))
+        E = np.where(X * X + Y * Y + Z * Z <= r * r, E + edep, E)
+        U[b, 0] = 1.0
+        U[b, 1:4] = 0.0
+        U[b, 4] = E
+        U[b, 5] = E ** (1.0 / GAMMA)
+    return U, np.array(parent, np.int32), np.array(child, np.int32)
+
+
+# ---- self-gravity: the binary's density on a properly nested octree ---------------------------
 def _binary_stars(q=0.7, r_donor=0.2):
     """centres, radii and central densities of the two n = 1.5 polytropes of binary_batch"""
     xi, th, xi1, om = lane_emden(1.5)

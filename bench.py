@@ -187,6 +187,8 @@ def extra_configs(amrhydro, torch, steps, warmup):
         log("  sedov tree %s: %.1f us per RK3 step" % (tag, 1e6 * wall / steps))
         B.close()
 
+    out.update(binary_octree(amrhydro, torch, steps, warmup, stream))
+
     log("self-gravity: multigrid Poisson on the binary's octree (%s)" % POISSON_TREE)
     parent, child, lv, loc = workloads.binary_tree(POISSON_LEVELS)
     D = amrhydro.Poisson(8, parent, child)
@@ -209,6 +211,82 @@ def extra_configs(amrhydro, torch, steps, warmup):
     log("  boundary %.2f ms; solve from phi = 0: %d V-cycles in %.1f ms; %.2f ms per V-cycle"
         % (1e3 * t_prep, it, 1e3 * t_solve, 1e3 * t_cyc))
     D.close()
+    return out
+
+
+def binary_octree(amrhydro, torch, steps, warmup, stream):
+    """The rotating binary on its octree (2,473 nodes of 8^3 over 6 levels, PPM, 13 fields, ideal
+    gas): whole RK3 steps of BinaryDriver with the stars' fixed potential (ghost exchange, CFL dt,
+    three substeps with reflux, each followed by the restriction; one captured graph per step), and
+    the same steps with self-gravity: after every substep S = 4 pi G rho, the Poisson solve from the
+    previous phi, pot_to_grid and the restriction (BinaryDriver::sub_step + solve,
+    binary/binary_driver.cpp:296-382), the hydro reading the solver's face forces by pointer."""
+    import workloads
+    out = {}
+    parent, child, lv, loc = workloads.binary_tree(POISSON_LEVELS)
+    S = workloads.binary_tree_state(lv, loc)
+    inner = set(parent[1:].tolist())
+    leaves = sum(1 for b in range(len(parent)) if b not in inner) * 8 ** 3
+    log("rotating binary on its octree (%s, %d leaf cells)" % (POISSON_TREE, leaves))
+    for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+        cfg = amrhydro.rotating_config(omega=S["omega"], precision=prec)
+        B = amrhydro.Batch(cfg, len(parent), 1.0)
+        B.set_stream(stream)
+        B.tree_set(parent, child)
+        B.upload(S["U"])
+        B.set_potential(S["phi"])
+        B.tree_fill_ghosts()
+        B.tree_inject()
+        dt = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
+        B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        B.run_steps(steps, dt.data_ptr(), rk=3, use_graph=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        r = {"workload": "rotating binary on " + POISSON_TREE + ", fixed potential", "nodes": len(parent),
+             "leaf_cells": leaves, "cell_updates_per_s": 3 * steps * leaves / wall,
+             "ms_per_step": 1e3 * wall / steps, "device_error_flags": B.error_flags()}
+        log("  binary octree %s: %.3e leaf cell-updates/s, %.3f ms per RK3 step"
+            % (tag, r["cell_updates_per_s"], r["ms_per_step"]))
+        if tag == "mixed":
+            # self-gravity: every substep solves for phi
+            P = amrhydro.Poisson(8, parent, child, bw=cfg.g)
+            P.source_from(B, 1.0)
+            P.solve(POISSON_TOLER)
+            B.set_potential_device(P.phi_device())
+            B.set_forces_device(P.forces_device())
+            its = []
+
+            def grav_step():
+                B.tree_fill_ghosts()
+                B.dt_device(1.0, 1.25, dt.data_ptr())
+                B.store()
+                for beta in BETAS:
+                    B.tree_substep_dev(dt.data_ptr(), beta)
+                    P.source_from(B, 1.0)
+                    its.append(P.solve(POISSON_TOLER)[0])
+                    B.set_potential_device(P.phi_device())
+                    B.pot_to_grid()
+                    B.tree_inject()
+            grav_step()
+            torch.cuda.synchronize()
+            its.clear()
+            ng = max(2, steps // 2)
+            t0 = time.perf_counter()
+            for _ in range(ng):
+                grav_step()
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            r["self_gravity"] = {"ms_per_step": 1e3 * wall / ng, "cell_updates_per_s": 3 * ng * leaves / wall,
+                                 "vcycles_per_solve": statistics.mean(its), "toler": POISSON_TOLER,
+                                 "device_error_flags": B.error_flags()}
+            log("  with self-gravity: %.2f ms per RK3 step, %.1f V-cycles per solve"
+                % (r["self_gravity"]["ms_per_step"], statistics.mean(its)))
+            B.set_forces_device(None)
+            P.close()
+        out["binary_octree_%s" % tag] = r
+        B.close()
     return out
 
 
@@ -314,6 +392,35 @@ def cpu_poisson_baseline():
             "sample": "one V-cycle of the FP64 oracle on the same tree and source"}
 
 
+def cpu_binary_octree_baseline():
+    """One RK3 substep of the FP64 oracle (OpenMP over nodes) on the binary's octree with the fixed
+    potential: ghost fill, fluxes with reflux, sources, floor, then the restriction."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    import workloads
+
+    parent, child, lv, loc = workloads.binary_tree(POISSON_LEVELS)
+    S = workloads.binary_tree_state(lv, loc)
+    T = orc.RotTree(orc.rot_cfg(omega=S["omega"]), parent, child)
+    T.set_state(S["U"])
+    T.set_phi(S["phi"])
+    dt = 0.01 * T.max_dt()
+    T.store()
+    t0 = time.perf_counter()
+    T.substep(dt, 1.0)
+    T.inject()
+    t = time.perf_counter() - t0
+    inner = set(parent[1:].tolist())
+    leaves = sum(1 for b in range(len(parent)) if b not in inner) * 8 ** 3
+    try:
+        import ctypes
+        cores = ctypes.CDLL("libgomp.so.1").omp_get_max_threads()
+    except OSError:
+        cores = os.cpu_count()
+    return {"value": leaves / t, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": "one substep + restriction of the FP64 oracle on the same octree, %.1f s" % t}
+
+
 def load_traffic(workload_key):
     p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload_key)
     if os.path.exists(p):
@@ -397,6 +504,11 @@ def main():
             pc = cpu_poisson_baseline()
             extra["poisson_binary_tree"]["cpu_baseline"] = pc
             log("  %.2f s per V-cycle (1 thread)" % pc["vcycle_s"])
+        if extra is not None and "binary_octree_fp64" in extra:
+            log("CPU oracle: one substep of the rotating binary on its octree")
+            bc = cpu_binary_octree_baseline()
+            extra["binary_octree_fp64"]["cpu_baseline"] = bc
+            log("  %.3e cell-updates/s on %d threads" % (bc["value"], bc["cores"]))
 
     head = results.get("mixed") or results.get("fp64")
     head_name = "mixed" if "mixed" in results else "fp64"
