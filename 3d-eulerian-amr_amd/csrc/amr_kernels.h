@@ -17,11 +17,12 @@ struct ExportItem {
   int slot;  // override slot written (owned by the coarse block)
 };
 
-hipError_t launch_tree_ghosts(int n, double* U, const int* nodes, int count, const int* ghost, const int* rel,
+hipError_t launch_tree_ghosts(int n, int g, double* U, const int* nodes, int count, const int* ghost, const int* rel,
                               int z_reflect, int* err, hipStream_t s);
 hipError_t launch_reflux_export(int n, bool mixed, const double* U, const ExportItem* items, int count,
                                 const int* slot, double* ovr, const PhysParams& P, hipStream_t s);
-hipError_t launch_restrict(int n, double* U, const int2* pairs, int count, const PhysParams& P, hipStream_t s);
+hipError_t launch_restrict(int n, int g, double* U, const int2* pairs, int count, const PhysParams& P,
+                           hipStream_t s);
 // the root's flow-off ledger of nf fields: FO = (FO + dFO dt) beta + FO0 (1 - beta), fo = [FO | FO0]
 hipError_t launch_ledger(int n, int nf, const double* ledger, double h, const double* dt_dev, double dt, double beta,
                          int z_reflect, double* fo, hipStream_t s);

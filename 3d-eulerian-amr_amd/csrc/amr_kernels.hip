@@ -51,33 +51,33 @@ __device__ __forceinline__ void prim_of(const double* u, typename PT<MIXED>::T* 
 }
 
 // decode a slab element r of face `face` into the ghost's padded coordinates
-template <int N>
+template <int N, int G>
 __device__ __forceinline__ void slab_coords(int face, int r, int* p, int* t) {
   if (face < 2) {
-    *t = r % GW;
-    p[0] = (face & 1) ? N + GW + *t : *t;
-    p[1] = (r / GW) % N + GW;
-    p[2] = r / (GW * N) + GW;
+    *t = r % G;
+    p[0] = (face & 1) ? N + G + *t : *t;
+    p[1] = (r / G) % N + G;
+    p[2] = r / (G * N) + G;
   } else if (face < 4) {
-    *t = (r / N) % GW;
-    p[0] = r % N + GW;
-    p[1] = (face & 1) ? N + GW + *t : *t;
-    p[2] = r / (GW * N) + GW;
+    *t = (r / N) % G;
+    p[0] = r % N + G;
+    p[1] = (face & 1) ? N + G + *t : *t;
+    p[2] = r / (G * N) + G;
   } else {
     *t = r / (N * N);
-    p[0] = r % N + GW;
-    p[1] = (r / N) % N + GW;
-    p[2] = (face & 1) ? N + GW + *t : *t;
+    p[0] = r % N + G;
+    p[1] = (r / N) % N + G;
+    p[2] = (face & 1) ? N + G + *t : *t;
   }
 }
 
-template <int N>
+template <int N, int G>
 __global__ __launch_bounds__(256) void tree_ghost_kernel(double* U, const int* nodes, int count,
                                                          const int* ghost, const int* rel, int z_reflect,
                                                          int* err) {
-  constexpr Layout L{N, GW, NFE};
+  constexpr Layout L{N, G, NFE};
   constexpr long R = L.rec();
-  constexpr int SC = GW * N * N;
+  constexpr int SC = G * N * N;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)count * 6 * SC) return;
   const int q = (int)(gid / (6 * SC));
@@ -89,21 +89,21 @@ __global__ __launch_bounds__(256) void tree_ghost_kernel(double* U, const int* n
   const int d = face >> 1;
   const bool upper = face & 1;
   int p[3], t;
-  slab_coords<N>(face, r, p, &t);
+  slab_coords<N, G>(face, r, p, &t);
   double u[NFE];
   double* Un = U + (long)node * NFE * R;
   if (type == GHOST_SIB) {
     // sibling's interior: lower face reads its cells N-G+t, upper face its cells t
-    int c[3] = {p[0] - GW, p[1] - GW, p[2] - GW};
-    c[d] = upper ? t : N - GW + t;
+    int c[3] = {p[0] - G, p[1] - G, p[2] - G};
+    c[d] = upper ? t : N - G + t;
     const double* Us = U + (long)src * NFE * R;
     const long e = L.interior(c[0], c[1], c[2]);
 #pragma unroll
     for (int l = 0; l < NFE; l++) u[l] = Us[l * R + e];
   } else if (type == GHOST_PHYS) {
-    int c[3] = {p[0] - GW, p[1] - GW, p[2] - GW};
+    int c[3] = {p[0] - G, p[1] - G, p[2] - G};
     const bool reflect = z_reflect && face == ZL;
-    c[d] = reflect ? GW - 1 - t : (upper ? N - 1 : 0);
+    c[d] = reflect ? G - 1 - t : (upper ? N - 1 : 0);
     const long e = L.interior(c[0], c[1], c[2]);
 #pragma unroll
     for (int l = 0; l < NFE; l++) u[l] = Un[l * R + e];
@@ -234,9 +234,9 @@ __global__ __launch_bounds__(N* N) void reflux_export_kernel(const double* U, co
   }
 }
 
-template <int N>
+template <int N, int G>
 __global__ __launch_bounds__(256) void restrict_kernel(double* U, const int2* pairs, int count, PhysParams P) {
-  constexpr Layout L{N, GW, NFE};
+  constexpr Layout L{N, G, NFE};
   constexpr long R = L.rec();
   constexpr int H = N / 2;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,15 +297,17 @@ __global__ void ledger_store_kernel(double* fo, int nf) {
 
 }  // namespace
 
-hipError_t launch_tree_ghosts(int n, double* U, const int* nodes, int count, const int* ghost, const int* rel,
+hipError_t launch_tree_ghosts(int n, int g, double* U, const int* nodes, int count, const int* ghost, const int* rel,
                               int z_reflect, int* err, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  const long total = (long)count * 6 * GW * n * n;
+  const long total = (long)count * 6 * g * n * n;
   const unsigned grid = (unsigned)((total + 255) / 256);
-  if (n == 8)
-    tree_ghost_kernel<8><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, z_reflect, err);
-  else if (n == 16)
-    tree_ghost_kernel<16><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, z_reflect, err);
+  if (n == 8 && g == 2)
+    tree_ghost_kernel<8, 2><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, z_reflect, err);
+  else if (n == 8 && g == 3)
+    tree_ghost_kernel<8, 3><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, z_reflect, err);
+  else if (n == 16 && g == 2)
+    tree_ghost_kernel<16, 2><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, z_reflect, err);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -330,14 +332,17 @@ hipError_t launch_reflux_export(int n, bool mixed, const double* U, const Export
   return hipGetLastError();
 }
 
-hipError_t launch_restrict(int n, double* U, const int2* pairs, int count, const PhysParams& P, hipStream_t s) {
+hipError_t launch_restrict(int n, int g, double* U, const int2* pairs, int count, const PhysParams& P,
+                           hipStream_t s) {
   if (count <= 0) return hipSuccess;
   const long total = (long)count * (n / 2) * (n / 2) * (n / 2);
   const unsigned grid = (unsigned)((total + 255) / 256);
-  if (n == 8)
-    restrict_kernel<8><<<grid, 256, 0, s>>>(U, pairs, count, P);
-  else if (n == 16)
-    restrict_kernel<16><<<grid, 256, 0, s>>>(U, pairs, count, P);
+  if (n == 8 && g == 2)
+    restrict_kernel<8, 2><<<grid, 256, 0, s>>>(U, pairs, count, P);
+  else if (n == 8 && g == 3)
+    restrict_kernel<8, 3><<<grid, 256, 0, s>>>(U, pairs, count, P);
+  else if (n == 16 && g == 2)
+    restrict_kernel<16, 2><<<grid, 256, 0, s>>>(U, pairs, count, P);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

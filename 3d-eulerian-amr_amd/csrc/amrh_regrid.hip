@@ -17,8 +17,6 @@ using namespace amrh;
 
 namespace {
 
-constexpr int GW = 2;
-
 // copy whole block records: dst block b <- src block idx[b] (idx < 0: left alone)
 __global__ __launch_bounds__(256) void gather_kernel(double* dst, const double* src, const int* idx, int nb,
                                                      long block_elems) {
@@ -34,7 +32,7 @@ __global__ __launch_bounds__(256) void gather_kernel(double* dst, const double* 
 
 // inject_from_parent for the nodes in `nodes` (node, parent, child index packed as int3): one thread
 // per 2x2x2 group of child cells, all fields
-template <int N>
+template <int N, int GW>
 __global__ __launch_bounds__(64) void prolong_kernel(double* U, const int3* nodes, int count, PhysParams P) {
   constexpr Layout L{N, GW, NFE};
   constexpr long R = L.rec();
@@ -84,7 +82,7 @@ __global__ __launch_bounds__(64) void prolong_kernel(double* U, const int3* node
 }
 
 // per block and octant: some interior cell has rho > threshold
-template <int N>
+template <int N, int GW>
 __global__ __launch_bounds__(256) void density_flags_kernel(const double* U, int nf, double threshold,
                                                            unsigned* flags) {
   constexpr long R = (long)N * N * N + 6L * GW * N * N;
@@ -151,12 +149,15 @@ int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index) {
     const long total = (long)fresh.size() * (n / 2) * (n / 2) * (n / 2);
     if (e == hipSuccess) {
       const unsigned grid = (unsigned)((total + 63) / 64);
-      if (n == 8)
-        prolong_kernel<8><<<grid, 64, 0, dst->stream>>>(dst->buf[dst->cur], nodes_dev, (int)fresh.size(),
-                                                         params(dst->cfg));
+      if (n == 8 && dst->cfg.g == 3)
+        prolong_kernel<8, 3><<<grid, 64, 0, dst->stream>>>(dst->buf[dst->cur], nodes_dev, (int)fresh.size(),
+                                                            params(dst->cfg));
+      else if (n == 8)
+        prolong_kernel<8, 2><<<grid, 64, 0, dst->stream>>>(dst->buf[dst->cur], nodes_dev, (int)fresh.size(),
+                                                            params(dst->cfg));
       else
-        prolong_kernel<16><<<grid, 64, 0, dst->stream>>>(dst->buf[dst->cur], nodes_dev, (int)fresh.size(),
-                                                          params(dst->cfg));
+        prolong_kernel<16, 2><<<grid, 64, 0, dst->stream>>>(dst->buf[dst->cur], nodes_dev, (int)fresh.size(),
+                                                             params(dst->cfg));
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(dst->stream);
@@ -170,16 +171,20 @@ int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index) {
 int amrh_tree_density_flags(amrh_ctx* c, double min_density, unsigned char* flags) {
   if (!c || !flags) return fail(AMRH_EINVAL, "null argument");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
-  if (c->cfg.g != 2 || (c->cfg.n != 8 && c->cfg.n != 16)) return fail(AMRH_EINVAL, "n = 8 or 16 with g = 2 only");
+  const int n = c->cfg.n, g = c->cfg.g;
+  if (!((g == 2 && (n == 8 || n == 16)) || (g == 3 && n == 8)))
+    return fail(AMRH_EINVAL, "n = 8 or 16 with g = 2, or n = 8 with g = 3");
   unsigned* f_dev = nullptr;
   HIPCHK(hipMalloc(&f_dev, sizeof(unsigned) * c->nb));
   std::vector<unsigned> f(c->nb, 0u);
   hipError_t e = hipMemsetAsync(f_dev, 0, sizeof(unsigned) * c->nb, c->stream);
   if (e == hipSuccess) {
-    if (c->cfg.n == 8)
-      density_flags_kernel<8><<<c->nb, 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, min_density, f_dev);
+    if (n == 8 && g == 3)
+      density_flags_kernel<8, 3><<<c->nb, 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, min_density, f_dev);
+    else if (n == 8)
+      density_flags_kernel<8, 2><<<c->nb, 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, min_density, f_dev);
     else
-      density_flags_kernel<16><<<c->nb, 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, min_density, f_dev);
+      density_flags_kernel<16, 2><<<c->nb, 256, 0, c->stream>>>(c->buf[c->cur], c->cfg.nf, min_density, f_dev);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(f.data(), f_dev, sizeof(unsigned) * c->nb, hipMemcpyDeviceToHost, c->stream);

@@ -232,7 +232,7 @@ int fill_ghosts(amrh_ctx* c) {
       HIPCHK(launch_rot_tree_ghosts(c->cfg.g, c->buf[c->cur], nodes, t->level_count[l], t->ghost_dev, t->rel_dev,
                                     c->x0_dev, c->h_dev, c->cfg.z_reflect, rp, c->err_dev, c->stream));
     else
-      HIPCHK(launch_tree_ghosts(c->cfg.n, c->buf[c->cur], nodes, t->level_count[l], t->ghost_dev, t->rel_dev,
+      HIPCHK(launch_tree_ghosts(c->cfg.n, c->cfg.g, c->buf[c->cur], nodes, t->level_count[l], t->ghost_dev, t->rel_dev,
                                 c->cfg.z_reflect, c->err_dev, c->stream));
   }
   return AMRH_OK;
@@ -250,8 +250,8 @@ int inject(amrh_ctx* c) {
   }
   const PhysParams P = params(c->cfg);
   for (int l = t->maxlevel - 1; l >= 0; l--)
-    HIPCHK(launch_restrict(c->cfg.n, c->buf[c->cur], t->pairs_dev + t->pair_start[l], t->pair_count[l], P,
-                           c->stream));
+    HIPCHK(launch_restrict(c->cfg.n, c->cfg.g, c->buf[c->cur], t->pairs_dev + t->pair_start[l], t->pair_count[l],
+                           P, c->stream));
   return AMRH_OK;
 }
 
@@ -289,9 +289,47 @@ int rot_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) 
   return AMRH_OK;
 }
 
+// The ideal gas with PPM: the block kernel level by level, finest first, applying and exporting the
+// reflux means as the rotating kernel does; the ledger; then the restriction
+int block_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
+  TreeState* t = c->tree;
+  if (beta != 1.0 && c->u0buf < 0) return fail(AMRH_ESTATE, "beta != 1 needs amrh_store() first");
+  if (int rc = fill_ghosts(c)) return rc;
+  SubstepArgs a{};
+  a.Uin = c->buf[c->cur];
+  int out = c->cur;
+  if (c->u0buf == c->cur) out = 1 - c->cur;  // never overwrite the RK base
+  a.Uout = c->buf[out];
+  a.U0 = c->u0buf >= 0 ? c->buf[c->u0buf] : c->buf[c->cur];
+  a.h = c->h_dev;
+  a.dt_dev = dt_dev;
+  a.dt = dt;
+  a.beta = beta;
+  a.P = params(c->cfg);
+  a.err = c->err_dev;
+  a.slot = t->slot_dev;
+  a.ovr = t->ovr_dev;
+  a.ovr_out = t->ovr_dev;
+  a.exp_slot = t->exp_dev;
+  a.ledger_block = 0;
+  a.ledger = t->ledger_dev;
+  a.z_reflect = c->cfg.z_reflect;
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  for (int l = t->maxlevel; l >= 0; l--) {
+    a.nodes = t->level_nodes_dev + t->level_start[l];
+    a.nblocks = t->level_count[l];
+    HIPCHK(launch_block_substep(c->cfg.n, c->cfg.g, c->cfg.recon, a, mixed, 0, c->stream));
+  }
+  c->cur = out;
+  HIPCHK(launch_ledger(c->cfg.n, c->cfg.nf, t->ledger_dev, t->h[0], dt_dev, dt, beta, c->cfg.z_reflect, t->fo_dev,
+                       c->stream));
+  return inject(c);
+}
+
 int tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
   TreeState* t = c->tree;
   if (c->cfg.physics == AMRH_PHYS_ROTATING) return rot_tree_substep(c, dt, beta, dt_dev);
+  if (c->cfg.recon == AMRH_RECON_PPM) return block_tree_substep(c, dt, beta, dt_dev);
   if (int rc = fill_ghosts(c)) return rc;
   const PhysParams P = params(c->cfg);
   const bool mixed = c->cfg.precision == AMRH_MIXED;
@@ -330,10 +368,13 @@ extern "C" {
 
 int amrh_tree_set(amrh_ctx* c, const int* parent, const int* child) {
   if (!c || !parent || !child) return fail(AMRH_EINVAL, "null argument");
-  const bool euler_ok = c->cfg.physics == AMRH_PHYS_EULER && c->cfg.recon == AMRH_RECON_MINMOD;
-  const bool rot_ok = c->cfg.physics == AMRH_PHYS_ROTATING;  // n = 8, minmod g = 2 or PPM g = 3
+  // the ideal gas: minmod (g = 2, n = 8 or 16) or PPM (g = 3, n = 8); the rotating-frame state: n = 8,
+  // minmod g = 2 or PPM g = 3 (amrh_create checks the pairs)
+  const bool euler_ok = c->cfg.physics == AMRH_PHYS_EULER &&
+                        (c->cfg.recon == AMRH_RECON_MINMOD || (c->cfg.recon == AMRH_RECON_PPM && c->cfg.n == 8));
+  const bool rot_ok = c->cfg.physics == AMRH_PHYS_ROTATING;
   if (!euler_ok && !rot_ok)
-    return fail(AMRH_EINVAL, "the octree coupling runs the ideal-gas minmod scheme or the rotating-frame state");
+    return fail(AMRH_EINVAL, "the octree coupling runs the ideal gas (minmod, or PPM on n = 8) or the rotating state");
   if (int rc = build(c, parent, child)) return rc;
   HIPCHK(hipMemcpy(c->h_dev, c->tree->h.data(), sizeof(double) * c->nb, hipMemcpyHostToDevice));
   c->have_h = true;

@@ -104,7 +104,9 @@ __device__ __forceinline__ double eig_of_q(const PhysParams& P, typename Prec<MI
 }
 
 // MODE 0: substep.  MODE 1: per-block max wavespeed (Grid::max_dt's scan, grid/grid.cpp:422-473).
-template <int N, int G, int RECON, bool MIXED, int MODE>
+// TREE: one level of the octree (a.nodes): reflux overrides applied, the block's own boundary
+// faces exported to the coarser nodes' slots, the root's boundary faces to the ledger.
+template <int N, int G, int RECON, bool MIXED, int MODE, bool TREE>
 __global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
   using T = typename Prec<MIXED>::T;
   constexpr int NT = N * N * N;
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEig
   const int ci = tid % N, cj = (tid / N) % N, ck = tid / (N * N);
 
   for (int bb = blockIdx.x; bb < nbl; bb += gridDim.x) {
-    const int b = MODE == 0 ? a.block0 + bb : bb;
+    const int b = MODE == 0 ? (TREE ? a.nodes[bb] : a.block0 + bb) : bb;
     const double* Ub = (MODE == 0 ? a.Uin : m.U) + (long)b * NFE * R;
 
     // stage the primitives; each thread keeps its own interior cell's conserved state
@@ -169,6 +171,53 @@ __global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEig
       }
       if constexpr (MODE == 0) {
         __syncthreads();
+        if constexpr (TREE) {
+          // adjust_*_flux (grid/grid_node.cpp:1076-1274): faces of the low, middle and high planes
+          // next to a finer region take the means the finer level exported
+#pragma unroll 1
+          for (int q2 = tid; q2 < 3 * N * N; q2 += NT) {
+            const int pl = q2 / (N * N), ta = q2 % N, tb = (q2 / N) % N;
+            const int qa = ta >= N / 2, qb = tb >= N / 2;
+            const int sl = a.slot[((b * 3 + d) * 3 + pl) * 4 + qa + 2 * qb];
+            if (sl < 0) continue;
+            const int f = pl == 0 ? 0 : (pl == 1 ? N / 2 : N);
+            const double* o =
+                a.ovr + (((long)sl * (N / 2) + (tb - qb * (N / 2))) * (N / 2) + (ta - qa * (N / 2))) * NFE;
+            const int q = f + (N + 1) * (ta + N * tb);
+#pragma unroll
+            for (int l = 0; l < NFE; l++) Fb[l][q] = (T)o[l];
+          }
+          __syncthreads();
+          if (tid < 4 * (N / 2) * (N / 2)) {
+            const int w = tid / ((N / 2) * (N / 2)), r = tid % ((N / 2) * (N / 2));
+            const int side = w >> 1;
+            const int sl = a.exp_slot[(b * 6 + 2 * d + side) * 2 + (w & 1)];
+            if (sl >= 0) {
+              const int al = r % (N / 2), bl = r / (N / 2);
+              const int q = (side ? N : 0) + (N + 1) * (2 * al + N * 2 * bl);
+              double* o = a.ovr_out + (((long)sl * (N / 2) + bl) * (N / 2) + al) * NFE;
+#pragma unroll
+              for (int l = 0; l < NFE; l++) {
+                double v = (double)Fb[l][q];
+                v += (double)Fb[l][q + (N + 1)];
+                v += (double)Fb[l][q + (N + 1) * N];
+                v += (double)Fb[l][q + (N + 1) * (N + 1)];
+                v *= 0.25;
+                o[l] = v;
+              }
+            }
+          }
+          if (b == a.ledger_block && a.ledger) {
+#pragma unroll 1
+            for (int q2 = tid; q2 < 2 * N * N; q2 += NT) {
+              const int side = q2 / (N * N), ta = q2 % N, tb = (q2 / N) % N;
+              const int q = (side ? N : 0) + (N + 1) * (ta + N * tb);
+              double* o = a.ledger + ((((long)d * 2 + side) * N + tb) * N + ta) * NFE;
+#pragma unroll
+              for (int l = 0; l < NFE; l++) o[l] = (double)Fb[l][q];
+            }
+          }
+        }
         const int p = d == 0 ? ci : (d == 1 ? cj : ck);
         const int ta = d == 0 ? cj : ci;
         const int tb = d == 2 ? cj : ck;
@@ -234,7 +283,10 @@ template <int N, int G, int RECON, bool MIXED, int MODE>
 hipError_t launch_one(const SubstepArgs& a, const MaxEigArgs& m, int nblocks, int grid, hipStream_t s) {
   const int g = grid > 0 && grid < nblocks ? grid : nblocks;
   if (nblocks <= 0) return hipSuccess;
-  kt_block_kernel<N, G, RECON, MIXED, MODE><<<g, N * N * N, 0, s>>>(a, m);
+  if (MODE == 0 && a.nodes)
+    kt_block_kernel<N, G, RECON, MIXED, MODE, MODE == 0><<<g, N * N * N, 0, s>>>(a, m);
+  else
+    kt_block_kernel<N, G, RECON, MIXED, MODE, false><<<g, N * N * N, 0, s>>>(a, m);
   return hipGetLastError();
 }
 

@@ -34,6 +34,11 @@ struct SubstepArgs {
   const double* ovr;       // override face fluxes [slot][N/2][N/2][NF] (fine-face averages)
   int ledger_block;        // block whose domain-boundary fluxes feed the flow-off ledger
   double* ledger;          // [3 dir][2 side][N][N][NF] boundary face fluxes of that block
+  // octree, level by level (the block kernel): the launch's blocks are nodes[0 .. nblocks), and
+  // each writes the 2x2 means of its boundary faces to the coarse slots exp_slot[node][face][2]
+  const int* nodes;
+  const int* exp_slot;
+  double* ovr_out;         // = ovr, written
   // isolated outflow blocks: write the next stage's ghosts with the interior
   int outflow;
   int z_reflect;

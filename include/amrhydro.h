@@ -200,7 +200,8 @@ int amrh_tree_flow_off(amrh_ctx* ctx, double* fo);
  * is filled from its parent by GridNode::inject_from_parent (grid/grid_node.cpp:193-233: minmod
  * slopes per axis, then the floor), level by level; the root's flow-off ledger carries over; ghosts
  * are filled at the end.  Nodes absent from the new tree are dropped (their parents already hold
- * the restricted values).  Synchronous. */
+ * the restricted values).  The ideal gas only (the rotating state's Binary::inject_from_parent is
+ * another prolongation).  Synchronous. */
 int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index);
 /* The density test of GridNode::set_refine_flags (grid/grid_node.cpp:647-667): flags[b] bit c is
  * set when an interior cell of node b's octant c has rho > min_density.  The flag propagation,

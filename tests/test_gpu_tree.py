@@ -25,7 +25,8 @@ def rel_l1_leaves(a, b, leaves, g=2):
 
 
 def gpu_tree(cfg_o, parent, child, U):
-    cfg = amrhydro.euler_config(n=cfg_o.n, grid_dim=cfg_o.grid_dim, z_reflect=cfg_o.z_reflect)
+    cfg = amrhydro.euler_config(n=cfg_o.n, g=cfg_o.g, grid_dim=cfg_o.grid_dim, z_reflect=cfg_o.z_reflect,
+                                recon=amrhydro.AMRH_RECON_PPM if cfg_o.recon == 1 else amrhydro.AMRH_RECON_MINMOD)
     B = amrhydro.Batch(cfg, len(parent), 1.0)
     B.tree_set(parent, child)
     B.upload(U)
@@ -62,7 +63,7 @@ def run_and_compare(cfg, spec, ic, nsub, label):
             done += 1
             out = B.download()
             ref = tc.padded_state(T)
-            e = rel_l1_leaves(out, ref, leaves)
+            e = rel_l1_leaves(out, ref, leaves, cfg.g)
             say("%s substep %d beta=%.3f rel-L1 %s" % (label, done, beta, np.array2string(e, precision=2)))
             assert np.all(e <= TOL), e
         B.tree_fill_ghosts()
@@ -96,6 +97,18 @@ def test_root_only_tree_matches_oracle():
     """A tree of one node: every face is physical, nothing to reflux or restrict."""
     cfg = orc.euler_cfg(n=8, g=2, grid_dim=0.5, z_reflect=0)
     run_and_compare(cfg, [], tc.blob_ic, 6, "root")
+
+
+@pytest.mark.parametrize("z_reflect", [0, 1])
+def test_partial_tree_ppm_matches_oracle(z_reflect):
+    """PPM (g = 3) on the octree: the block kernel level by level with its reflux exports"""
+    cfg = orc.euler_cfg(n=8, g=3, recon=1, grid_dim=0.5, z_reflect=z_reflect)
+    run_and_compare(cfg, tc.partial_spec(), tc.blob_ic, 6, "ppm zr=%d" % z_reflect)
+
+
+def test_config1_sedov_tree_ppm_matches_oracle():
+    cfg = orc.euler_cfg(n=8, g=3, recon=1, grid_dim=0.5, z_reflect=0)
+    run_and_compare(cfg, tc.full_spec(2), lambda T: tc.sedov_ic(T, 2), 6, "sedov ppm")
 
 
 def test_n16_tree_matches_oracle():

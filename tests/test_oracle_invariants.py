@@ -146,11 +146,13 @@ def test_tree_offsets_and_widths():
     assert off == (2 * 2 + 2 + 8,) * 3
 
 
-def test_amr_ghosts_reproduce_normal_linear_profile():
+@pytest.mark.parametrize("g,recon", [(2, 0), (3, 1)])
+def test_amr_ghosts_reproduce_normal_linear_profile(g, recon):
     """Coarse->fine ghost = u -/+ minmod(normal slope)/4 (grid/grid_amr_bound.cpp:27-78): exact for a
     profile linear along the face normal; piecewise constant (the parent cell's value) along the
-    face.  Node 9+1 (child 1 of the root's child 0) has a coarse neighbour across its x-upper face."""
-    cfg = orc.euler_cfg(n=8, grid_dim=0.5)
+    face.  Node 9+1 (child 1 of the root's child 0) has a coarse neighbour across its x-upper face.
+    Ghost width 2 (minmod) and 3 (PPM)."""
+    cfg = orc.euler_cfg(n=8, g=g, recon=recon, grid_dim=0.5)
     T = partial_tree(cfg)
     node = 9 + 1
 
@@ -160,9 +162,12 @@ def test_amr_ghosts_reproduce_normal_linear_profile():
 
     fill_tree(T, lin)
     want = T.U(node).copy()
-    T.U(node)[:, :, :, -2:] = -99.0
+    T.U(node)[:, :, :, -g:] = -99.0
     T.enforce_boundaries()
-    np.testing.assert_allclose(T.U(node)[0, 2:-2, 2:-2, -2:], want[0, 2:-2, 2:-2, -2:], rtol=0, atol=1e-14)
+    I = (0, slice(g, -g), slice(g, -g), slice(-g, None))
+    np.testing.assert_allclose(T.U(node)[I], want[I], rtol=0, atol=1e-14)
+    if g != 2:
+        return
 
     def tang(X, Y, Z):
         rho = 2.0 + 0.25 * Y + 0 * X
@@ -201,11 +206,12 @@ def test_restriction_is_child_mean():
             np.testing.assert_allclose(blk[f], mean[f], rtol=0, atol=2e-15 * np.abs(K[f]).max())
 
 
-def test_tree_conserves_mass_and_energy_with_reflux():
+@pytest.mark.parametrize("g,recon", [(2, 0), (3, 1)])
+def test_tree_conserves_mass_and_energy_with_reflux(g, recon):
     """Sum over leaf cells of U*h^3 plus the root's boundary-flux ledger is constant to roundoff
     across RK3 steps of a tree with coarse/fine interfaces: reflux (grid/grid_node.cpp:1076-1274)
-    and restriction keep the scheme conservative."""
-    cfg = orc.euler_cfg(n=8, grid_dim=0.5)
+    and restriction keep the scheme conservative, with minmod and with PPM."""
+    cfg = orc.euler_cfg(n=8, g=g, recon=recon, grid_dim=0.5)
     T = partial_tree(cfg)
 
     def blob(X, Y, Z):
