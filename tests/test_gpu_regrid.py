@@ -21,20 +21,22 @@ n, g = 8, 2
 G = n + 2 * g
 
 
-def oracle_tree(spec):
-    cfg = orc.euler_cfg(n=n, g=g, grid_dim=0.5, z_reflect=0)
+def oracle_tree(spec, g=g):
+    cfg = orc.euler_cfg(n=n, g=g, recon=1 if g == 3 else 0, grid_dim=0.5, z_reflect=0)
     return tc.build(cfg, spec)
 
 
-def gpu_tree(parent, child, U=None):
-    B = amrhydro.Batch(amrhydro.euler_config(n=n, grid_dim=0.5), len(parent), 1.0)
+def gpu_tree(parent, child, U=None, g=g):
+    recon = amrhydro.AMRH_RECON_PPM if g == 3 else amrhydro.AMRH_RECON_MINMOD
+    B = amrhydro.Batch(amrhydro.euler_config(n=n, g=g, grid_dim=0.5, recon=recon), len(parent), 1.0)
     B.tree_set(parent, child)
     if U is not None:
         B.upload(U)
     return B
 
 
-def slab_mask():
+def slab_mask(g=g):
+    G = n + 2 * g
     m = np.zeros((G, G, G), bool)
     s = slice(g, G - g)
     m[:, s, s] = True
@@ -43,9 +45,10 @@ def slab_mask():
     return m
 
 
-def test_repack_after_refinement_matches_oracle():
+@pytest.mark.parametrize("g", [2, 3])
+def test_repack_after_refinement_matches_oracle(g):
     spec1 = tc.partial_spec()
-    T1, p1, c1 = oracle_tree(spec1)
+    T1, p1, c1 = oracle_tree(spec1, g)
     tc.blob_ic(T1)
     T1.enforce_boundaries()
     T1.inject_from_children()
@@ -54,7 +57,7 @@ def test_repack_after_refinement_matches_oracle():
     # refine two leaves: node 3 (level 1) and node 20 (level 2)
     leaves = [3, 20]
     spec2 = spec1 + [(p, c) for p in leaves for c in range(8)]
-    T2, p2, c2 = oracle_tree(spec2)
+    T2, p2, c2 = oracle_tree(spec2, g)
     n1 = len(p1)
     for b in range(n1):
         T2.U(b)[:] = U1[b]
@@ -62,8 +65,8 @@ def test_repack_after_refinement_matches_oracle():
     for b in new:  # parents are old nodes with filled ghosts
         T2.prolong(b)
     ref = tc.padded_state(T2)
-    src = gpu_tree(p1, c1, U1)
-    dst = gpu_tree(p2, c2)
+    src = gpu_tree(p1, c1, U1, g)
+    dst = gpu_tree(p2, c2, g=g)
     idx = np.array(list(range(n1)) + [-1] * len(new), np.int32)
     dst.tree_repack_from(src, idx)
     out = dst.download()
@@ -74,19 +77,20 @@ def test_repack_after_refinement_matches_oracle():
     np.testing.assert_array_equal(out[n1:][J], ref[n1:][J])
     T_ = (slice(None), 5, slice(g, -g), slice(g, -g), slice(g, -g))
     np.testing.assert_allclose(out[n1:][T_], ref[n1:][T_], rtol=1e-15, atol=0)
-    say("repack: %d old nodes carried, %d new nodes prolongated, match" % (n1, len(new)))
+    say("repack g=%d: %d old nodes carried, %d new nodes prolongated, match" % (g, n1, len(new)))
     # the new tree steps on: ghosts were filled by the repack (same as the oracle's fill)
     T2.enforce_boundaries()
-    m = slab_mask()
+    m = slab_mask(g)
     np.testing.assert_allclose(out[:, :, m], tc.padded_state(T2)[:, :, m], rtol=1e-15, atol=0)
 
 
-def test_density_flags_match_numpy():
+@pytest.mark.parametrize("g", [2, 3])
+def test_density_flags_match_numpy(g):
     spec = tc.partial_spec()
-    T, p, c = oracle_tree(spec)
+    T, p, c = oracle_tree(spec, g)
     tc.blob_ic(T)
     U = tc.padded_state(T)
-    B = gpu_tree(p, c, U)
+    B = gpu_tree(p, c, U, g)
     thr = 1.5
     got = B.tree_density_flags(thr)
     rho = U[:, 0, g:-g, g:-g, g:-g]
