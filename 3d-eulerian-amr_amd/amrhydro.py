@@ -128,6 +128,7 @@ SIGNATURES = {
     "amrh_poisson_download": (C.c_int, [C.c_void_p, C.c_int, DP]),
     "amrh_poisson_phi_device": (C.c_void_p, [C.c_void_p]),
     "amrh_poisson_forces_device": (C.c_void_p, [C.c_void_p]),
+    "amrh_poisson_phi_from_state": (C.c_int, [C.c_void_p, C.c_void_p]),
     "amrh_poisson_source_from_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
     "amrh_poisson_prepare": (C.c_int, [C.c_void_p]),
     "amrh_poisson_boundary_tables": (C.c_int, [C.c_void_p, DP, C.c_int]),
@@ -510,6 +511,10 @@ class Poisson:
 
     def forces_device(self):
         return lib().amrh_poisson_forces_device(self.h)
+
+    def phi_from_state(self, batch):
+        """Binary::pot_from_grid: phi = pot / rho + R^2 omega0^2 / 2, then the ghosts"""
+        _ck(lib().amrh_poisson_phi_from_state(self.h, batch.ctx))
 
     def source_from(self, batch, G=1.0):
         _ck(lib().amrh_poisson_source_from_state(self.h, batch.ctx, float(G)))

@@ -737,6 +737,26 @@ __global__ __launch_bounds__(NTHREADS) void source_from_state_kernel(PArgs A, co
   }
 }
 
+// Binary::pot_from_grid (binary/binary.cpp:857-872): phi = pot / rho + R^2 Omega0^2 / 2 on the
+// interior, from a rotating-state batch of the same nodes
+__global__ __launch_bounds__(NTHREADS) void phi_from_state_kernel(PArgs A, const double* U, long rec, int nf,
+                                                                  const double* x0s, const double* hs,
+                                                                  double omega0) {
+  const int b = blockIdx.x;
+  const int P = A.P, M = P - 2;
+  const double* rho = U + (long)b * nf * rec;
+  const double* pot = rho + 6 * rec;  // the pot field (binary/state_ztwd.cpp:15-28)
+  double* phi = A.f[PHI] + (long)b * A.P3;
+  const double h = hs[b];
+  for (int t = threadIdx.x; t < M * M * M; t += blockDim.x) {
+    const int i = t % M, j = (t / M) % M;
+    const double x = x0s[3L * b] + (double)i * h + 0.5 * h;
+    const double y = x0s[3L * b + 1] + (double)j * h + 0.5 * h;
+    const double R2 = x * x + y * y;
+    phi[cell(P, i + 1, j + 1, t / (M * M) + 1)] = pot[t] / rho[t] + 0.5 * R2 * omega0 * omega0;
+  }
+}
+
 }  // namespace
 
 struct amrh_poisson {
@@ -1104,6 +1124,21 @@ int amrh_poisson_source_from_state(amrh_poisson* p, const amrh_ctx* c, double G)
   PCHK(hipGetLastError());
   PCHK(hipStreamSynchronize(p->stream));
   return AMRH_OK;
+}
+
+int amrh_poisson_phi_from_state(amrh_poisson* p, const amrh_ctx* c) {
+  if (!p || !c) return fail(AMRH_EINVAL, "null argument");
+  if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "phi from pot needs the rotating-frame state");
+  if (!c->have_u || !c->have_x0) return fail(AMRH_ESTATE, "the hydro context holds no state or positions");
+  if (c->nb != p->nn || c->cfg.n != p->n) return fail(AMRH_EINVAL, "poisson: the hydro batch must hold the same nodes");
+  PCHK(hipStreamSynchronize(c->stream));
+  phi_from_state_kernel<<<p->nn, NTHREADS, 0, p->stream>>>(p->args(p->all_start), c->buf[c->cur], c->L.rec(),
+                                                            c->cfg.nf, c->x0_dev, c->h_dev, c->cfg.omega0);
+  PCHK(hipGetLastError());
+  PCHK(hipStreamSynchronize(p->stream));
+  // the root's enforce_phi_boundaries for every level (binary/binary.cpp:878-883), once the
+  // physical faces have their tables; before that the solve's own fill follows
+  return p->have_tables ? amrh_poisson_phi_boundaries(p) : AMRH_OK;
 }
 
 int amrh_poisson_prepare(amrh_poisson* p) {

@@ -106,8 +106,10 @@ int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index) {
   if (!src->have_u) return fail(AMRH_ESTATE, "the source holds no state");
   if (dst->cfg.n != src->cfg.n || dst->cfg.g != src->cfg.g || dst->cfg.nf != src->cfg.nf)
     return fail(AMRH_EINVAL, "block shapes differ");
-  if (dst->cfg.physics != AMRH_PHYS_EULER || src->cfg.physics != AMRH_PHYS_EULER)
-    return fail(AMRH_EINVAL, "re-packing prolongs the ideal-gas state (GridNode::inject_from_parent) only");
+  if (dst->cfg.physics != src->cfg.physics) return fail(AMRH_EINVAL, "the two contexts run different physics");
+  const bool rot = dst->cfg.physics == AMRH_PHYS_ROTATING;
+  if (rot && dst->cfg.eos == AMRH_EOS_HELMHOLTZ && !dst->eos)
+    return fail(AMRH_ESTATE, "eos = HELMHOLTZ needs amrh_attach_eos on the new context");
   const TreeState* t = dst->tree;
   const int nb = dst->nb, n = dst->cfg.n, nf = dst->cfg.nf;
   std::vector<int> idx(src_index, src_index + nb);
@@ -149,7 +151,11 @@ int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index) {
     const long total = (long)fresh.size() * (n / 2) * (n / 2) * (n / 2);
     if (e == hipSuccess) {
       const unsigned grid = (unsigned)((total + 63) / 64);
-      if (n == 8 && dst->cfg.g == 3)
+      if (rot) {
+        const RotArgs ra = ctx_rot_args(dst);
+        e = launch_rot_prolong(dst->cfg.g, dst->cfg.eos, dst->buf[dst->cur], nodes_dev, (int)fresh.size(), dst->x0_dev,
+                               dst->h_dev, ra.c, ra.H, dst->stream);
+      } else if (n == 8 && dst->cfg.g == 3)
         prolong_kernel<8, 3><<<grid, 64, 0, dst->stream>>>(dst->buf[dst->cur], nodes_dev, (int)fresh.size(),
                                                             params(dst->cfg));
       else if (n == 8)
@@ -158,7 +164,7 @@ int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index) {
       else
         prolong_kernel<16, 2><<<grid, 64, 0, dst->stream>>>(dst->buf[dst->cur], nodes_dev, (int)fresh.size(),
                                                              params(dst->cfg));
-      e = hipGetLastError();
+      if (e == hipSuccess) e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(dst->stream);
     (void)hipFree(nodes_dev);

@@ -45,6 +45,9 @@ hipError_t launch_rot_tree_ghosts(int g, double* U, const int* nodes, int count,
 // inject_from_children for one parent level: pairs (parent, child << 24 | child node)
 hipError_t launch_rot_restrict(int g, int eos, double* U, const int2* pairs, int count, const double* x0,
                                const double* h, const rot::RotParams& c, const helm::HelmDev& H, hipStream_t s);
+// inject_from_parent of new nodes (node, parent, child index), floored at the child cells
+hipError_t launch_rot_prolong(int g, int eos, double* U, const int3* nodes, int count, const double* x0,
+                              const double* h, const rot::RotParams& c, const helm::HelmDev& H, hipStream_t s);
 // Binary::pot_to_grid on every block: pot = rho (phi - R^2 Omega0^2 / 2)
 hipError_t launch_rot_pot_to_grid(int g, double* U, const double* phi, const double* x0, const double* h,
                                   int nblocks, const rot::RotParams& c, hipStream_t s);

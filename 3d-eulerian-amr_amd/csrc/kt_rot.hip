@@ -596,6 +596,70 @@ __global__ __launch_bounds__(256) void rot_restrict_kernel(double* U, const int2
   for (int l = 0; l < NF; l++) Up[(long)l * R + at(pi, pj, pk)] = u[l];
 }
 
+// GridNode::inject_from_parent (grid/grid_node.cpp:193-233) of the rotating state: one thread per
+// 2x2x2 group of a new child's cells, minmod slopes of the parent's cells per axis, each child cell
+// floored at its own position
+template <int G, int EOS>
+__global__ __launch_bounds__(64) void rot_prolong_kernel(double* U, const int3* nodes, int count, const double* x0s,
+                                                         const double* hs, RotParams c, helm::HelmDev H) {
+  constexpr int R = N3 + 6 * G * N2;
+  constexpr int HN = N / 2, GP = N + 2 * G;
+  constexpr Layout L{N, G, NF};
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)count * HN * HN * HN) return;
+  const int3 t = nodes[gid / (HN * HN * HN)];
+  const int r = (int)(gid % (HN * HN * HN));
+  const int node = t.x, parent = t.y, ch = t.z;
+  const int cx = ch & 1, cy = (ch >> 1) & 1, cz = (ch >> 2) & 1;
+  const int ii = r % HN, jj = (r / HN) % HN, kk = r / (HN * HN);
+  // the parent's padded cell of the group (grid/grid_node.cpp:204-206)
+  const int i0 = (G + G + 2 * ii) / 2 + cx * (GP / 2 - G);
+  const int j0 = (G + G + 2 * jj) / 2 + cy * (GP / 2 - G);
+  const int k0 = (G + G + 2 * kk) / 2 + cz * (GP / 2 - G);
+  const double* Up = U + (long)parent * NF * R;
+  double* Uc = U + (long)node * NF * R;
+  const long e0 = L.from_padded(i0, j0, k0);
+  const long exm = L.from_padded(i0 - 1, j0, k0), exp_ = L.from_padded(i0 + 1, j0, k0);
+  const long eym = L.from_padded(i0, j0 - 1, k0), eyp = L.from_padded(i0, j0 + 1, k0);
+  const long ezm = L.from_padded(i0, j0, k0 - 1), ezp = L.from_padded(i0, j0, k0 + 1);
+  double s1[NF], s2[NF], s3[NF], u0[NF];
+#pragma unroll
+  for (int l = 0; l < NF; l++) {
+    const double* f = Up + (long)l * R;
+    const double u = f[e0];
+    u0[l] = u;
+    s1[l] = x64::minmod2(f[exp_] - u, u - f[exm]);
+    s2[l] = x64::minmod2(f[eyp] - u, u - f[eym]);
+    s3[l] = x64::minmod2(f[ezp] - u, u - f[ezm]);
+  }
+#pragma unroll 1
+  for (int q = 0; q < 8; q++) {
+    const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
+    double v[NF];
+#pragma unroll
+    for (int l = 0; l < NF; l++) {
+      const double u = u0[l];
+      switch (q) {
+        case 0: v[l] = u - (s1[l] + s2[l] + s3[l]) * 0.25; break;
+        case 1: v[l] = u + (s1[l] - s2[l] - s3[l]) * 0.25; break;
+        case 2: v[l] = u - (s1[l] - s2[l] + s3[l]) * 0.25; break;
+        case 3: v[l] = u + (s1[l] + s2[l] - s3[l]) * 0.25; break;
+        case 4: v[l] = u - (s1[l] + s2[l] - s3[l]) * 0.25; break;
+        case 5: v[l] = u + (s1[l] - s2[l] + s3[l]) * 0.25; break;
+        case 6: v[l] = u - (s1[l] - s2[l] - s3[l]) * 0.25; break;
+        default: v[l] = u + (s1[l] + s2[l] + s3[l]) * 0.25; break;
+      }
+    }
+    const int ci = 2 * ii + qx, cj = 2 * jj + qy, ck = 2 * kk + qz;
+    double X[3];
+    cell_pos(x0s + 3L * node, hs[node], ci, cj, ck, X);
+    floor_state<EOS>(H, c, v, X);
+    const long e = ((long)ck * N + cj) * N + ci;
+#pragma unroll
+    for (int l = 0; l < NF; l++) Uc[(long)l * R + e] = v[l];
+  }
+}
+
 // Binary::pot_to_grid (binary/binary.cpp:825-849)
 template <int G>
 __global__ __launch_bounds__(256) void rot_pot_kernel(double* U, const double* phi, const double* x0s,
@@ -676,6 +740,25 @@ hipError_t launch_rot_restrict(int g, int eos, double* U, const int2* pairs, int
     rot_restrict_kernel<2, EOS_HELMHOLTZ><<<grid, 256, 0, s>>>(U, pairs, count, x0, h, c, H);
   else if (g == 2)
     rot_restrict_kernel<2, EOS_IDEAL><<<grid, 256, 0, s>>>(U, pairs, count, x0, h, c, H);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_rot_prolong(int g, int eos, double* U, const int3* nodes, int count, const double* x0,
+                              const double* h, const RotParams& c, const helm::HelmDev& H, hipStream_t s) {
+  const long total = (long)count * (N / 2) * (N / 2) * (N / 2);
+  if (total == 0) return hipSuccess;
+  const unsigned grid = (unsigned)((total + 63) / 64);
+  const bool helm = eos == EOS_HELMHOLTZ;
+  if (g == 3 && helm)
+    rot_prolong_kernel<3, EOS_HELMHOLTZ><<<grid, 64, 0, s>>>(U, nodes, count, x0, h, c, H);
+  else if (g == 3)
+    rot_prolong_kernel<3, EOS_IDEAL><<<grid, 64, 0, s>>>(U, nodes, count, x0, h, c, H);
+  else if (g == 2 && helm)
+    rot_prolong_kernel<2, EOS_HELMHOLTZ><<<grid, 64, 0, s>>>(U, nodes, count, x0, h, c, H);
+  else if (g == 2)
+    rot_prolong_kernel<2, EOS_IDEAL><<<grid, 64, 0, s>>>(U, nodes, count, x0, h, c, H);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

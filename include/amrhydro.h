@@ -338,6 +338,13 @@ const void* amrh_poisson_forces_device(const amrh_poisson* p);
  * nodes (same n): S = 4 pi G rho on the interior (State::poisson_source,
  * binary/state_ztwd.cpp:219-222).  Synchronous. */
 int amrh_poisson_source_from_state(amrh_poisson* p, const amrh_ctx* hydro, double G);
+/* Binary::pot_from_grid (binary/binary.cpp:857-884) from a rotating-state context whose blocks are
+ * the same nodes: phi = pot / rho + R^2 omega0^2 / 2 on every interior cell, then, once
+ * amrh_poisson_prepare has made the boundary tables, the ghosts of every level
+ * (enforce_phi_boundaries).  Used for the first
+ * solve and after a regrid, where Binary::inject_from_parent recomputes a new node's phi this way
+ * (binary/binary.cpp:852-855).  Synchronous. */
+int amrh_poisson_phi_from_state(amrh_poisson* p, const amrh_ctx* hydro);
 /* Binary::compute_source_sums and compute_physical_boundaries (binary/binary.cpp:938-1019): per
  * node and octant the source's monopole, and the boundary phi of every physical face as the
  * tree's sum of -alpha m / r (alpha = 2 with z reflection).  Synchronous. */

@@ -124,6 +124,9 @@ int orc_rot_tree_store(orc_rot_tree* t);
 int orc_rot_tree_substep(orc_rot_tree* t, double dt, double beta);
 int orc_rot_tree_pot_to_grid(orc_rot_tree* t);
 int orc_rot_tree_inject(orc_rot_tree* t);
+/* GridNode::inject_from_parent (grid/grid_node.cpp:193-233) into node from its parent (ghosts
+ * filled), floored at the child's cells; returns the floor's EOS flags */
+int orc_rot_tree_prolong(orc_rot_tree* t, int node);
 double orc_rot_tree_max_dt(orc_rot_tree* t);
 int orc_rot_tree_flow_off(const orc_rot_tree* t, double* fo);
 int orc_rot_tree_state_sum(const orc_rot_tree* t, double* sum);

@@ -721,9 +721,57 @@ int rot_inject_child(RT* t, int pi, int q) {
   return err;
 }
 
+// GridNode::inject_from_parent (grid/grid_node.cpp:193-233) of the rotating state: a new child's
+// interior from its parent's cells (ghosts filled) with minmod slopes per axis, each child cell
+// floored at its own position (Binary::inject_from_parent, binary/binary.cpp:852-855, adds only
+// the potential, which the solver owns)
+int rot_prolong_child(RT* t, int ni) {
+  const RC& c = t->c;
+  RT::RNode& C = t->nodes[ni];
+  const RT::RNode& P = t->nodes[C.parent];
+  const int q = C.cidx;
+  const int cx = q & 1, cy = (q >> 1) & 1, cz = (q >> 2) & 1;
+  int err = 0;
+  auto Pu = [&](int l, int i, int j, int k) { return P.U[l * c.G3 + c.at(i, j, k)]; };
+  for (int k = c.g; k <= c.G - c.g - 1; k += 2)
+    for (int j = c.g; j <= c.G - c.g - 1; j += 2) {
+      const int k0 = (c.g + k) / 2 + cz * (c.G / 2 - c.g);
+      const int j0 = (c.g + j) / 2 + cy * (c.G / 2 - c.g);
+      int i0 = c.g + cx * (c.G / 2 - c.g);
+      for (int i = c.g; i < c.G - c.g; i += 2, i0++) {
+        double v[8][NFR];
+        for (int l = 0; l < NFR; l++) {
+          const double u = Pu(l, i0, j0, k0);
+          const double s1 = minmod2(Pu(l, i0 + 1, j0, k0) - u, u - Pu(l, i0 - 1, j0, k0));
+          const double s2 = minmod2(Pu(l, i0, j0 + 1, k0) - u, u - Pu(l, i0, j0 - 1, k0));
+          const double s3 = minmod2(Pu(l, i0, j0, k0 + 1) - u, u - Pu(l, i0, j0, k0 - 1));
+          v[0][l] = u - (s1 + s2 + s3) * 0.25;
+          v[1][l] = u + (s1 - s2 - s3) * 0.25;
+          v[2][l] = u - (s1 - s2 + s3) * 0.25;
+          v[3][l] = u + (s1 + s2 - s3) * 0.25;
+          v[4][l] = u - (s1 + s2 - s3) * 0.25;
+          v[5][l] = u + (s1 - s2 + s3) * 0.25;
+          v[6][l] = u - (s1 - s2 - s3) * 0.25;
+          v[7][l] = u + (s1 + s2 + s3) * 0.25;
+        }
+        for (int m = 0; m < 8; m++) {
+          const int ii = i + (m & 1), jj = j + ((m >> 1) & 1), kk = k + (m >> 2);
+          err |= floor_state(c, v[m], pos(c, C.x0, C.h, ii, jj, kk, -1));
+          for (int l = 0; l < NFR; l++) C.U[l * c.G3 + c.at(ii, jj, kk)] = v[m][l];
+        }
+      }
+    }
+  return err;
+}
+
 }  // namespace
 
 extern "C" {
+
+int orc_rot_tree_prolong(orc_rot_tree* t, int node) {
+  if (node <= 0 || node >= (int)t->nodes.size()) return -1;
+  return rot_prolong_child(t, node);
+}
 
 orc_rot_tree* orc_rot_tree_create(const orc_rot_cfg* cfg, double grid_dim, int nn, const int* parent,
                                   const int* cidx) {

@@ -344,6 +344,7 @@ class RotTree:
             L.orc_rot_tree_use_forces.restype = None
             for f in ("fill_ghosts", "store", "pot_to_grid", "inject"):
                 getattr(L, "orc_rot_tree_" + f).argtypes = [vp]
+            L.orc_rot_tree_prolong.argtypes = [vp, C.c_int]
             L.orc_rot_tree_substep.argtypes = [vp, C.c_double, C.c_double]
             L.orc_rot_tree_max_dt.argtypes = [vp]
             L.orc_rot_tree_max_dt.restype = C.c_double
@@ -421,6 +422,9 @@ class RotTree:
 
     def inject(self):
         return self.L.orc_rot_tree_inject(self.t)
+
+    def prolong(self, node):
+        return self.L.orc_rot_tree_prolong(self.t, node)
 
     def max_dt(self):
         return self.L.orc_rot_tree_max_dt(self.t)

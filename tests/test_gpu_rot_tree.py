@@ -186,3 +186,57 @@ def test_rot_tree_mixed_stays_near_fp64_oracle():
     say("rot tree mixed after 5 RK3 steps: rel-L1 rho %.2e, max over fields %.2e" % (e[0], e.max()))
     assert e[0] <= 1e-6 and e.max() <= 1e-5
     B.close()
+
+
+def test_rot_tree_repack_prolongs_like_the_oracle():
+    """A regrid of the rotating state: two leaves refined (a level-1 and a level-2 node); surviving
+    nodes carry over bit for bit, the new nodes take GridNode::inject_from_parent with the rotating
+    floor at their own cells (grid/grid_node.cpp:193-233), as the oracle does"""
+    from test_oracle_poisson import tree
+    from test_oracle_rot_tree import partial_spec
+    T1, cfg = binary_tree_case("partial", seed=27)
+    T1.fill_ghosts()
+    U1 = T1.state()
+    spec2 = partial_spec() + [(p, c) for p in (3, 10) for c in range(8)]
+    p2, c2 = tree(spec2)
+    T2 = orc.RotTree(cfg, p2, c2)
+    n1 = T1.nn
+    for b in range(n1):
+        T2.U(b)[:] = U1[b]
+    for b in range(n1, T2.nn):
+        assert T2.prolong(b) == 0
+    src = device_batch(T1, cfg)
+    d = amrhydro.rotating_config(eos=cfg.eos, recon=cfg.recon, omega=cfg.Omega, omega0=cfg.Omega0)
+    dst = amrhydro.Batch(d, T2.nn, 1.0)
+    dst.tree_set(p2, c2)
+    dst.tree_repack_from(src, np.array(list(range(n1)) + [-1] * (T2.nn - n1), np.int32))
+    out, ref = dst.download(), T2.state()
+    g = cfg.g
+    I = (slice(None), slice(None), slice(g, -g), slice(g, -g), slice(g, -g))
+    np.testing.assert_array_equal(out[:n1][I], U1[I])
+    e = rel_l1_rot(out[n1:], ref[n1:], g)
+    same = np.mean(out[n1:][I] == ref[n1:][I])
+    say("rot repack: %d carried, %d prolongated, rel-L1 max %.1e, bit-identical %.6f" % (n1, T2.nn - n1, e.max(), same))
+    assert np.all(e <= TOL), e
+    assert dst.error_flags() == 0
+    src.close()
+    dst.close()
+
+
+def test_poisson_phi_from_the_rotating_state():
+    """Binary::pot_from_grid (binary/binary.cpp:857-872): phi = pot / rho + R^2 Omega0^2 / 2"""
+    T, cfg = binary_tree_case("one", seed=28)
+    B = device_batch(T, cfg)
+    P = amrhydro.Poisson(8, T.parent, T.child, grid_dim=1.0, bw=cfg.g)
+    P.phi_from_state(B)
+    phi = P.download("phi")
+    g = cfg.g
+    for b in range(T.nn):
+        x0, h = T.x0(b)
+        ax = [x0[d] + (np.arange(8) + 0.5) * h for d in range(3)]
+        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        U = T.U(b)[:, g:-g, g:-g, g:-g]
+        want = U[6] / U[0] + 0.5 * (X * X + Y * Y) * cfg.Omega0 * cfg.Omega0
+        np.testing.assert_allclose(phi[b, 1:-1, 1:-1, 1:-1], want, rtol=1e-15, atol=1e-15 * np.abs(want).max())
+    B.close()
+    P.close()

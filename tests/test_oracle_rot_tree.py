@@ -332,3 +332,62 @@ def test_max_dt_is_the_min_over_nodes():
         m = orc.rot_max_eig(cfg, T.U(b)[None].copy(), x0[None].copy(), np.array([h]))[0]
         dts.append(cfg.cfl * h / m)
     assert abs(dt - min(dts)) <= 4e-16 * min(dts)
+
+
+def test_prolongation_continues_linear_profiles_and_restricts_back():
+    """GridNode::inject_from_parent (grid/grid_node.cpp:193-233) of the rotating state: a new child
+    of node 16 (the partial tree's level-2 node, siblings on its low faces, coarser nodes on its
+    high ones) takes its parent's linear profile at its own cell centres (rho, s_R, l_z, s_z, pot,
+    E, which the floor keeps), T solved at the child's cell, and the 8 children restrict back to
+    the parent (grid/grid_node.cpp:1035-1074)"""
+    parent, child = TREES["partial"]
+    T = orc.RotTree(orc.rot_cfg(recon=1, omega=0.4), parent, child)
+    g, G = 3, 14
+    coef = {0: (1.0, 0.125, -0.0625, 0.25), 3: (0.5, 0.25, 0.125, -0.125), 4: (0.75, -0.125, 0.25, 0.0625),
+            5: (-0.25, 0.0625, 0.125, 0.25), 6: (-0.5, 0.125, 0.0, -0.25), 7: (8.0, 0.5, -0.25, 0.125)}
+    lin = lambda c, X, Y, Z: c[0] + c[1] * X + c[2] * Y + c[3] * Z
+    for b in range(T.nn):
+        x0, h = T.x0(b)
+        X, Y, Z = centres(x0, h, G, g)
+        U = T.U(b)
+        for f, c in coef.items():
+            U[f] = lin(c, X, Y, Z)
+        U[1], U[2] = 0.5 * U[0], 0.5 * U[0]
+        U[8] = 1.0
+        U[9] = 1.0
+        U[10], U[11], U[12] = U[0], 0.0 * U[0], 0.0 * U[0]
+    T.fill_ghosts()
+    node16 = T.U(16).copy()
+    for b in range(17, 25):
+        T.U(b)[:] = -7.0
+        assert T.prolong(b) == 0
+    checked = 0
+    for b in range(17, 25):
+        x0, h = T.x0(b)
+        X, Y, Z = centres(x0, h, G, g)
+        U = T.U(b)
+        q = T.child[b]
+        # child cells whose parent stencil stays off node 16's high (coarse-fine) faces
+        keep = np.zeros((G, G, G), bool)
+        for k in range(g, G - g):
+            for j in range(g, G - g):
+                for i in range(g, G - g):
+                    p = [(g + c) // 2 + ((q >> d) & 1) * (G // 2 - g) for d, c in enumerate((i - (i - g) % 2, j - (j - g) % 2, k - (k - g) % 2))]
+                    keep[k, j, i] = all(pp < G - g - 1 for pp in p)
+        for f, c in coef.items():
+            want = lin(c, X, Y, Z)
+            np.testing.assert_allclose(U[f][keep], want[keep], rtol=0, atol=1e-14, err_msg="field %d" % f)
+        checked += keep.sum()
+        # T at the child's own cell (ideal gas, closed form), every interior cell
+        I = (slice(g, -g),) * 3
+        R = np.hypot(X, Y)[I]
+        st = U[4][I] / R - U[0][I] * R * 0.4
+        ei = U[7][I] - 0.5 * (U[3][I] ** 2 + st ** 2 + U[5][I] ** 2) / U[0][I]
+        ei = np.where(ei < 0.001 * U[7][I], U[8][I], ei)  # the dual-energy switch: tau
+        abar = U[0][I] / (U[10][I] / 4.0 + U[11][I] / 14.12 + U[12][I] / 17.52)
+        np.testing.assert_allclose(U[9][I], (ei / U[0][I]) * (GAMMA - 1.0) * abar / (1.0 + abar / 2.0), rtol=1e-12)
+    assert checked > 1000
+    T.inject()
+    I = (slice(None),) + (slice(g, -g),) * 3
+    for f in coef:
+        np.testing.assert_allclose(T.U(16)[I][f], node16[I][f], rtol=1e-14, atol=1e-15)
