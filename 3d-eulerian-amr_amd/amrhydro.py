@@ -118,6 +118,10 @@ SIGNATURES = {
     "amrh_tree_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, IP]),
     "amrh_tree_checkpoint_read": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(C.c_void_p),
                                             IP, DP, DP]),
+    "amrh_tree_checkpoint_write_ex": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, IP,
+                                                C.c_void_p, C.c_long]),
+    "amrh_tree_checkpoint_read_ex": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(C.c_void_p),
+                                               IP, DP, DP, C.c_void_p, C.c_long]),
     "amrh_tree_output_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "amrh_tree_output_mesh": (C.c_int, [C.c_void_p, DP, DP, DP, IP, DP]),
     "amrh_tree_output_vtk": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
@@ -376,24 +380,31 @@ class Batch:
         _ck(lib().amrh_tree_density_flags(self.ctx, min_density, f.ctypes.data_as(C.POINTER(C.c_ubyte))))
         return f
 
-    def tree_checkpoint_write(self, path, time=0.0, refine_alpha=1.0, ages=None):
+    def tree_checkpoint_write(self, path, time=0.0, refine_alpha=1.0, ages=None, root_prefix=None):
+        """root_prefix: bytes written before the root's record (Binary::write's parameters)"""
         a = None if ages is None else np.ascontiguousarray(ages, dtype=np.int32)
-        _ck(lib().amrh_tree_checkpoint_write(self.ctx, path.encode(), time, refine_alpha,
-                                             a.ctypes.data_as(IP) if a is not None else None))
+        pre = b"" if root_prefix is None else bytes(root_prefix)
+        buf = C.create_string_buffer(pre, len(pre)) if pre else None
+        _ck(lib().amrh_tree_checkpoint_write_ex(self.ctx, path.encode(), time, refine_alpha,
+                                                a.ctypes.data_as(IP) if a is not None else None, buf, len(pre)))
 
     @classmethod
-    def tree_checkpoint_read(cls, path, cfg):
-        """A new Batch holding the checkpoint's tree and state; returns (batch, time, refine_alpha)."""
+    def tree_checkpoint_read(cls, path, cfg, prefix_bytes=0):
+        """A new Batch holding the checkpoint's tree and state; returns (batch, time, refine_alpha),
+        and the root prefix's bytes as a fourth item when prefix_bytes > 0."""
         ctx = C.c_void_p()
         nb = C.c_int()
         t, a = C.c_double(), C.c_double()
-        _ck(lib().amrh_tree_checkpoint_read(path.encode(), C.byref(cfg), C.byref(ctx), C.byref(nb), C.byref(t),
-                                            C.byref(a)))
+        buf = C.create_string_buffer(prefix_bytes) if prefix_bytes else None
+        _ck(lib().amrh_tree_checkpoint_read_ex(path.encode(), C.byref(cfg), C.byref(ctx), C.byref(nb), C.byref(t),
+                                               C.byref(a), buf, prefix_bytes))
         B = cls.__new__(cls)
         B.cfg = cfg
         B.ctx = ctx
         B.G = cfg.n + 2 * cfg.g
         B.nb = nb.value
+        if prefix_bytes:
+            return B, t.value, a.value, buf.raw
         return B, t.value, a.value
 
     def tree_output_mesh(self):

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "amrh_ctx.h"
@@ -238,7 +239,13 @@ int amrh_tree_output_vtk(amrh_ctx* c, const char* path, double time) {
 
 
 int amrh_tree_checkpoint_write(amrh_ctx* c, const char* path, double time, double refine_alpha, const int* ages) {
+  return amrh_tree_checkpoint_write_ex(c, path, time, refine_alpha, ages, nullptr, 0);
+}
+
+int amrh_tree_checkpoint_write_ex(amrh_ctx* c, const char* path, double time, double refine_alpha, const int* ages,
+                                  const void* root_prefix, long prefix_bytes) {
   if (!c || !path) return fail(AMRH_EINVAL, "null argument");
+  if (prefix_bytes < 0 || (prefix_bytes > 0 && !root_prefix)) return fail(AMRH_EINVAL, "bad root prefix");
   if (!c->tree) return fail(AMRH_ESTATE, "checkpoints hold an octree: call amrh_tree_set");
   const int n = c->cfg.n, g = c->cfg.g, nf = c->cfg.nf, G = n + 2 * g, nb = c->nb;
   const TreeState* t = c->tree;
@@ -249,6 +256,7 @@ int amrh_tree_checkpoint_write(amrh_ctx* c, const char* path, double time, doubl
   FILE* fp = fopen(path, "wb");
   if (!fp) return fail(AMRH_EINVAL, "cannot open %s for writing", path);
   Writer w{fp};
+  if (prefix_bytes > 0 && fwrite(root_prefix, 1, (size_t)prefix_bytes, fp) != (size_t)prefix_bytes) w.ok = false;
   const double h0 = t->h[0];
   std::vector<int> stack = {0};
   while (!stack.empty() && w.ok) {
@@ -280,13 +288,27 @@ int amrh_tree_checkpoint_write(amrh_ctx* c, const char* path, double time, doubl
 
 int amrh_tree_checkpoint_read(const char* path, const amrh_config* cfg, amrh_ctx** out, int* nblocks,
                               double* time, double* refine_alpha) {
+  return amrh_tree_checkpoint_read_ex(path, cfg, out, nblocks, time, refine_alpha, nullptr, 0);
+}
+
+int amrh_tree_checkpoint_read_ex(const char* path, const amrh_config* cfg, amrh_ctx** out, int* nblocks,
+                                 double* time, double* refine_alpha, void* root_prefix, long prefix_bytes) {
   if (!path || !cfg || !out) return fail(AMRH_EINVAL, "null argument");
+  if (prefix_bytes < 0) return fail(AMRH_EINVAL, "bad root prefix");
   *out = nullptr;
   const int n = cfg->n, g = cfg->g, nf = cfg->nf, G = n + 2 * g;
   if (nf <= 0 || nf > 16 || G <= 0) return fail(AMRH_EINVAL, "bad configuration");
   FILE* fp = fopen(path, "rb");
   if (!fp) return fail(AMRH_EINVAL, "cannot open %s", path);
   Reader r{fp};
+  if (prefix_bytes > 0) {
+    std::vector<unsigned char> pre((size_t)prefix_bytes);
+    if (fread(pre.data(), 1, pre.size(), fp) != pre.size()) {
+      fclose(fp);
+      return fail(AMRH_EINVAL, "%s: shorter than its root prefix", path);
+    }
+    if (root_prefix) memcpy(root_prefix, pre.data(), pre.size());
+  }
   std::vector<int> parent, child;
   std::vector<double> U;
   double fo[16] = {0}, t0 = 0.0, alpha = 1.0;

@@ -218,6 +218,15 @@ int amrh_tree_checkpoint_write(amrh_ctx* ctx, const char* path, double time, dou
  * are returned. */
 int amrh_tree_checkpoint_read(const char* path, const amrh_config* cfg, amrh_ctx** out, int* nblocks,
                               double* time, double* refine_alpha);
+/* The same with a caller-owned block of bytes before the root's record: Binary::write /
+ * Binary::read (binary/binary.cpp:70-93, :175-203) put the binary's parameters there (a bool, 15
+ * doubles: l1x, h, M1, fill_factor, n, R1, q, a, K1, K2, phi0_1, phi0_2, Omega, phase, Omega0, and
+ * two ints: accretor and donor type; 129 bytes).  The library writes the bytes as given and reads
+ * them back into root_prefix (NULL: skipped). */
+int amrh_tree_checkpoint_write_ex(amrh_ctx* ctx, const char* path, double time, double refine_alpha,
+                                  const int* ages, const void* root_prefix, long prefix_bytes);
+int amrh_tree_checkpoint_read_ex(const char* path, const amrh_config* cfg, amrh_ctx** out, int* nblocks,
+                                 double* time, double* refine_alpha, void* root_prefix, long prefix_bytes);
 /* The mesh GridNode::output hands to Silo (grid/grid_node.cpp:1315-1356, :1422-1487): nodes in
  * depth-first order; per node the (GNX+1)^3 cell-corner points xf(i), yf(j), zf(k), x fastest; one
  * hexahedral zone per interior cell not covered by a child, in k, j, i order, with its 8 corner

@@ -240,3 +240,61 @@ def test_poisson_phi_from_the_rotating_state():
         np.testing.assert_allclose(phi[b, 1:-1, 1:-1, 1:-1], want, rtol=1e-15, atol=1e-15 * np.abs(want).max())
     B.close()
     P.close()
+
+
+def test_rot_tree_checkpoint_with_the_binary_header(tmp_path):
+    """A rotating-state checkpoint: Binary::write's root parameters (binary/binary.cpp:70-93) before
+    GridNode::write's depth-first records (grid/grid_node.cpp:29-43), parsed independently; read
+    back, the run continues bit for bit"""
+    import struct
+    T, cfg = binary_tree_case("partial", seed=29)
+    B = device_batch(T, cfg)
+    B.tree_fill_ghosts()
+    dt = 0.5 * B.tree_max_dt()
+    B.store()
+    B.tree_substep(dt, 1.0)
+    B.tree_inject()
+    header = struct.pack("<?15d2i", False, *[0.125 * (q + 1) for q in range(12)], cfg.Omega, 0.3, cfg.Omega0, 1, 2)
+    assert len(header) == 129
+    path = str(tmp_path / "X.chk")
+    B.tree_checkpoint_write(path, time=0.5, root_prefix=header)
+    U, fo = B.download(), B.tree_flow_off()
+    buf = open(path, "rb").read()
+    assert buf[:129] == header
+    G, nf = 14, 13
+    rec = 3 * 8 + 3 * (4 + 8) + nf * 8 + G ** 3 * nf * 8 + 8 + 8 + 4
+    assert len(buf) == 129 + T.nn * rec
+    root = np.frombuffer(buf, "<f8", count=G ** 3 * nf, offset=129 + 3 * 8 + 36 + nf * 8).reshape(G, G, G, nf)
+    g = cfg.g
+    np.testing.assert_array_equal(np.moveaxis(root, [0, 1, 2, 3], [3, 2, 1, 0])[:, g:-g, g:-g, g:-g],
+                                  U[0][:, g:-g, g:-g, g:-g])
+    np.testing.assert_array_equal(np.frombuffer(buf, "<f8", count=nf, offset=129 + 60), fo)
+    d = amrhydro.rotating_config(eos=cfg.eos, recon=cfg.recon, omega=cfg.Omega, omega0=cfg.Omega0)
+    B2, t, alpha, pre = amrhydro.Batch.tree_checkpoint_read(path, d, prefix_bytes=129)
+    assert pre == header and t == 0.5
+    # the file's node order is depth first: map it back to the context's order
+    order = []
+
+    def dfs(b):
+        order.append(b)
+        for q in range(8):
+            for x in range(T.nn):
+                if T.parent[x] == b and T.child[x] == q:
+                    dfs(x)
+    dfs(0)
+    U2 = B2.download()
+    I = (slice(None), slice(g, -g), slice(g, -g), slice(g, -g))
+    for new, old in enumerate(order):
+        np.testing.assert_array_equal(U2[new][I], U[old][I])
+    np.testing.assert_array_equal(B2.tree_flow_off(), fo)
+    B2.set_potential(np.stack([T.phi(b).copy() for b in order]))
+    for X in (B, B2):
+        X.tree_fill_ghosts()
+        X.store()
+        X.tree_substep(dt, 1.0)
+        X.tree_inject()
+    Ua, Ub = B.download(), B2.download()
+    for new, old in enumerate(order):
+        np.testing.assert_array_equal(Ub[new][I], Ua[old][I])
+    B.close()
+    B2.close()
