@@ -37,6 +37,32 @@ constexpr int N = 8;
 constexpr int N2 = N * N, N3 = N * N * N;
 constexpr int NFACE = (N + 1) * N * N;
 
+// Face q of direction d: plane f (0..N) along d, tangential cell indices (ta, tb).  The tangential
+// index that is contiguous in the staged pencils varies fastest, so a wavefront's reads of the
+// staged primitives and of the face buffer are unit-stride in LDS for every direction.
+template <int d>
+__device__ __forceinline__ int face_index(int f, int ta, int tb) {
+  if constexpr (d == 0) return f + (N + 1) * (ta + N * tb);
+  if constexpr (d == 1) return ta + N * (f + (N + 1) * tb);
+  return ta + N * (tb + N * f);
+}
+template <int d>
+__device__ __forceinline__ void face_decode(int q, int& f, int& ta, int& tb) {
+  if constexpr (d == 0) {
+    f = q % (N + 1);
+    ta = (q / (N + 1)) % N;
+    tb = q / ((N + 1) * N);
+  } else if constexpr (d == 1) {
+    ta = q % N;
+    f = (q / N) % (N + 1);
+    tb = q / (N * (N + 1));
+  } else {
+    ta = q % N;
+    tb = (q / N) % N;
+    f = q / (N * N);
+  }
+}
+
 // record element -> interior-relative coordinates (i, j, k), any of them in [-G, N+G)
 template <int G>
 __device__ __forceinline__ void elem_ijk(int e, int& i, int& j, int& k) {
@@ -101,11 +127,14 @@ __device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int 
 // MIXED: the staged primitives, the reconstruction, the face states' EOS and fluxes and the KT
 // combination in FP32 (rot::sp); the flux differences, sources, RK update and floor in FP64.
 // TREE: the octree's reflux overrides, exports and the root's ledger (a.slot, a.exp_slot, a.ledger)
+// The mixed form stages the whole record (interior + 6 slabs, 119 KB of LDS with the face buffers)
+// once per block; the FP64 form stages one direction's pencils at a time (154 KB).
 template <int G, int RECON, int EOS, int MODE, bool MIXED, bool TREE>
 __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   using T = typename std::conditional<MIXED, float, double>::type;
-  constexpr int RS = N3 + 2 * G * N2;  // staged elements per field
+  constexpr bool ALL = MIXED;          // stage the whole record once
   constexpr int R = N3 + 6 * G * N2;   // record elements per field
+  constexpr int RS = ALL ? R : N3 + 2 * G * N2;  // staged elements per field
   __shared__ T P[NF][RS];
   __shared__ T Fb[MODE == 0 ? NF : 1][MODE == 0 ? NFACE : 1];
   __shared__ T Vf[MODE == 0 ? NFACE : 1];
@@ -133,12 +162,15 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
     double emax = 0.0;
     const double hinv = 1.0 / h;
 
-    auto sweep = [&](auto dc) {
-      constexpr int d = decltype(dc)::value;
-      // stage the primitives of this direction's pencils
+    // the staged primitives of the cells of pencil position p of direction d
+    auto staged = [&](int d, int p, int ta, int tb) {
+      const int r = pencil_region<G>(d, p, ta, tb);
+      return (ALL && (p < 0 || p >= N)) ? r + 2 * d * G * N2 : r;
+    };
+    auto stage = [&](int d) {
 #pragma unroll 1
       for (int r = tid; r < RS; r += N3) {
-        const int e = region_to_elem<G>(d, r);
+        const int e = ALL ? r : region_to_elem<G>(d, r);
         double u[NF];
 #pragma unroll
         for (int l = 0; l < NF; l++) u[l] = Ub[(long)l * R + e];
@@ -151,15 +183,22 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         for (int l = 0; l < NF; l++) P[l][r] = (T)u[l];
       }
       __syncthreads();
+    };
+    if constexpr (ALL) stage(0);
+
+    auto sweep = [&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      if constexpr (!ALL) stage(d);  // this direction's pencils
 #pragma unroll 1
       for (int q = tid; q < NFACE; q += N3) {
-        const int f = q % (N + 1), t = q / (N + 1), ta = t % N, tb = t / N;
+        int f, ta, tb;
+        face_decode<d>(q, f, ta, tb);
         if constexpr (MIXED) {
           float ql[NF], qr[NF];
           if constexpr (RECON == AMRH_RECON_PPM) {
             int e6[6];
 #pragma unroll
-            for (int m = 0; m < 6; m++) e6[m] = pencil_region<G>(d, f - 3 + m, ta, tb);
+            for (int m = 0; m < 6; m++) e6[m] = staged(d, f - 3 + m, ta, tb);
 #pragma unroll
             for (int l = 0; l < NF; l++) {
               float v[6];
@@ -170,7 +209,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           } else {
             int e4[4];
 #pragma unroll
-            for (int m = 0; m < 4; m++) e4[m] = pencil_region<G>(d, f - 2 + m, ta, tb);
+            for (int m = 0; m < 4; m++) e4[m] = staged(d, f - 2 + m, ta, tb);
 #pragma unroll
             for (int l = 0; l < NF; l++) {
               float v[4];
@@ -211,7 +250,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         if constexpr (RECON == AMRH_RECON_PPM) {
           int e6[6];
 #pragma unroll
-          for (int m = 0; m < 6; m++) e6[m] = pencil_region<G>(d, f - 3 + m, ta, tb);
+          for (int m = 0; m < 6; m++) e6[m] = staged(d, f - 3 + m, ta, tb);
 #pragma unroll
           for (int l = 0; l < NF; l++) {
             double v[6];
@@ -223,7 +262,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         } else {
           int e4[4];
 #pragma unroll
-          for (int m = 0; m < 4; m++) e4[m] = pencil_region<G>(d, f - 2 + m, ta, tb);
+          for (int m = 0; m < 4; m++) e4[m] = staged(d, f - 2 + m, ta, tb);
 #pragma unroll
           for (int l = 0; l < NF; l++) {
             double v[4];
@@ -274,7 +313,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
             if (sl < 0) continue;
             const int f = pl == 0 ? 0 : (pl == 1 ? N / 2 : N);
             const double* o = a.ovr + (((long)sl * (N / 2) + (tb - qb * (N / 2))) * (N / 2) + (ta - qa * (N / 2))) * NF;
-            const int q = f + (N + 1) * (ta + N * tb);
+            const int q = face_index<d>(f, ta, tb);
 #pragma unroll
             for (int l = 0; l < NF; l++) Fb[l][q] = (T)o[l];
           }
@@ -286,14 +325,16 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
             const int sl = a.exp_slot[(b * 6 + 2 * d + side) * 2 + (w & 1)];
             if (sl >= 0) {
               const int al = r % (N / 2), bl = r / (N / 2);
-              const int q = (side ? N : 0) + (N + 1) * (2 * al + N * 2 * bl);
+              const int fp = side ? N : 0;
+              const int q00 = face_index<d>(fp, 2 * al, 2 * bl), q10 = face_index<d>(fp, 2 * al + 1, 2 * bl);
+              const int q01 = face_index<d>(fp, 2 * al, 2 * bl + 1), q11 = face_index<d>(fp, 2 * al + 1, 2 * bl + 1);
               double* o = a.ovr + (((long)sl * (N / 2) + bl) * (N / 2) + al) * NF;
 #pragma unroll
               for (int l = 0; l < NF; l++) {
-                double v = (double)Fb[l][q];
-                v += (double)Fb[l][q + (N + 1)];
-                v += (double)Fb[l][q + (N + 1) * N];
-                v += (double)Fb[l][q + (N + 1) * (N + 1)];
+                double v = (double)Fb[l][q00];
+                v += (double)Fb[l][q10];
+                v += (double)Fb[l][q01];
+                v += (double)Fb[l][q11];
                 v *= 0.25;
                 o[l] = v;
               }
@@ -304,7 +345,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
 #pragma unroll 1
             for (int q2 = tid; q2 < 2 * N2; q2 += N3) {
               const int side = q2 / N2, ta = q2 % N, tb = (q2 / N) % N;
-              const int q = (side ? N : 0) + (N + 1) * (ta + N * tb);
+              const int q = face_index<d>(side ? N : 0, ta, tb);
               double* o = a.ledger + ((((long)d * 2 + side) * N + tb) * N + ta) * NF;
 #pragma unroll
               for (int l = 0; l < NF; l++) o[l] = (double)Fb[l][q];
@@ -314,12 +355,12 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         const int p = d == 0 ? ci : (d == 1 ? cj : ck);
         const int ta = d == 0 ? cj : ci;
         const int tb = d == 2 ? cj : ck;
-        const int q0 = p + (N + 1) * (ta + N * tb);
+        const int q0 = face_index<d>(p, ta, tb), q1 = face_index<d>(p + 1, ta, tb);
 #pragma unroll
         // (with FP32 fluxes the FP64 difference is exact: the divergence still telescopes)
-        for (int l = 0; l < NF; l++) D[l] -= ((double)Fb[l][q0 + 1] - (double)Fb[l][q0]) * hinv;
+        for (int l = 0; l < NF; l++) D[l] -= ((double)Fb[l][q1] - (double)Fb[l][q0]) * hinv;
         vlo[d] = (double)Vf[q0];
-        vhi[d] = (double)Vf[q0 + 1];
+        vhi[d] = (double)Vf[q1];
         __syncthreads();  // P, Fb and Vf are restaged for the next direction
       }
     };
