@@ -17,6 +17,7 @@
 #include "amrh_internal.h"
 #include "amrhydro.h"
 #include "cell_ops.h"
+#include "face_order.h"
 #include "kt_kernels.h"
 #include "recon.h"
 
@@ -157,7 +158,8 @@ __global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEig
       constexpr int d = decltype(dc)::value;
 #pragma unroll 1
       for (int q = tid; q < NFACE; q += NT) {
-        const int f = q % (N + 1), t = q / (N + 1), ta = t % N, tb = t / N;
+        int f, ta, tb;
+        face_decode<N, d>(q, f, ta, tb);
         T ql[NFE], qr[NFE];
         reconstruct<N, G, RECON, MIXED>(P, d, f, ta, tb, ql, qr, PP.theta);
         if constexpr (MODE == 0) {
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEig
             const int f = pl == 0 ? 0 : (pl == 1 ? N / 2 : N);
             const double* o =
                 a.ovr + (((long)sl * (N / 2) + (tb - qb * (N / 2))) * (N / 2) + (ta - qa * (N / 2))) * NFE;
-            const int q = f + (N + 1) * (ta + N * tb);
+            const int q = face_index<N, d>(f, ta, tb);
 #pragma unroll
             for (int l = 0; l < NFE; l++) Fb[l][q] = (T)o[l];
           }
@@ -194,14 +196,17 @@ __global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEig
             const int sl = a.exp_slot[(b * 6 + 2 * d + side) * 2 + (w & 1)];
             if (sl >= 0) {
               const int al = r % (N / 2), bl = r / (N / 2);
-              const int q = (side ? N : 0) + (N + 1) * (2 * al + N * 2 * bl);
+              const int fp = side ? N : 0;
+              const int q00 = face_index<N, d>(fp, 2 * al, 2 * bl), q10 = face_index<N, d>(fp, 2 * al + 1, 2 * bl);
+              const int q01 = face_index<N, d>(fp, 2 * al, 2 * bl + 1);
+              const int q11 = face_index<N, d>(fp, 2 * al + 1, 2 * bl + 1);
               double* o = a.ovr_out + (((long)sl * (N / 2) + bl) * (N / 2) + al) * NFE;
 #pragma unroll
               for (int l = 0; l < NFE; l++) {
-                double v = (double)Fb[l][q];
-                v += (double)Fb[l][q + (N + 1)];
-                v += (double)Fb[l][q + (N + 1) * N];
-                v += (double)Fb[l][q + (N + 1) * (N + 1)];
+                double v = (double)Fb[l][q00];
+                v += (double)Fb[l][q10];
+                v += (double)Fb[l][q01];
+                v += (double)Fb[l][q11];
                 v *= 0.25;
                 o[l] = v;
               }
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEig
 #pragma unroll 1
             for (int q2 = tid; q2 < 2 * N * N; q2 += NT) {
               const int side = q2 / (N * N), ta = q2 % N, tb = (q2 / N) % N;
-              const int q = (side ? N : 0) + (N + 1) * (ta + N * tb);
+              const int q = face_index<N, d>(side ? N : 0, ta, tb);
               double* o = a.ledger + ((((long)d * 2 + side) * N + tb) * N + ta) * NFE;
 #pragma unroll
               for (int l = 0; l < NFE; l++) o[l] = (double)Fb[l][q];
@@ -221,13 +226,13 @@ __global__ __launch_bounds__(N* N* N) void kt_block_kernel(SubstepArgs a, MaxEig
         const int p = d == 0 ? ci : (d == 1 ? cj : ck);
         const int ta = d == 0 ? cj : ci;
         const int tb = d == 2 ? cj : ck;
-        const int q0 = p + (N + 1) * (ta + N * tb);
+        const int q0 = face_index<N, d>(p, ta, tb), q1 = face_index<N, d>(p + 1, ta, tb);
 #pragma unroll
         for (int l = 0; l < NFE; l++) {
           if constexpr (!MIXED)
-            D[l] -= (Fb[l][q0 + 1] - Fb[l][q0]) * hinv;
+            D[l] -= (Fb[l][q1] - Fb[l][q0]) * hinv;
           else  // FP32 fluxes differenced in FP64 (exact), so the divergence telescopes
-            D[l] -= ((double)Fb[l][q0 + 1] - (double)Fb[l][q0]) * hinv;
+            D[l] -= ((double)Fb[l][q1] - (double)Fb[l][q0]) * hinv;
         }
         __syncthreads();  // the next direction rewrites the face buffer
       }

@@ -23,6 +23,7 @@
 #include "amrh_internal.h"
 #include "amrh_layout.h"
 #include "amrhydro.h"
+#include "face_order.h"
 #include "kt_rot.h"
 #include "recon.h"
 #include "rot_phys.h"
@@ -36,32 +37,6 @@ using namespace rot;
 constexpr int N = 8;
 constexpr int N2 = N * N, N3 = N * N * N;
 constexpr int NFACE = (N + 1) * N * N;
-
-// Face q of direction d: plane f (0..N) along d, tangential cell indices (ta, tb).  The tangential
-// index that is contiguous in the staged pencils varies fastest, so a wavefront's reads of the
-// staged primitives and of the face buffer are unit-stride in LDS for every direction.
-template <int d>
-__device__ __forceinline__ int face_index(int f, int ta, int tb) {
-  if constexpr (d == 0) return f + (N + 1) * (ta + N * tb);
-  if constexpr (d == 1) return ta + N * (f + (N + 1) * tb);
-  return ta + N * (tb + N * f);
-}
-template <int d>
-__device__ __forceinline__ void face_decode(int q, int& f, int& ta, int& tb) {
-  if constexpr (d == 0) {
-    f = q % (N + 1);
-    ta = (q / (N + 1)) % N;
-    tb = q / ((N + 1) * N);
-  } else if constexpr (d == 1) {
-    ta = q % N;
-    f = (q / N) % (N + 1);
-    tb = q / (N * (N + 1));
-  } else {
-    ta = q % N;
-    tb = (q / N) % N;
-    f = q / (N * N);
-  }
-}
 
 // record element -> interior-relative coordinates (i, j, k), any of them in [-G, N+G)
 template <int G>
@@ -192,7 +167,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
 #pragma unroll 1
       for (int q = tid; q < NFACE; q += N3) {
         int f, ta, tb;
-        face_decode<d>(q, f, ta, tb);
+        face_decode<N, d>(q, f, ta, tb);
         if constexpr (MIXED) {
           float ql[NF], qr[NF];
           if constexpr (RECON == AMRH_RECON_PPM) {
@@ -313,7 +288,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
             if (sl < 0) continue;
             const int f = pl == 0 ? 0 : (pl == 1 ? N / 2 : N);
             const double* o = a.ovr + (((long)sl * (N / 2) + (tb - qb * (N / 2))) * (N / 2) + (ta - qa * (N / 2))) * NF;
-            const int q = face_index<d>(f, ta, tb);
+            const int q = face_index<N, d>(f, ta, tb);
 #pragma unroll
             for (int l = 0; l < NF; l++) Fb[l][q] = (T)o[l];
           }
@@ -326,8 +301,8 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
             if (sl >= 0) {
               const int al = r % (N / 2), bl = r / (N / 2);
               const int fp = side ? N : 0;
-              const int q00 = face_index<d>(fp, 2 * al, 2 * bl), q10 = face_index<d>(fp, 2 * al + 1, 2 * bl);
-              const int q01 = face_index<d>(fp, 2 * al, 2 * bl + 1), q11 = face_index<d>(fp, 2 * al + 1, 2 * bl + 1);
+              const int q00 = face_index<N, d>(fp, 2 * al, 2 * bl), q10 = face_index<N, d>(fp, 2 * al + 1, 2 * bl);
+              const int q01 = face_index<N, d>(fp, 2 * al, 2 * bl + 1), q11 = face_index<N, d>(fp, 2 * al + 1, 2 * bl + 1);
               double* o = a.ovr + (((long)sl * (N / 2) + bl) * (N / 2) + al) * NF;
 #pragma unroll
               for (int l = 0; l < NF; l++) {
@@ -345,7 +320,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
 #pragma unroll 1
             for (int q2 = tid; q2 < 2 * N2; q2 += N3) {
               const int side = q2 / N2, ta = q2 % N, tb = (q2 / N) % N;
-              const int q = face_index<d>(side ? N : 0, ta, tb);
+              const int q = face_index<N, d>(side ? N : 0, ta, tb);
               double* o = a.ledger + ((((long)d * 2 + side) * N + tb) * N + ta) * NF;
 #pragma unroll
               for (int l = 0; l < NF; l++) o[l] = (double)Fb[l][q];
@@ -355,7 +330,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         const int p = d == 0 ? ci : (d == 1 ? cj : ck);
         const int ta = d == 0 ? cj : ci;
         const int tb = d == 2 ? cj : ck;
-        const int q0 = face_index<d>(p, ta, tb), q1 = face_index<d>(p + 1, ta, tb);
+        const int q0 = face_index<N, d>(p, ta, tb), q1 = face_index<N, d>(p + 1, ta, tb);
 #pragma unroll
         // (with FP32 fluxes the FP64 difference is exact: the divergence still telescopes)
         for (int l = 0; l < NF; l++) D[l] -= ((double)Fb[l][q1] - (double)Fb[l][q0]) * hinv;
