@@ -285,20 +285,25 @@ def binary_tree_state(level, loc, n=8, g=3, eos="ideal"):
     M = _binary_model()
     G, P = n + 2 * g, n + 2
     nn = len(level)
+    level = np.asarray(level)
+    loc = np.asarray(loc, dtype=np.float64).reshape(nn, 3)
+    h = 2.0 / n / 2.0 ** level
+    x0 = -1.0 + loc * n * h[:, None]
     U = np.empty((nn, 13, G, G, G))
     phi = np.empty((nn, P, P, P))
-    x0 = np.empty((nn, 3))
-    h = np.empty(nn)
-    for b, (l, L) in enumerate(zip(level, loc)):
-        h[b] = 2.0 / n / 2 ** l
-        x0[b] = [-1.0 + L[d] * n * h[b] for d in range(3)]
-        ax = [x0[b, d] + (np.arange(G) - g + 0.5) * h[b] for d in range(3)]
-        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
-        U[b] = _binary_fields(M, X, Y, Z, eos)
-        ax = [x0[b, d] + (np.arange(P) - 1 + 0.5) * h[b] for d in range(3)]
-        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
-        phi[b] = _binary_phi(M, X, Y, Z)
-    return dict(U=U, phi=phi, x0=x0, h=h, omega=M["omega"])
+    for s in range(0, nn, 2048):
+        e = min(nn, s + 2048)
+        hh = h[s:e, None]
+
+        def grid(off):
+            x = x0[s:e, 0:1] + off[None, :] * hh
+            y = x0[s:e, 1:2] + off[None, :] * hh
+            z = x0[s:e, 2:3] + off[None, :] * hh
+            return x[:, None, None, :], y[:, None, :, None], z[:, :, None, None]
+
+        U[s:e] = np.moveaxis(_binary_fields(M, *grid(np.arange(G) - g + 0.5), eos), 0, 1)
+        phi[s:e] = _binary_phi(M, *grid(np.arange(P) - 1 + 0.5))
+    return dict(U=U, phi=phi, x0=np.ascontiguousarray(x0), h=h, omega=M["omega"])
 
 
 def sedov_tree(maxlevel=2, n=8, g=2, grid_dim=0.5):

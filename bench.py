@@ -214,79 +214,87 @@ def extra_configs(amrhydro, torch, steps, warmup):
     return out
 
 
+C4_TREE = "binary octree for C4, 25737 nodes of 8^3 (22520 leaves), 7 levels"
+
+
 def binary_octree(amrhydro, torch, steps, warmup, stream):
-    """The rotating binary on its octree (2,473 nodes of 8^3 over 6 levels, PPM, 13 fields, ideal
-    gas): whole RK3 steps of BinaryDriver with the stars' fixed potential (ghost exchange, CFL dt,
-    three substeps with reflux, each followed by the restriction; one captured graph per step), and
-    the same steps with self-gravity: after every substep S = 4 pi G rho, the Poisson solve from the
-    previous phi, pot_to_grid and the restriction (BinaryDriver::sub_step + solve,
-    binary/binary_driver.cpp:296-382), the hydro reading the solver's face forces by pointer."""
+    """The rotating binary on its octree, PPM, 13 fields, ideal gas: whole RK3 steps of BinaryDriver
+    with the stars' fixed potential (ghost exchange, CFL dt, three substeps with reflux, each
+    followed by the restriction; one captured graph per step) on the 2,473-node tree of the Poisson
+    line and on a C4-sized tree (binary_tree(6, margin 2): 22,520 leaves over 6 refined levels);
+    and on the smaller tree the same steps with self-gravity: after every substep S = 4 pi G rho,
+    the Poisson solve from the previous phi, pot_to_grid and the restriction (BinaryDriver::sub_step
+    + solve, binary/binary_driver.cpp:296-382), the hydro reading the solver's face forces by
+    pointer."""
     import workloads
     out = {}
-    parent, child, lv, loc = workloads.binary_tree(POISSON_LEVELS)
-    S = workloads.binary_tree_state(lv, loc)
-    inner = set(parent[1:].tolist())
-    leaves = sum(1 for b in range(len(parent)) if b not in inner) * 8 ** 3
-    log("rotating binary on its octree (%s, %d leaf cells)" % (POISSON_TREE, leaves))
-    for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
-        cfg = amrhydro.rotating_config(omega=S["omega"], precision=prec)
-        B = amrhydro.Batch(cfg, len(parent), 1.0)
-        B.set_stream(stream)
-        B.tree_set(parent, child)
-        B.upload(S["U"])
-        B.set_potential(S["phi"])
-        B.tree_fill_ghosts()
-        B.tree_inject()
-        dt = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
-        B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        B.run_steps(steps, dt.data_ptr(), rk=3, use_graph=True)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        r = {"workload": "rotating binary on " + POISSON_TREE + ", fixed potential", "nodes": len(parent),
-             "leaf_cells": leaves, "cell_updates_per_s": 3 * steps * leaves / wall,
-             "ms_per_step": 1e3 * wall / steps, "device_error_flags": B.error_flags()}
-        log("  binary octree %s: %.3e leaf cell-updates/s, %.3f ms per RK3 step"
-            % (tag, r["cell_updates_per_s"], r["ms_per_step"]))
-        if tag == "mixed":
-            # self-gravity: every substep solves for phi
-            P = amrhydro.Poisson(8, parent, child, bw=cfg.g)
-            P.source_from(B, 1.0)
-            P.solve(POISSON_TOLER)
-            B.set_potential_device(P.phi_device())
-            B.set_forces_device(P.forces_device())
-            its = []
-
-            def grav_step():
-                B.tree_fill_ghosts()
-                B.dt_device(1.0, 1.25, dt.data_ptr())
-                B.store()
-                for beta in BETAS:
-                    B.tree_substep_dev(dt.data_ptr(), beta)
-                    P.source_from(B, 1.0)
-                    its.append(P.solve(POISSON_TOLER)[0])
-                    B.set_potential_device(P.phi_device())
-                    B.pot_to_grid()
-                    B.tree_inject()
-            grav_step()
+    for key, label, args in (("binary_octree", POISSON_TREE, dict(maxlevel=POISSON_LEVELS)),
+                             ("binary_octree_c4", C4_TREE, dict(maxlevel=6, margin=2.0))):
+        parent, child, lv, loc = workloads.binary_tree(**args)
+        S = workloads.binary_tree_state(lv, loc)
+        inner = set(parent[1:].tolist())
+        leaves = sum(1 for b in range(len(parent)) if b not in inner) * 8 ** 3
+        log("rotating binary on its octree (%s, %d leaf cells)" % (label, leaves))
+        for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+            cfg = amrhydro.rotating_config(omega=S["omega"], precision=prec)
+            B = amrhydro.Batch(cfg, len(parent), 1.0)
+            B.set_stream(stream)
+            B.tree_set(parent, child)
+            B.upload(S["U"])
+            B.set_potential(S["phi"])
+            B.tree_fill_ghosts()
+            B.tree_inject()
+            dt = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
+            B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
             torch.cuda.synchronize()
-            its.clear()
-            ng = max(2, steps // 2)
             t0 = time.perf_counter()
-            for _ in range(ng):
-                grav_step()
+            B.run_steps(steps, dt.data_ptr(), rk=3, use_graph=True)
             torch.cuda.synchronize()
             wall = time.perf_counter() - t0
-            r["self_gravity"] = {"ms_per_step": 1e3 * wall / ng, "cell_updates_per_s": 3 * ng * leaves / wall,
-                                 "vcycles_per_solve": statistics.mean(its), "toler": POISSON_TOLER,
-                                 "device_error_flags": B.error_flags()}
-            log("  with self-gravity: %.2f ms per RK3 step, %.1f V-cycles per solve"
-                % (r["self_gravity"]["ms_per_step"], statistics.mean(its)))
-            B.set_forces_device(None)
-            P.close()
-        out["binary_octree_%s" % tag] = r
-        B.close()
+            r = {"workload": "rotating binary on " + label + ", fixed potential", "nodes": len(parent),
+                 "leaf_cells": leaves, "cell_updates_per_s": 3 * steps * leaves / wall,
+                 "ms_per_step": 1e3 * wall / steps, "device_error_flags": B.error_flags()}
+            log("  %s %s: %.3e leaf cell-updates/s, %.3f ms per RK3 step"
+                % (key, tag, r["cell_updates_per_s"], r["ms_per_step"]))
+            if tag == "mixed" and key == "binary_octree":
+                # self-gravity: every substep solves for phi
+                P = amrhydro.Poisson(8, parent, child, bw=cfg.g)
+                P.source_from(B, 1.0)
+                P.solve(POISSON_TOLER)
+                B.set_potential_device(P.phi_device())
+                B.set_forces_device(P.forces_device())
+                its = []
+
+                def grav_step():
+                    B.tree_fill_ghosts()
+                    B.dt_device(1.0, 1.25, dt.data_ptr())
+                    B.store()
+                    for beta in BETAS:
+                        B.tree_substep_dev(dt.data_ptr(), beta)
+                        P.source_from(B, 1.0)
+                        its.append(P.solve(POISSON_TOLER)[0])
+                        B.set_potential_device(P.phi_device())
+                        B.pot_to_grid()
+                        B.tree_inject()
+                grav_step()
+                torch.cuda.synchronize()
+                its.clear()
+                ng = max(2, steps // 2)
+                t0 = time.perf_counter()
+                for _ in range(ng):
+                    grav_step()
+                torch.cuda.synchronize()
+                wall = time.perf_counter() - t0
+                r["self_gravity"] = {"ms_per_step": 1e3 * wall / ng, "cell_updates_per_s": 3 * ng * leaves / wall,
+                                     "vcycles_per_solve": statistics.mean(its), "toler": POISSON_TOLER,
+                                     "device_error_flags": B.error_flags()}
+                log("  with self-gravity: %.2f ms per RK3 step, %.1f V-cycles per solve"
+                    % (r["self_gravity"]["ms_per_step"], statistics.mean(its)))
+                B.set_forces_device(None)
+                P.close()
+            out["%s_%s" % (key, tag)] = r
+            B.close()
+        del S
     return out
 
 
