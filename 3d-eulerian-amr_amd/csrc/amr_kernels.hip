@@ -272,23 +272,28 @@ __global__ __launch_bounds__(256) void restrict_kernel(double* U, const int2* pa
   for (int l = 0; l < NFE; l++) Up[l * R + e] = u[l];
 }
 
+// The boundary terms tmp = (F_up - F_lo) h^2 (the upper face only for z with z_reflect) are
+// formed by all threads into LDS; one thread per field then adds them in the reference's order
+// (direction, then pencil row b, then pencil a), so the ledger is bit-identical to the serial sum.
 template <int N, int NFL>
-__global__ void ledger_kernel(const double* ledger, double h, const double* dt_dev, double dt, double beta,
-                              int z_reflect, double* fo /* [FO | FO0] */) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(256) void ledger_kernel(const double* ledger, double h, const double* dt_dev, double dt,
+                                                     double beta, int z_reflect, double* fo /* [FO | FO0] */) {
+  constexpr int TERMS = 3 * N * N;
+  __shared__ double tmp[TERMS * NFL];
+  for (int t = threadIdx.x; t < TERMS * NFL; t += blockDim.x) {
+    const int q = t / NFL, l = t % NFL;  // q = (d * N + b) * N + a
+    const int d = q / (N * N), ba = q % (N * N);
+    const double lo = ledger[((d * 2 + 0) * N * N + ba) * NFL + l];
+    const double up = ledger[((d * 2 + 1) * N * N + ba) * NFL + l];
+    tmp[t] = (d == 2 && z_reflect) ? up * h * h : (up - lo) * h * h;
+  }
+  __syncthreads();
+  if (threadIdx.x >= NFL) return;
   if (dt_dev) dt = *dt_dev;
-  double dfo[NFL];
-  for (int l = 0; l < NFL; l++) dfo[l] = 0.0;
-  for (int d = 0; d < 3; d++)
-    for (int b = 0; b < N; b++)
-      for (int a = 0; a < N; a++)
-        for (int l = 0; l < NFL; l++) {
-          const double lo = ledger[(((d * 2 + 0) * N + b) * N + a) * NFL + l];
-          const double up = ledger[(((d * 2 + 1) * N + b) * N + a) * NFL + l];
-          const double tmp = (d == 2 && z_reflect) ? up * h * h : (up - lo) * h * h;
-          dfo[l] += tmp;
-        }
-  for (int l = 0; l < NFL; l++) fo[l] = (fo[l] + dfo[l] * dt) * beta + fo[NFL + l] * (1.0 - beta);
+  const int l = threadIdx.x;
+  double dfo = 0.0;
+  for (int q = 0; q < TERMS; q++) dfo += tmp[q * NFL + l];
+  fo[l] = (fo[l] + dfo * dt) * beta + fo[NFL + l] * (1.0 - beta);
 }
 
 __global__ void ledger_store_kernel(double* fo, int nf) {
@@ -351,11 +356,11 @@ hipError_t launch_restrict(int n, int g, double* U, const int2* pairs, int count
 hipError_t launch_ledger(int n, int nf, const double* ledger, double h, const double* dt_dev, double dt, double beta,
                          int z_reflect, double* fo, hipStream_t s) {
   if (n == 8 && nf == NFE)
-    ledger_kernel<8, NFE><<<1, 64, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
+    ledger_kernel<8, NFE><<<1, 256, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
   else if (n == 16 && nf == NFE)
-    ledger_kernel<16, NFE><<<1, 64, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
+    ledger_kernel<16, NFE><<<1, 256, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
   else if (n == 8 && nf == 13)
-    ledger_kernel<8, 13><<<1, 64, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
+    ledger_kernel<8, 13><<<1, 256, 0, s>>>(ledger, h, dt_dev, dt, beta, z_reflect, fo);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
