@@ -679,49 +679,42 @@ __global__ __launch_bounds__(NTHREADS) void source_sums_kernel(PArgs A, double* 
   }
 }
 
-// PoissonPhysBound::compute_boundaries with Binary::compute_phi (binary/binary.cpp:958-983): one
-// thread per table entry; the tree's sum is formed in the reference's recursion order (own
-// octants, then each child's subtree in index order) with a stack over the pre-order walk.
+// PoissonPhysBound::compute_boundaries with Binary::compute_phi (binary/binary.cpp:958-983): the
+// sum over every node's octant monopoles of -alpha m / r at each table entry.  32 lanes share an
+// entry, each summing every 32nd node, and the partial sums are combined by a fixed butterfly, so
+// the result is deterministic; it differs from the reference's recursion order only in rounding
+// (all terms have one sign).
+constexpr int TAB_LANES = 32;
 __global__ __launch_bounds__(NTHREADS) void tables_kernel(PArgs A, int nslots) {
   const int P = A.P;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)nslots * P * P) return;
-  const int slot = (int)(t / (P * P)), r = (int)(t % (P * P)), a = r / P, c = r % P;
-  const int b = A.slots[slot].x, f = A.slots[slot].y, d = f >> 1, g = (f & 1) ? P - 1 : 0;
-  double x, y, z;
-  if (d == 0) {
-    x = centre(A, b, 0, g), y = centre(A, b, 1, a), z = centre(A, b, 2, c);
-  } else if (d == 1) {
-    x = centre(A, b, 0, a), y = centre(A, b, 1, g), z = centre(A, b, 2, c);
-  } else {
-    x = centre(A, b, 0, a), y = centre(A, b, 1, c), z = centre(A, b, 2, g);
-  }
-  const double alpha = A.z_reflect ? 1.0 * 2.0 : 1.0;
-  double stack[MAXDEPTH];
-  int sp = 0;
-  for (int q = 0; q < A.nn; q++) {
-    const int nb = A.pre[q], lv = A.nd[nb].level;
-    while (sp > lv) {
-      const double v = stack[--sp];
-      stack[sp - 1] += v;
+  const long t = (long)blockIdx.x * (NTHREADS / TAB_LANES) + threadIdx.x / TAB_LANES;
+  const int lane = threadIdx.x % TAB_LANES;
+  const bool valid = t < (long)nslots * P * P;
+  double part = 0.0;
+  if (valid) {
+    const int slot = (int)(t / (P * P)), r = (int)(t % (P * P)), a = r / P, c = r % P;
+    const int b = A.slots[slot].x, f = A.slots[slot].y, d = f >> 1, g = (f & 1) ? P - 1 : 0;
+    double x, y, z;
+    if (d == 0) {
+      x = centre(A, b, 0, g), y = centre(A, b, 1, a), z = centre(A, b, 2, c);
+    } else if (d == 1) {
+      x = centre(A, b, 0, a), y = centre(A, b, 1, g), z = centre(A, b, 2, c);
+    } else {
+      x = centre(A, b, 0, a), y = centre(A, b, 1, c), z = centre(A, b, 2, g);
     }
-    const double* s = A.src + (long)nb * 32;
-    double own = 0.0;
-    const bool allr = A.allref[nb] != 0;
-    for (int o = 0; o < 8; o++) {
-      const double dx = x - s[4 * o + 1], dy = y - s[4 * o + 2], dz = z - s[4 * o + 3];
-      if (!allr)
-        own -= alpha * s[4 * o] / sqrt(dx * dx + dy * dy + dz * dz);
-      else
-        own = 0.0;
+    const double alpha = A.z_reflect ? 1.0 * 2.0 : 1.0;
+    for (int nb = lane; nb < A.nn; nb += TAB_LANES) {
+      if (A.allref[nb]) continue;  // every octant refined: the children hold the mass
+      const double* s = A.src + (long)nb * 32;
+#pragma unroll
+      for (int o = 0; o < 8; o++) {
+        const double dx = x - s[4 * o + 1], dy = y - s[4 * o + 2], dz = z - s[4 * o + 3];
+        part -= alpha * s[4 * o] * rsqrt(dx * dx + dy * dy + dz * dz);
+      }
     }
-    stack[sp++] = own;
   }
-  while (sp > 1) {
-    const double v = stack[--sp];
-    stack[sp - 1] += v;
-  }
-  A.tab[t] = stack[0];
+  for (int off = TAB_LANES / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, TAB_LANES);
+  if (valid && lane == 0) A.tab[t] = part;
 }
 
 // Poisson::set_source from a hydro batch: S = 4 pi G rho on the interior (binary/state_ztwd.cpp:219-222)
@@ -1147,7 +1140,8 @@ int amrh_poisson_prepare(amrh_poisson* p) {
   PCHK(hipGetLastError());
   const long entries = (long)p->slots.size() * p->P * p->P;
   if (entries) {
-    tables_kernel<<<(unsigned)((entries + NTHREADS - 1) / NTHREADS), NTHREADS, 0, p->stream>>>(
+    const long per_block = NTHREADS / TAB_LANES;
+    tables_kernel<<<(unsigned)((entries + per_block - 1) / per_block), NTHREADS, 0, p->stream>>>(
         p->args(p->all_start), (int)p->slots.size());
     PCHK(hipGetLastError());
   }
