@@ -298,3 +298,52 @@ def test_rot_tree_checkpoint_with_the_binary_header(tmp_path):
         np.testing.assert_array_equal(Ub[new][I], Ua[old][I])
     B.close()
     B2.close()
+
+
+@pytest.mark.parametrize("z_reflect", [0, 1])
+def test_rot_tree_output_mesh(tmp_path, z_reflect):
+    """GridNode::output's mesh of a rotating-state tree with PPM blocks (g = 3): corner points from
+    each node's origin, the uncovered cells' zones depth first in k, j, i order, and the 13 fields
+    named as State::field_name (binary/state_ztwd.cpp:58-89) in the VTK file"""
+    from test_gpu_output import read_vtk
+    T, cfg = binary_tree_case("partial", seed=30, z_reflect=z_reflect)
+    B = device_batch(T, cfg)
+    x, y, z, nl, v = B.tree_output_mesh()
+    g, G = cfg.g, 14
+    kids = {b: [None] * 8 for b in range(T.nn)}
+    for b in range(1, T.nn):
+        kids[int(T.parent[b])][int(T.child[b])] = b
+    xs, vals, zones = [], [], 0
+
+    def rec(b):
+        nonlocal zones
+        x0, h = T.x0(b)
+        f = [x0[d] + (np.arange(G + 1) - g) * h for d in range(3)]
+        Z, Y, X = np.meshgrid(f[2], f[1], f[0], indexing="ij")
+        xs.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()]))
+        U = T.U(b)
+        for k in range(g, G - g):
+            for j in range(g, G - g):
+                for i in range(g, G - g):
+                    c = (2 * i) // G | ((2 * j) // G) << 1 | ((2 * k) // G) << 2
+                    if kids[b][c] is None:
+                        vals.append(U[:, k, j, i].copy())
+        for c in range(8):
+            if kids[b][c] is not None:
+                rec(kids[b][c])
+
+    rec(0)
+    P = np.concatenate(xs, axis=1)
+    np.testing.assert_array_equal(x, P[0])
+    np.testing.assert_array_equal(y, P[1])
+    np.testing.assert_array_equal(z, P[2])
+    np.testing.assert_array_equal(v, np.array(vals).T)
+    assert nl.shape == (len(vals), 8)
+    path = str(tmp_path / "rot.vtk")
+    B.tree_output_vtk(path, time=0.5)
+    _, pts, cells, _, fields = read_vtk(path)
+    assert list(fields) == ["d_tot", "d_acc", "d_don", "sr", "lz", "sz", "pot", "et", "tau", "T", "s1", "s2", "s3"]
+    np.testing.assert_array_equal(fields["lz"], np.array(vals)[:, 4])
+    vol = np.prod(pts[cells[:, 1:]][:, 6] - pts[cells[:, 1:]][:, 0], axis=1).sum()
+    assert abs(vol - 8.0) < 1e-12  # [-1, 1]^2 x [-1, 1] or [0, 2] under z reflection
+    B.close()
