@@ -68,3 +68,40 @@ def test_tree_set_rejects_bad_arrays(parent, child):
     with pytest.raises(amrhydro.AmrhError, match="bad parent"):
         B.tree_set(np.array(parent, np.int32), np.array(child, np.int32))
     B.close()
+
+
+def test_rotating_entry_points_reject_misuse(tmp_path):
+    """The rotating-state and self-gravity entry points check their preconditions before any
+    device work: forces and pot_to_grid belong to the rotating state, the substep needs the
+    potential, phi from pot needs a rotating state, re-packing needs one block shape and one physics
+    on both sides, and
+    a checkpoint shorter than its root prefix is refused."""
+    E = amrhydro.Batch(amrhydro.euler_config(n=8), 9, 1.0)
+    E.tree_set(np.array([-1] + [0] * 8, np.int32), np.array([-1] + list(range(8)), np.int32))
+    with pytest.raises(amrhydro.AmrhError, match="ROTATING"):
+        E.set_forces(np.zeros((3, 9, 10, 10, 10)))
+    with pytest.raises(amrhydro.AmrhError, match="ROTATING"):
+        E.pot_to_grid()
+    P = amrhydro.Poisson(8, [-1] + [0] * 8, [-1] + list(range(8)))
+    E.upload(np.ones((9, 6, 12, 12, 12)))
+    with pytest.raises(amrhydro.AmrhError, match="rotating"):
+        P.phi_from_state(E)
+    R = amrhydro.Batch(amrhydro.rotating_config(), 9, 1.0)
+    R.tree_set(np.array([-1] + [0] * 8, np.int32), np.array([-1] + list(range(8)), np.int32))
+    U = np.zeros((9, 13, 14, 14, 14))
+    U[:, 0] = 1.0
+    U[:, 7] = 1.0
+    U[:, 8] = 1.0
+    U[:, 9] = 1.0
+    U[:, 10] = 1.0
+    R.upload(U)
+    with pytest.raises(amrhydro.AmrhError, match="potential"):
+        R.tree_substep(1e-4, 1.0)
+    with pytest.raises(amrhydro.AmrhError, match="shapes differ|different physics"):
+        R.tree_repack_from(E, np.arange(9, dtype=np.int32))
+    path = str(tmp_path / "short.chk")
+    R.tree_checkpoint_write(path, root_prefix=b"\x00" * 16)
+    with pytest.raises(amrhydro.AmrhError):
+        amrhydro.Batch.tree_checkpoint_read(path, amrhydro.rotating_config(), prefix_bytes=1 << 30)
+    for X in (E, R, P):
+        X.close()
