@@ -154,12 +154,16 @@ __device__ __forceinline__ void put_ghost(double* Uo, const Layout& L, int face,
 // Per plane k: stage the plane's primitives (with x/y ghosts) in LDS; x- and y-slopes of every
 // cell once; the x/y faces from two cells and their slopes; the z-face from the column window;
 // then each thread updates its cell.  Three barriers per plane.
-// FP64: two waves per SIMD.  The FP64 substep would otherwise take more than 256 registers and run
-// one wave per SIMD; a small spill at two waves is measured 20% faster.  The mixed form is left to
-// the compiler: it fits two waves without spilling, and three (168 registers) spill ~300 B per lane
-// and run slower.
-template <int N, bool MIXED, int MODE>
-__global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 2))) void kt_sweep_kernel(
+// Occupancy, measured on the flat 4096 x 16^3 batch: the FP64 substep runs two waves per SIMD and
+// reads its cell's conserved state back from memory (L2) at each plane instead of carrying three
+// planes of FP64 column state in registers, which removes its spills (1.88 -> 1.54 ms; on the
+// 8^3 multilevel batch 4.17 -> 3.39 ms); the mixed form keeps the column state, and runs three
+// waves per SIMD on 16^3 blocks (168 registers, a small spill: 2% faster than two without
+// spilling) but two on 8^3 blocks (three are 7% slower there).
+// TREE: the octree's reflux overrides and the root's flow-off ledger (a.slot, a.ledger); the flat
+// instantiation carries neither, which keeps its register count down.
+template <int N, bool MIXED, int MODE, bool TREE>
+__global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N == 16 ? 3 : 1) : 2))) void kt_sweep_kernel(
     SubstepArgs a, MaxEigArgs m) {
   using T = typename Prec<MIXED>::T;
   constexpr int PW = N + 2 * GW;  // padded plane edge
@@ -212,7 +216,9 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 
   const double hinv = MODE == 0 ? 1.0 / a.h[b] : 0.0;
 
   // ---- prologue: column cells k = -2 .. 2 ------------------------------------------------
-  double ucur[NFE], uc1[NFE], uc2[NFE];
+  // the conserved FP64 state of cells k, k+1, k+2 (the mixed form carries it; FP64 reloads cell k)
+  constexpr bool CARRY = MIXED;
+  double ucur[NFE], uc1[CARRY ? NFE : 1], uc2[CARRY ? NFE : 1];
   T c0[NFE], cp1[NFE], cp2[NFE], s0[NFE], Fzlo[NFE];
   double emax = 0.0;
   {
@@ -232,18 +238,24 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 
     make_prim<MIXED>(ucur, c0);
     e = col_elem<N>(L, i, j, 1);
 #pragma unroll
-    for (int l = 0; l < NFE; l++) uc1[l] = Ub[l * R + e];
-    make_prim<MIXED>(uc1, cp1);
+    for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
+    if constexpr (CARRY)
+#pragma unroll
+      for (int l = 0; l < NFE; l++) uc1[l] = u[l];
+    make_prim<MIXED>(u, cp1);
     e = col_elem<N>(L, i, j, 2);
 #pragma unroll
-    for (int l = 0; l < NFE; l++) uc2[l] = Ub[l * R + e];
-    make_prim<MIXED>(uc2, cp2);
+    for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
+    if constexpr (CARRY)
+#pragma unroll
+      for (int l = 0; l < NFE; l++) uc2[l] = u[l];
+    make_prim<MIXED>(u, cp2);
     slope_of<MIXED>(P, cm2, cm1, c0, sm1);
     slope_of<MIXED>(P, cm1, c0, cp1, s0);
     if constexpr (MODE == 0) {
       face_of<MIXED, 2>(P, cm1, sm1, c0, s0, Fzlo);  // face k = 0
-      if (a.slot) apply_override<N>(a, b, 2, 0, i, j, Fzlo);
-      if (b == a.ledger_block) {
+      if (TREE && a.slot) apply_override<N>(a, b, 2, 0, i, j, Fzlo);
+      if (TREE && b == a.ledger_block) {
 #pragma unroll
         for (int l = 0; l < NFE; l++) a.ledger[(((2 * 2 + 0) * N + j) * N + i) * NFE + l] = (double)Fzlo[l];
       }
@@ -294,6 +306,11 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 
       const long e = col_elem<N>(L, i, j, k + 3);
 #pragma unroll
       for (int l = 0; l < NFE; l++) craw[l] = Ub[l * R + e];
+    }
+    if (MODE == 0 && !CARRY && k > 0) {  // cell k's conserved state again (read 3 planes ago: L2)
+      const long e = L.interior(i, j, k);
+#pragma unroll
+      for (int l = 0; l < NFE; l++) ucur[l] = Ub[l * R + e];
     }
     double u0v[NFE];
     if (MODE == 0 && has_u0) {
@@ -357,7 +374,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 
         if constexpr (MODE == 0) {
           T F[NFE];
           face_of<MIXED, 0>(P, pl, sl, pr, sr, F);
-          if (a.slot) apply_override<N>(a, b, 0, fi, jj, k, F);
+          if (TREE && a.slot) apply_override<N>(a, b, 0, fi, jj, k, F);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fx[l][jj][fi] = F[l];
         } else {
@@ -377,7 +394,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 
         if constexpr (MODE == 0) {
           T F[NFE];
           face_of<MIXED, 1>(P, pl, sl, pr, sr, F);
-          if (a.slot) apply_override<N>(a, b, 1, fj, ii, k, F);
+          if (TREE && a.slot) apply_override<N>(a, b, 1, fj, ii, k, F);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fy[l][fj][ii] = F[l];
         } else {
@@ -389,7 +406,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 
     slope_of<MIXED>(P, c0, cp1, cp2, sp1);
     if constexpr (MODE == 0) {
       face_of<MIXED, 2>(P, c0, s0, cp1, sp1, Fzhi);
-      if (a.slot) apply_override<N>(a, b, 2, k + 1, i, j, Fzhi);
+      if (TREE && a.slot) apply_override<N>(a, b, 2, k + 1, i, j, Fzhi);
     } else {
       emax = x64::rmax(emax, eig_of<MIXED, 2>(P, c0, s0, cp1, sp1));
     }
@@ -425,7 +442,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 
       else
         floor_mixed(P, u);
       if (!(u[RHO] > 0.0) && a.err) atomicOr(a.err, 1);
-      if (b == a.ledger_block) {
+      if (TREE && b == a.ledger_block) {
         // boundary faces of the ledger block (grid/grid.cpp:350-404 sums them per pencil)
         double* lg = a.ledger;
         if (i == 0) {
@@ -495,12 +512,18 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 
       cp1[l] = cp2[l];
       s0[l] = sp1[l];
       Fzlo[l] = Fzhi[l];
-      ucur[l] = uc1[l];
-      uc1[l] = uc2[l];
+    }
+    if constexpr (CARRY) {
+#pragma unroll
+      for (int l = 0; l < NFE; l++) {
+        ucur[l] = uc1[l];
+        uc1[l] = uc2[l];
+      }
     }
     if (have_next) {
+      if constexpr (CARRY)
 #pragma unroll
-      for (int l = 0; l < NFE; l++) uc2[l] = craw[l];
+        for (int l = 0; l < NFE; l++) uc2[l] = craw[l];
       make_prim<MIXED>(craw, cp2);
     }
   }
@@ -659,10 +682,15 @@ static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, int grid, h
   MaxEigArgs m{};
   if (a.nblocks <= 0) return hipSuccess;
   const int g = (grid > 0 && grid < a.nblocks) ? grid : a.nblocks;
-  if (mixed)
-    kt_sweep_kernel<N, true, 0><<<g, N * N, 0, s>>>(a, m);
+  const bool tree = a.slot != nullptr || a.ledger != nullptr;
+  if (mixed && tree)
+    kt_sweep_kernel<N, true, 0, true><<<g, N * N, 0, s>>>(a, m);
+  else if (mixed)
+    kt_sweep_kernel<N, true, 0, false><<<g, N * N, 0, s>>>(a, m);
+  else if (tree)
+    kt_sweep_kernel<N, false, 0, true><<<g, N * N, 0, s>>>(a, m);
   else
-    kt_sweep_kernel<N, false, 0><<<g, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, false, 0, false><<<g, N * N, 0, s>>>(a, m);
   return hipGetLastError();
 }
 
@@ -678,9 +706,9 @@ static hipError_t launch_maxeig_n(const MaxEigArgs& m, bool mixed, int grid, hip
   if (m.nblocks <= 0) return hipSuccess;
   const int g = (grid > 0 && grid < m.nblocks) ? grid : m.nblocks;
   if (mixed)
-    kt_sweep_kernel<N, true, 1><<<g, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, true, 1, false><<<g, N * N, 0, s>>>(a, m);
   else
-    kt_sweep_kernel<N, false, 1><<<g, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, false, 1, false><<<g, N * N, 0, s>>>(a, m);
   return hipGetLastError();
 }
 
