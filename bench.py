@@ -544,7 +544,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic("flat%d_%s" % (n, head_name)),
-                     "kernel": "kt_sweep_kernel<%d,%s,0>" % (n, "true" if head_name == "mixed" else "false"),
+                     "kernel": "kt_sweep_kernel<%d,%s,0,false>" % (n, "true" if head_name == "mixed" else "false"),
                      "bytes_per_launch": head["bytes_per_launch_avg"]},
         "cpu_baseline": cpu,
         "other_workloads": extra,
