@@ -103,16 +103,19 @@ def time_steps(B, torch, cells, steps, warmup):
     return res
 
 
-def extra_configs(amrhydro, torch, steps, warmup):
+def extra_configs(amrhydro, torch, steps, warmup, only=None):
     """The other workloads of the hot path, one line each (FP64 and, where built, mixed): the
     uneven multilevel leaf batch (59,518 leaves of 8^3), PPM on 8^3 blocks (g = 3), the rotating
     binary (20,000 leaves, PPM) with the ideal-gas and the Helmholtz EOS (mixed and FP64), the
     self-gravity V-cycles on the binary's octree, and the 73-node octree
     of the Sedov blast (ghost exchange, reflux and restriction included, one captured graph per
-    step)."""
+    step).  only: run just that workload (a key of EXTRA_WORKLOADS)."""
     import workloads
     out = {}
     OUT = amrhydro.AMRH_BOUNDARY_OUTFLOW
+
+    def want(key):
+        return only is None or only == key
     stream = torch.cuda.current_stream().cuda_stream
 
     def flat(name, cfg, U, h, extra=None):
@@ -129,91 +132,99 @@ def extra_configs(amrhydro, torch, steps, warmup):
             % (name, r["cell_updates_per_s"], r["substep_kernel_ms_avg"], r["achieved_GBs"]))
         return r
 
-    log("multilevel batch (59,518 leaves of 8^3)")
-    U, h, lev = workloads.multilevel()
-    for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
-        cfg = amrhydro.euler_config(n=8, precision=prec, boundary=OUT)
-        out["multilevel_8_%s" % tag] = flat("multilevel %s" % tag, cfg, U, h)
-    del U
+    if want("multilevel"):
+        log("multilevel batch (59,518 leaves of 8^3)")
+        U, h, lev = workloads.multilevel()
+        for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+            cfg = amrhydro.euler_config(n=8, precision=prec, boundary=OUT)
+            out["multilevel_8_%s" % tag] = flat("multilevel %s" % tag, cfg, U, h)
+        del U
 
-    log("PPM flat batch (8192 blocks of 8^3, g = 3)")
-    U = workloads.flat_batch(8192, n=8, g=3)
-    for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
-        cfg = amrhydro.euler_config(n=8, g=3, precision=prec, boundary=OUT, recon=amrhydro.AMRH_RECON_PPM)
-        out["ppm_flat_8_%s" % tag] = flat("ppm %s" % tag, cfg, U, 1.0 / 8)
-    del U
+    if want("ppm_flat"):
+        log("PPM flat batch (8192 blocks of 8^3, g = 3)")
+        U = workloads.flat_batch(8192, n=8, g=3)
+        for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+            cfg = amrhydro.euler_config(n=8, g=3, precision=prec, boundary=OUT, recon=amrhydro.AMRH_RECON_PPM)
+            out["ppm_flat_8_%s" % tag] = flat("ppm %s" % tag, cfg, U, 1.0 / 8)
+        del U
 
-    eos_tab = None
-    for eos, tag in ((amrhydro.AMRH_EOS_IDEAL, "ideal"), (amrhydro.AMRH_EOS_HELMHOLTZ, "helmholtz")):
-        log("rotating binary, %s EOS (20,000 leaves of 8^3, PPM, 13 fields)" % tag)
-        Bn = workloads.binary_batch(eos="ideal" if eos == amrhydro.AMRH_EOS_IDEAL else "helm")
-        if eos == amrhydro.AMRH_EOS_HELMHOLTZ:
-            eos_tab = amrhydro.HelmholtzEos(workloads.helmholtz_table())
-            eos_tab.set_units(*workloads.HELM_UNITS)
-
-        def prep(B, Bn=Bn, eos=eos):
-            B.set_block_origin(Bn["x0"])
-            B.set_potential(Bn["phi"])
+    if want("rotating"):
+        eos_tab = None
+        for eos, tag in ((amrhydro.AMRH_EOS_IDEAL, "ideal"), (amrhydro.AMRH_EOS_HELMHOLTZ, "helmholtz")):
+            log("rotating binary, %s EOS (20,000 leaves of 8^3, PPM, 13 fields)" % tag)
+            Bn = workloads.binary_batch(eos="ideal" if eos == amrhydro.AMRH_EOS_IDEAL else "helm")
             if eos == amrhydro.AMRH_EOS_HELMHOLTZ:
-                B.attach_eos(eos_tab)
-        for prec, ptag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
-            cfg = amrhydro.rotating_config(eos=eos, omega=Bn["omega"], boundary=OUT, precision=prec)
-            out["rotating_%s_%s" % (tag, ptag)] = flat("rotating %s %s" % (tag, ptag), cfg, Bn["U"], Bn["h"], prep)
-        del Bn
-    if eos_tab is not None:
-        eos_tab.close()
+                eos_tab = amrhydro.HelmholtzEos(workloads.helmholtz_table())
+                eos_tab.set_units(*workloads.HELM_UNITS)
 
-    log("Sedov octree (73 nodes of 8^3, fully refined to level 2)")
-    U, parent, child = workloads.sedov_tree(maxlevel=2)
-    for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
-        cfg = amrhydro.euler_config(n=8, grid_dim=0.5, precision=prec)
-        B = amrhydro.Batch(cfg, len(parent), 1.0)
-        B.set_stream(stream)
-        B.tree_set(parent, child)
-        B.upload(U)
-        B.tree_fill_ghosts()
-        B.tree_inject()
-        dt = torch.full((1,), 1e-6, dtype=torch.float64, device="cuda")
-        B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
-        torch.cuda.synchronize()
+            def prep(B, Bn=Bn, eos=eos):
+                B.set_block_origin(Bn["x0"])
+                B.set_potential(Bn["phi"])
+                if eos == amrhydro.AMRH_EOS_HELMHOLTZ:
+                    B.attach_eos(eos_tab)
+            for prec, ptag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+                cfg = amrhydro.rotating_config(eos=eos, omega=Bn["omega"], boundary=OUT, precision=prec)
+                out["rotating_%s_%s" % (tag, ptag)] = flat("rotating %s %s" % (tag, ptag), cfg, Bn["U"], Bn["h"], prep)
+            del Bn
+        if eos_tab is not None:
+            eos_tab.close()
+
+    if want("sedov_tree"):
+        log("Sedov octree (73 nodes of 8^3, fully refined to level 2)")
+        U, parent, child = workloads.sedov_tree(maxlevel=2)
+        for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+            cfg = amrhydro.euler_config(n=8, grid_dim=0.5, precision=prec)
+            B = amrhydro.Batch(cfg, len(parent), 1.0)
+            B.set_stream(stream)
+            B.tree_set(parent, child)
+            B.upload(U)
+            B.tree_fill_ghosts()
+            B.tree_inject()
+            dt = torch.full((1,), 1e-6, dtype=torch.float64, device="cuda")
+            B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            B.run_steps(steps, dt.data_ptr(), rk=3, use_graph=True)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            leaves = 64 * 8 ** 3
+            out["sedov_tree_73_%s" % tag] = {"cell_updates_per_s": 3 * steps * leaves / wall,
+                                             "us_per_step": 1e6 * wall / steps,
+                                             "device_error_flags": B.error_flags()}
+            log("  sedov tree %s: %.1f us per RK3 step" % (tag, 1e6 * wall / steps))
+            B.close()
+
+    if want("binary_octree"):
+        out.update(binary_octree(amrhydro, torch, steps, warmup, stream))
+
+    if want("poisson"):
+        log("self-gravity: multigrid Poisson on the binary's octree (%s)" % POISSON_TREE)
+        parent, child, lv, loc = workloads.binary_tree(POISSON_LEVELS)
+        D = amrhydro.Poisson(8, parent, child)
+        D.upload("S", workloads.binary_source(lv, loc))
         t0 = time.perf_counter()
-        B.run_steps(steps, dt.data_ptr(), rk=3, use_graph=True)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        leaves = 64 * 8 ** 3
-        out["sedov_tree_73_%s" % tag] = {"cell_updates_per_s": 3 * steps * leaves / wall,
-                                         "us_per_step": 1e6 * wall / steps,
-                                         "device_error_flags": B.error_flags()}
-        log("  sedov tree %s: %.1f us per RK3 step" % (tag, 1e6 * wall / steps))
-        B.close()
+        D.prepare()
+        t_prep = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        it, err = D.solve(POISSON_TOLER, prepare=False)
+        t_solve = time.perf_counter() - t0
+        ncyc = max(3, steps)
+        t0 = time.perf_counter()
+        for _ in range(ncyc):
+            D.vcycle()
+        t_cyc = (time.perf_counter() - t0) / ncyc
+        out["poisson_binary_tree"] = {
+            "workload": POISSON_TREE, "nodes": len(parent), "levels": int(lv.max()) + 1,
+            "boundary_ms": 1e3 * t_prep, "solve_ms": 1e3 * t_solve, "solve_vcycles": it,
+            "solve_error": err, "toler": POISSON_TOLER, "vcycle_ms": 1e3 * t_cyc}
+        log("  boundary %.2f ms; solve from phi = 0: %d V-cycles in %.1f ms; %.2f ms per V-cycle"
+            % (1e3 * t_prep, it, 1e3 * t_solve, 1e3 * t_cyc))
+        D.close()
 
-    out.update(binary_octree(amrhydro, torch, steps, warmup, stream))
-
-    log("self-gravity: multigrid Poisson on the binary's octree (%s)" % POISSON_TREE)
-    parent, child, lv, loc = workloads.binary_tree(POISSON_LEVELS)
-    D = amrhydro.Poisson(8, parent, child)
-    D.upload("S", workloads.binary_source(lv, loc))
-    t0 = time.perf_counter()
-    D.prepare()
-    t_prep = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    it, err = D.solve(POISSON_TOLER, prepare=False)
-    t_solve = time.perf_counter() - t0
-    ncyc = max(3, steps)
-    t0 = time.perf_counter()
-    for _ in range(ncyc):
-        D.vcycle()
-    t_cyc = (time.perf_counter() - t0) / ncyc
-    out["poisson_binary_tree"] = {
-        "workload": POISSON_TREE, "nodes": len(parent), "levels": int(lv.max()) + 1,
-        "boundary_ms": 1e3 * t_prep, "solve_ms": 1e3 * t_solve, "solve_vcycles": it,
-        "solve_error": err, "toler": POISSON_TOLER, "vcycle_ms": 1e3 * t_cyc}
-    log("  boundary %.2f ms; solve from phi = 0: %d V-cycles in %.1f ms; %.2f ms per V-cycle"
-        % (1e3 * t_prep, it, 1e3 * t_solve, 1e3 * t_cyc))
-    D.close()
     return out
 
 
+EXTRA_WORKLOADS = ("multilevel", "ppm_flat", "rotating", "sedov_tree", "binary_octree", "poisson")
 C4_TREE = "binary octree for C4, 25737 nodes of 8^3 (22520 leaves), 7 levels"
 
 
@@ -448,6 +459,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other workloads' lines")
     ap.add_argument("--only", choices=["mixed", "fp64", "both"], default="both")
+    ap.add_argument("--workload", choices=("headline",) + EXTRA_WORKLOADS, default="headline",
+                    help="headline: the contract line (flat batch, plus the other workloads unless "
+                    "--no-extra); any other: only that workload's lines, for profiling")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -473,6 +487,11 @@ def main():
     amrhydro.lib()
     # a real (non-null) stream shared by the library and the timing events
     torch.cuda.set_stream(torch.cuda.Stream())
+    if args.workload != "headline":
+        out = extra_configs(amrhydro, torch, args.steps, args.warmup, only=args.workload)
+        print(json.dumps({"workload": args.workload, "steps": args.steps, "warmup": args.warmup,
+                          "results": out}))
+        return
     n, nb = args.n, args.nblocks
     log("generating workload: %d blocks of %d^3 (+2 ghosts)" % (nb, n))
     U = workloads.flat_batch(nb, n=n)
