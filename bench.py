@@ -385,7 +385,11 @@ def cpu_baseline(n, target_s=10.0):
         cores = gomp.omp_get_max_threads()
     except OSError:
         cores = int(threads) if threads else os.cpu_count()
-    return {"value": rate, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"value": rate, "unit": "cell-updates/s", "cores": cores, "kind": "port", "nproc": nproc,
             "sample": "%d blocks of %d^3 (+2 ghosts) x %d RK3 steps (3 substeps each), FP64 oracle, "
                       "OpenMP over blocks, %.1f s timed" % (nb, n, steps, sum(times))}
 
