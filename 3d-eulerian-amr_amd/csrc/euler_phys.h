@@ -236,10 +236,15 @@ __device__ __forceinline__ double rcp64(double x) {
   return fma(r, e, r);  // one Newton step: ~2^-52
 }
 
+// libm pow, out of line: the kernels that inline pow_inv_gamma would otherwise hold pow's FP64
+// polynomial constants in registers for the whole block (they spilled them), although this path
+// runs only for gamma != 5/3
+__device__ __attribute__((noinline)) inline double pow_out_of_line(double x, double y) { return pow(x, y); }
+
 // x^(1/gamma) for x >= 1e-20: FP32 estimate, then for gamma = 5/3 one FP64 Newton step on
 // y^5 = x^3 (relative error ~1e-13); other gamma fall back to pow.
 __device__ __forceinline__ double pow_inv_gamma(double x, double gamma) {
-  if (gamma != 5.0 / 3.0) return pow(x, 1.0 / gamma);
+  if (gamma != 5.0 / 3.0) return pow_out_of_line(x, 1.0 / gamma);
   const float xf = (float)x;
   const double y0 = (double)__builtin_amdgcn_exp2f(0.6f * __builtin_amdgcn_logf(xf));
   const double y2 = y0 * y0;

@@ -72,6 +72,24 @@ __device__ __forceinline__ void apply_override(const SubstepArgs& a, int b, int 
   for (int l = 0; l < NFE; l++) F[l] = (T)o[l];
 }
 
+// A block record (NF fields of R elements, FP64) as a buffer resource: element e of field l is
+// read and written at the per-thread byte offset e*8 plus the uniform offset l*R*8, so a thread
+// keeps one 32-bit offset per cell instead of a 64-bit pointer per field (the record is far below
+// 2 GiB; every access is in range).
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+template <long R>
+struct RecBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ explicit RecBuf(const double* blk)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(blk), (short)0, (int)(NFE * R * 8), 0x00020000)) {}
+  __device__ double ld(int l, unsigned e) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, e * 8u, (int)(l * R * 8), 0));
+  }
+  __device__ void st(int l, unsigned e, double v) const {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, e * 8u, (int)(l * R * 8), 0);
+  }
+};
+
 // element offset of the column cell (i, j, kk), kk in [-G, N+G)
 template <int N>
 __device__ __forceinline__ long col_elem(const Layout& L, int i, int j, int kk) {
@@ -132,7 +150,7 @@ __device__ __forceinline__ void outflow_fix(double* u, int face) {
 }
 
 template <int N>
-__device__ __forceinline__ void put_ghost(double* Uo, const Layout& L, int face, int t, int i, int j, int k,
+__device__ __forceinline__ void put_ghost(const RecBuf<(long)N * N * N + 6L * GW * N * N>& Uo, const Layout& L, int face, int t, int i, int j, int k,
                                           const double* u, bool fix) {
   constexpr long R = (long)N * N * N + 6L * GW * N * N;
   double g[NFE];
@@ -147,7 +165,7 @@ __device__ __forceinline__ void put_ghost(double* Uo, const Layout& L, int face,
   else
     e = L.slab(face) + ((long)t * N + j) * N + i;
 #pragma unroll
-  for (int l = 0; l < NFE; l++) Uo[l * R + e] = g[l];
+  for (int l = 0; l < NFE; l++) Uo.st(l, (unsigned)e, g[l]);
 }
 
 // MODE 0: substep (flux + divergence + update).  MODE 1: max wavespeed scan only.
@@ -158,8 +176,10 @@ __device__ __forceinline__ void put_ghost(double* Uo, const Layout& L, int face,
 // reads its cell's conserved state back from memory (L2) at each plane instead of carrying three
 // planes of FP64 column state in registers, which removes its spills (1.88 -> 1.54 ms; on the
 // 8^3 multilevel batch 4.17 -> 3.39 ms); the mixed form keeps the column state, and runs three
-// waves per SIMD on 16^3 blocks (168 registers, a small spill: 2% faster than two without
-// spilling) but two on 8^3 blocks (three are 7% slower there).
+// waves per SIMD on 16^3 blocks (168 registers) but two on 8^3 blocks (three are no faster there).
+// Every record access goes through RecBuf (one 32-bit offset per cell, not a 64-bit pointer per
+// field) and libm pow stays out of line; together they took the mixed 16^3 substep from 50 spilled
+// registers to 4 (0.763 -> 0.711 ms).
 // TREE: the octree's reflux overrides and the root's flow-off ledger (a.slot, a.ledger); the flat
 // instantiation carries neither, which keeps its register count down.
 template <int N, bool MIXED, int MODE, bool TREE>
@@ -201,6 +221,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
   const int gw = tid - (NT - NGW);
   const bool is_gw = MODE == 0 && a.outflow && gw >= 0;
   double* const Uob = MODE == 0 ? a.Uout + (long)b * NFE * R : nullptr;
+  const RecBuf<R> ub(Ub), uo(Uob), u0(MODE == 0 && a.U0 ? a.U0 + (long)b * NFE * R : nullptr);
   auto write_xy_ghost = [&](int kk) {
     const int face = gw / (GW * N), r = gw % (GW * N);
     const double* src = face < 2 ? gb[face][r / GW] : gb[face][r % N];
@@ -211,7 +232,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
     // X slabs: (kk*N + jj)*G + t = kk*G*N + r;  Y slabs: (kk*G + t)*N + ii = kk*G*N + r
     const long e = L.slab(face) + (long)kk * GW * N + r;
 #pragma unroll
-    for (int l = 0; l < NFE; l++) Uob[l * R + e] = g[l];
+    for (int l = 0; l < NFE; l++) uo.st(l, (unsigned)e, g[l]);
   };
   const double hinv = MODE == 0 ? 1.0 / a.h[b] : 0.0;
 
@@ -226,26 +247,26 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
     T cm2[NFE], cm1[NFE], sm1[NFE];
     long e = col_elem<N>(L, i, j, -2);
 #pragma unroll
-    for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
+    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
     make_prim<MIXED>(u, cm2);
     e = col_elem<N>(L, i, j, -1);
 #pragma unroll
-    for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
+    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
     make_prim<MIXED>(u, cm1);
     e = col_elem<N>(L, i, j, 0);
 #pragma unroll
-    for (int l = 0; l < NFE; l++) ucur[l] = Ub[l * R + e];
+    for (int l = 0; l < NFE; l++) ucur[l] = ub.ld(l, (unsigned)e);
     make_prim<MIXED>(ucur, c0);
     e = col_elem<N>(L, i, j, 1);
 #pragma unroll
-    for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
+    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
     if constexpr (CARRY)
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc1[l] = u[l];
     make_prim<MIXED>(u, cp1);
     e = col_elem<N>(L, i, j, 2);
 #pragma unroll
-    for (int l = 0; l < NFE; l++) u[l] = Ub[l * R + e];
+    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
     if constexpr (CARRY)
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc2[l] = u[l];
@@ -268,7 +289,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
   double hraw[NFE];
   const bool is_halo = tid < 4 * HN;
   int hpx = 0, hpy = 0;
-  long hbase = 0;
+  unsigned hbase = 0;
   if (is_halo) {
     const int face = tid / HN, r = tid % HN;
     if (face < 2) {
@@ -280,9 +301,9 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
       hpx = ii + GW;
       hpy = face == YL ? t : N + GW + t;
     }
-    hbase = L.slab(face) + r;  // plane k adds k*HN (both slab kinds are plane-major)
+    hbase = (unsigned)(L.slab(face) + r);  // plane k adds k*HN (both slab kinds are plane-major)
 #pragma unroll
-    for (int l = 0; l < NFE; l++) hraw[l] = Ub[l * R + hbase];
+    for (int l = 0; l < NFE; l++) hraw[l] = ub.ld(l, hbase);
   }
 
   for (int k = 0; k < N; k++) {
@@ -298,26 +319,25 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
     // prefetch: halo of plane k+1, column cell k+3, U0 at k
     if (is_halo && k + 1 < N) {
 #pragma unroll
-      for (int l = 0; l < NFE; l++) hraw[l] = Ub[l * R + hbase + (long)(k + 1) * HN];
+      for (int l = 0; l < NFE; l++) hraw[l] = ub.ld(l, hbase + (unsigned)((k + 1) * HN));
     }
     double craw[NFE];
     const bool have_next = (k + 3) < N + GW;
     if (have_next) {
       const long e = col_elem<N>(L, i, j, k + 3);
 #pragma unroll
-      for (int l = 0; l < NFE; l++) craw[l] = Ub[l * R + e];
+      for (int l = 0; l < NFE; l++) craw[l] = ub.ld(l, (unsigned)e);
     }
     if (MODE == 0 && !CARRY && k > 0) {  // cell k's conserved state again (read 3 planes ago: L2)
       const long e = L.interior(i, j, k);
 #pragma unroll
-      for (int l = 0; l < NFE; l++) ucur[l] = Ub[l * R + e];
+      for (int l = 0; l < NFE; l++) ucur[l] = ub.ld(l, (unsigned)e);
     }
     double u0v[NFE];
     if (MODE == 0 && has_u0) {
-      const double* U0b = a.U0 + (long)b * NFE * R;
       const long e = L.interior(i, j, k);
 #pragma unroll
-      for (int l = 0; l < NFE; l++) u0v[l] = U0b[l * R + e];
+      for (int l = 0; l < NFE; l++) u0v[l] = u0.ld(l, (unsigned)e);
     }
     __syncthreads();
 
@@ -464,10 +484,9 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
           for (int l = 0; l < NFE; l++) lg[(((2 * 2 + 1) * N + j) * N + i) * NFE + l] = (double)Fzhi[l];
         }
       }
-      double* Uo = Uob;
       const long e = L.interior(i, j, k);
 #pragma unroll
-      for (int l = 0; l < NFE; l++) Uo[l * R + e] = u[l];
+      for (int l = 0; l < NFE; l++) uo.st(l, (unsigned)e, u[l]);
       if (a.outflow) {
         // The next stage's ghosts of an isolated outflow block are functions of its own boundary
         // cells (grid/grid_phys_bound.cpp:27-54), so they are written here instead of in a separate
@@ -500,7 +519,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
             outflow_fix(g, f);
 #pragma unroll 1
           for (int t = mirror ? GW - 1 - k : 0; t < (mirror ? GW - k : GW); t++)
-            put_ghost<N>(Uo, L, f, t, i, j, k, g, false);
+            put_ghost<N>(uo, L, f, t, i, j, k, g, false);
         }
       }
     }
