@@ -25,6 +25,7 @@
 #include "amrhydro.h"
 #include "face_order.h"
 #include "kt_rot.h"
+#include "rec_buf.h"
 #include "recon.h"
 #include "rot_phys.h"
 
@@ -37,6 +38,12 @@ using namespace rot;
 constexpr int N = 8;
 constexpr int N2 = N * N, N3 = N * N * N;
 constexpr int NFACE = (N + 1) * N * N;
+
+// libm calls of the frame source, out of line: inlined, their FP64 constants are hoisted out of the
+// block loop into registers that stay allocated for the whole kernel
+__device__ __attribute__((noinline)) double atan2_ool(double y, double x) { return atan2(y, x); }
+__device__ __attribute__((noinline)) double cos_ool(double x) { return cos(x); }
+__device__ __attribute__((noinline)) double sin_ool(double x) { return sin(x); }
 
 // record element -> interior-relative coordinates (i, j, k), any of them in [-G, N+G)
 template <int G>
@@ -128,6 +135,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   for (int bi = blockIdx.x; bi < a.nblocks; bi += gridDim.x) {
     const int b = a.nodes ? a.nodes[bi] : bi;
     const double* Ub = a.Uin + (long)b * NF * R;
+    const RecBuf<R, NF> ub(Ub);
     const double h = a.h[b];
     const double x0[3] = {a.x0[3 * b], a.x0[3 * b + 1], a.x0[3 * b + 2]};
     double D[NF];
@@ -148,7 +156,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         const int e = ALL ? r : region_to_elem<G>(d, r);
         double u[NF];
 #pragma unroll
-        for (int l = 0; l < NF; l++) u[l] = Ub[(long)l * R + e];
+        for (int l = 0; l < NF; l++) u[l] = ub.ld(l, (unsigned)e);
         int i, j, k;
         elem_ijk<G>(e, i, j, k);
         double X[3];
@@ -347,7 +355,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       const long e = (long)tid;
       double s[NF];
 #pragma unroll
-      for (int l = 0; l < NF; l++) s[l] = Ub[(long)l * R + e];
+      for (int l = 0; l < NF; l++) s[l] = ub.ld(l, (unsigned)e);
       double X[3];
       cell_pos(x0, h, ci, cj, ck, X);
       // face-averaged force of the potential (poisson/poisson.cpp:223-247, binary/binary.cpp:225-227)
@@ -414,20 +422,21 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       for (int l = 0; l < NF; l++) sv[l] = 0.0;
       {
         const double Rr = Rc(X);
-        sv[SR] += (p + pow(s[LZ] / Rr, 2.0) / s[D0]) / Rr;
-        const double th = atan2(X[1], X[0]);
-        sv[SR] += s[D0] * c.fR * cos(th - c.ftheta);
-        sv[LZ] -= s[D0] * c.fR * sin(th - c.ftheta) * Rr;
-        sv[rot::ET] += s[SR] * c.fR * cos(th - c.ftheta);
-        sv[rot::ET] -= st(c, s, X) * c.fR * sin(th - c.ftheta) * Rr;
+        sv[SR] += (p + f32::pow_out_of_line(s[LZ] / Rr, 2.0) / s[D0]) / Rr;
+        const double th = atan2_ool(X[1], X[0]);
+        const double cth = cos_ool(th - c.ftheta), sth = sin_ool(th - c.ftheta);
+        sv[SR] += s[D0] * c.fR * cth;
+        sv[LZ] -= s[D0] * c.fR * sth * Rr;
+        sv[rot::ET] += s[SR] * c.fR * cth;
+        sv[rot::ET] -= st(c, s, X) * c.fR * sth * Rr;
       }
 #pragma unroll
       for (int l = 0; l < NF; l++) D[l] += sv[l];
       double u[NF];
       if (has_u0) {
-        const double* U0b = a.U0 + (long)b * NF * R;
+        const RecBuf<R, NF> u0(a.U0 + (long)b * NF * R);
 #pragma unroll
-        for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + U0b[(long)l * R + e] * (1.0 - beta);
+        for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + u0.ld(l, (unsigned)e) * (1.0 - beta);
       } else {
 #pragma unroll
         for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + s[l] * (1.0 - beta);
@@ -436,9 +445,9 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       if (!(u[D0] > 0.0)) atomicOr(a.err, 1);
       const double R2 = X[0] * X[0] + X[1] * X[1];
       u[POT] = u[D0] * (pc - 0.5 * R2 * c.Omega0 * c.Omega0);
-      double* Uo = a.Uout + (long)b * NF * R;
+      const RecBuf<R, NF> uo(a.Uout + (long)b * NF * R);
 #pragma unroll
-      for (int l = 0; l < NF; l++) Uo[(long)l * R + e] = u[l];
+      for (int l = 0; l < NF; l++) uo.st(l, (unsigned)e, u[l]);
     } else {
       for (int off = 32; off > 0; off >>= 1) emax = rmax(emax, __shfl_xor(emax, off, 64));
       if ((tid & 63) == 0) red[tid >> 6] = emax;

@@ -7,6 +7,7 @@
 #include "amrh_internal.h"
 #include "cell_ops.h"
 #include "kt_kernels.h"
+#include "rec_buf.h"
 
 namespace amrh {
 
@@ -72,24 +73,6 @@ __device__ __forceinline__ void apply_override(const SubstepArgs& a, int b, int 
   for (int l = 0; l < NFE; l++) F[l] = (T)o[l];
 }
 
-// A block record (NF fields of R elements, FP64) as a buffer resource: element e of field l is
-// read and written at the per-thread byte offset e*8 plus the uniform offset l*R*8, so a thread
-// keeps one 32-bit offset per cell instead of a 64-bit pointer per field (the record is far below
-// 2 GiB; every access is in range).
-typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-template <long R>
-struct RecBuf {
-  __amdgpu_buffer_rsrc_t r;
-  __device__ explicit RecBuf(const double* blk)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(blk), (short)0, (int)(NFE * R * 8), 0x00020000)) {}
-  __device__ double ld(int l, unsigned e) const {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, e * 8u, (int)(l * R * 8), 0));
-  }
-  __device__ void st(int l, unsigned e, double v) const {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, e * 8u, (int)(l * R * 8), 0);
-  }
-};
-
 // element offset of the column cell (i, j, kk), kk in [-G, N+G)
 template <int N>
 __device__ __forceinline__ long col_elem(const Layout& L, int i, int j, int kk) {
@@ -150,8 +133,8 @@ __device__ __forceinline__ void outflow_fix(double* u, int face) {
 }
 
 template <int N>
-__device__ __forceinline__ void put_ghost(const RecBuf<(long)N * N * N + 6L * GW * N * N>& Uo, const Layout& L, int face, int t, int i, int j, int k,
-                                          const double* u, bool fix) {
+__device__ __forceinline__ void put_ghost(const RecBuf<(long)N * N * N + 6L * GW * N * N, NFE>& Uo, const Layout& L,
+                                          int face, int t, int i, int j, int k, const double* u, bool fix) {
   constexpr long R = (long)N * N * N + 6L * GW * N * N;
   double g[NFE];
 #pragma unroll
@@ -221,7 +204,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N
   const int gw = tid - (NT - NGW);
   const bool is_gw = MODE == 0 && a.outflow && gw >= 0;
   double* const Uob = MODE == 0 ? a.Uout + (long)b * NFE * R : nullptr;
-  const RecBuf<R> ub(Ub), uo(Uob), u0(MODE == 0 && a.U0 ? a.U0 + (long)b * NFE * R : nullptr);
+  const RecBuf<R, NFE> ub(Ub), uo(Uob), u0(MODE == 0 && a.U0 ? a.U0 + (long)b * NFE * R : nullptr);
   auto write_xy_ghost = [&](int kk) {
     const int face = gw / (GW * N), r = gw % (GW * N);
     const double* src = face < 2 ? gb[face][r / GW] : gb[face][r % N];
