@@ -19,6 +19,7 @@
 #include "cell_ops.h"
 #include "face_order.h"
 #include "kt_kernels.h"
+#include "rec_buf.h"
 #include "recon.h"
 
 namespace amrh {
@@ -133,6 +134,7 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
   for (int bb = blockIdx.x; bb < nbl; bb += gridDim.x) {
     const int b = MODE == 0 ? (TREE ? a.nodes[bb] : a.block0 + bb) : bb;
     const double* Ub = (MODE == 0 ? a.Uin : m.U) + (long)b * NFE * R;
+    const RecBuf<R, NFE> ub(Ub);
 
     // stage the primitives; each thread keeps its own interior cell's conserved state
     double ucur[NFE];
@@ -140,7 +142,7 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
     for (int e = tid; e < R; e += NT) {
       double u[NFE];
 #pragma unroll
-      for (int l = 0; l < NFE; l++) u[l] = Ub[(long)l * R + e];
+      for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
       if (e == tid) {  // the first pass: e = tid < N^3 is this thread's interior cell
 #pragma unroll
         for (int l = 0; l < NFE; l++) ucur[l] = u[l];
@@ -249,9 +251,9 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
       const long e = (long)tid;  // interior element of cell (ci, cj, ck)
       double u0v[NFE];
       if (has_u0) {
-        const double* U0b = a.U0 + (long)b * NFE * R;
+        const RecBuf<R, NFE> u0(a.U0 + (long)b * NFE * R);
 #pragma unroll
-        for (int l = 0; l < NFE; l++) u0v[l] = U0b[(long)l * R + e];
+        for (int l = 0; l < NFE; l++) u0v[l] = u0.ld(l, (unsigned)e);
       }
       double u[NFE];
 #pragma unroll
@@ -267,9 +269,9 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
       else
         floor_mixed(PP, u);
       if (!(u[RHO] > 0.0) && a.err) atomicOr(a.err, 1);
-      double* Uo = a.Uout + (long)b * NFE * R;
+      const RecBuf<R, NFE> uo(a.Uout + (long)b * NFE * R);
 #pragma unroll
-      for (int l = 0; l < NFE; l++) Uo[(long)l * R + e] = u[l];
+      for (int l = 0; l < NFE; l++) uo.st(l, (unsigned)e, u[l]);
     } else {
       for (int off = 32; off > 0; off >>= 1) emax = x64::rmax(emax, __shfl_xor(emax, off, 64));
       if ((tid & 63) == 0) red[tid >> 6] = emax;
