@@ -27,6 +27,7 @@ struct F64Buf {
 // a block record: NF fields of R elements each
 template <long R, int NF>
 struct RecBuf : F64Buf {
+  static_assert((long)NF * R * 8 < (1L << 31), "a block record must fit one buffer resource");
   __device__ explicit RecBuf(const double* blk) : F64Buf(blk, NF * R) {}
   __device__ double ld(int l, unsigned e) const { return F64Buf::ld(e, l * R); }
   __device__ void st(int l, unsigned e, double v) const { F64Buf::st(e, v, l * R); }
