@@ -116,6 +116,37 @@ def test_rk3_fp64_trajectory_matches_oracle():
     assert np.all(e <= FP64_TOL), e
 
 
+def test_full_size_flat_batch_rk3_step():
+    """The headline workload at its full size (4096 blocks of 16^3 + 2 ghosts, workloads.flat_batch):
+    one RK3 step with the reference's first dt, the FP64 form against the oracle (<= 1e-12 relative
+    L1 per field) and the mixed form against the FP64 form (density <= 1e-6)."""
+    import workloads
+
+    n, nb = 16, 4096
+    U = workloads.flat_batch(nb, n=n)
+    cfg = orc.euler_cfg(n=n)
+    orc.outflow_ghosts(cfg, U)
+    h = np.full(nb, 1.0 / n)
+    dt = oracle_dt(cfg, U, h[0], None)
+    ref = U.copy()
+    orc.rk_steps(cfg, ref, h, dt, 3, 1)
+    outs = []
+    for prec in (amrhydro.AMRH_FP64, amrhydro.AMRH_MIXED):
+        B = gpu_batch(n, nb, h, precision=prec)
+        B.upload(U)
+        B.rk_step(dt, 3)
+        B.fill_outflow_ghosts()
+        outs.append(B.download())
+        assert B.error_flags() == 0
+        B.close()
+    e64 = rel_l1(outs[0], ref, 2)
+    e32 = rel_l1(outs[1], outs[0], 2)
+    say("full C2 batch, one RK3 step: fp64 vs oracle %s; mixed vs fp64 %s"
+        % (np.array2string(e64, precision=2), np.array2string(e32, precision=2)))
+    assert np.all(e64 <= FP64_TOL), e64
+    assert e32[0] <= 1e-6, e32
+
+
 def test_mixed_precision_error_after_rk3_steps():
     """Mixed form (FP32 reconstruction/flux/wavespeed, FP64 accumulate): 30 RK3 steps of a C2-shaped
     batch against the FP64 oracle.  Density rel-L1 must stay <= 1e-6."""
