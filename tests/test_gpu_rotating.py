@@ -85,6 +85,49 @@ def test_rotating_substeps_match_oracle(eos, recon):
     G.close()
 
 
+def test_full_size_rotating_batch_rk3_step():
+    """C4 at its full size (workloads.binary_batch: 20,000 leaves of 8^3, ideal gas, PPM, outflow):
+    one RK3 step of the whole batch on the GPU; every 40th block (blocks are independent) against
+    the oracle, the FP64 form <= 1e-12 relative L1 per field, the mixed form density <= 1e-6."""
+    import workloads
+
+    Bn = workloads.binary_batch(eos="ideal")
+    U, phi, x0, h = Bn["U"], Bn["phi"], Bn["x0"], Bn["h"]
+    om = Bn["omega"]
+    cfg_o = orc.rot_cfg(eos=0, recon=1, omega=om)
+    sel = np.arange(0, U.shape[0], 40)
+    ref = np.ascontiguousarray(U[sel])
+    ps, xs, hs = np.ascontiguousarray(phi[sel]), np.ascontiguousarray(x0[sel]), np.ascontiguousarray(h[sel])
+    orc.rot_outflow_ghosts(cfg_o, ref, xs, hs)
+    m = orc.rot_max_eig(cfg_o, ref, xs, hs)
+    dt = 0.1 * cfg_o.cfl * (hs / m).min()
+    ref0 = ref.copy()
+    for beta in (1.0, 0.25, 2.0 / 3.0):
+        orc.rot_outflow_ghosts(cfg_o, ref, xs, hs)
+        assert not orc.rot_substep(cfg_o, ref, ref0, ps, xs, hs, dt, beta).any()
+    say("full C4 batch: %d blocks, %d checked against the oracle" % (U.shape[0], len(sel)))
+    outs = []
+    for prec in (amrhydro.AMRH_FP64, amrhydro.AMRH_MIXED):
+        cfg = amrhydro.rotating_config(eos=0, omega=om, precision=prec)
+        G = amrhydro.Batch(cfg, U.shape[0], h)
+        G.set_block_origin(x0)
+        G.set_potential(phi)
+        G.upload(U)
+        G.store()
+        for beta in (1.0, 0.25, 2.0 / 3.0):
+            G.fill_outflow_ghosts()
+            G.substep(dt, beta)
+        assert G.error_flags() == 0
+        outs.append(G.download()[sel])
+        G.close()
+    e64 = rel_l1_rot(outs[0], ref, 3)
+    e32 = rel_l1_rot(outs[1], ref, 3)
+    say("  fp64 vs oracle %s" % np.array2string(e64, precision=1))
+    say("  mixed vs oracle %s" % np.array2string(e32, precision=1))
+    assert np.all(e64 <= TOL), e64
+    assert e32[rot_field("rho")] <= 1e-6, e32
+
+
 def test_rotating_ghosts_match_oracle():
     B, cfg, G = setup(0, 1, seed=12)
     U = B["U"].copy()
