@@ -85,16 +85,17 @@ def test_rotating_substeps_match_oracle(eos, recon):
     G.close()
 
 
-def test_full_size_rotating_batch_rk3_step():
-    """C4 at its full size (workloads.binary_batch: 20,000 leaves of 8^3, ideal gas, PPM, outflow):
-    one RK3 step of the whole batch on the GPU; every 40th block (blocks are independent) against
-    the oracle, the FP64 form <= 1e-12 relative L1 per field, the mixed form density <= 1e-6."""
-    import workloads
-
-    Bn = workloads.binary_batch(eos="ideal")
+@pytest.mark.parametrize("eos", [0, 1])
+def test_full_size_rotating_batch_rk3_step(eos):
+    """C4 and C5 at their full size (workloads.binary_batch: 20,000 leaves of 8^3, ideal gas or
+    Helmholtz EOS, PPM, outflow): one RK3 step of the whole batch on the GPU; every 40th block
+    (blocks are independent) against the oracle, the FP64 form <= 1e-12 relative L1 per field, the
+    mixed form density <= 1e-6."""
+    Bn = workloads.binary_batch(eos="helm" if eos == 1 else "ideal")
     U, phi, x0, h = Bn["U"], Bn["phi"], Bn["x0"], Bn["h"]
     om = Bn["omega"]
-    cfg_o = orc.rot_cfg(eos=0, recon=1, omega=om)
+    cfg_o = orc.rot_cfg(eos=eos, recon=1, omega=om)
+    E = helm_eos() if eos == 1 else None  # the table, for the oracle and the GPU
     sel = np.arange(0, U.shape[0], 40)
     ref = np.ascontiguousarray(U[sel])
     ps, xs, hs = np.ascontiguousarray(phi[sel]), np.ascontiguousarray(x0[sel]), np.ascontiguousarray(h[sel])
@@ -105,13 +106,15 @@ def test_full_size_rotating_batch_rk3_step():
     for beta in (1.0, 0.25, 2.0 / 3.0):
         orc.rot_outflow_ghosts(cfg_o, ref, xs, hs)
         assert not orc.rot_substep(cfg_o, ref, ref0, ps, xs, hs, dt, beta).any()
-    say("full C4 batch: %d blocks, %d checked against the oracle" % (U.shape[0], len(sel)))
+    say("full batch, eos=%d: %d blocks, %d checked against the oracle" % (eos, U.shape[0], len(sel)))
     outs = []
     for prec in (amrhydro.AMRH_FP64, amrhydro.AMRH_MIXED):
-        cfg = amrhydro.rotating_config(eos=0, omega=om, precision=prec)
+        cfg = amrhydro.rotating_config(eos=eos, omega=om, precision=prec)
         G = amrhydro.Batch(cfg, U.shape[0], h)
         G.set_block_origin(x0)
         G.set_potential(phi)
+        if E is not None:
+            G.attach_eos(E)
         G.upload(U)
         G.store()
         for beta in (1.0, 0.25, 2.0 / 3.0):
