@@ -78,6 +78,7 @@ SIGNATURES = {
     "amrh_download": (C.c_int, [C.c_void_p, DP]),
     "amrh_download_device": (C.c_int, [C.c_void_p, C.c_void_p]),
     "amrh_store": (C.c_int, [C.c_void_p]),
+    "amrh_restore": (C.c_int, [C.c_void_p]),
     "amrh_fill_outflow_ghosts": (C.c_int, [C.c_void_p]),
     "amrh_substep": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "amrh_substep_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
@@ -90,6 +91,8 @@ SIGNATURES = {
     "amrh_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "amrh_run_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "amrh_tree_set": (C.c_int, [C.c_void_p, IP, IP]),
+    "amrh_tree_set_origin": (C.c_int, [C.c_void_p, DP]),
+    "amrh_tree_get_origin": (C.c_int, [C.c_void_p, DP]),
     "amrh_tree_fill_ghosts": (C.c_int, [C.c_void_p]),
     "amrh_tree_substep": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "amrh_tree_substep_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
@@ -231,6 +234,10 @@ class Batch:
     def store(self):
         _ck(lib().amrh_store(self.ctx))
 
+    def restore(self):
+        """Grid::restore over every block (grid/grid.cpp:187-199): U = U0, FO = FO0."""
+        _ck(lib().amrh_restore(self.ctx))
+
     def fill_outflow_ghosts(self):
         _ck(lib().amrh_fill_outflow_ghosts(self.ctx))
 
@@ -345,6 +352,16 @@ class Batch:
         c = np.ascontiguousarray(child, dtype=np.int32)
         assert len(p) == self.nb and len(c) == self.nb
         _ck(lib().amrh_tree_set(self.ctx, p.ctypes.data_as(IP), c.ctypes.data_as(IP)))
+
+    def tree_set_origin(self, origin):
+        """GridNode::set_origin (grid/grid_node.cpp:20-26)."""
+        o = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
+        _ck(lib().amrh_tree_set_origin(self.ctx, _dp(o)))
+
+    def tree_origin(self):
+        o = np.zeros(3)
+        _ck(lib().amrh_tree_get_origin(self.ctx, _dp(o)))
+        return o
 
     def tree_fill_ghosts(self):
         _ck(lib().amrh_tree_fill_ghosts(self.ctx))

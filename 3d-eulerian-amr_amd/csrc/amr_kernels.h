@@ -26,6 +26,7 @@ hipError_t launch_restrict(int n, int g, double* U, const int2* pairs, int count
 // the root's flow-off ledger of nf fields: FO = (FO + dFO dt) beta + FO0 (1 - beta), fo = [FO | FO0]
 hipError_t launch_ledger(int n, int nf, const double* ledger, double h, const double* dt_dev, double dt, double beta,
                          int z_reflect, double* fo, hipStream_t s);
-hipError_t launch_ledger_store(double* fo, int nf, hipStream_t s);
+hipError_t launch_ledger_store(double* fo, int nf, hipStream_t s);     // FO0 = FO
+hipError_t launch_ledger_restore(double* fo, int nf, hipStream_t s);   // FO = FO0
 
 }  // namespace amrh

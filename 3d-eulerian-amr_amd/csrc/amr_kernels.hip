@@ -300,6 +300,10 @@ __global__ void ledger_store_kernel(double* fo, int nf) {
   if (threadIdx.x < nf) fo[nf + threadIdx.x] = fo[threadIdx.x];
 }
 
+__global__ void ledger_restore_kernel(double* fo, int nf) {
+  if (threadIdx.x < nf) fo[threadIdx.x] = fo[nf + threadIdx.x];
+}
+
 }  // namespace
 
 hipError_t launch_tree_ghosts(int n, int g, double* U, const int* nodes, int count, const int* ghost, const int* rel,
@@ -368,6 +372,11 @@ hipError_t launch_ledger(int n, int nf, const double* ledger, double h, const do
 
 hipError_t launch_ledger_store(double* fo, int nf, hipStream_t s) {
   ledger_store_kernel<<<1, 64, 0, s>>>(fo, nf);
+  return hipGetLastError();
+}
+
+hipError_t launch_ledger_restore(double* fo, int nf, hipStream_t s) {
+  ledger_restore_kernel<<<1, 64, 0, s>>>(fo, nf);
   return hipGetLastError();
 }
 

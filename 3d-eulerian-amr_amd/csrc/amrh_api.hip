@@ -126,6 +126,7 @@ int amrh_create(const amrh_config* cfg, int nblocks, amrh_ctx** out) {
 void amrh_destroy(amrh_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   for (int q = 0; q < 2; q++)
     if (c->buf[q]) (void)hipFree(c->buf[q]);
   if (c->h_dev) (void)hipFree(c->h_dev);
@@ -217,6 +218,17 @@ int amrh_store(amrh_ctx* c) {
   return AMRH_OK;
 }
 
+int amrh_restore(amrh_ctx* c) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
+  if (c->u0buf < 0) return fail(AMRH_ESTATE, "restore needs amrh_store() first");
+  // U = U0: the RK base buffer becomes the current stage again (it stays the RK base, so the next
+  // substep writes the other buffer and U0 survives for another restore)
+  c->cur = c->u0buf;
+  if (c->tree) HIPCHK(launch_ledger_restore(c->tree->fo_dev, c->cfg.nf, c->stream));  // FO = FO0
+  return AMRH_OK;
+}
+
 int amrh_fill_outflow_ghosts(amrh_ctx* c) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
@@ -228,7 +240,10 @@ int amrh_fill_outflow_ghosts(amrh_ctx* c) {
 int amrh_set_block_origin(amrh_ctx* c, const double* x0) {
   if (!c || !x0) return fail(AMRH_EINVAL, "null argument");
   if (c->tree) return fail(AMRH_EINVAL, "an octree's positions follow from amrh_tree_set");
-  if (!c->x0_dev) HIPCHK(hipMalloc(&c->x0_dev, sizeof(double) * 3 * c->nb));
+  if (!c->x0_dev) {
+    drop_graphs(c);  // a new device array
+    HIPCHK(hipMalloc(&c->x0_dev, sizeof(double) * 3 * c->nb));
+  }
   HIPCHK(hipMemcpyAsync(c->x0_dev, x0, sizeof(double) * 3 * c->nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->have_x0 = true;
@@ -260,6 +275,7 @@ int amrh_set_potential_device(amrh_ctx* c, const void* phi) {
 int amrh_set_forces(amrh_ctx* c, const double* f) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "the forces belong to ROTATING physics");
+  drop_graphs(c);  // captured substeps hold the force pointer
   if (!f) {
     c->force_dev = nullptr;
     return AMRH_OK;
@@ -274,6 +290,7 @@ int amrh_set_forces(amrh_ctx* c, const double* f) {
 int amrh_set_forces_device(amrh_ctx* c, const void* f) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (c->cfg.physics != AMRH_PHYS_ROTATING) return fail(AMRH_EINVAL, "the forces belong to ROTATING physics");
+  if (c->force_dev != (const double*)f) drop_graphs(c);  // captured substeps hold the force pointer
   c->force_dev = (const double*)f;
   return AMRH_OK;
 }
@@ -292,6 +309,7 @@ int amrh_attach_eos(amrh_ctx* c, amrh_eos* eos) {
   if (!c || !eos) return fail(AMRH_EINVAL, "null argument");
   if (c->cfg.physics != AMRH_PHYS_ROTATING || c->cfg.eos != AMRH_EOS_HELMHOLTZ)
     return fail(AMRH_EINVAL, "an EOS table belongs to ROTATING physics with eos = AMRH_EOS_HELMHOLTZ");
+  if (c->eos != eos) drop_graphs(c);  // captured substeps hold the table's pointers
   c->eos = eos;
   return AMRH_OK;
 }

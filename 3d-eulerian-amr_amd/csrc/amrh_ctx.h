@@ -59,6 +59,7 @@ struct amrh_ctx {
   hipStream_t own_stream = nullptr;
   bool have_h = false, have_u = false;
   amrh::TreeState* tree = nullptr;
+  double origin[3] = {0.0, 0.0, 0.0};  // Grid::origin of every node of the tree (grid/grid.h:24)
   // AMRH_PHYS_ROTATING
   double* x0_dev = nullptr;    // [nb][3] block origins
   double* phi_dev = nullptr;   // [nb][(n+2)^3] potential
@@ -90,6 +91,17 @@ int ctx_ready(const amrh_ctx* c);
 hipError_t ctx_outflow_ghosts(const amrh_ctx* c, double* U);
 // the rotating kernels' arguments common to every launch: potential, forces, positions, constants
 RotArgs ctx_rot_args(const amrh_ctx* c);
+// forget the captured step graphs (their kernel arguments hold pointers and tables that a call
+// is about to replace); frees them
+void drop_graphs(amrh_ctx* c);
+// the low interior face of every node of the tree, Grid::xf(bw) with the context's origin, to the
+// device (amrh_tree.hip)
+int tree_upload_x0(amrh_ctx* c);
+// state_sum / max / min reductions (amrh_diag.hip): the checks, the size of the per-block partials,
+// and the two passes (no allocation; asynchronous on the context's stream)
+int diag_check(const amrh_ctx* c);
+long diag_part_elems(const amrh_ctx* c);
+int diag_launch(const amrh_ctx* c, double* part, double* out, bool sums_layout);
 
 }  // namespace amrh
 

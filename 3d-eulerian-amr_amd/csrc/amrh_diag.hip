@@ -74,10 +74,14 @@ __global__ __launch_bounds__(DT) void diag_block_kernel(const double* __restrict
   }
 }
 
-__global__ void diag_combine_kernel(const double* __restrict__ part, int nb, int nf, double* __restrict__ out) {
+// sums_layout = 0: out[l*NQ + q] for every quantity; 1: only the moments, as amrh_state_sum
+// returns them, out[q*nf + l] for q < 4
+__global__ void diag_combine_kernel(const double* __restrict__ part, int nb, int nf, double* __restrict__ out,
+                                    int sums_layout) {
   const int t = threadIdx.x;
   if (t >= nf * NQ) return;
   const int l = t / NQ, q = t % NQ;
+  if (sums_layout && q >= 4) return;
   double v = q < 4 ? 0.0 : (q == 4 ? -1.0e+99 : 1.0e+99);
   for (int b = 0; b < nb; b++) {
     const double p = part[((long)b * nf + l) * NQ + q];
@@ -88,32 +92,52 @@ __global__ void diag_combine_kernel(const double* __restrict__ part, int nb, int
     else
       v = dmin(p, v);
   }
-  out[t] = v;
+  out[sums_layout ? q * nf + l : t] = v;
 }
 
-int run_diag(amrh_ctx* c, std::vector<double>& res) {
-  if (!c) return fail(AMRH_EINVAL, "null ctx");
+}  // namespace
+
+int amrh::diag_check(const amrh_ctx* c) {
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
+  if (c->cfg.nf * NQ > 1024) return fail(AMRH_EINVAL, "too many fields");
+  const int n = c->cfg.n, g = c->cfg.g;
+  if (!((n == 8 && (g == 2 || g == 3)) || (n == 16 && g == 2)))
+    return fail(AMRH_EINVAL, "no diagnostics for n = %d, g = %d", n, g);
+  return AMRH_OK;
+}
+
+long amrh::diag_part_elems(const amrh_ctx* c) { return (long)c->nb * c->cfg.nf * NQ; }
+
+// pass 1 into part (diag_part_elems doubles), pass 2 into out: the state_sum layout (4*nf) when
+// sums_layout, else [nf][NQ]; asynchronous on the context's stream, no allocation
+int amrh::diag_launch(const amrh_ctx* c, double* part, double* out, bool sums_layout) {
   const int nf = c->cfg.nf, nb = c->nb;
-  if (nf * NQ > 1024) return fail(AMRH_EINVAL, "too many fields");
-  if (!c->diag_part) {
-    HIPCHK(hipMalloc(&c->diag_part, sizeof(double) * ((size_t)nb * nf * NQ + (size_t)nf * NQ)));
-  }
-  double* out = c->diag_part + (size_t)nb * nf * NQ;
   const unsigned char* cov = c->tree ? c->tree->covered_dev : nullptr;
   const double* x0 = c->have_x0 ? c->x0_dev : nullptr;
   const int n = c->cfg.n, g = c->cfg.g;
   if (n == 8 && g == 2)
-    diag_block_kernel<8, 2><<<nb, DT, 0, c->stream>>>(c->buf[c->cur], nf, x0, c->h_dev, cov, c->diag_part);
+    diag_block_kernel<8, 2><<<nb, DT, 0, c->stream>>>(c->buf[c->cur], nf, x0, c->h_dev, cov, part);
   else if (n == 8 && g == 3)
-    diag_block_kernel<8, 3><<<nb, DT, 0, c->stream>>>(c->buf[c->cur], nf, x0, c->h_dev, cov, c->diag_part);
-  else if (n == 16 && g == 2)
-    diag_block_kernel<16, 2><<<nb, DT, 0, c->stream>>>(c->buf[c->cur], nf, x0, c->h_dev, cov, c->diag_part);
+    diag_block_kernel<8, 3><<<nb, DT, 0, c->stream>>>(c->buf[c->cur], nf, x0, c->h_dev, cov, part);
   else
-    return fail(AMRH_EINVAL, "no diagnostics for n = %d, g = %d", n, g);
+    diag_block_kernel<16, 2><<<nb, DT, 0, c->stream>>>(c->buf[c->cur], nf, x0, c->h_dev, cov, part);
   HIPCHK(hipGetLastError());
-  diag_combine_kernel<<<1, 1024, 0, c->stream>>>(c->diag_part, nb, nf, out);
+  diag_combine_kernel<<<1, 1024, 0, c->stream>>>(part, nb, nf, out, sums_layout ? 1 : 0);
   HIPCHK(hipGetLastError());
+  return AMRH_OK;
+}
+
+namespace {
+
+int run_diag(amrh_ctx* c, std::vector<double>& res) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (int rc = diag_check(c)) return rc;
+  const int nf = c->cfg.nf, nb = c->nb;
+  if (!c->diag_part) {
+    HIPCHK(hipMalloc(&c->diag_part, sizeof(double) * ((size_t)nb * nf * NQ + (size_t)nf * NQ)));
+  }
+  double* out = c->diag_part + (size_t)nb * nf * NQ;
+  if (int rc = diag_launch(c, c->diag_part, out, false)) return rc;
   res.assign((size_t)nf * NQ, 0.0);
   HIPCHK(hipMemcpyAsync(res.data(), out, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

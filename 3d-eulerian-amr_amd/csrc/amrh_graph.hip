@@ -46,16 +46,6 @@ int time_launches(amrh_ctx* c, int reps, F fn, float* ms) {
   return AMRH_OK;
 }
 
-void drop_graphs(amrh_ctx* c) {
-  for (int p = 0; p < 2; p++)
-    if (c->step_exec[p]) {
-      (void)hipGraphExecDestroy(c->step_exec[p]);
-      c->step_exec[p] = nullptr;
-    }
-  c->step_exec_rk = 0;
-  c->step_exec_dt = nullptr;
-}
-
 const double kBetas[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
 
 // one RK step with the device dt control (scale 1, growth MAXDTINC = 1.25), as launched
@@ -77,6 +67,16 @@ int step_once(amrh_ctx* c, int rk, double* dt_dev) {
 }
 
 }  // namespace
+
+void amrh::drop_graphs(amrh_ctx* c) {
+  for (int p = 0; p < 2; p++)
+    if (c->step_exec[p]) {
+      (void)hipGraphExecDestroy(c->step_exec[p]);
+      c->step_exec[p] = nullptr;
+    }
+  c->step_exec_rk = 0;
+  c->step_exec_dt = nullptr;
+}
 
 extern "C" {
 

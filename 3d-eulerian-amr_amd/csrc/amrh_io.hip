@@ -130,15 +130,16 @@ int amrh_tree_output_mesh(amrh_ctx* c, double* x, double* y, double* z, int* nod
     const double h = t->h[b];
     zone0[q] = zacc;
     zacc += open_octants(t, b) * (long)H * H * H;
-    // Grid::xf / yf / zf (grid/grid.cpp:510-526), origin 0
+    // Grid::xf / yf / zf (grid/grid.cpp:510-526); z subtracts the origin only under Z_REFLECT
     if (x && y && z) {
+      const double* o = c->origin;
       for (int k = 0; k < G1; k++)
         for (int j = 0; j < G1; j++)
           for (int i = 0; i < G1; i++) {
             const long p = pi + ((long)k * G1 + j) * G1 + i;
-            x[p] = double(t->off[b][0] + i) * h - gd - double(g) * h0 - 0.0;
-            y[p] = double(t->off[b][1] + j) * h - gd - double(g) * h0 - 0.0;
-            z[p] = c->cfg.z_reflect ? double(t->off[b][2] + k) * h - double(g) * h0 - 0.0
+            x[p] = double(t->off[b][0] + i) * h - gd - double(g) * h0 - o[0];
+            y[p] = double(t->off[b][1] + j) * h - gd - double(g) * h0 - o[1];
+            z[p] = c->cfg.z_reflect ? double(t->off[b][2] + k) * h - double(g) * h0 - o[2]
                                     : double(t->off[b][2] + k) * h - gd - double(g) * h0;
           }
     }
@@ -267,7 +268,7 @@ int amrh_tree_checkpoint_write_ex(amrh_ctx* c, const char* path, double time, do
     w.put(time);
     for (int d = 0; d < 3; d++) {
       w.put(t->off[b][d]);
-      w.put(0.0);  // origin
+      w.put(c->origin[d]);  // every node carries the tree's origin (grid/grid_node.cpp:20-26, :281)
     }
     for (int l = 0; l < nf; l++) w.put(b == 0 ? fo[l] : 0.0);  // only the root keeps a ledger
     const double* Ub = U.data() + (size_t)b * nf * G * G * G;
@@ -311,7 +312,7 @@ int amrh_tree_checkpoint_read_ex(const char* path, const amrh_config* cfg, amrh_
   }
   std::vector<int> parent, child;
   std::vector<double> U;
-  double fo[16] = {0}, t0 = 0.0, alpha = 1.0;
+  double fo[16] = {0}, t0 = 0.0, alpha = 1.0, origin[3] = {0.0, 0.0, 0.0};
   // depth first; each frame remembers which child flags are still to be read
   struct Frame {
     int node;
@@ -328,8 +329,9 @@ int amrh_tree_checkpoint_read_ex(const char* path, const amrh_config* cfg, amrh_
     const double tm = r.get<double>();
     if (b == 0) t0 = tm;
     for (int d = 0; d < 3; d++) {
-      (void)r.get<int>();     // offset: implied by the tree
-      (void)r.get<double>();  // origin
+      (void)r.get<int>();  // offset: implied by the tree
+      const double o = r.get<double>();
+      if (b == 0) origin[d] = o;  // the root's; GridNode::set_origin gives every node the same
     }
     for (int l = 0; l < nf; l++) {
       const double v = r.get<double>();
@@ -364,7 +366,8 @@ int amrh_tree_checkpoint_read_ex(const char* path, const amrh_config* cfg, amrh_
   amrh_ctx* c = nullptr;
   const int nb = (int)parent.size();
   if (int rc = amrh_create(cfg, nb, &c)) return rc;
-  int rc = amrh_tree_set(c, parent.data(), child.data());
+  int rc = amrh_tree_set_origin(c, origin);
+  if (!rc) rc = amrh_tree_set(c, parent.data(), child.data());
   if (!rc) rc = amrh_upload(c, U.data());
   if (!rc) {
     hipError_t e = hipMemcpy(c->tree->fo_dev, fo, sizeof(double) * nf, hipMemcpyHostToDevice);

@@ -110,6 +110,8 @@ int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index) {
   const bool rot = dst->cfg.physics == AMRH_PHYS_ROTATING;
   if (rot && dst->cfg.eos == AMRH_EOS_HELMHOLTZ && !dst->eos)
     return fail(AMRH_ESTATE, "eos = HELMHOLTZ needs amrh_attach_eos on the new context");
+  // new nodes take their parent's origin (grid/grid_node.cpp:281): the new tree keeps the old one's
+  if (int rc = amrh_tree_set_origin(dst, src->origin)) return rc;
   const TreeState* t = dst->tree;
   const int nb = dst->nb, n = dst->cfg.n, nf = dst->cfg.nf;
   std::vector<int> idx(src_index, src_index + nb);

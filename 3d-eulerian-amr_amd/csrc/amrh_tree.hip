@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <tuple>
 
@@ -364,6 +365,33 @@ double max_dt_rec(const TreeState* t, const std::vector<double>& eig, double cfl
 
 }  // namespace
 
+// The low interior face of every node, Grid::xf(bw) (grid/grid.cpp:512-526):
+//   (offset + bw) h - GRID_DIM - bw h0 - origin   in x and y;
+//   z: the same without -origin_z, or, when the lower z boundary reflects, without -GRID_DIM
+//   (the reference subtracts origin_z only under Z_REFLECT).
+int amrh::tree_upload_x0(amrh_ctx* c) {
+  const TreeState* t = c->tree;
+  const int g = c->cfg.g;
+  const double h0 = t->h[0];
+  std::vector<double> x0(3 * (size_t)c->nb);
+  for (int b = 0; b < c->nb; b++)
+    for (int d = 0; d < 3; d++) {
+      double x = double(t->off[b][d] + g) * t->h[b];
+      if (d < 2) {
+        x = x - c->cfg.grid_dim - g * h0 - c->origin[d];
+      } else if (c->cfg.z_reflect) {
+        x = x - g * h0 - c->origin[2];
+      } else {
+        x = x - c->cfg.grid_dim - g * h0;
+      }
+      x0[3 * b + d] = x;
+    }
+  if (!c->x0_dev) HIPCHK(hipMalloc(&c->x0_dev, sizeof(double) * 3 * c->nb));
+  HIPCHK(hipMemcpy(c->x0_dev, x0.data(), sizeof(double) * x0.size(), hipMemcpyHostToDevice));
+  c->have_x0 = true;
+  return AMRH_OK;
+}
+
 extern "C" {
 
 int amrh_tree_set(amrh_ctx* c, const int* parent, const int* child) {
@@ -375,23 +403,26 @@ int amrh_tree_set(amrh_ctx* c, const int* parent, const int* child) {
   const bool rot_ok = c->cfg.physics == AMRH_PHYS_ROTATING;
   if (!euler_ok && !rot_ok)
     return fail(AMRH_EINVAL, "the octree coupling runs the ideal gas (minmod, or PPM on n = 8) or the rotating state");
+  // captured steps hold the old tree's tables, which build() frees
+  drop_graphs(c);
   if (int rc = build(c, parent, child)) return rc;
   HIPCHK(hipMemcpy(c->h_dev, c->tree->h.data(), sizeof(double) * c->nb, hipMemcpyHostToDevice));
   c->have_h = true;
-  // low interior face of every node, Grid::xf(bw) (grid/grid.cpp:512-526; no -GRID_DIM in z when
-  // the lower z boundary reflects)
-  const TreeState* t = c->tree;
-  const int g = c->cfg.g;
-  const double h0 = t->h[0];
-  std::vector<double> x0(3 * (size_t)c->nb);
-  for (int b = 0; b < c->nb; b++)
-    for (int d = 0; d < 3; d++) {
-      const double shift = (d == 2 && c->cfg.z_reflect) ? 0.0 : c->cfg.grid_dim;
-      x0[3 * b + d] = double(t->off[b][d] + g) * t->h[b] - shift - g * h0;
-    }
-  if (!c->x0_dev) HIPCHK(hipMalloc(&c->x0_dev, sizeof(double) * 3 * c->nb));
-  HIPCHK(hipMemcpy(c->x0_dev, x0.data(), sizeof(double) * x0.size(), hipMemcpyHostToDevice));
-  c->have_x0 = true;
+  return tree_upload_x0(c);
+}
+
+int amrh_tree_set_origin(amrh_ctx* c, const double* origin) {
+  if (!c || !origin) return fail(AMRH_EINVAL, "null argument");
+  for (int d = 0; d < 3; d++)
+    if (!std::isfinite(origin[d])) return fail(AMRH_EINVAL, "origin[%d] is not finite", d);
+  for (int d = 0; d < 3; d++) c->origin[d] = origin[d];
+  if (!c->tree) return AMRH_OK;  // applied by amrh_tree_set
+  return tree_upload_x0(c);
+}
+
+int amrh_tree_get_origin(const amrh_ctx* c, double* origin) {
+  if (!c || !origin) return fail(AMRH_EINVAL, "null argument");
+  for (int d = 0; d < 3; d++) origin[d] = c->origin[d];
   return AMRH_OK;
 }
 

@@ -107,6 +107,12 @@ int amrh_download_device(amrh_ctx* ctx, void* U_padded_dev);
 /* U0 = U for every block (Grid::store, grid/grid.cpp:173-185).  No copy is made: the current
  * buffer becomes the RK base and the next substep writes the other buffer. */
 int amrh_store(amrh_ctx* ctx);
+/* U = U0 for every block and, on an octree, the root's flow-off ledger FO = FO0 (Grid::restore,
+ * grid/grid.cpp:187-199, recursed by GridNode::restore, grid/grid_node.cpp:486-493).  No copy is
+ * made: the RK base buffer becomes the current stage again and stays the RK base, so a restored
+ * step can be redone and restored again -- BinaryDriver::step's dt rejection (REDO,
+ * binary/binary_driver.cpp:414-423).  AMRH_ESTATE before the first amrh_store. */
+int amrh_restore(amrh_ctx* ctx);
 
 /* Ghost fill of isolated blocks whose six neighbours are physical outflow boundaries
  * (GridPhysBound, grid/grid_phys_bound.cpp:27-54 with State::enforce_outflow,
@@ -170,6 +176,13 @@ int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_gr
  * (g = 2) and the rotating-frame state (n = 8; minmod g = 2 or PPM g = 3), whose node origins
  * (Grid::xf, grid/grid.cpp:512-526) follow from the tree. */
 int amrh_tree_set(amrh_ctx* ctx, const int* parent, const int* child);
+/* GridNode::set_origin (grid/grid_node.cpp:20-26): the origin subtracted from every node's cell and
+ * face positions, Grid::xf = (offset + i) h - GRID_DIM - bw h0 - origin (grid/grid.cpp:512-524); as
+ * there, z subtracts origin[2] only when the lower z boundary reflects.  BinaryDriver moves it to
+ * the centre of mass (binary/binary_driver.cpp:544-551).  Default 0; kept across amrh_tree_set,
+ * carried into a repacked tree and through checkpoints.  Synchronous. */
+int amrh_tree_set_origin(amrh_ctx* ctx, const double* origin);
+int amrh_tree_get_origin(const amrh_ctx* ctx, double* origin);
 /* GridNode::enforce_boundaries (grid/grid_node.cpp:59-112), level by level: same-level copy,
  * coarse->fine interpolation (grid/grid_amr_bound.cpp:27-78), physical boundary
  * (grid/grid_phys_bound.cpp:27-54). */
@@ -200,8 +213,10 @@ int amrh_tree_flow_off(amrh_ctx* ctx, double* fo);
  * is filled from its parent by GridNode::inject_from_parent (grid/grid_node.cpp:193-233: minmod
  * slopes per axis, then the floor), level by level; the root's flow-off ledger carries over; ghosts
  * are filled at the end.  Nodes absent from the new tree are dropped (their parents already hold
- * the restricted values).  The ideal gas only (the rotating state's Binary::inject_from_parent is
- * another prolongation).  Synchronous. */
+ * the restricted values).  The rotating-frame state is prolonged the same way, with the species
+ * renormalisation and the Newton T of its floor at the child's cells; its potential for the new
+ * nodes comes from amrh_poisson_phi_from_state (Binary::inject_from_parent,
+ * binary/binary.cpp:852-855).  The new tree takes the old one's origin.  Synchronous. */
 int amrh_tree_repack(amrh_ctx* dst, amrh_ctx* src, const int* src_index);
 /* The density test of GridNode::set_refine_flags (grid/grid_node.cpp:647-667): flags[b] bit c is
  * set when an interior cell of node b's octant c has rho > min_density.  The flag propagation,

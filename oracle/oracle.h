@@ -119,6 +119,7 @@ double* orc_rot_tree_force(orc_rot_tree* t, int node);
 const double* orc_rot_tree_F(const orc_rot_tree* t, int node, int d);
 int orc_rot_tree_node_x0(const orc_rot_tree* t, int node, double* x0, double* h);
 void orc_rot_tree_use_forces(orc_rot_tree* t, int on);
+void orc_rot_tree_set_origin(orc_rot_tree* t, const double* origin);
 int orc_rot_tree_fill_ghosts(orc_rot_tree* t);
 int orc_rot_tree_store(orc_rot_tree* t);
 int orc_rot_tree_substep(orc_rot_tree* t, double dt, double beta);

@@ -522,9 +522,23 @@ struct orc_rot_tree {
     double DFO[NFR] = {0}, FO[NFR] = {0}, FO0[NFR] = {0};
   };
   RC c;
-  double h0 = 0.0;
+  double h0 = 0.0, grid_dim = 1.0, origin[3] = {0.0, 0.0, 0.0};
   bool use_forces = false;
   std::vector<RNode> nodes;
+  // low interior face of node nd, Grid::xf(bw) (grid/grid.cpp:512-524): x and y subtract the
+  // origin; z subtracts GRID_DIM but not the origin, or under Z_REFLECT the origin but not GRID_DIM
+  void set_x0(RNode& nd) const {
+    for (int d = 0; d < 3; d++) {
+      double x = double(nd.off[d] + c.g) * nd.h;
+      if (d < 2)
+        x = x - grid_dim - c.g * h0 - origin[d];
+      else if (c.zref)
+        x = x - c.g * h0 - origin[2];
+      else
+        x = x - grid_dim - c.g * h0;
+      nd.x0[d] = x;
+    }
+  }
   explicit orc_rot_tree(const orc_rot_cfg* cfg) : c(cfg) {}
   int sibling(int b, int face) const {
     const RNode& nd = nodes[b];
@@ -778,6 +792,7 @@ orc_rot_tree* orc_rot_tree_create(const orc_rot_cfg* cfg, double grid_dim, int n
   RC c(cfg);
   if (c.G > MAXG || nn < 1 || parent[0] != -1) return nullptr;
   orc_rot_tree* t = new orc_rot_tree(cfg);
+  t->grid_dim = grid_dim;
   t->h0 = 2.0 * grid_dim / double(c.n);  // grid/grid.cpp:70
   const int P = c.n + 2;
   t->nodes.resize(nn);
@@ -803,11 +818,7 @@ orc_rot_tree* orc_rot_tree_create(const orc_rot_cfg* cfg, double grid_dim, int n
     } else {
       nd.h = t->h0;
     }
-    // low interior face, Grid::xf(bw) (grid/grid.cpp:512-526)
-    for (int d = 0; d < 3; d++) {
-      const double shift = (d == 2 && c.zref) ? 0.0 : grid_dim;
-      nd.x0[d] = double(nd.off[d] + c.g) * nd.h - shift - c.g * t->h0;
-    }
+    t->set_x0(nd);
     nd.U.assign((size_t)NFR * c.G3, 0.0);
     nd.U0 = nd.U;
     nd.D = nd.U;
@@ -836,6 +847,12 @@ int orc_rot_tree_node_x0(const orc_rot_tree* t, int node, double* x0, double* h)
 }
 
 void orc_rot_tree_use_forces(orc_rot_tree* t, int on) { t->use_forces = on != 0; }
+
+// GridNode::set_origin (grid/grid_node.cpp:20-26): every node takes the origin
+void orc_rot_tree_set_origin(orc_rot_tree* t, const double* origin) {
+  for (int d = 0; d < 3; d++) t->origin[d] = origin[d];
+  for (auto& nd : t->nodes) t->set_x0(nd);
+}
 
 int orc_rot_tree_fill_ghosts(orc_rot_tree* t) {
   // level order: a node reads same-level interiors, its own interior and its parent's cells with

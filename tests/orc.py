@@ -82,6 +82,10 @@ def lib():
     return _lib
 
 
+def ref_lib_built(kind="euler"):
+    return os.path.exists(os.path.join(REF_DIR, "libref_%s.so" % kind))
+
+
 def ref_lib(kind="euler"):
     """The reference-built harness; only in this container (needs /root/reference to build)."""
     path = os.path.join(REF_DIR, "libref_%s.so" % kind)
@@ -342,6 +346,8 @@ class RotTree:
             L.orc_rot_tree_node_x0.argtypes = [vp, C.c_int, DP, DP]
             L.orc_rot_tree_use_forces.argtypes = [vp, C.c_int]
             L.orc_rot_tree_use_forces.restype = None
+            L.orc_rot_tree_set_origin.argtypes = [vp, DP]
+            L.orc_rot_tree_set_origin.restype = None
             for f in ("fill_ghosts", "store", "pot_to_grid", "inject"):
                 getattr(L, "orc_rot_tree_" + f).argtypes = [vp]
             L.orc_rot_tree_prolong.argtypes = [vp, C.c_int]
@@ -407,6 +413,10 @@ class RotTree:
 
     def use_forces(self, on):
         self.L.orc_rot_tree_use_forces(self.t, int(on))
+
+    def set_origin(self, origin):
+        o = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
+        self.L.orc_rot_tree_set_origin(self.t, dptr(o))
 
     def fill_ghosts(self):
         self.L.orc_rot_tree_fill_ghosts(self.t)

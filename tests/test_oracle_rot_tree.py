@@ -391,3 +391,21 @@ def test_prolongation_continues_linear_profiles_and_restricts_back():
     I = (slice(None),) + (slice(g, -g),) * 3
     for f in coef:
         np.testing.assert_allclose(T.U(16)[I][f], node16[I][f], rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("z_reflect", [0, 1])
+def test_origin_enters_the_positions_as_grid_xf(z_reflect):
+    """GridNode::set_origin (grid/grid_node.cpp:20-26) with Grid::xf = (offset + i) h - GRID_DIM -
+    bw h0 - origin (grid/grid.cpp:512-524): x and y move by -origin; z moves by -origin_z only under
+    Z_REFLECT (without it zf has no origin term)"""
+    parent, child = TREES["partial"]
+    cfg = orc.rot_cfg(z_reflect=z_reflect)
+    T0 = orc.RotTree(cfg, parent, child)
+    T1 = orc.RotTree(cfg, parent, child)
+    o = np.array([0.13, -0.07, 0.05])
+    T1.set_origin(o)
+    for b in range(len(parent)):
+        (a, h0), (c, h1) = T0.x0(b), T1.x0(b)
+        assert h0 == h1
+        assert c[0] == a[0] - o[0] and c[1] == a[1] - o[1]
+        assert c[2] == (a[2] - o[2] if z_reflect else a[2])

@@ -53,9 +53,11 @@ def test_to_con_inverts_to_prim(omega):
         assert abs(p[5] - u[5] / u[0]) <= 1e-15 * abs(u[5] / u[0]) + 1e-300
 
 
-@pytest.mark.parametrize("omega", [0.0, 0.45])
-def test_fluxes_are_the_cartesian_fluxes_in_cylindrical_components(omega):
-    cfg = orc.rot_cfg(omega=omega, omega0=omega)
+@pytest.mark.parametrize("omega,omega0", [(0.0, 0.0), (0.45, 0.45), (0.45, 0.3), (0.0, -0.2)])
+def test_fluxes_are_the_cartesian_fluxes_in_cylindrical_components(omega, omega0):
+    """Omega0 != Omega: the energy flux carries the pressure work of the frame velocity difference
+    (Omega - Omega0) x r (State::dvx/dvy, binary/state_ztwd.cpp:256-262, used at :360-388)"""
+    cfg = orc.rot_cfg(omega=omega, omega0=omega0)
     rng = np.random.default_rng(2)
     for _ in range(100):
         X = rng.uniform(-1, 1, 3)
@@ -79,8 +81,9 @@ def test_fluxes_are_the_cartesian_fluxes_in_cylindrical_components(omega):
             assert np.isclose(F[4], lz_flux, rtol=1e-12, atol=1e-14)
             assert np.isclose(F[5], mom[2], rtol=1e-12, atol=1e-14)
             assert np.isclose(F[0], rho * vd, rtol=1e-13)
-            # energy: (E + p) v_d
-            assert np.isclose(F[7], (u[7] + p) * vd, rtol=1e-12, atol=1e-14)
+            # energy: E v_d + p (v_d + dv_d), dv = (Omega - Omega0) z x r
+            dv = np.array([-X[1], X[0], 0.0]) * (omega - omega0)
+            assert np.isclose(F[7], u[7] * vd + p * (vd + dv[d]), rtol=1e-12, atol=1e-14)
             cs = np.sqrt(5.0 / 3.0 * p / rho)
             assert np.isclose(F[13], abs(vd) + cs, rtol=1e-13)
 
