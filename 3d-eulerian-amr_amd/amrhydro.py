@@ -39,6 +39,7 @@ class Config(C.Structure):
 
 AMRH_BOUNDARY_EXTERNAL = 0
 AMRH_BOUNDARY_OUTFLOW = 1
+STATE_SUM_DEV_MAX_PARTIALS = 3145728  # AMRH_STATE_SUM_DEV_MAX_PARTIALS
 
 
 def euler_config(n=8, g=2, precision=AMRH_FP64, z_reflect=0, grid_dim=1.0,
@@ -116,6 +117,7 @@ SIGNATURES = {
     "amrh_state_sum": (C.c_int, [C.c_void_p, DP]),
     "amrh_state_max": (C.c_int, [C.c_void_p, DP]),
     "amrh_state_min": (C.c_int, [C.c_void_p, DP]),
+    "amrh_state_sum_dev": (C.c_int, [C.c_void_p, C.c_void_p]),
     "amrh_tree_repack": (C.c_int, [C.c_void_p, C.c_void_p, IP]),
     "amrh_tree_density_flags": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_ubyte)]),
     "amrh_tree_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, IP]),
@@ -298,6 +300,10 @@ class Batch:
         s = np.empty(4 * self.cfg.nf)
         _ck(lib().amrh_state_sum(self.ctx, _dp(s)))
         return s.reshape(4, self.cfg.nf)
+
+    def state_sum_dev(self, sums_ptr):
+        """amrh_state_sum into device memory (4*nf doubles at sums_ptr), allocation-free, async"""
+        _ck(lib().amrh_state_sum_dev(self.ctx, C.c_void_p(sums_ptr)))
 
     def state_max(self):
         s = np.empty(self.cfg.nf)

@@ -266,6 +266,16 @@ int amrh_state_sum(amrh_ctx* ctx, double* sums);
 /* GridNode::state_max / state_min (grid/grid_node.cpp:1529-1589): per field over the same cells. */
 int amrh_state_max(amrh_ctx* ctx, double* out);
 int amrh_state_min(amrh_ctx* ctx, double* out);
+/* amrh_state_sum for callers that cannot allocate device memory (work captured into a HIP graph,
+ * per-step loops that forbid hipMalloc).  No counterpart in the reference.  The same two reductions
+ * and the same result, written to sums_dev (4*nf doubles in device memory, amrh_state_sum's
+ * order) asynchronously on the context's stream; the call allocates nothing and copies nothing to
+ * the host.  The per-block partials live in a fixed array of the library, one per GPU, of
+ * AMRH_STATE_SUM_DEV_MAX_PARTIALS doubles (24 MiB): nblocks * nf * 6 may not exceed it (AMRH_EINVAL
+ * otherwise; e.g. at most 87,381 blocks of 6 fields or 40,329 of 13).  All calls on one GPU share
+ * that array, so calls that overlap in time on one GPU race. */
+#define AMRH_STATE_SUM_DEV_MAX_PARTIALS 3145728L
+int amrh_state_sum_dev(amrh_ctx* ctx, double* sums_dev);
 
 /* ---- block positions, rotating-frame state (AMRH_PHYS_ROTATING) --------------------------------
  * Block b's interior spans [x0, x0 + n h) on each axis (x0: nblocks * 3 doubles, host): the cell
