@@ -24,6 +24,10 @@ def euler_tree(spec):
     cfg = orc.euler_cfg(n=8, g=2, grid_dim=0.5)
     T, parent, child = tc.build(cfg, spec)
     tc.sedov_ic(T, 2)
+    for b in range(T.nnodes):  # a uniform drift in x, so mass and energy cross the domain boundary
+        U = T.U(b)
+        U[4] += 0.5 * 0.04 * U[0]
+        U[1] = 0.2 * U[0]
     T.enforce_boundaries()
     T.inject_from_children()
     d = amrhydro.euler_config(n=8, g=2, grid_dim=0.5)
