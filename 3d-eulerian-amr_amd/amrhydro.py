@@ -34,7 +34,7 @@ class Config(C.Structure):
                 ("theta", C.c_double), ("cfl", C.c_double), ("grid_dim", C.c_double),
                 ("boundary", C.c_int), ("eos", C.c_int), ("omega", C.c_double),
                 ("omega0", C.c_double), ("f_r", C.c_double), ("f_theta", C.c_double),
-                ("abar", C.c_double * 3)]
+                ("abar", C.c_double * 3), ("newton_tol", C.c_double)]
 
 
 AMRH_BOUNDARY_EXTERNAL = 0
@@ -53,7 +53,7 @@ def euler_config(n=8, g=2, precision=AMRH_FP64, z_reflect=0, grid_dim=1.0,
 
 def rotating_config(eos=AMRH_EOS_IDEAL, recon=AMRH_RECON_PPM, precision=AMRH_FP64, omega=0.0, omega0=None,
                     f_r=0.0, f_theta=0.0, z_reflect=0, boundary=AMRH_BOUNDARY_EXTERNAL, gamma=5.0 / 3.0,
-                    cfl=0.3):
+                    cfl=0.3, abar=(4.0, 14.12, 17.52), newton_tol=0.0):
     """Rotating-frame binary-star state (binary/state_ztwd.cpp) on 8^3 blocks: 13 fields, PPM with
     g = 3 (binary/binary_defs.h:29-30) or minmod with g = 2; rho_floor 1e-10 (:31), species mass
     numbers 4, 14.12, 17.52 (:28), GRID_CFL_FACTOR 0.3 (binary/binary_defs.h:58)."""
@@ -61,8 +61,8 @@ def rotating_config(eos=AMRH_EOS_IDEAL, recon=AMRH_RECON_PPM, precision=AMRH_FP6
     cfg = Config(n=8, g=g, nf=13, physics=AMRH_PHYS_ROTATING, recon=recon, precision=precision,
                  z_reflect=z_reflect, gamma=gamma, rho_floor=1.0e-10, ei_floor=1.0e-20, theta=1.3,
                  cfl=cfl, grid_dim=1.0, boundary=boundary, eos=eos, omega=omega,
-                 omega0=omega if omega0 is None else omega0, f_r=f_r, f_theta=f_theta)
-    cfg.abar[0], cfg.abar[1], cfg.abar[2] = 4.0, 14.12, 17.52
+                 omega0=omega if omega0 is None else omega0, f_r=f_r, f_theta=f_theta, newton_tol=newton_tol)
+    cfg.abar[0], cfg.abar[1], cfg.abar[2] = abar
     return cfg
 
 

@@ -62,6 +62,7 @@ int check_cfg(const amrh_config* c) {
     for (int i = 0; i < 3; i++)
       if (!(c->abar[i] > 0.0)) return fail(AMRH_EINVAL, "abar[%d] must be positive", i);
     if (!(c->rho_floor > 0.0)) return fail(AMRH_EINVAL, "rho_floor must be positive");
+    if (!(c->newton_tol >= 0.0 && c->newton_tol < 1.0)) return fail(AMRH_EINVAL, "newton_tol must be in [0, 1)");
   } else {
     return fail(AMRH_EINVAL, "physics %d not supported", c->physics);
   }
@@ -344,6 +345,7 @@ RotArgs amrh::ctx_rot_args(const amrh_ctx* c) {
   r.c = rot_params(c->cfg);
   if (c->eos) r.H = *eos_device(c->eos);
   r.H.err = c->err_dev;
+  r.H.toler = c->cfg.newton_tol > 0.0 ? c->cfg.newton_tol : 1.0e-9;
   return r;
 }
 

@@ -17,6 +17,7 @@ using namespace amrh;
 struct amrh_eos {
   helm::HelmDev H{};
   double* cell = nullptr;
+  float* cell32 = nullptr;  // the FP32 form's coefficients (helm::p_cs32)
   double* small = nullptr;  // t, d and the eight delta arrays
   int* err = nullptr;
   double cm = 1.0, g = 1.0, s = 1.0, K = 1.0;
@@ -123,8 +124,63 @@ int amrh_eos_create(const double* rows, amrh_eos** out) {
         c[4 * q + 3] = rows[((size_t)(j + 1) * IMAX + i + 1) * 9 + col];
       }
     }
+  // the FP32 form (helm::p_cs32): per cell, the interpolant in monomials a_mn xt^m xd^n.  With the
+  // Hermite data F[T][D] (T = 2 b + s_t, D = 2 a + s_d: the coefficient of derivative orders (a in d,
+  // b in T) at node sides (s_d, s_t), times dd^a dt^b) and M[m][2 o + s] the x^m coefficient of the
+  // weight of order o at side s -- psi_o(x) for s = 0, (-1)^o psi_o(1 - x) for s = 1
+  // (helmholtz/helmeos.cpp:14-59, :255-321) -- the cell is a = M F M^T.  a_00 is stored as 0 (only
+  // derivatives are formed); the cell is normalised by 2^E, E in slot 36.
+  static const int q_of[3][3] = {{0, 1, 2}, {3, 5, 7}, {4, 6, 8}};  // [a][b] -> q
+  double M[6][6] = {};
+  {
+    const double psi[3][6] = {{1.0, 0.0, 0.0, -10.0, 15.0, -6.0},  // psi0 (:14-15)
+                              {0.0, 1.0, 0.0, -6.0, 8.0, -3.0},     // psi1 (:20-21)
+                              {0.0, 0.0, 0.5, -1.5, 1.5, -0.5}};    // psi2 (:26-27)
+    const double binom[6][6] = {{1}, {1, 1}, {1, 2, 1}, {1, 3, 3, 1}, {1, 4, 6, 4, 1}, {1, 5, 10, 10, 5, 1}};
+    for (int o = 0; o < 3; o++) {
+      for (int m = 0; m < 6; m++) M[m][2 * o] = psi[o][m];
+      // (-1)^o psi_o(1 - x) = (-1)^o sum_k c_k sum_m C(k, m) (-x)^m
+      for (int k = 0; k < 6; k++)
+        for (int m = 0; m <= k; m++) M[m][2 * o + 1] += ((o & 1) ? -1.0 : 1.0) * psi[o][k] * binom[k][m] * ((m & 1) ? -1.0 : 1.0);
+    }
+  }
+  std::vector<float> cell32(ncell * helm::CELL32, 0.0f);
+  for (int j = 0; j < JMAX - 1; j++)
+    for (int i = 0; i < IMAX - 1; i++) {
+      const size_t ci = (size_t)j * (IMAX - 1) + i;
+      const double* c = &cell[ci * 36];
+      double F[6][6];
+      for (int b = 0; b < 3; b++)
+        for (int st = 0; st < 2; st++)
+          for (int a = 0; a < 3; a++)
+            for (int sd = 0; sd < 2; sd++)
+              F[2 * b + st][2 * a + sd] = c[4 * q_of[a][b] + sd + 2 * st] * std::pow(dt_sav[j], b) * std::pow(dd_sav[i], a);
+      double FM[6][6], A[6][6], mx = 0.0;
+      for (int t = 0; t < 6; t++)
+        for (int n = 0; n < 6; n++) {
+          double acc = 0.0;
+          for (int d = 0; d < 6; d++) acc += F[t][d] * M[n][d];
+          FM[t][n] = acc;
+        }
+      for (int m = 0; m < 6; m++)
+        for (int n = 0; n < 6; n++) {
+          double acc = 0.0;
+          for (int t = 0; t < 6; t++) acc += M[m][t] * FM[t][n];
+          A[m][n] = (m == 0 && n == 0) ? 0.0 : acc;
+          mx = std::fmax(mx, std::fabs(A[m][n]));
+        }
+      int ex = 0;
+      if (mx > 0.0 && std::isfinite(mx)) (void)std::frexp(mx, &ex);
+      float* o = &cell32[ci * helm::CELL32];
+      for (int m = 0; m < 6; m++)
+        for (int n = 0; n < 6; n++) o[6 * m + n] = (float)std::ldexp(A[m][n], -ex);
+      o[36] = (float)ex;
+    }
   amrh_eos* E = new amrh_eos;
   hipError_t e = hipMalloc(&E->cell, sizeof(double) * cell.size());
+  if (e == hipSuccess) e = hipMalloc(&E->cell32, sizeof(float) * cell32.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(E->cell32, cell32.data(), sizeof(float) * cell32.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&E->small, sizeof(double) * small.size());
   if (e == hipSuccess) e = hipMalloc(&E->err, sizeof(int));
   if (e == hipSuccess) e = hipMemcpy(E->cell, cell.data(), sizeof(double) * cell.size(), hipMemcpyHostToDevice);
@@ -136,6 +192,7 @@ int amrh_eos_create(const double* rows, amrh_eos** out) {
   }
   helm::HelmDev& H = E->H;
   H.cell = E->cell;
+  H.cell32 = E->cell32;
   const long off_dt = JMAX, off_d = 5L * JMAX;
   H.t = E->small;
   H.dt_sav = E->small + off_dt;
@@ -152,6 +209,7 @@ int amrh_eos_create(const double* rows, amrh_eos** out) {
   H.dlo = dlo;
   H.dstpi = dstpi;
   H.err = E->err;
+  H.toler = 1.0e-9;  // helmholtz/helmeos.cpp:449
   set_factors(E);
   *out = E;
   return AMRH_OK;
@@ -178,6 +236,7 @@ int amrh_eos_load(const char* path, amrh_eos** out) {
 void amrh_eos_destroy(amrh_eos* E) {
   if (!E) return;
   if (E->cell) (void)hipFree(E->cell);
+  if (E->cell32) (void)hipFree(E->cell32);
   if (E->small) (void)hipFree(E->small);
   if (E->err) (void)hipFree(E->err);
   delete E;

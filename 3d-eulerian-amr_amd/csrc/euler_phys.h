@@ -142,11 +142,11 @@ __device__ __forceinline__ float fast_pow(float x, float y) {
   return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
 }
 
-// minmod(theta*a, theta*b, (a+b)/2): nonzero only when all three agree in sign
+// minmod(theta*a, theta*b, (a+b)/2): nonzero only when all three agree in sign, and then
+// sign(a) * theta * min(|a|, |b|, |a+b| / (2 theta)) -- one multiply by theta instead of two
 __device__ __forceinline__ float minmod_theta(float a, float b, float th) {
 #pragma clang fp contract(fast)
-  const float c = 0.5f * (a + b);
-  const float m = fminf(fminf(fabsf(th * a), fabsf(th * b)), fabsf(c));
+  const float m = th * fminf(fminf(fabsf(a), fabsf(b)), fabsf((a + b) * (0.5f / th)));
   return (a * b > 0.0f) ? copysignf(m, a) : 0.0f;
 }
 __device__ __forceinline__ void slope(const float* a, const float* b, const float* c, float* s, float th) {
@@ -155,44 +155,41 @@ __device__ __forceinline__ void slope(const float* a, const float* b, const floa
   for (int l = 0; l < NFE; l++) s[l] = minmod_theta(c[l] - b[l], b[l] - a[l], th);
 }
 
-// one reconstructed face state (primitive q: rho, vx, vy, vz, eps, tau) -> conserved U, flux, eig
+// total energy, pressure and signal speed |v_d| + c_s of one reconstructed primitive face state
+// (the dual-energy branch is a real branch, so whole wavefronts that do not need it skip the
+// transcendental pair)
 template <int D>
-__device__ __forceinline__ void state_flux(const PhysF& P, const float* q, float* U, float* f, float& e) {
+__device__ __forceinline__ void face_side(const PhysF& P, const float* q, float& E, float& p, float& e) {
 #pragma clang fp contract(fast)
-  const float rho = q[RHO];
   const float v2 = q[SX] * q[SX] + q[SY] * q[SY] + q[SZ] * q[SZ];
-  const float ekin = 0.5f * rho * v2;
-  const float E = q[ET] + ekin;
-  float p = P.gm1 * q[ET];
-  // dual-energy branch (rare): a real branch, so the transcendental pair is skipped by whole
-  // wavefronts that do not need it
+  E = q[ET] + 0.5f * q[RHO] * v2;
+  p = P.gm1 * q[ET];
   if (__builtin_expect(q[ET] < 0.001f * E, 0)) p = P.gm1 * fast_pow(q[TAU], P.gamma);
-  const float vd = q[SX + D];
-  const float cs = fast_sqrt(P.gamma * p * fast_rcp(rho));
-  e = fabsf(vd) + cs;
-  U[RHO] = rho;
-  U[SX] = rho * q[SX];
-  U[SY] = rho * q[SY];
-  U[SZ] = rho * q[SZ];
-  U[ET] = E;
-  U[TAU] = q[TAU];
-#pragma unroll
-  for (int l = 0; l < NFE; l++) f[l] = U[l] * vd;
-  f[SX + D] += p;
-  f[ET] += vd * p;
+  e = fabsf(q[SX + D]) + fast_sqrt(P.gamma * p * fast_rcp(q[RHO]));
 }
 
-// KT face flux from the two reconstructed primitive face states
+// KT face flux from the two reconstructed primitive face states.  The combination
+// F = (f_L + f_R)/2 - a (U_R - U_L)/2 with f = U v_d + pressure terms is regrouped as
+// F = U_L (v_L + a)/2 + U_R (v_R - a)/2 + (pressure terms)/2, and U_L (v_L + a)/2 for the momenta as
+// (rho_L (v_L + a)/2) v_L: two operations per field instead of six (same value in exact arithmetic)
 template <int D>
 __device__ __forceinline__ void kt_flux(const PhysF& P, const float* ql, const float* qr, float* F, float& eig) {
 #pragma clang fp contract(fast)
-  float UL[NFE], UR[NFE], fl[NFE], fr[NFE], el, er;
-  state_flux<D>(P, ql, UL, fl, el);
-  state_flux<D>(P, qr, UR, fr, er);
-  const float a = fmaxf(el, er);
+  float EL, pL, eL, ER, pR, eR;
+  face_side<D>(P, ql, EL, pL, eL);
+  face_side<D>(P, qr, ER, pR, eR);
+  const float a = fmaxf(eL, eR);
   eig = a;
-#pragma unroll
-  for (int l = 0; l < NFE; l++) F[l] = 0.5f * ((fl[l] + fr[l]) - a * (UR[l] - UL[l]));
+  const float vL = ql[SX + D], vR = qr[SX + D];
+  const float cL = 0.5f * (vL + a), cR = 0.5f * (vR - a);
+  const float wL = ql[RHO] * cL, wR = qr[RHO] * cR;
+  F[RHO] = wL + wR;
+  F[SX] = wL * ql[SX] + wR * qr[SX];
+  F[SY] = wL * ql[SY] + wR * qr[SY];
+  F[SZ] = wL * ql[SZ] + wR * qr[SZ];
+  F[SX + D] += 0.5f * (pL + pR);
+  F[ET] = EL * cL + ER * cR + 0.5f * (vL * pL + vR * pR);
+  F[TAU] = ql[TAU] * cL + qr[TAU] * cR;
 }
 template <int D>
 __device__ __forceinline__ void kt_face(const PhysF& P, const float* pl, const float* sl, const float* pr,
@@ -209,9 +206,9 @@ __device__ __forceinline__ void kt_face(const PhysF& P, const float* pl, const f
 template <int D>
 __device__ __forceinline__ float eig_q(const PhysF& P, const float* ql, const float* qr) {
 #pragma clang fp contract(fast)
-  float U[NFE], f[NFE], el, er;
-  state_flux<D>(P, ql, U, f, el);
-  state_flux<D>(P, qr, U, f, er);
+  float E, p, el, er;
+  face_side<D>(P, ql, E, p, el);
+  face_side<D>(P, qr, E, p, er);
   return fmaxf(el, er);
 }
 
@@ -219,13 +216,13 @@ template <int D>
 __device__ __forceinline__ float face_eig(const PhysF& P, const float* pl, const float* sl, const float* pr,
                                           const float* sr) {
 #pragma clang fp contract(fast)
-  float U[NFE], f[NFE], el, er, q[NFE];
+  float E, p, el, er, q[NFE];
 #pragma unroll
   for (int l = 0; l < NFE; l++) q[l] = pl[l] + 0.5f * sl[l];
-  state_flux<D>(P, q, U, f, el);
+  face_side<D>(P, q, E, p, el);
 #pragma unroll
   for (int l = 0; l < NFE; l++) q[l] = pr[l] - 0.5f * sr[l];
-  state_flux<D>(P, q, U, f, er);
+  face_side<D>(P, q, E, p, er);
   return fmaxf(el, er);
 }
 

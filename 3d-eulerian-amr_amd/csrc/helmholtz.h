@@ -44,7 +44,13 @@ struct HelmDev {
   // unit factors of helmholtz_set_cgs_units (:133-142) as eos_from_code / eos_to_code apply them
   double f_p, f_e, f_T, f_cs, f_cv, f_rho, f_s, f_dsdt;
   int* err;
+  double toler;  // relative tolerance of compute_T's Newton solve (the reference's 1e-9, :449)
+  // the FP32 form's coefficients (p_cs32): [(JMAX-1)*(IMAX-1)][CELL32], see p_cs32
+  const float* cell32;
 };
+
+// floats per cell of the FP32 coefficients: 36, then the cell's power-of-two exponent, then padding
+constexpr int CELL32 = 40;
 
 // eos_t (helmholtz/helmholtz.h:4-6)
 struct Eos {
@@ -274,6 +280,205 @@ __device__ inline void eos(const HelmDev& H, Eos& q) {
   q.dsdt /= H.f_dsdt;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Pressure and sound speed in FP32, for the mixed form's face states (not bit-matched to the
+// reference).  The same biquintic Hermite interpolant of the same table, written per cell in the
+// monomial basis, f = sum_{m,n<6} a_mn xt^m xd^n (xt, xd in [0,1] across the cell): the host
+// converts each cell's 36 Hermite coefficients in FP64.  Derivatives taken by Horner sums in this
+// basis do not cancel, where the Hermite weights' second derivatives cancel terms some 10^3 times
+// larger than the result -- which FP32 coefficients could not carry.  a_00 (the only coefficient a
+// constant in f reaches) is stored as 0: only derivatives are formed, so the degenerate part of f
+// leaves no large value to subtract.  The cell is normalised by a power of two (slot 36 holds its
+// exponent) so that no table overflows FP32; the scale, the node spacings and the final p, c_s
+// assembly are FP64.  Only the four sums p and c_s need (f_d, f_dd, f_dt, f_tt) are formed.
+// ------------------------------------------------------------------------------------------------
+
+// pressure and sound speed (code units) of a state given in code units; same range and sign
+// checks as eos() (the flags are raised, p and cs are NaN)
+__device__ inline void p_cs32(const HelmDev& H, float rho_c, float T_c, float abar_f, float zbar_f, float& p_out,
+                              float& cs_out) {
+  const double den = (double)rho_c * H.f_rho;
+  const double temp = hmax((double)T_c * H.f_T, TMIN);
+  const double abar = (double)abar_f, zbar = (double)zbar_f;
+  const double ytot1 = 1.0 / abar;
+  const double ye = hmax(-15.0, ytot1 * zbar);
+  const double din = ye * den;
+  if ((temp > H.t[JMAX - 1]) || (temp < H.t[0]) || (din > H.d[IMAX - 1]) || (din < H.d[0]) || !(den > 0.0)) {
+    atomicOr(H.err, ERR_RANGE);
+    p_out = cs_out = __builtin_nanf("");
+    return;
+  }
+  // the cell: a hardware log2 estimate, then settled on the exact node values
+  constexpr float LOG10_2 = 0.30102999566398120f;
+  int jat = (int)(((float)__builtin_amdgcn_logf((float)temp) * LOG10_2 - (float)H.tlo) * (float)H.tstpi);
+  int iat = (int)(((float)__builtin_amdgcn_logf((float)din) * LOG10_2 - (float)H.dlo) * (float)H.dstpi);
+  jat = jat < 0 ? 0 : (jat > JMAX - 2 ? JMAX - 2 : jat);
+  iat = iat < 0 ? 0 : (iat > IMAX - 2 ? IMAX - 2 : iat);
+  if (jat > 0 && temp < H.t[jat]) jat--;
+  if (jat < JMAX - 2 && temp >= H.t[jat + 1]) jat++;
+  if (iat > 0 && din < H.d[iat]) iat--;
+  if (iat < IMAX - 2 && din >= H.d[iat + 1]) iat++;
+  const double dtis = H.dti_sav[jat], ddis = H.ddi_sav[iat];
+  const float xt = (float)hmax((temp - H.t[jat]) * dtis, 0.0);
+  const float xd = (float)hmax((din - H.d[iat]) * ddis, 0.0);
+
+  // a[6 m + n]: the cell's monomial coefficients of xt^m xd^n
+  float a[36];
+  const float4* src = reinterpret_cast<const float4*>(H.cell32 + ((long)jat * (IMAX - 1) + iat) * CELL32);
+#pragma unroll
+  for (int r = 0; r < 9; r++) {
+    const float4 v = src[r];
+    a[4 * r] = v.x;
+    a[4 * r + 1] = v.y;
+    a[4 * r + 2] = v.z;
+    a[4 * r + 3] = v.w;
+  }
+  const float ex = H.cell32[((long)jat * (IMAX - 1) + iat) * CELL32 + 36];
+  float s01, s11, s02, s20;  // (T, d) derivative orders, times dt^T dd^d
+  {
+#pragma clang fp contract(fast)
+    float p0[6], p1[6], p2[6];  // per T power m: sum_n a_mn xd^n and its first two xd-derivatives
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const float* r = a + 6 * m;
+      p0[m] = ((((r[5] * xd + r[4]) * xd + r[3]) * xd + r[2]) * xd + r[1]) * xd + r[0];
+      p1[m] = (((5.0f * r[5] * xd + 4.0f * r[4]) * xd + 3.0f * r[3]) * xd + 2.0f * r[2]) * xd + r[1];
+      p2[m] = ((20.0f * r[5] * xd + 12.0f * r[4]) * xd + 6.0f * r[3]) * xd + 2.0f * r[2];
+    }
+    s01 = ((((p1[5] * xt + p1[4]) * xt + p1[3]) * xt + p1[2]) * xt + p1[1]) * xt + p1[0];
+    s02 = ((((p2[5] * xt + p2[4]) * xt + p2[3]) * xt + p2[2]) * xt + p2[1]) * xt + p2[0];
+    s11 = (((5.0f * p1[5] * xt + 4.0f * p1[4]) * xt + 3.0f * p1[3]) * xt + 2.0f * p1[2]) * xt + p1[1];
+    s20 = ((20.0f * p0[5] * xt + 12.0f * p0[4]) * xt + 6.0f * p0[3]) * xt + 2.0f * p0[2];
+  }
+  const double sc = ldexp(1.0, (int)ex);
+  const double df_d = (double)s01 * sc * ddis;
+  const double df_dd = (double)s02 * sc * H.dd2i_sav[iat];
+  const double df_dt = (double)s11 * sc * ddis * dtis;
+  const double df_tt = (double)s20 * sc * H.dt2i_sav[jat];
+
+  // ions and electrons as in eos() (:186-210, :380-410), no entropy
+  const double deni = 1.0 / den;
+  const double xni = avo * ytot1 * den;
+  const double pion = xni * kerg * temp;
+  const double dpiondd = avo * ytot1 * kerg * temp;
+  const double dpiondt = xni * kerg;
+  const double deiondt = 1.5 * dpiondt * deni;
+  const double x2 = din * din;
+  const double pele = x2 * df_d;
+  const double dpepdd = ye * (x2 * df_dd + 2.0 * din * df_d);
+  const double dpepdt = x2 * df_dt;
+  const double deepdt = -temp * df_tt * ye;
+  const double p = pele + pion;
+  const double cv = deiondt + deepdt;
+  const double x = (p * deni * deni) * ((dpiondt + dpepdt) / cv) + (dpiondd + dpepdd);
+  if (cv < 0.0) {
+    atomicOr(H.err, ERR_CV);
+    p_out = cs_out = __builtin_nanf("");
+    return;
+  }
+  p_out = (float)(p / H.f_p);
+  if (x < 0.0) {
+    atomicOr(H.err, ERR_CS);
+    cs_out = 0.0f;
+  } else {
+    cs_out = (float)(sqrt(x) / H.f_cs);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Specific internal energy and heat capacity in FP64 (the mixed form's Newton solve for T; not
+// bit-matched to the reference): the same interpolant, summed separably -- free, f_t and f_tt share
+// one density contraction, 54 products instead of 3 x 36 in the reference's order -- and nothing
+// but e and c_v (the Newton iteration reads no other output).  Returns the range flags as eos().
+// ------------------------------------------------------------------------------------------------
+__device__ inline bool e_cv_sep(const HelmDev& H, Eos& q) {
+  const double den = q.rho * H.f_rho;
+  const double temp = hmax(q.T * H.f_T, TMIN);
+  const double ytot1 = 1.0 / q.abar;
+  const double ye = hmax(-15.0, ytot1 * q.zbar);
+  const double din = ye * den;
+  if ((temp > H.t[JMAX - 1]) || (temp < H.t[0]) || (din > H.d[IMAX - 1]) || (din < H.d[0])) {
+    atomicOr(H.err, ERR_RANGE);
+    q.e = q.cv = __builtin_nan("");
+    return false;
+  }
+  int jat = int((log10(temp) - H.tlo) * H.tstpi) + 1;
+  jat = jat > JMAX - 1 ? JMAX - 1 : jat;
+  jat = jat < 1 ? 1 : jat;
+  int iat = int((log10(din) - H.dlo) * H.dstpi) + 1;
+  iat = iat > IMAX - 1 ? IMAX - 1 : iat;
+  iat = iat < 1 ? 1 : iat;
+  --jat;
+  --iat;
+  const double xt = hmax((temp - H.t[jat]) * H.dti_sav[jat], 0.0);
+  const double xd = hmax((din - H.d[iat]) * H.ddi_sav[iat], 0.0);
+  const double mxt = 1.0 - xt, mxd = 1.0 - xd;
+  const double dts = H.dt_sav[jat], dt2s = H.dt2_sav[jat], dtis = H.dti_sav[jat], dt2is = H.dt2i_sav[jat];
+  const double dds = H.dd_sav[iat], dd2s = H.dd2_sav[iat];
+  // density weights of the value (index of fi within a quantity's corner pair: lower, upper node)
+  const double wd[6] = {psi0(xd), psi0(mxd), psi1(xd) * dds, -psi1(mxd) * dds, psi2(xd) * dd2s, psi2(mxd) * dd2s};
+  // G[b][st]: the density contraction of the quantities of T-derivative order b at T node st;
+  // the cell's coefficients: quantity q (f, ft, ftt, fd, fdd, fdt, fddt, fdtt, fddtt), corners
+  // (i,j), (i+1,j), (i,j+1), (i+1,j+1) at 4 q + corner
+  const double* c = H.cell + ((long)jat * (IMAX - 1) + iat) * 36;
+  auto contract = [&](int q0, int q1, int q2, int st) {
+    const double2 a = *reinterpret_cast<const double2*>(c + 4 * q0 + 2 * st);
+    const double2 b = *reinterpret_cast<const double2*>(c + 4 * q1 + 2 * st);
+    const double2 e = *reinterpret_cast<const double2*>(c + 4 * q2 + 2 * st);
+    return a.x * wd[0] + a.y * wd[1] + b.x * wd[2] + b.y * wd[3] + e.x * wd[4] + e.y * wd[5];
+  };
+  // T-derivative order 0: (f, fd, fdd); 1: (ft, fdt, fddt); 2: (ftt, fdtt, fddtt)
+  const double g00 = contract(0, 3, 4, 0), g01 = contract(0, 3, 4, 1);
+  const double g10 = contract(1, 5, 6, 0), g11 = contract(1, 5, 6, 1);
+  const double g20 = contract(2, 7, 8, 0), g21 = contract(2, 7, 8, 1);
+  const double free = g00 * psi0(xt) + g01 * psi0(mxt) + g10 * (psi1(xt) * dts) + g11 * (-psi1(mxt) * dts) +
+                      g20 * (psi2(xt) * dt2s) + g21 * (psi2(mxt) * dt2s);
+  const double df_t = g00 * (dpsi0(xt) * dtis) + g01 * (-dpsi0(mxt) * dtis) + g10 * dpsi1(xt) + g11 * dpsi1(mxt) +
+                      g20 * (dpsi2(xt) * dts) + g21 * (-dpsi2(mxt) * dts);
+  const double df_tt = g00 * (ddpsi0(xt) * dt2is) + g01 * (ddpsi0(mxt) * dt2is) + g10 * (ddpsi1(xt) * dtis) +
+                       g11 * (-ddpsi1(mxt) * dtis) + g20 * ddpsi2(xt) + g21 * ddpsi2(mxt);
+  // ions (:186-197) and electrons (:380-391): e = eion + ye (free - T f_t), cv = deiondt - T f_tt ye
+  const double kt = kerg * temp;
+  const double eion = 1.5 * avo * ytot1 * kt;
+  const double deiondt = 1.5 * avo * ytot1 * kerg;
+  q.e = (eion + ye * (free - temp * df_t)) / H.f_e;
+  q.cv = (deiondt - temp * df_tt * ye) / H.f_cv;
+  return true;
+}
+
+// helmholtz_compute_T's iteration (:436-486) on e_cv_sep, FP64: the same start at T, TMIN test,
+// damping schedule, floor and tolerance; the final full evaluation is skipped (it changes no T)
+__device__ inline void compute_T_sep(const HelmDev& H, Eos& q) {
+  double T0 = q.T;
+  const double e0 = q.e;
+  q.T = TMIN;  // in code units, as the reference sets eos->T (:443)
+  if (!e_cv_sep(H, q)) return;
+  q.T = T0;
+  const double tmin = TMIN;
+  double a = 1.0;
+  if (q.e >= e0) {
+    q.T = tmin;
+    return;
+  }
+  int iters = 0;
+  double f;
+  do {
+    T0 = hmax(q.T, tmin);
+    if (!e_cv_sep(H, q)) return;
+    f = q.e - e0;
+    const double dT = -f / q.cv;
+    q.T += a * dT;
+    q.T = hmax(q.T, (T0 + tmin) / 2.0);
+    iters++;
+    if (iters % 10 == 0) a *= 0.9;
+    if (q.T < tmin) {
+      atomicOr(H.err, ERR_TMIN);
+      return;
+    }
+  } while (fabs(f / e0) >= H.toler && iters < 100);
+  if (iters >= 100) atomicOr(H.err, ERR_NEWTON);
+}
+
 // helmholtz_compute_T, energy form (:436-486): damped Newton on e(T) = e0 from the state's T.
 // Where the reference aborts (T below TMIN, no convergence in 100 iterations) a flag is raised.
 __device__ inline void compute_T(const HelmDev& H, Eos& q) {
@@ -282,7 +487,7 @@ __device__ inline void compute_T(const HelmDev& H, Eos& q) {
   q.T = TMIN;
   eos(H, q);
   q.T = T0;
-  const double toler = 1.0e-9;
+  const double toler = H.toler;
   double a = 1.0;
   if (q.e >= e0) {
     q.T = TMIN;

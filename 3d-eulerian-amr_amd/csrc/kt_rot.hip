@@ -409,7 +409,16 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       du -= vlo[2];
       du /= h;
       double p, cs;
-      eos_p_cs<EOS>(H, c, s, p, cs);
+      if constexpr (MIXED && EOS == EOS_HELMHOLTZ) {
+        // the mixed form's cell pressure for -p div u and the geometric source: FP32 EOS
+        const double A = Abar(c, s);
+        float pf, csf;
+        helm::p_cs32(H, (float)s[D0], (float)s[TT], (float)A, (float)(A / 2.0), pf, csf);
+        p = pf;
+        cs = csf;
+      } else {
+        eos_p_cs<EOS>(H, c, s, p, cs);
+      }
 #pragma unroll
       for (int l = 0; l < NF; l++) sv[l] = 0.0;
       sv[rot::TAU] = -p * du;
@@ -441,7 +450,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
 #pragma unroll
         for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + s[l] * (1.0 - beta);
       }
-      floor_state<EOS>(H, c, u, X);
+      floor_state<EOS, MIXED>(H, c, u, X);
       if (!(u[D0] > 0.0)) atomicOr(a.err, 1);
       const double R2 = X[0] * X[0] + X[1] * X[1];
       u[POT] = u[D0] * (pc - 0.5 * R2 * c.Omega0 * c.Omega0);

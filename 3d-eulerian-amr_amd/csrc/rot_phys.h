@@ -77,10 +77,13 @@ __device__ __forceinline__ void eos_eval(const helm::HelmDev& H, const RotParams
   }
 }
 
-// helmholtz_compute_T (helmholtz/helmeos.cpp:436-486); the ideal gas runs the same iteration
-template <int EOS>
+// helmholtz_compute_T (helmholtz/helmeos.cpp:436-486); the ideal gas runs the same iteration.
+// SEP: the Helmholtz iteration on the separable FP64 e and c_v (the mixed form)
+template <int EOS, bool SEP = false>
 __device__ __forceinline__ void eos_compute_T(const helm::HelmDev& H, const RotParams& c, helm::Eos& q) {
-  if constexpr (EOS == EOS_HELMHOLTZ) {
+  if constexpr (EOS == EOS_HELMHOLTZ && SEP) {
+    helm::compute_T_sep(H, q);
+  } else if constexpr (EOS == EOS_HELMHOLTZ) {
     helm::compute_T(H, q);
   } else {
     double T0 = q.T;
@@ -109,7 +112,7 @@ __device__ __forceinline__ void eos_compute_T(const helm::HelmDev& H, const RotP
         atomicOr(H.err, helm::ERR_TMIN);
         return;
       }
-    } while (fabs(f / e0) >= 1.0e-9 && iters < 100);
+    } while (fabs(f / e0) >= H.toler && iters < 100);
     eos_eval<EOS>(H, c, q);
     if (iters >= 100) atomicOr(H.err, helm::ERR_NEWTON);
   }
@@ -191,8 +194,8 @@ __device__ __forceinline__ double flux(const helm::HelmDev& H, const RotParams& 
   return fabs(v) + cs;
 }
 
-// binary/state_ztwd.cpp:482-554
-template <int EOS>
+// binary/state_ztwd.cpp:482-554 (SEP: the mixed form's Newton solve, eos_compute_T)
+template <int EOS, bool SEP = false>
 __device__ __forceinline__ void floor_state(const helm::HelmDev& H, const RotParams& c, double* u, const double* X) {
   if (u[D0] < c.rho_floor) {
     u[D0] = c.rho_floor;
@@ -236,7 +239,7 @@ __device__ __forceinline__ void floor_state(const helm::HelmDev& H, const RotPar
   q.rho = d;
   q.e = ue / d;
   q.T = u[TT];
-  eos_compute_T<EOS>(H, c, q);
+  eos_compute_T<EOS, SEP>(H, c, q);
   u[TT] = q.T;
   if (ue > 0.1 * u[ET]) u[TAU] = ue;
 }
@@ -271,8 +274,8 @@ __device__ __forceinline__ void enforce_outflow(const RotParams& c, double* u, c
 
 // ------------------------------------------------------------------------------------------
 // The mixed form's face states: reconstructed primitives back to conserved, EOS pressure and
-// sound speed, and the directional flux in FP32 (the Helmholtz EOS itself runs in FP64 on the
-// FP32 face state).  Same formulas as above; not bit-matched to the reference.
+// sound speed (the Helmholtz one by helm::p_cs32), and the directional flux in FP32.  Same formulas
+// as above; not bit-matched to the reference.
 // ------------------------------------------------------------------------------------------
 namespace sp {
 
@@ -329,14 +332,7 @@ __device__ __forceinline__ void p_cs(const helm::HelmDev& H, const RotF& c, cons
   const float abar = u[D0] * rcp(u[SPEC] * c.abar_inv[0] + u[SPEC + 1] * c.abar_inv[1] + u[SPEC + 2] * c.abar_inv[2]);
   const float zbar = 0.5f * abar;
   if constexpr (EOS == EOS_HELMHOLTZ) {
-    helm::Eos q{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    q.rho = u[D0];
-    q.T = u[TT];
-    q.abar = abar;
-    q.zbar = zbar;
-    helm::eos(H, q);
-    p = (float)q.p;
-    cs = (float)q.cs;
+    helm::p_cs32(H, u[D0], u[TT], abar, zbar, p, cs);
   } else {
     const float T = u[TT] > (float)TMIN_IDEAL ? u[TT] : (float)TMIN_IDEAL;
     p = u[D0] * T * ((1.0f + zbar) * rcp(abar));

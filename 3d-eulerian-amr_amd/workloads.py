@@ -142,6 +142,90 @@ def helmholtz_table():
     return np.ascontiguousarray(rows.reshape(J * I, 9))
 
 
+# cgs constants of helmholtz/const.h used by the double-white-dwarf table and model
+HELM_CLIGHT, HELM_ASOL = 2.99792458e10, 4.0 * 5.67051e-5 / 2.99792458e10
+HELM_G, HELM_MSOL = 6.6742867e-8, 1.9892e33
+# the cold ideal Fermi gas of electrons: n_e = N_A d (d = ye rho), x = p_F/(m_e c) = (d/FERMI_B)^(1/3),
+# P = FERMI_A * 8 int_0^x t^4/sqrt(1+t^2) dt (Chandrasekhar)
+FERMI_B = 8.0 * np.pi * HELM_ME ** 3 * HELM_CLIGHT ** 3 / (3.0 * HELM_H ** 3 * HELM_AVO)
+FERMI_A = np.pi * HELM_ME ** 4 * HELM_CLIGHT ** 5 / (3.0 * HELM_H ** 3)
+
+
+def cold_electrons(d):
+    """The T = 0 ideal (non-interacting) electron gas at electron density N_A d (any degeneracy and
+    relativity), x = (d/B)^(1/3): pressure P = A f(x) and kinetic energy density u = A g(x) with
+        f(x) = 8 int_0^x t^4 (1+t^2)^-1/2 dt = x (2x^2 - 3) sqrt(1+x^2) + 3 asinh(x),
+        g(x) = 24 int_0^x t^2 (sqrt(1+t^2) - 1) dt = 8 x^3 (sqrt(1+x^2) - 1) - f(x),
+    and dP/dd = 8 A x^5 / (3 d sqrt(1+x^2)).  Below x = 0.1 the closed forms cancel, so the
+    integrands' binomial series are integrated term by term there (truncation below 1e-20); above,
+    sqrt(1+x^2) - 1 is formed as x^2/(sqrt(1+x^2) + 1).  Returns (P, u, dP/dd) in erg/cm^3 and
+    erg/cm^3 per g/cm^3."""
+    from scipy.special import binom
+    d = np.asarray(d, dtype=np.float64)
+    x = np.cbrt(d / FERMI_B)
+    sq = np.sqrt(1.0 + x * x)
+    f = x * (2.0 * x * x - 3.0) * sq + 3.0 * np.arcsinh(x)
+    g = 8.0 * x ** 3 * (x * x / (sq + 1.0)) - f
+    small = x < 0.1
+    if np.any(small):
+        k = np.arange(12)
+        cf = 8.0 * binom(-0.5, k) / (2 * k + 5)        # t^4 (1+t^2)^-1/2
+        cg = 24.0 * binom(0.5, k + 1) / (2 * k + 5)    # t^2 (sqrt(1+t^2) - 1)
+        xs = x[small][..., None]
+        f = np.where(small, 0.0, f)
+        g = np.where(small, 0.0, g)
+        f[small] = (cf * xs ** (2 * k + 5)).sum(axis=-1)
+        g[small] = (cg * xs ** (2 * k + 5)).sum(axis=-1)
+    P, u = FERMI_A * f, FERMI_A * g
+    dPdd = 8.0 * FERMI_A * x ** 5 / (3.0 * d * sq)
+    return P, u, dPdd
+
+
+def helmholtz_table_dwd():
+    """The electron + radiation free-energy table of the double-white-dwarf workload, at the
+    reference's dimensions and ranges (helmholtz/helmeos.cpp:7-8, :78-86), per unit ye*rho = d as
+    helmholtz_eos reads it (p = d^2 f_d, e = ye (f - T f_t), :380-387):
+      * the cold ideal Fermi gas, f = u(d)/d with u the kinetic energy density of cold_electrons:
+        f_d = P/d^2, f_dd = P'(d)/d^2 - 2P/d^3, no T dependence -- the degenerate part;
+      * the non-degenerate ideal electron gas of helmholtz_table (A T (ln d - 1.5 ln T + C));
+      * radiation, f = -a T^4/(3d): p = a T^4/3, e = a T^4/rho, s = 4 a T^3/(3 rho) per gram.
+    Every column is the closed-form derivative of that sum, so the table is thermodynamically
+    consistent.  Ions stay analytic in the EOS."""
+    I, J = HELM_IMAX, HELM_JMAX
+    rows = helmholtz_table().reshape(J, I, 9)
+    d = 10.0 ** (-12.0 + np.arange(I) * (27.0 / (I - 1)))
+    T = 10.0 ** (3.0 + np.arange(J) * (10.0 / (J - 1)))
+    P, u, dPdd = cold_electrons(d)
+    rows[..., 0] += (u / d)[None, :]
+    rows[..., 1] += (P / d ** 2)[None, :]
+    rows[..., 3] += (dPdd / d ** 2 - 2.0 * P / d ** 3)[None, :]
+    a = HELM_ASOL
+    dd, TT = np.meshgrid(d, T)
+    rows[..., 0] += -a * TT ** 4 / (3.0 * dd)              # f
+    rows[..., 1] += a * TT ** 4 / (3.0 * dd ** 2)          # f_d
+    rows[..., 2] += -4.0 * a * TT ** 3 / (3.0 * dd)        # f_t
+    rows[..., 3] += -2.0 * a * TT ** 4 / (3.0 * dd ** 3)   # f_dd
+    rows[..., 4] += -4.0 * a * TT ** 2 / dd                # f_tt
+    rows[..., 5] += 4.0 * a * TT ** 3 / (3.0 * dd ** 2)    # f_dt
+    rows[..., 6] += -8.0 * a * TT ** 3 / (3.0 * dd ** 3)   # f_ddt
+    rows[..., 7] += 4.0 * a * TT ** 2 / dd ** 2            # f_dtt
+    rows[..., 8] += -8.0 * a * TT ** 2 / dd ** 3           # f_ddtt
+    return np.ascontiguousarray(rows.reshape(J * I, 9))
+
+
+def dwd_eos(rho, T, abar, zbar):
+    """Closed-form p (dyn/cm^2) and e (erg/g) of the free energy helmholtz_table_dwd tabulates, with
+    the EOS's analytic ions (helmholtz/helmeos.cpp:186-198): the values its interpolation approaches."""
+    rho, T = np.asarray(rho, np.float64), np.asarray(T, np.float64)
+    ye = zbar / abar
+    d = ye * rho
+    A = HELM_AVO * HELM_KERG
+    P, u, _ = cold_electrons(d)
+    p = P + A * T * d + HELM_ASOL * T ** 4 / 3.0 + A * T * rho / abar
+    e = u / rho + 1.5 * A * T * ye + HELM_ASOL * T ** 4 / rho + 1.5 * A * T / abar
+    return p, e
+
+
 def write_helmholtz_table(path, rows):
     """The rows as a helmholtz.table text file (9 values per line, 17 significant digits, so
     %le parsing restores them exactly)."""
@@ -276,6 +360,164 @@ def binary_batch(counts=BINARY_COUNTS, n=8, g=3, eos="ideal", q=0.7, r_donor=0.2
         phi[s:e] = _binary_phi(M, *grid(np.arange(P) - 1 + 0.5))
     x0 = np.ascontiguousarray(org)
     return dict(U=U, phi=phi, x0=x0, h=h, level=lev, omega=M["omega"])
+
+
+# ---- the double white dwarf of the Helmholtz-EOS workload ------------------------------------------
+DWD_MASSES = (0.6, 0.9)  # solar masses: donor, accretor
+DWD_ABAR = 96.0 / 7.0    # C/O 50/50 by mass: 1/abar = 0.5/12 + 0.5/16; zbar = abar/2 = 6.857
+DWD_ROT_ABAR = (4.0, DWD_ABAR, 17.52)  # the C/O species slot carries abar 13.714
+DWD_T = 1.0e7            # K, isothermal stars
+DWD_RHO_BG, DWD_T_BG = 1.0, 1.0e8  # g/cm^3, K: a hot tenuous envelope
+
+
+def white_dwarf(mass, ye=0.5, n=4000):
+    """A cold white dwarf of `mass` (solar masses) in hydrostatic equilibrium, dP/dr = -G m rho/r^2,
+    dm/dr = 4 pi r^2 rho, on the T = 0 ideal electron gas of cold_electrons (electron pressure only:
+    the ions and the finite temperature add < 1% at 1e7 K).  The central density is found by
+    bisection.  Returns (r, rho, m, phi) on n radii from the centre to the surface (cgs), phi the
+    star's own potential, -G M/r outside."""
+    from scipy.integrate import solve_ivp
+
+    def rhs(r, y):
+        m, x = y
+        x = max(x, 0.0)
+        rho = FERMI_B * x ** 3 / ye
+        dPdx = 8.0 * FERMI_A * x ** 4 / np.sqrt(1.0 + x * x)
+        return [4.0 * np.pi * r * r * rho, -HELM_G * m * rho / (r * r * dPdx) if x > 0.0 else 0.0]
+
+    def integrate(rho_c, dense=False):
+        xc = np.cbrt(ye * rho_c / FERMI_B)
+        r0 = 1.0e3
+        m0 = 4.0 / 3.0 * np.pi * r0 ** 3 * rho_c
+        surf = lambda r, y: y[1] - 1e-4 * xc  # noqa: E731
+        surf.terminal = True
+        sol = solve_ivp(rhs, (r0, 1e11), [m0, xc], events=surf, rtol=1e-10, atol=[1e-12 * HELM_MSOL, 1e-14],
+                        dense_output=dense, max_step=1e7)
+        return sol
+
+    lo, hi = 1e4, 1e10
+    for _ in range(48):  # log-bisection: 6 decades to 1e-13 relative
+        mid = np.sqrt(lo * hi)
+        M = integrate(mid).y[0, -1] / HELM_MSOL
+        lo, hi = (mid, hi) if M < mass else (lo, mid)
+    sol = integrate(np.sqrt(lo * hi), dense=True)
+    R = sol.t[-1]
+    r = np.linspace(sol.t[0], R, n)
+    m, x = sol.sol(r)
+    rho = FERMI_B * np.maximum(x, 0.0) ** 3 / ye
+    # phi(r) = -G M/R - int_r^R G m/r'^2 dr'
+    gm = HELM_G * m / (r * r)
+    tail = np.concatenate([np.cumsum((0.5 * (gm[1:] + gm[:-1]) * np.diff(r))[::-1])[::-1], [0.0]])
+    phi = -HELM_G * m[-1] / R - tail
+    return r, rho, m, phi
+
+
+_DWD_MODEL = None
+
+
+def dwd_model():
+    """The 0.6 + 0.9 solar-mass C/O pair: radii, centres (centre of mass at the origin), separation
+    3 donor radii (the donor near its Roche lobe, 0.344 a at q = 2/3), the Keplerian Omega, and the
+    code units: length L = R_donor / 0.2 (so the donor spans 0.2 of the root domain's half edge, as
+    in the polytrope workload), mass 1 solar mass, time (L^3 / (G Msun))^1/2; T in K.  units =
+    (cm, g, s, K) as amrh_eos_set_units takes them."""
+    global _DWD_MODEL
+    if _DWD_MODEL is not None:
+        return _DWD_MODEL
+    stars = [white_dwarf(mm) for mm in DWD_MASSES]
+    (rd, rhod, md, phid), (ra, rhoa, ma, phia) = stars
+    Rd, Ra = rd[-1], ra[-1]
+    Md, Ma = md[-1], ma[-1]
+    a = 3.0 * Rd
+    xd, xa = a * Ma / (Ma + Md), -a * Md / (Ma + Md)
+    L = Rd / 0.2
+    t = np.sqrt(L ** 3 / (HELM_G * HELM_MSOL))
+    omega = np.sqrt(HELM_G * (Ma + Md) / a ** 3) * t  # code units
+    _DWD_MODEL = dict(stars=stars, R=(Rd, Ra), M=(Md, Ma), centres=(xd, xa), a=a, L=L, t=t, omega=omega,
+                      units=(L, HELM_MSOL, t, 1.0))
+    return _DWD_MODEL
+
+
+def _dwd_phi(M, X, Y, Z):
+    """The two stars' own spherical potentials at (X, Y, Z) (code units), in code units."""
+    L, tu = M["L"], M["t"]
+    phi = 0.0
+    for (r_, rho_, m_, phi_), xc in zip(M["stars"], M["centres"]):
+        rr = np.sqrt((X * L - xc) ** 2 + (Y * L) ** 2 + (Z * L) ** 2)
+        phi = phi + np.where(rr < r_[-1], np.interp(rr, r_, phi_), -HELM_G * m_[-1] / np.maximum(rr, 1.0))
+    return phi / (L / tu) ** 2
+
+
+def _dwd_fields(M, X, Y, Z):
+    """The 13 conserved fields (code units) of the double white dwarf at cell centres (code units),
+    at rest in the frame rotating at Omega; T = DWD_T in the stars, DWD_T_BG in the envelope."""
+    L, tu = M["L"], M["t"]
+    rho_u, e_u = HELM_MSOL / L ** 3, (L / tu) ** 2
+    rho = np.full(np.broadcast(X, Y, Z).shape, DWD_RHO_BG)
+    rho_acc = np.zeros_like(rho)
+    for (r_, rho_, m_, phi_), xc, acc in zip(M["stars"], M["centres"], (False, True)):
+        rr = np.sqrt((X * L - xc) ** 2 + (Y * L) ** 2 + (Z * L) ** 2)
+        rs = np.where(rr < r_[-1], np.interp(rr, r_, rho_), 0.0)
+        rho = rho + rs
+        if acc:
+            rho_acc = rho_acc + rs
+    phi_c = _dwd_phi(M, X, Y, Z)
+    star = rho > 10.0 * DWD_RHO_BG
+    T = np.where(star, DWD_T, DWD_T_BG)
+    e = np.empty_like(rho)
+    e[star] = dwd_eos(rho[star], DWD_T, DWD_ABAR, DWD_ABAR / 2.0)[1]
+    rest = ~star
+    e[rest] = dwd_eos(rho[rest], DWD_T_BG, DWD_ABAR, DWD_ABAR / 2.0)[1]
+    omega = M["omega"]
+    R2 = X * X + Y * Y
+    rho_c = rho / rho_u
+    u = np.empty((13,) + rho.shape)
+    u[0] = rho_c
+    u[1] = rho_acc / rho_u + 0.5 * DWD_RHO_BG / rho_u
+    u[2] = rho_c - u[1]
+    u[3] = 0.0
+    u[4] = rho_c * R2 * omega
+    u[5] = 0.0
+    u[6] = rho_c * (phi_c - 0.5 * R2 * omega * omega)
+    u[7] = rho_c * e / e_u
+    u[8] = u[7]
+    u[9] = T
+    u[10] = 0.0
+    u[11] = rho_c
+    u[12] = 0.0
+    return u
+
+
+def dwd_batch(counts=BINARY_COUNTS, n=8, g=3, seed=SEED):
+    """The Helmholtz-EOS workload: a 0.6 + 0.9 solar-mass C/O double white dwarf (abar 13.714,
+    zbar 6.857), each star a cold white dwarf in hydrostatic equilibrium (white_dwarf) at 1e7 K,
+    in a 1 g/cm^3, 1e8 K envelope, at rest in the frame rotating at the Keplerian Omega, on
+    BINARY_COUNTS leaves of 8^3 (multilevel_layout, nearest the origin finest) in the code units of
+    dwd_model.  The given potential is the two stars' own spherical potentials.  Energies follow
+    from the closed-form free energy that helmholtz_table_dwd tabulates.  Returns binary_batch's
+    dict plus units (for amrh_eos_set_units) and abar (the species' mass numbers)."""
+    lev, org = multilevel_layout(counts, seed)
+    nb = len(lev)
+    G = n + 2 * g
+    M = dwd_model()
+    h = 2.0 / (n * 2.0 ** (lev + 1))
+    U = np.empty((nb, 13, G, G, G))
+    P = n + 2
+    phi = np.empty((nb, P, P, P))
+    for s in range(0, nb, 1024):
+        e = min(nb, s + 1024)
+        hh = h[s:e, None]
+
+        def grid(off):
+            x = org[s:e, 0:1] + off[None, :] * hh
+            y = org[s:e, 1:2] + off[None, :] * hh
+            z = org[s:e, 2:3] + off[None, :] * hh
+            return x[:, None, None, :], y[:, None, :, None], z[:, :, None, None]
+
+        U[s:e] = np.moveaxis(_dwd_fields(M, *grid(np.arange(G) - g + 0.5)), 0, 1)
+        phi[s:e] = _dwd_phi(M, *grid(np.arange(P) - 1 + 0.5))
+    return dict(U=U, phi=phi, x0=np.ascontiguousarray(org), h=h, level=lev, omega=M["omega"], units=M["units"],
+                abar=DWD_ROT_ABAR)
 
 
 def binary_tree_state(level, loc, n=8, g=3, eos="ideal"):

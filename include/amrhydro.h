@@ -76,6 +76,8 @@ typedef struct {
   double f_r;        /* State::fR, centre-of-mass correction force (binary/binary_driver.cpp:194) */
   double f_theta;    /* State::ftheta (binary/binary_driver.cpp:195) */
   double abar[3];    /* species mass numbers (binary/state_ztwd.cpp:28) */
+  double newton_tol; /* relative tolerance |e(T) - e|/e of the floor's Newton solve for T; 0 = the
+                        reference's 1e-9 (helmholtz/helmeos.cpp:449) */
 } amrh_config;
 
 typedef struct amrh_ctx amrh_ctx;

@@ -188,14 +188,13 @@ int eos(St& q) {
   return err;
 }
 
-// helmholtz_compute_T, energy form (:436-486)
-int compute_T(St& q) {
+// helmholtz_compute_T, energy form (:436-486); toler is the reference's 1e-9 unless a caller sets it
+int compute_T(St& q, double toler = 1.0e-9) {
   double T0 = q.T;
   const double e0 = q.e;
   q.T = TMIN;
   int err = eos(q);
   q.T = T0;
-  const double toler = 1.0e-9;
   double a = 1.0;
   if (q.e >= e0) {
     q.T = TMIN;
@@ -231,9 +230,9 @@ int helm_eos_state(double* r) {
   for (int c = 0; c < 10; c++) r[c] = o[c];
   return e;
 }
-int helm_compute_T_state(double* r) {
+int helm_compute_T_state(double* r, double toler) {
   St q{r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9]};
-  const int e = compute_T(q);
+  const int e = compute_T(q, toler);
   const double o[10] = {q.p, q.e, q.cs, q.cv, q.abar, q.zbar, q.rho, q.T, q.s, q.dsdt};
   for (int c = 0; c < 10; c++) r[c] = o[c];
   return e;

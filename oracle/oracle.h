@@ -96,6 +96,8 @@ typedef struct {
   double gamma;       /* ideal-gas EOS only */
   double cfl;
   double abar[3];     /* species mass numbers (binary/state_ztwd.cpp:28) */
+  double newton_tol;  /* relative tolerance of the T Newton solve; 0 = the reference's 1e-9
+                         (helmholtz/helmeos.cpp:449) */
 } orc_rot_cfg;
 
 int orc_rot_batch_substep(const orc_rot_cfg* c, int nb, double* U, const double* U0, const double* phi,

@@ -28,7 +28,7 @@
 
 namespace orc {
 int helm_eos_state(double* r);
-int helm_compute_T_state(double* r);
+int helm_compute_T_state(double* r, double toler);
 }  // namespace orc
 
 using namespace orc;
@@ -44,10 +44,12 @@ struct RC {
   long G3;
   double theta, rho_floor, Omega, Omega0, fR, ftheta, gamma, cfl;
   double abar[NSPEC];
+  double toler;  // Newton tolerance (helmholtz/helmeos.cpp:449 uses 1e-9)
   explicit RC(const orc_rot_cfg* c)
       : n(c->n), g(c->g), G(c->n + 2 * c->g), recon(c->recon), zref(c->z_reflect), eos(c->eos),
         G3((long)G * G * G), theta(c->theta), rho_floor(c->rho_floor), Omega(c->Omega), Omega0(c->Omega0),
-        fR(c->fR), ftheta(c->ftheta), gamma(c->gamma), cfl(c->cfl) {
+        fR(c->fR), ftheta(c->ftheta), gamma(c->gamma), cfl(c->cfl),
+        toler(c->newton_tol > 0.0 ? c->newton_tol : 1.0e-9) {
     for (int i = 0; i < NSPEC; i++) abar[i] = c->abar[i];
   }
   long at(int i, int j, int k) const { return ((long)k * G + j) * G + i; }
@@ -123,7 +125,7 @@ int ideal_compute_T(const RC& c, double* q) {
     iters++;
     if (iters % 10 == 0) a *= 0.9;
     if (q[7] < TMIN_IDEAL) return 128;
-  } while (std::fabs(f / e0) >= 1.0e-9 && iters < 100);
+  } while (std::fabs(f / e0) >= c.toler && iters < 100);
   ideal_eos(c, q);
   return iters >= 100 ? 64 : 0;
 }
@@ -235,7 +237,7 @@ int floor_state(const RC& c, double* u, const X3& X) {
   q[6] = d;
   q[1] = ue / d;
   q[7] = u[TT];
-  const int err = c.eos == 1 ? helm_compute_T_state(q) : ideal_compute_T(c, q);
+  const int err = c.eos == 1 ? helm_compute_T_state(q, c.toler) : ideal_compute_T(c, q);
   u[TT] = q[7];
   if (ue > 0.1 * u[ETR]) u[TAUR] = ue;
   return err;

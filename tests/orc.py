@@ -269,15 +269,16 @@ class OrcRotCfg(C.Structure):
                 ("eos", C.c_int), ("theta", C.c_double), ("rho_floor", C.c_double),
                 ("Omega", C.c_double), ("Omega0", C.c_double), ("fR", C.c_double),
                 ("ftheta", C.c_double), ("gamma", C.c_double), ("cfl", C.c_double),
-                ("abar", C.c_double * 3)]
+                ("abar", C.c_double * 3), ("newton_tol", C.c_double)]
 
 
-def rot_cfg(eos=0, recon=1, omega=0.0, omega0=None, f_r=0.0, f_theta=0.0, z_reflect=0):
+def rot_cfg(eos=0, recon=1, omega=0.0, omega0=None, f_r=0.0, f_theta=0.0, z_reflect=0, abar=(4.0, 14.12, 17.52),
+            newton_tol=0.0):
     """The constants of binary/state_ztwd.cpp:28-31 and binary/binary_defs.h:29-30."""
     c = OrcRotCfg(n=8, g=3 if recon == 1 else 2, recon=recon, z_reflect=z_reflect, eos=eos, theta=1.3,
                   rho_floor=1.0e-10, Omega=omega, Omega0=omega if omega0 is None else omega0, fR=f_r,
-                  ftheta=f_theta, gamma=5.0 / 3.0, cfl=0.3)
-    c.abar[0], c.abar[1], c.abar[2] = 4.0, 14.12, 17.52
+                  ftheta=f_theta, gamma=5.0 / 3.0, cfl=0.3, newton_tol=newton_tol)
+    c.abar[0], c.abar[1], c.abar[2] = abar
     return c
 
 

@@ -33,12 +33,14 @@ _eos = {}
 
 
 def helm_eos():
+    """The device table of the polytrope workload in its units; the oracle's (global) table and
+    units are set to the same on every call, since other tests load their own."""
+    rows = hc.table_rows()
+    orc.helm_table(rows)
+    orc.helm_set_units(*workloads.HELM_UNITS)
     if "E" not in _eos:
-        rows = hc.table_rows()
-        orc.helm_table(rows)
         E = amrhydro.HelmholtzEos(rows)
         E.set_units(*workloads.HELM_UNITS)
-        orc.helm_set_units(*workloads.HELM_UNITS)
         _eos["E"] = E
     return _eos["E"]
 
