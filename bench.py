@@ -169,6 +169,27 @@ def extra_configs(amrhydro, torch, steps, warmup, only=None):
         if eos_tab is not None:
             eos_tab.close()
 
+    if want("dwd"):
+        log("double white dwarf, Helmholtz EOS on the degenerate + radiation table (0.6 + 0.9 Msun C/O, "
+            "20,000 leaves of 8^3, PPM, 13 fields, Newton to 1e-10)")
+        Bn = workloads.dwd_batch()
+        tab = amrhydro.HelmholtzEos(workloads.helmholtz_table_dwd())
+        tab.set_units(*Bn["units"])
+
+        def prep_dwd(B, Bn=Bn):
+            B.set_block_origin(Bn["x0"])
+            B.set_potential(Bn["phi"])
+            B.attach_eos(tab)
+        for prec, ptag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
+            cfg = amrhydro.rotating_config(eos=amrhydro.AMRH_EOS_HELMHOLTZ, omega=Bn["omega"], boundary=OUT,
+                                           precision=prec, abar=Bn["abar"], newton_tol=1e-10)
+            r = flat("rotating helmholtz dwd %s" % ptag, cfg, Bn["U"], Bn["h"], prep_dwd)
+            r["workload"] = ("double white dwarf 0.6 + 0.9 Msun C/O (abar 13.714, zbar 6.857), 20,000 leaves of "
+                             "8^3, Helmholtz EOS on the cold-Fermi-gas + radiation table, Newton to 1e-10")
+            out["rotating_helmholtz_dwd_%s" % ptag] = r
+        del Bn
+        tab.close()
+
     if want("sedov_tree"):
         log("Sedov octree (73 nodes of 8^3, fully refined to level 2)")
         U, parent, child = workloads.sedov_tree(maxlevel=2)
@@ -224,7 +245,7 @@ def extra_configs(amrhydro, torch, steps, warmup, only=None):
     return out
 
 
-EXTRA_WORKLOADS = ("multilevel", "ppm_flat", "rotating", "sedov_tree", "binary_octree", "poisson")
+EXTRA_WORKLOADS = ("multilevel", "ppm_flat", "rotating", "dwd", "sedov_tree", "binary_octree", "poisson")
 C4_TREE = "binary octree for C4, 25737 nodes of 8^3 (22520 leaves), 7 levels"
 
 
