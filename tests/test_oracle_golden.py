@@ -39,3 +39,30 @@ def test_floor_and_outflow_match_reference_vectors(gold):
             a = gold["cells"][c].copy()
             orc.lib().orc_outflow(orc.C.byref(cfg), face, orc.dptr(a))
             np.testing.assert_array_equal(a, gold["outflow"][c, face])
+
+
+HELM = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_helm.npz")
+
+
+@pytest.mark.parametrize("kind", ["ideal", "dwd"])
+def test_helmholtz_matches_reference_vectors(kind):
+    """helmholtz_eos and helmholtz_compute_T of the reference's helmeos.cpp
+    (tests/golden/make_helm_golden.py) on the two synthetic tables, in code units: the oracle gives
+    the same bits in all ten eos_t fields"""
+    from gpu_util import amrhydro  # noqa: F401  (puts the package on the path)
+    import workloads
+    if not os.path.exists(HELM):
+        pytest.fail("golden fixture missing: run tests/golden/make_helm_golden.py")
+    gold = np.load(HELM)
+    rows = workloads.helmholtz_table() if kind == "ideal" else workloads.helmholtz_table_dwd()
+    orc.helm_table(rows)
+    orc.helm_set_units(*gold[kind + "_units"])
+    try:
+        a = np.ascontiguousarray(gold[kind + "_eos_in"].copy())
+        assert not orc.helm_eos(a).any()
+        np.testing.assert_array_equal(a, gold[kind + "_eos_out"])
+        b = np.ascontiguousarray(gold[kind + "_T_in"].copy())
+        assert not orc.helm_eos(b, solve_T=True).any()
+        np.testing.assert_array_equal(b, gold[kind + "_T_out"])
+    finally:
+        orc.helm_set_units()
