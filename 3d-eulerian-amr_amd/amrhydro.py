@@ -88,8 +88,11 @@ SIGNATURES = {
     "amrh_dt_device": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
     "amrh_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "amrh_substep_bytes": (C.c_double, [C.c_void_p, C.c_int]),
+    "amrh_ghost_write_bytes": (C.c_double, [C.c_void_p]),
     "amrh_autotune": (C.c_int, [C.c_void_p, IP, IP]),
     "amrh_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "amrh_launch_info": (C.c_int, [C.c_void_p, IP, IP, IP, DP, DP]),
+    "amrh_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "amrh_run_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "amrh_tree_set": (C.c_int, [C.c_void_p, IP, IP]),
     "amrh_tree_set_origin": (C.c_int, [C.c_void_p, DP]),
@@ -270,6 +273,9 @@ class Batch:
         _ck(lib().amrh_error(self.ctx, C.byref(f)))
         return f.value
 
+    def ghost_write_bytes(self):
+        return lib().amrh_ghost_write_bytes(self.ctx)
+
     def substep_bytes(self, beta_is_one):
         return lib().amrh_substep_bytes(self.ctx, 1 if beta_is_one else 0)
 
@@ -287,6 +293,16 @@ class Batch:
         gs, gm = C.c_int(), C.c_int()
         _ck(lib().amrh_autotune(self.ctx, C.byref(gs), C.byref(gm)))
         return gs.value, gm.value
+
+    def launch_info(self):
+        """(grid_substep, grid_maxeig, variant, ms_default, ms_tuned) of the autotuner"""
+        gs, gm, v = C.c_int(), C.c_int(), C.c_int()
+        d, t = C.c_double(), C.c_double()
+        _ck(lib().amrh_launch_info(self.ctx, C.byref(gs), C.byref(gm), C.byref(v), C.byref(d), C.byref(t)))
+        return gs.value, gm.value, v.value, d.value, t.value
+
+    def set_variant(self, variant):
+        _ck(lib().amrh_set_variant(self.ctx, variant))
 
     def set_launch(self, grid_substep, grid_maxeig):
         _ck(lib().amrh_set_launch(self.ctx, grid_substep, grid_maxeig))

@@ -367,7 +367,7 @@ hipError_t amrh::ctx_outflow_ghosts(const amrh_ctx* c, double* U) {
   return launch_outflow_ghosts(c->cfg.n, c->cfg.g, U, c->nb, c->cfg.z_reflect, c->stream);
 }
 
-hipError_t amrh::ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int grid) {
+hipError_t amrh::ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int grid, int variant) {
   const bool mixed = c->cfg.precision == AMRH_MIXED;
   if (c->cfg.physics == AMRH_PHYS_ROTATING) {
     RotArgs r = rot_args(c);
@@ -381,7 +381,8 @@ hipError_t amrh::ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int
     if (e == hipSuccess && a.outflow) e = ctx_outflow_ghosts(c, a.Uout);
     return e;
   }
-  if (c->cfg.recon == AMRH_RECON_MINMOD) return launch_substep(c->cfg.n, a, mixed, grid, c->stream);
+  if (c->cfg.recon == AMRH_RECON_MINMOD)
+    return launch_substep(c->cfg.n, a, mixed, grid, c->stream, variant >= 0 ? variant : c->variant);
   hipError_t e = launch_block_substep(c->cfg.n, c->cfg.g, c->cfg.recon, a, mixed, grid, c->stream);
   if (e == hipSuccess && a.outflow) e = launch_outflow_ghosts(c->cfg.n, c->cfg.g, a.Uout, c->nb, c->cfg.z_reflect, c->stream);
   return e;
@@ -504,12 +505,16 @@ double amrh_substep_bytes(const amrh_ctx* c, int beta_is_one) {
   const double n = c->cfg.n, g = c->cfg.g, nf = c->cfg.nf;
   const double n3 = n * n * n;
   const double read_u = n3 + 6.0 * g * n * n;
-  // with AMRH_BOUNDARY_OUTFLOW the substep also writes the next stage's face ghost slabs
-  const double ghost_w = c->cfg.boundary == AMRH_BOUNDARY_OUTFLOW ? 6.0 * g * n * n : 0.0;
-  double per_block = 8.0 * nf * (read_u + n3 + ghost_w + (beta_is_one ? 0.0 : n3));
-  // the rotating state also reads the potential with its ghost layer
-  if (c->cfg.physics == AMRH_PHYS_ROTATING) per_block += 8.0 * (n + 2.0) * (n + 2.0) * (n + 2.0);
+  double per_block = 8.0 * nf * (read_u + n3 + (beta_is_one ? 0.0 : n3));
+  // the rotating state also reads the potential over the interior and its six face halos
+  if (c->cfg.physics == AMRH_PHYS_ROTATING) per_block += 8.0 * (n3 + 6.0 * n * n);
   return per_block * c->nb;
+}
+
+double amrh_ghost_write_bytes(const amrh_ctx* c) {
+  if (!c || c->cfg.boundary != AMRH_BOUNDARY_OUTFLOW) return 0.0;
+  const double n = c->cfg.n, g = c->cfg.g, nf = c->cfg.nf;
+  return 8.0 * nf * 6.0 * g * n * n * c->nb;
 }
 
 }  // extern "C"

@@ -70,6 +70,8 @@ struct amrh_ctx {
   double* diag_part = nullptr;  // state_sum/max/min partials (allocated on first use)
   int grid_substep = 0;  // persistent grid of the substep kernel (0: one workgroup per block)
   int grid_maxeig = 0;   // same for the CFL scan
+  int variant = 0;       // register-budget variant of the flat minmod substep (launch_substep)
+  float tune_ms_default = 0.f, tune_ms_best = 0.f;  // the autotuner's timings of the substep
   // captured RK steps, one graph per parity of the ping-pong buffers (amrh_graph.hip)
   hipGraphExec_t step_exec[2] = {nullptr, nullptr};
   int step_exec_rk = 0;
@@ -83,7 +85,8 @@ PhysParams params(const amrh_config& c);
 int substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool tree);
 // the substep / CFL-scan kernels of the context's scheme (the column-sweep kernel for minmod,
 // the block kernel for PPM); a block-kernel substep in outflow mode is followed by the ghost fill
-hipError_t ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int grid);
+// variant: the flat minmod substep's register-budget variant; -1 = the context's
+hipError_t ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int grid, int variant = -1);
 hipError_t ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid);
 // the physics' extra inputs are in place (0) or an AMRH_ESTATE failure
 int ctx_ready(const amrh_ctx* c);

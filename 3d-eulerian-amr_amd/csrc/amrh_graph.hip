@@ -90,6 +90,9 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   std::vector<int> cand = {0};
   for (int k : {1, 2, 3, 4, 6, 8, 12, 16})
     if (k * cus < c->nb) cand.push_back(k * cus);
+  // the register-budget variants exist for the flat minmod sweep of the ideal gas
+  const bool variants = c->cfg.physics == AMRH_PHYS_EULER && c->cfg.recon == AMRH_RECON_MINMOD;
+  const int nvar = variants ? kSweepVariants : 1;
   SubstepArgs a{};
   a.Uin = c->buf[c->cur];
   a.Uout = c->buf[other];
@@ -109,16 +112,25 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   m.block_eig = c->eig_dev;
   m.nblocks = c->nb;
   m.P = params(c->cfg);
-  float best_s = 1e30f, best_m = 1e30f;
-  int gs = 0, gm = 0;
+  float best_s = 1e30f, best_m = 1e30f, def_s = 0.f;
+  int gs = 0, gm = 0, vs = 0;
+  for (int v = 0; v < nvar; v++)
+    for (int g : cand) {
+      float ms = 0.f;
+      if (int rc = time_launches(c, 3, [&]() -> int {
+            HIPCHK(ctx_substep_launch(c, a, g, v));
+            return AMRH_OK;
+          }, &ms))
+        return rc;
+      if (v == 0 && g == 0) def_s = ms;
+      if (ms < best_s) {
+        best_s = ms;
+        gs = g;
+        vs = v;
+      }
+    }
   for (int g : cand) {
     float ms = 0.f;
-    if (int rc = time_launches(c, 3, [&]() -> int {
-          HIPCHK(ctx_substep_launch(c, a, g));
-          return AMRH_OK;
-        }, &ms))
-      return rc;
-    if (ms < best_s) { best_s = ms; gs = g; }
     if (int rc = time_launches(c, 3, [&]() -> int {
           HIPCHK(ctx_maxeig_launch(c, m, g));
           return AMRH_OK;
@@ -128,9 +140,31 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   }
   c->grid_substep = gs;
   c->grid_maxeig = gm;
+  c->variant = vs;
+  c->tune_ms_default = def_s;
+  c->tune_ms_best = best_s;
   drop_graphs(c);
   if (grid_substep) *grid_substep = gs;
   if (grid_maxeig) *grid_maxeig = gm;
+  return AMRH_OK;
+}
+
+int amrh_launch_info(const amrh_ctx* c, int* grid_substep, int* grid_maxeig, int* variant, double* ms_default,
+                     double* ms_tuned) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (grid_substep) *grid_substep = c->grid_substep;
+  if (grid_maxeig) *grid_maxeig = c->grid_maxeig;
+  if (variant) *variant = c->variant;
+  if (ms_default) *ms_default = c->tune_ms_default;
+  if (ms_tuned) *ms_tuned = c->tune_ms_best;
+  return AMRH_OK;
+}
+
+int amrh_set_variant(amrh_ctx* c, int variant) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (variant < 0 || variant >= kSweepVariants) return fail(AMRH_EINVAL, "variant must be 0..%d", kSweepVariants - 1);
+  c->variant = variant;
+  drop_graphs(c);
   return AMRH_OK;
 }
 

@@ -6,8 +6,11 @@
 
 namespace amrh {
 
-// grid <= 0: one workgroup per block; otherwise a persistent grid of that many workgroups
-hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s);
+// grid <= 0: one workgroup per block; otherwise a persistent grid of that many workgroups.
+// variant (flat batches of the minmod sweep): 0 the default register budget, 1..3 the kernel held to
+// 2, 3, 4 waves per SIMD
+constexpr int kSweepVariants = 4;
+hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s, int variant = 0);
 hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s);
 hipError_t launch_outflow_ghosts(int n, int g, double* U, int nblocks, int z_reflect, hipStream_t s);
 hipError_t launch_pack(int n, int g, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s);

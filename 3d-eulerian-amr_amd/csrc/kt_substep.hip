@@ -165,8 +165,11 @@ __device__ __forceinline__ void put_ghost(const RecBuf<(long)N * N * N + 6L * GW
 // registers to 4 (0.763 -> 0.711 ms).
 // TREE: the octree's reflux overrides and the root's flow-off ledger (a.slot, a.ledger); the flat
 // instantiation carries neither, which keeps its register count down.
-template <int N, bool MIXED, int MODE, bool TREE>
-__global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(MIXED ? (N == 16 ? 3 : 1) : 2))) void kt_sweep_kernel(
+// WPE: the waves per SIMD the register allocation is held to (the register budget: 2 -> 256 VGPRs,
+// 3 -> 168, 4 -> 128); 0 takes the measured default above.  The flat substep is instantiated for
+// 2, 3 and 4 as well, and the autotuner picks among them with the grid.
+template <int N, bool MIXED, int MODE, bool TREE, int WPE = 0>
+__global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : (MIXED ? (N == 16 ? 3 : 1) : 2)))) void kt_sweep_kernel(
     SubstepArgs a, MaxEigArgs m) {
   using T = typename Prec<MIXED>::T;
   constexpr int PW = N + 2 * GW;  // padded plane edge
@@ -679,8 +682,19 @@ __global__ void reset_u64_kernel(unsigned long long* p) { *p = 0ull; }
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
+template <int N, bool MIXED>
+static void launch_flat_variant(const SubstepArgs& a, int g, int variant, hipStream_t s) {
+  MaxEigArgs m{};
+  switch (variant) {
+    case 1: kt_sweep_kernel<N, MIXED, 0, false, 2><<<g, N * N, 0, s>>>(a, m); break;
+    case 2: kt_sweep_kernel<N, MIXED, 0, false, 3><<<g, N * N, 0, s>>>(a, m); break;
+    case 3: kt_sweep_kernel<N, MIXED, 0, false, 4><<<g, N * N, 0, s>>>(a, m); break;
+    default: kt_sweep_kernel<N, MIXED, 0, false><<<g, N * N, 0, s>>>(a, m); break;
+  }
+}
+
 template <int N>
-static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, int grid, hipStream_t s) {
+static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, int grid, int variant, hipStream_t s) {
   MaxEigArgs m{};
   if (a.nblocks <= 0) return hipSuccess;
   const int g = (grid > 0 && grid < a.nblocks) ? grid : a.nblocks;
@@ -688,17 +702,18 @@ static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, int grid, h
   if (mixed && tree)
     kt_sweep_kernel<N, true, 0, true><<<g, N * N, 0, s>>>(a, m);
   else if (mixed)
-    kt_sweep_kernel<N, true, 0, false><<<g, N * N, 0, s>>>(a, m);
+    launch_flat_variant<N, true>(a, g, variant, s);
   else if (tree)
     kt_sweep_kernel<N, false, 0, true><<<g, N * N, 0, s>>>(a, m);
   else
-    kt_sweep_kernel<N, false, 0, false><<<g, N * N, 0, s>>>(a, m);
+    launch_flat_variant<N, false>(a, g, variant, s);
   return hipGetLastError();
 }
 
-hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s) {
-  if (n == 8) return launch_substep_n<8>(a, mixed, grid, s);
-  if (n == 16) return launch_substep_n<16>(a, mixed, grid, s);
+hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s, int variant) {
+  if (variant < 0 || variant >= kSweepVariants) return hipErrorInvalidValue;
+  if (n == 8) return launch_substep_n<8>(a, mixed, grid, variant, s);
+  if (n == 16) return launch_substep_n<16>(a, mixed, grid, variant, s);
   return hipErrorInvalidValue;
 }
 

@@ -7,16 +7,21 @@ A "step" is one full RK3 time step over the batch: CFL wavespeed reduction to a 
 three fused substeps (reconstruction + KT flux + divergence + RK combine + floor).  Every block is an
 isolated domain with outflow faces (grid/grid_phys_bound.cpp semantics); each substep writes the
 next stage's ghost slabs as it stores the boundary cells, so the ghost refresh costs no separate
-pass.  `value` is leaf cell-updates/s (3 substeps x cells / wall time of the whole step) of the
-mixed-precision form; the FP64 form is measured the same way and reported beside it.
+pass.  The launch shape (grid and register-budget variant) comes from the library's autotuner;
+whole steps are replayed as captured hipGraphs.  `value` is leaf cell-updates/s (3 substeps x
+cells / wall time of whole steps) of the mixed-precision form; the FP64 form is measured the same
+way and reported beside it.  The roofline's `achieved` is the substep's algorithmic bytes (interior
++ face ghost slabs read, interior written, U0 read when beta != 1) over its kernel time, measured
+with HIP events around direct launches of the same tuned kernel on the library's stream.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
 import statistics
-import subprocess
 import sys
 import time
 
@@ -32,28 +37,42 @@ BETAS = (1.0, 0.25, 2.0 / 3.0)
 POISSON_LEVELS = 5  # 2,473 nodes of 8^3 over 6 levels, refined around the two stars
 POISSON_TREE = "binary octree, 2473 nodes of 8^3, 6 levels"
 POISSON_TOLER = 1e-7  # ELL_TOLER (binary/binary_defs.h:64)
+TUNE_KEYS = ("grid_substep", "grid_maxeig", "variant", "substep_ms_default_launch", "substep_ms_tuned_launch")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
-    # isolated outflow blocks: each substep writes the next stage's ghosts as it stores the
-    # boundary cells, so no separate ghost pass runs between substeps
-    cfg = amrhydro.euler_config(n=n, precision=precision, boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW)
-    B = amrhydro.Batch(cfg, nb, 1.0 / n)
-    B.set_stream(torch.cuda.current_stream().cuda_stream)
-    B.upload_device(U_dev_ptr)
-    B.fill_outflow_ghosts()
-    res = time_steps(B, torch, nb * n ** 3, steps, warmup)
-    B.close()
-    return res
+def kernel_build_id():
+    """A hash of the kernel sources: the tag a traffic profile carries, so that a profile of an
+    older build is not reported as this build's traffic."""
+    h = hashlib.sha1()
+    for f in sorted(glob.glob(os.path.join(PKG, "csrc", "*")) + [os.path.join(ROOT, "include", "amrhydro.h")]):
+        with open(f, "rb") as fh:
+            h.update(os.path.basename(f).encode())
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
-def time_steps(B, torch, cells, steps, warmup):
-    """RK3 steps on a prepared batch (state uploaded, ghosts filled, outflow boundary mode): the
-    device CFL dt, store, three substeps; HIP events around every substep launch."""
+def load_traffic(key):
+    """HBM bytes per launch of a substep from profiles/traffic_<key>.json (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes, scaled by a plain copy's counts), when it was measured on this build."""
+    p = os.path.join(ROOT, "profiles", "traffic_%s.json" % key)
+    if not os.path.exists(p):
+        return None, "no profile"
+    with open(p) as f:
+        d = json.load(f)
+    if d.get("kernel_build") != kernel_build_id():
+        return None, "profile of another kernel build (%s)" % d.get("kernel_build")
+    return d.get("bytes_per_launch"), "measured on this build"
+
+
+def kernel_times(B, torch, cells, steps, warmup):
+    """RK3 steps on a prepared batch (state uploaded, ghosts filled, outflow boundary mode) with
+    direct launches: the device CFL dt, store, three substeps; HIP events around every substep
+    launch on the library's stream (torch's current stream).  Returns the substep kernel times and
+    the algorithmic bytes behind the roofline."""
     stream = torch.cuda.current_stream()
     dt = torch.zeros(1, dtype=torch.float64, device="cuda")
     first = [True]
@@ -78,32 +97,105 @@ def time_steps(B, torch, cells, steps, warmup):
         step()
     torch.cuda.synchronize()
     events = []
-    t0 = time.perf_counter()
     for _ in range(steps):
         step(events)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     k_ms = [(a.elapsed_time(b), beta) for a, b, beta in events]
     bytes_one = B.substep_bytes(True)
     bytes_other = B.substep_bytes(False)
     tot_bytes = sum(bytes_one if beta == 1.0 else bytes_other for _, beta in k_ms)
     tot_ms = sum(t for t, _ in k_ms)
-    flags = B.error_flags()
-    res = {
-        "cell_updates_per_s": 3 * steps * cells / wall,
-        "ms_per_step": 1e3 * wall / steps,
+    return {
         "substep_kernel_ms_avg": tot_ms / len(k_ms),
         "substep_kernel_ms_beta1": statistics.mean(t for t, b in k_ms if b == 1.0),
         "substep_kernel_ms_beta_ne1": statistics.mean(t for t, b in k_ms if b != 1.0),
         "rhs_cell_updates_per_s": len(k_ms) * cells / (tot_ms * 1e-3),
         "achieved_GBs": tot_bytes / (tot_ms * 1e-3) / 1e9,
         "bytes_per_launch_avg": tot_bytes / len(k_ms),
-        "device_error_flags": flags,
+        "bytes_per_cell_update": tot_bytes / len(k_ms) / cells,
+        "ghost_write_bytes_per_launch": B.ghost_write_bytes(),
     }
+
+
+def graph_steps(B, torch, cells, steps, warmup):
+    """Whole RK3 steps replayed as captured hipGraphs (amrh_run_steps: CFL scan, device dt with the
+    reference's growth cap, store, three substeps), timed by the wall clock."""
+    dt = torch.zeros(1, dtype=torch.float64, device="cuda")
+    B.dt_device(1e-2, 0.0, dt.data_ptr())  # the first step's dt (euler/hydro_driver.cpp:108-111)
+    B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.run_steps(steps, dt.data_ptr(), rk=3, use_graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"cell_updates_per_s": 3 * steps * cells / wall, "ms_per_step": 1e3 * wall / steps}
+
+
+def measure(B, torch, cells, steps, warmup, tune=True):
+    """The autotuner's launch shape, then the substep kernel times (direct launches) and the
+    throughput of whole steps (graph replays) on it."""
+    res = {}
+    if tune:
+        B.autotune()
+        res["launch"] = dict(zip(TUNE_KEYS, B.launch_info()))
+    res.update(kernel_times(B, torch, cells, steps, warmup))
+    res.update(graph_steps(B, torch, cells, steps, warmup))
+    res["device_error_flags"] = B.error_flags()
     return res
 
 
-def extra_configs(amrhydro, torch, steps, warmup, only=None):
+def roofline(r, traffic_key=None):
+    traffic, note = load_traffic(traffic_key) if traffic_key else (None, "not profiled")
+    return {"bound": "hbm", "achieved": r["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": r["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": note,
+            "bytes_per_launch": r["bytes_per_launch_avg"], "bytes_per_cell_update": r["bytes_per_cell_update"],
+            "ghost_write_bytes_per_launch": r["ghost_write_bytes_per_launch"]}
+
+
+def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
+    # isolated outflow blocks: each substep writes the next stage's ghosts as it stores the
+    # boundary cells, so no separate ghost pass runs between substeps
+    cfg = amrhydro.euler_config(n=n, precision=precision, boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW)
+    B = amrhydro.Batch(cfg, nb, 1.0 / n)
+    B.set_stream(torch.cuda.current_stream().cuda_stream)
+    B.upload_device(U_dev_ptr)
+    B.fill_outflow_ghosts()
+    res = measure(B, torch, nb * n ** 3, steps, warmup)
+    B.close()
+    return res
+
+
+def rotating_accuracy(amrhydro, Bn, prep, steps):
+    """C4 at its full size: `steps` RK3 steps of the mixed and the FP64 form from the same state with
+    the same dt sequence (the FP64 form's CFL dt under the reference's control,
+    euler/hydro_driver.cpp:107-113); relative L1 per field of mixed against FP64 over the
+    interiors.  The FP64 form equals the CPU oracle per step to 1e-12 (tests)."""
+    forms = []
+    for prec in (amrhydro.AMRH_FP64, amrhydro.AMRH_MIXED):
+        cfg = amrhydro.rotating_config(omega=Bn["omega"], boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW, precision=prec)
+        B = amrhydro.Batch(cfg, Bn["U"].shape[0], Bn["h"])
+        prep(B)
+        B.upload(Bn["U"])
+        B.fill_outflow_ghosts()
+        forms.append(B)
+    B64, B32 = forms
+    dt = None
+    for _ in range(steps):
+        dt = B64.cfl_dt(dt)
+        for B in forms:
+            B.rk_step(dt, 3, fill_ghosts=False)
+    g = 3
+    a, b = B32.download(), B64.download()
+    I = (slice(None), slice(None), slice(g, -g), slice(g, -g), slice(g, -g))
+    A, R = a[I], b[I]
+    rel = [float(np.abs(A[:, f] - R[:, f]).sum() / max(np.abs(R[:, f]).sum(), 1e-300)) for f in range(A.shape[1])]
+    flags = [B.error_flags() for B in forms]
+    for B in forms:
+        B.close()
+    return {"steps": steps, "rel_l1_per_field": rel, "fields": list(amrhydro.ROT_FIELDS), "device_error_flags": flags}
+
+
+def extra_configs(amrhydro, torch, steps, warmup, only=None, cpu=False, accuracy_steps=100):
     """The other workloads of the hot path, one line each (FP64 and, where built, mixed): the
     uneven multilevel leaf batch (59,518 leaves of 8^3), PPM on 8^3 blocks (g = 3), the rotating
     binary (20,000 leaves, PPM) with the ideal-gas and the Helmholtz EOS (mixed and FP64), the
@@ -118,7 +210,7 @@ def extra_configs(amrhydro, torch, steps, warmup, only=None):
         return only is None or only == key
     stream = torch.cuda.current_stream().cuda_stream
 
-    def flat(name, cfg, U, h, extra=None):
+    def flat(name, cfg, U, h, extra=None, traffic_key=None):
         B = amrhydro.Batch(cfg, U.shape[0], h)
         B.set_stream(stream)
         if extra:
@@ -126,18 +218,28 @@ def extra_configs(amrhydro, torch, steps, warmup, only=None):
         B.upload(U)
         B.fill_outflow_ghosts()
         cells = U.shape[0] * cfg.n ** 3
-        r = time_steps(B, torch, cells, steps, warmup)
+        r = measure(B, torch, cells, steps, warmup)
         B.close()
-        log("  %s: %.3e cell-updates/s, substep kernel %.3f ms, %.0f GB/s"
-            % (name, r["cell_updates_per_s"], r["substep_kernel_ms_avg"], r["achieved_GBs"]))
+        r["roofline"] = roofline(r, traffic_key)
+        log("  %s: %.3e cell-updates/s, substep kernel %.3f ms, %.0f GB/s (%.1f%% of HBM); launch %s"
+            % (name, r["cell_updates_per_s"], r["substep_kernel_ms_avg"], r["achieved_GBs"],
+               100 * r["achieved_GBs"] / HBM_PEAK_GBS, r.get("launch")))
         return r
 
     if want("multilevel"):
-        log("multilevel batch (59,518 leaves of 8^3)")
+        log("uneven multilevel leaf batch (%d leaves of 8^3 over %d levels: %s)"
+            % (sum(workloads.LEVEL_COUNTS), len(workloads.LEVEL_COUNTS), list(workloads.LEVEL_COUNTS)))
         U, h, lev = workloads.multilevel()
         for prec, tag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
             cfg = amrhydro.euler_config(n=8, precision=prec, boundary=OUT)
-            out["multilevel_8_%s" % tag] = flat("multilevel %s" % tag, cfg, U, h)
+            r = flat("multilevel %s" % tag, cfg, U, h, traffic_key="multilevel8_%s" % tag)
+            r["workload"] = ("uneven multilevel octree leaves: %d leaves of 8^3 (+2 ghosts), per level %s, Gaussian "
+                             "core with solid-body rotation, RK3, one launch per substep (all levels), whole steps "
+                             "as hipGraphs, autotuned launch" % (U.shape[0], list(workloads.LEVEL_COUNTS)))
+            out["multilevel_8_%s" % tag] = r
+        if cpu:
+            log("CPU baseline (FP64 oracle) on a sample of the multilevel leaves")
+            out["multilevel_8_mixed"]["cpu_baseline"] = cpu_baseline(8, U=U, h=h, label="multilevel leaves")
         del U
 
     if want("ppm_flat"):
@@ -165,6 +267,11 @@ def extra_configs(amrhydro, torch, steps, warmup, only=None):
             for prec, ptag in ((amrhydro.AMRH_MIXED, "mixed"), (amrhydro.AMRH_FP64, "fp64")):
                 cfg = amrhydro.rotating_config(eos=eos, omega=Bn["omega"], boundary=OUT, precision=prec)
                 out["rotating_%s_%s" % (tag, ptag)] = flat("rotating %s %s" % (tag, ptag), cfg, Bn["U"], Bn["h"], prep)
+            if eos == amrhydro.AMRH_EOS_IDEAL and accuracy_steps > 0:
+                log("  accuracy: %d RK3 steps of the whole batch, mixed vs FP64" % accuracy_steps)
+                acc = rotating_accuracy(amrhydro, Bn, prep, accuracy_steps)
+                out["rotating_ideal_mixed"]["accuracy_mixed_vs_fp64"] = acc
+                log("  rel-L1 per field: %s" % ["%.1e" % x for x in acc["rel_l1_per_field"]])
             del Bn
         if eos_tab is not None:
             eos_tab.close()
@@ -367,36 +474,47 @@ def accuracy(amrhydro, torch, U_dev_ptr, nb, n, steps):
     return out
 
 
-def cpu_baseline(n, target_s=10.0):
-    """The FP64 CPU oracle (OpenMP over blocks) on a bounded sample of the same workload: whole
-    RK3 steps (3 substeps each, ghost refresh excluded from the timing) until ~target_s."""
+def cpu_baseline(n, target_s=10.0, U=None, h=None, label="flat batch blocks"):
+    """The FP64 CPU oracle (OpenMP over blocks) on a bounded sample of the same workload: whole RK3
+    steps timed as the GPU's are -- the CFL scan of the step, then 3 substeps; the ghost refresh
+    between substeps is outside the timing (the GPU writes ghosts inside its substep) -- until
+    ~target_s of CPU work.  U, h: the workload's own blocks to sample (default: flat_batch)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
     import workloads
 
     cfg = orc.euler_cfg(n=n)
     threads = os.environ.get("OMP_NUM_THREADS")
-    nb = 32
-    U = workloads.flat_batch(nb, n=n, seed=5)
-    orc.outflow_ghosts(cfg, U)
-    h = np.full(nb, 1.0 / n)
-    dt = 1e-2 * orc.block_dt(cfg, orc.batch_max_eig(cfg, U).max(), h[0])
+
+    def sample(nb, seed):
+        if U is None:
+            return workloads.flat_batch(nb, n=n, seed=seed), np.full(nb, 1.0 / n)
+        idx = np.random.default_rng(seed).choice(U.shape[0], size=min(nb, U.shape[0]), replace=False)
+        return np.ascontiguousarray(U[idx]), np.ascontiguousarray(h[idx])
+
+    Us, hs = sample(32, 5)
+    orc.outflow_ghosts(cfg, Us)
     t0 = time.perf_counter()
-    orc.batch_substep(cfg, U, U.copy(), h, dt, 1.0)
-    t1 = time.perf_counter()
-    rate = nb * n ** 3 / (t1 - t0)
-    nb = max(32, min(1024, int(rate * target_s / 3 / n ** 3)))
-    U = workloads.flat_batch(nb, n=n, seed=6)
-    orc.outflow_ghosts(cfg, U)
-    h = np.full(nb, 1.0 / n)
+    orc.batch_substep(cfg, Us, Us.copy(), hs, 1e-9, 1.0)
+    rate = Us.shape[0] * n ** 3 / (time.perf_counter() - t0)
+    nb = max(32, min(4096, int(rate * target_s / 3 / n ** 3)))
+    Us, hs = sample(nb, 6)
+    nb = Us.shape[0]
+    orc.outflow_ghosts(cfg, Us)
     times = []
     steps = 0
+    dt = None
     while sum(times) < target_s and steps < 50:
-        U0 = U.copy()
+        U0 = Us.copy()
+        t0 = time.perf_counter()
+        m = orc.batch_max_eig(cfg, Us)  # the step's CFL scan (grid/grid.cpp:422-483)
+        d = 0.4 * (hs / m).min()
+        dt = 1e-2 * d if dt is None else min(1.25 * dt, d)
+        times.append(time.perf_counter() - t0)
         for beta in BETAS:
-            orc.outflow_ghosts(cfg, U)
+            orc.outflow_ghosts(cfg, Us)
             t0 = time.perf_counter()
-            orc.batch_substep(cfg, U, U0, h, dt, beta)
+            orc.batch_substep(cfg, Us, U0, hs, dt, beta)
             times.append(time.perf_counter() - t0)
         steps += 1
     rate = 3 * steps * nb * n ** 3 / sum(times)
@@ -411,8 +529,8 @@ def cpu_baseline(n, target_s=10.0):
     except AttributeError:
         nproc = os.cpu_count()
     return {"value": rate, "unit": "cell-updates/s", "cores": cores, "kind": "port", "nproc": nproc,
-            "sample": "%d blocks of %d^3 (+2 ghosts) x %d RK3 steps (3 substeps each), FP64 oracle, "
-                      "OpenMP over blocks, %.1f s timed" % (nb, n, steps, sum(times))}
+            "sample": "%d %s of %d^3 (+2 ghosts) x %d RK3 steps (CFL scan + 3 substeps each), FP64 oracle, "
+                      "OpenMP over blocks, %.1f s timed" % (nb, label, n, steps, sum(times))}
 
 
 def cpu_poisson_baseline():
@@ -465,14 +583,6 @@ def cpu_binary_octree_baseline():
             "sample": "one substep + restriction of the FP64 oracle on the same octree, %.1f s" % t}
 
 
-def load_traffic(workload_key):
-    p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload_key)
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
-    return None
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,7 +623,8 @@ def main():
     # a real (non-null) stream shared by the library and the timing events
     torch.cuda.set_stream(torch.cuda.Stream())
     if args.workload != "headline":
-        out = extra_configs(amrhydro, torch, args.steps, args.warmup, only=args.workload)
+        out = extra_configs(amrhydro, torch, args.steps, args.warmup, only=args.workload, cpu=not args.no_cpu,
+                            accuracy_steps=args.accuracy_steps)
         print(json.dumps({"workload": args.workload, "steps": args.steps, "warmup": args.warmup,
                           "results": out}))
         return
@@ -530,9 +641,11 @@ def main():
         log("timing %s form: %d RK3 steps after %d warmup" % (name, args.steps, args.warmup))
         results[name] = run_form(amrhydro, torch, U_dev.data_ptr(), nb, n, forms[name], args.steps,
                                  args.warmup)
-        log("  %s: %.3e cell-updates/s, substep kernel %.3f ms, %.0f GB/s"
-            % (name, results[name]["cell_updates_per_s"], results[name]["substep_kernel_ms_avg"],
-               results[name]["achieved_GBs"]))
+        results[name]["roofline"] = roofline(results[name], "flat%d_%s" % (n, name))
+        log("  %s: %.3e cell-updates/s (%.3f ms per step), substep kernel %.3f ms, %.0f GB/s (%.1f%% of HBM); "
+            "launch %s" % (name, results[name]["cell_updates_per_s"], results[name]["ms_per_step"],
+                           results[name]["substep_kernel_ms_avg"], results[name]["achieved_GBs"],
+                           100 * results[name]["achieved_GBs"] / HBM_PEAK_GBS, results[name]["launch"]))
 
     acc = None
     if args.accuracy_steps > 0 and args.only == "both":
@@ -544,7 +657,8 @@ def main():
     if not args.no_extra:
         del U_dev
         torch.cuda.empty_cache()
-        extra = extra_configs(amrhydro, torch, max(3, args.steps // 4), 1)
+        extra = extra_configs(amrhydro, torch, max(3, args.steps // 4), 1, cpu=not args.no_cpu,
+                              accuracy_steps=args.accuracy_steps)
 
     cpu = None
     if not args.no_cpu:
@@ -564,7 +678,9 @@ def main():
 
     head = results.get("mixed") or results.get("fp64")
     head_name = "mixed" if "mixed" in results else "fp64"
-    achieved = head["achieved_GBs"]
+    wpe = {0: "default", 1: 2, 2: 3, 3: 4}[head["launch"]["variant"]]
+    roof = dict(head["roofline"])
+    roof["kernel"] = "kt_sweep_kernel<%d, MIXED=%s, substep, flat, waves/SIMD %s>" % (n, head_name == "mixed", wpe)
     out = {
         "metric": METRIC,
         "value": head["cell_updates_per_s"],
@@ -585,11 +701,8 @@ def main():
         "mixed_over_fp64": (results["mixed"]["cell_updates_per_s"] / results["fp64"]["cell_updates_per_s"]
                             if "mixed" in results and "fp64" in results else None),
         "accuracy_mixed_vs_fp64": acc,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic("flat%d_%s" % (n, head_name)),
-                     "kernel": "kt_sweep_kernel<%d,%s,0,false>" % (n, "true" if head_name == "mixed" else "false"),
-                     "bytes_per_launch": head["bytes_per_launch_avg"]},
+        "roofline": roof,
+        "kernel_build": kernel_build_id(),
         "cpu_baseline": cpu,
         "other_workloads": extra,
     }

@@ -146,10 +146,12 @@ int amrh_dt_device(amrh_ctx* ctx, double scale, double growth, double* dt_dev);
 /* Device error flags raised since the last call (AMRH_DERR_*); clears them.  Synchronous. */
 int amrh_error(amrh_ctx* ctx, int* flags);
 
-/* Bytes of HBM one substep over the batch moves by the algorithmic count (interior + face
- * ghost slabs read, interior written, U0 read when beta != 1; plus the face ghost slabs written
- * when boundary = AMRH_BOUNDARY_OUTFLOW). */
+/* Bytes of HBM one substep over the batch moves by the algorithmic count: the interior and its six
+ * face ghost slabs read, the interior written, U0's interior read when beta != 1; for the rotating
+ * state also the potential over the interior and its six face halos.  The next stage's ghost slabs
+ * an AMRH_BOUNDARY_OUTFLOW substep writes belong to the ghost refresh: amrh_ghost_write_bytes. */
 double amrh_substep_bytes(const amrh_ctx* ctx, int beta_is_one);
+double amrh_ghost_write_bytes(const amrh_ctx* ctx);
 
 /* ---- launch shape and whole steps (no counterpart in the reference: its loops are OpenMP) -----
  * The sweep kernels walk the batch with a grid stride.  amrh_autotune times one workgroup per
@@ -158,6 +160,14 @@ double amrh_substep_bytes(const amrh_ctx* ctx, int beta_is_one);
  * amrh_store).  amrh_set_launch sets them by hand (0 = one workgroup per block). */
 int amrh_autotune(amrh_ctx* ctx, int* grid_substep, int* grid_maxeig);
 int amrh_set_launch(amrh_ctx* ctx, int grid_substep, int grid_maxeig);
+/* The flat minmod substep of the ideal gas also comes in register-budget variants -- 0 the
+ * default, 1..3 the kernel held to 2, 3, 4 waves per SIMD (256, 168, 128 registers) -- and
+ * amrh_autotune times every variant with every grid.  amrh_launch_info returns the choice and the
+ * autotuner's substep times of the default launch and of the pick (ms; 0 before amrh_autotune);
+ * amrh_set_variant sets the variant by hand.  Results do not depend on the launch shape. */
+int amrh_launch_info(const amrh_ctx* ctx, int* grid_substep, int* grid_maxeig, int* variant, double* ms_default,
+                     double* ms_tuned);
+int amrh_set_variant(amrh_ctx* ctx, int variant);
 /* HydroDriver::step repeated nsteps times (euler/hydro_driver.cpp:171-194, regrid excluded) with
  * the reference's dt control on the device: dt_dev[0] holds the previous dt and becomes
  * min(MAXDTINC*dt, CFL dt).  For the octree (after amrh_tree_set) each step fills ghosts, scans

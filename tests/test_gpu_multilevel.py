@@ -73,6 +73,46 @@ def test_graph_and_persistent_launch_match_direct(precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n", [8, 16])
+def test_register_budget_variants_give_the_same_bits(precision, n):
+    """Every register-budget variant of the flat sweep (2, 3, 4 waves per SIMD) and the autotuner's
+    pick give the bits of the default launch; amrh_launch_info reports the pick"""
+    if n == 8:
+        U, h = small_multilevel()
+    else:
+        U = workloads.flat_batch(40, n=16, seed=4)
+        h = np.full(40, 1.0 / 16)
+    outs = []
+    for v in range(4):
+        B = amrhydro.Batch(amrhydro.euler_config(n=n, precision=precision,
+                                                 boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW), U.shape[0], h)
+        B.upload(U)
+        B.fill_outflow_ghosts()
+        B.set_variant(v)
+        dt = dt_buffer(1e-5)
+        B.run_steps(3, dt.data_ptr(), rk=3, use_graph=True)
+        B.sync()
+        outs.append(B.download())
+        if v == 0:
+            B2 = amrhydro.Batch(amrhydro.euler_config(n=n, precision=precision,
+                                                      boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW), U.shape[0], h)
+            B2.upload(U)
+            B2.fill_outflow_ghosts()
+            B2.autotune()
+            gs, gm, var, ms0, ms1 = B2.launch_info()
+            say("n=%d prec=%d: autotuned grid %d, variant %d, substep %.3f -> %.3f ms" % (n, precision, gs, var, ms0, ms1))
+            assert 0 <= var < 4 and ms1 <= ms0 and ms1 > 0.0
+            dt2 = dt_buffer(1e-5)
+            B2.run_steps(3, dt2.data_ptr(), rk=3, use_graph=True)
+            B2.sync()
+            np.testing.assert_array_equal(B2.download(), outs[0])
+            B2.close()
+        B.close()
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_tree_graph_matches_direct(precision):
     import tree_cases as tc
     cfg_o = orc.euler_cfg(n=8, g=2, grid_dim=0.5, z_reflect=0)
