@@ -13,8 +13,9 @@
 // zeroes D[pot] and D[T], adds the geometric source and forms U = (U + D dt) beta + U0 (1 - beta)
 // (grid/grid.cpp:406-420), floors it with the Newton solve for T (binary/state_ztwd.cpp:482-554) and
 // resets pot = rho (phi - R^2 Omega0^2 / 2) (binary/binary.cpp:825-849).
-// LDS: 13 x 896 staged primitives + 13 x 576 face fluxes + 576 face velocities: 154 KB in FP64,
-// 77 KB in the mixed form.
+// LDS (PPM, G = 3): FP64 stages one direction's 13 x 896 primitives beside 13 x 576 face fluxes
+// and 576 face velocities (154 KB); the mixed form stages the whole record, 13 x 1664 FP32
+// primitives, once per block beside two such face buffers in FP32 (148 KB).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -109,8 +110,8 @@ __device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int 
 // MIXED: the staged primitives, the reconstruction, the face states' EOS and fluxes and the KT
 // combination in FP32 (rot::sp); the flux differences, sources, RK update and floor in FP64.
 // TREE: the octree's reflux overrides, exports and the root's ledger (a.slot, a.exp_slot, a.ledger)
-// The mixed form stages the whole record (interior + 6 slabs, 119 KB of LDS with the face buffers)
-// once per block; the FP64 form stages one direction's pencils at a time (154 KB).
+// The mixed form stages the whole record (interior + 6 slabs) once per block and computes the faces
+// of all three directions as one list; the FP64 form stages one direction's pencils at a time.
 template <int G, int RECON, int EOS, int MODE, bool MIXED, bool TREE>
 __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   using T = typename std::conditional<MIXED, float, double>::type;
@@ -118,8 +119,9 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   constexpr int R = N3 + 6 * G * N2;   // record elements per field
   constexpr int RS = ALL ? R : N3 + 2 * G * N2;  // staged elements per field
   __shared__ T P[NF][RS];
-  __shared__ T Fb[MODE == 0 ? NF : 1][MODE == 0 ? NFACE : 1];
-  __shared__ T Vf[MODE == 0 ? NFACE : 1];
+  constexpr int NFB = MODE == 0 ? (ALL ? 2 : 1) : 1;  // face buffers (double-buffered in the pooled form)
+  __shared__ T Fb[NFB][MODE == 0 ? NF : 1][MODE == 0 ? NFACE : 1];
+  __shared__ T Vf[NFB][MODE == 0 ? NFACE : 1];
   __shared__ double red[N3 / 64];
   const sp::RotF cf = sp::make(a.c);
 
@@ -169,66 +171,65 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
     };
     if constexpr (ALL) stage(0);
 
-    auto sweep = [&](auto dc) {
+    // face q (face_order.h) of direction d: both sides reconstructed, converted back at the face
+    // position and given to the EOS; the KT flux and the mean state's face velocity go to Fb[fb] and
+    // Vf[fb] (MODE 0), the larger wavespeed to emax (MODE 1)
+    auto face = [&](auto dc, int q, int fb) {
       constexpr int d = decltype(dc)::value;
-      if constexpr (!ALL) stage(d);  // this direction's pencils
-#pragma unroll 1
-      for (int q = tid; q < NFACE; q += N3) {
-        int f, ta, tb;
-        face_decode<N, d>(q, f, ta, tb);
-        if constexpr (MIXED) {
-          float ql[NF], qr[NF];
-          if constexpr (RECON == AMRH_RECON_PPM) {
-            int e6[6];
+      int f, ta, tb;
+      face_decode<N, d>(q, f, ta, tb);
+      if constexpr (MIXED) {
+        float ql[NF], qr[NF];
+        if constexpr (RECON == AMRH_RECON_PPM) {
+          int e6[6];
 #pragma unroll
-            for (int m = 0; m < 6; m++) e6[m] = staged(d, f - 3 + m, ta, tb);
+          for (int m = 0; m < 6; m++) e6[m] = staged(d, f - 3 + m, ta, tb);
 #pragma unroll
-            for (int l = 0; l < NF; l++) {
-              float v[6];
+          for (int l = 0; l < NF; l++) {
+            float v[6];
 #pragma unroll
-              for (int m = 0; m < 6; m++) v[m] = (float)P[l][e6[m]];
-              f32::ppm_face(v, ql[l], qr[l], l == POT, l == POT);
-            }
-          } else {
-            int e4[4];
-#pragma unroll
-            for (int m = 0; m < 4; m++) e4[m] = staged(d, f - 2 + m, ta, tb);
-#pragma unroll
-            for (int l = 0; l < NF; l++) {
-              float v[4];
-#pragma unroll
-              for (int m = 0; m < 4; m++) v[m] = (float)P[l][e4[m]];
-              f32::minmod_face(v, ql[l], qr[l], (float)c.theta);
-            }
+            for (int m = 0; m < 6; m++) v[m] = (float)P[l][e6[m]];
+            f32::ppm_face(v, ql[l], qr[l], l == POT, l == POT);
           }
-          float X[3];
-          {
-            const int ii = d == 0 ? f : ta;
-            const int jj = d == 0 ? ta : (d == 1 ? f : tb);
-            const int kk = d == 2 ? f : tb;
-            double Xd[3];
-            cell_pos(x0, h, ii, jj, kk, Xd);
-            Xd[d] = x0[d] + (double)f * h;
-            X[0] = (float)Xd[0], X[1] = (float)Xd[1], X[2] = (float)Xd[2];
-          }
-          sp::to_con(cf, ql, X);
-          sp::to_con(cf, qr, X);
-          float fl[NF], fr[NF];
-          const float el = sp::flux<EOS, d>(H, cf, ql, X, fl);
-          const float er = sp::flux<EOS, d>(H, cf, qr, X, fr);
-          const float amax = fmaxf(el, er);
-          if constexpr (MODE == 0) {
-            float uf[NF];
+        } else {
+          int e4[4];
 #pragma unroll
-            for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5f;
-            Vf[q] = (T)sp::vel<d>(cf, uf, X, sp::Rc(X));
+          for (int m = 0; m < 4; m++) e4[m] = staged(d, f - 2 + m, ta, tb);
 #pragma unroll
-            for (int l = 0; l < NF; l++) Fb[l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5f);
-          } else {
-            emax = rmax(emax, (double)amax);
+          for (int l = 0; l < NF; l++) {
+            float v[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) v[m] = (float)P[l][e4[m]];
+            f32::minmod_face(v, ql[l], qr[l], (float)c.theta);
           }
-          continue;
         }
+        float X[3];
+        {
+          const int ii = d == 0 ? f : ta;
+          const int jj = d == 0 ? ta : (d == 1 ? f : tb);
+          const int kk = d == 2 ? f : tb;
+          double Xd[3];
+          cell_pos(x0, h, ii, jj, kk, Xd);
+          Xd[d] = x0[d] + (double)f * h;
+          X[0] = (float)Xd[0], X[1] = (float)Xd[1], X[2] = (float)Xd[2];
+        }
+        sp::to_con(cf, ql, X);
+        sp::to_con(cf, qr, X);
+        float fl[NF], fr[NF];
+        const float el = sp::flux<EOS, d>(H, cf, ql, X, fl);
+        const float er = sp::flux<EOS, d>(H, cf, qr, X, fr);
+        const float amax = fmaxf(el, er);
+        if constexpr (MODE == 0) {
+          float uf[NF];
+#pragma unroll
+          for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5f;
+          Vf[fb][q] = (T)sp::vel<d>(cf, uf, X, sp::Rc(X));
+#pragma unroll
+          for (int l = 0; l < NF; l++) Fb[fb][l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5f);
+        } else {
+          emax = rmax(emax, (double)amax);
+        }
+      } else {
         double ql[NF], qr[NF];
         if constexpr (RECON == AMRH_RECON_PPM) {
           int e6[6];
@@ -269,13 +270,13 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           double uf[NF];
 #pragma unroll
           for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5;
-          Vf[q] = (T)(d == 0 ? vx(c, uf, X) : (d == 1 ? vy(c, uf, X) : vz(uf)));
+          Vf[fb][q] = (T)(d == 0 ? vx(c, uf, X) : (d == 1 ? vy(c, uf, X) : vz(uf)));
           double fl[NF], fr[NF];
           const double el = flux<EOS, d>(H, c, ql, X, fl);
           const double er = flux<EOS, d>(H, c, qr, X, fr);
           const double amax = rmax(el, er);
 #pragma unroll
-          for (int l = 0; l < NF; l++) Fb[l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5);
+          for (int l = 0; l < NF; l++) Fb[fb][l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5);
         } else {
           double F[NF];
           const double el = flux<EOS, d>(H, c, ql, X, F);
@@ -283,73 +284,133 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           emax = rmax(emax, rmax(el, er));
         }
       }
-      __syncthreads();
-      if constexpr (MODE == 0) {
-        if constexpr (TREE) {
-          // adjust_*_flux (grid/grid_node.cpp:1076-1274): faces of the node's low, middle and high
-          // planes next to a finer region take the means its fine faces exported
+    };
+
+    // MODE 0, once every face of direction d is in Fb[fb]: the octree's reflux overrides, exports
+    // and ledger, then each cell subtracts its flux difference (D takes x, then y, then z:
+    // grid/grid.cpp:304-344) and keeps its two face velocities
+    auto finish = [&](auto dc, int fb) {
+      constexpr int d = decltype(dc)::value;
+      if constexpr (TREE) {
+        // adjust_*_flux (grid/grid_node.cpp:1076-1274): faces of the node's low, middle and high
+        // planes next to a finer region take the means its fine faces exported
 #pragma unroll 1
-          for (int q2 = tid; q2 < 3 * N2; q2 += N3) {
-            const int pl = q2 / N2, ta = q2 % N, tb = (q2 / N) % N;
-            const int qa = ta >= N / 2, qb = tb >= N / 2;
-            const int sl = a.slot[((b * 3 + d) * 3 + pl) * 4 + qa + 2 * qb];
-            if (sl < 0) continue;
-            const int f = pl == 0 ? 0 : (pl == 1 ? N / 2 : N);
-            const double* o = a.ovr + (((long)sl * (N / 2) + (tb - qb * (N / 2))) * (N / 2) + (ta - qa * (N / 2))) * NF;
-            const int q = face_index<N, d>(f, ta, tb);
+        for (int q2 = tid; q2 < 3 * N2; q2 += N3) {
+          const int pl = q2 / N2, ta = q2 % N, tb = (q2 / N) % N;
+          const int qa = ta >= N / 2, qb = tb >= N / 2;
+          const int sl = a.slot[((b * 3 + d) * 3 + pl) * 4 + qa + 2 * qb];
+          if (sl < 0) continue;
+          const int f = pl == 0 ? 0 : (pl == 1 ? N / 2 : N);
+          const double* o = a.ovr + (((long)sl * (N / 2) + (tb - qb * (N / 2))) * (N / 2) + (ta - qa * (N / 2))) * NF;
+          const int q = face_index<N, d>(f, ta, tb);
 #pragma unroll
-            for (int l = 0; l < NF; l++) Fb[l][q] = (T)o[l];
-          }
-          __syncthreads();
-          // this node's boundary faces, 2x2 means, to the coarse node's slot
-          if (tid < 4 * (N / 2) * (N / 2)) {
-            const int w = tid / ((N / 2) * (N / 2)), r = tid % ((N / 2) * (N / 2));
-            const int side = w >> 1;
-            const int sl = a.exp_slot[(b * 6 + 2 * d + side) * 2 + (w & 1)];
-            if (sl >= 0) {
-              const int al = r % (N / 2), bl = r / (N / 2);
-              const int fp = side ? N : 0;
-              const int q00 = face_index<N, d>(fp, 2 * al, 2 * bl), q10 = face_index<N, d>(fp, 2 * al + 1, 2 * bl);
-              const int q01 = face_index<N, d>(fp, 2 * al, 2 * bl + 1), q11 = face_index<N, d>(fp, 2 * al + 1, 2 * bl + 1);
-              double* o = a.ovr + (((long)sl * (N / 2) + bl) * (N / 2) + al) * NF;
+          for (int l = 0; l < NF; l++) Fb[fb][l][q] = (T)o[l];
+        }
+        __syncthreads();
+        // this node's boundary faces, 2x2 means, to the coarse node's slot
+        if (tid < 4 * (N / 2) * (N / 2)) {
+          const int w = tid / ((N / 2) * (N / 2)), r = tid % ((N / 2) * (N / 2));
+          const int side = w >> 1;
+          const int sl = a.exp_slot[(b * 6 + 2 * d + side) * 2 + (w & 1)];
+          if (sl >= 0) {
+            const int al = r % (N / 2), bl = r / (N / 2);
+            const int fp = side ? N : 0;
+            const int q00 = face_index<N, d>(fp, 2 * al, 2 * bl), q10 = face_index<N, d>(fp, 2 * al + 1, 2 * bl);
+            const int q01 = face_index<N, d>(fp, 2 * al, 2 * bl + 1), q11 = face_index<N, d>(fp, 2 * al + 1, 2 * bl + 1);
+            double* o = a.ovr + (((long)sl * (N / 2) + bl) * (N / 2) + al) * NF;
 #pragma unroll
-              for (int l = 0; l < NF; l++) {
-                double v = (double)Fb[l][q00];
-                v += (double)Fb[l][q10];
-                v += (double)Fb[l][q01];
-                v += (double)Fb[l][q11];
-                v *= 0.25;
-                o[l] = v;
-              }
-            }
-          }
-          // the root's boundary faces for the flow-off ledger (grid/grid.cpp:350-404)
-          if (b == 0 && a.ledger) {
-#pragma unroll 1
-            for (int q2 = tid; q2 < 2 * N2; q2 += N3) {
-              const int side = q2 / N2, ta = q2 % N, tb = (q2 / N) % N;
-              const int q = face_index<N, d>(side ? N : 0, ta, tb);
-              double* o = a.ledger + ((((long)d * 2 + side) * N + tb) * N + ta) * NF;
-#pragma unroll
-              for (int l = 0; l < NF; l++) o[l] = (double)Fb[l][q];
+            for (int l = 0; l < NF; l++) {
+              double v = (double)Fb[fb][l][q00];
+              v += (double)Fb[fb][l][q10];
+              v += (double)Fb[fb][l][q01];
+              v += (double)Fb[fb][l][q11];
+              v *= 0.25;
+              o[l] = v;
             }
           }
         }
-        const int p = d == 0 ? ci : (d == 1 ? cj : ck);
-        const int ta = d == 0 ? cj : ci;
-        const int tb = d == 2 ? cj : ck;
-        const int q0 = face_index<N, d>(p, ta, tb), q1 = face_index<N, d>(p + 1, ta, tb);
+        // the root's boundary faces for the flow-off ledger (grid/grid.cpp:350-404)
+        if (b == 0 && a.ledger) {
+#pragma unroll 1
+          for (int q2 = tid; q2 < 2 * N2; q2 += N3) {
+            const int side = q2 / N2, ta = q2 % N, tb = (q2 / N) % N;
+            const int q = face_index<N, d>(side ? N : 0, ta, tb);
+            double* o = a.ledger + ((((long)d * 2 + side) * N + tb) * N + ta) * NF;
 #pragma unroll
-        // (with FP32 fluxes the FP64 difference is exact: the divergence still telescopes)
-        for (int l = 0; l < NF; l++) D[l] -= ((double)Fb[l][q1] - (double)Fb[l][q0]) * hinv;
-        vlo[d] = (double)Vf[q0];
-        vhi[d] = (double)Vf[q1];
-        __syncthreads();  // P, Fb and Vf are restaged for the next direction
+            for (int l = 0; l < NF; l++) o[l] = (double)Fb[fb][l][q];
+          }
+        }
       }
+      const int p = d == 0 ? ci : (d == 1 ? cj : ck);
+      const int ta = d == 0 ? cj : ci;
+      const int tb = d == 2 ? cj : ck;
+      const int q0 = face_index<N, d>(p, ta, tb), q1 = face_index<N, d>(p + 1, ta, tb);
+      // (with FP32 fluxes the FP64 difference is exact: the divergence still telescopes)
+#pragma unroll
+      for (int l = 0; l < NF; l++) D[l] -= ((double)Fb[fb][l][q1] - (double)Fb[fb][l][q0]) * hinv;
+      vlo[d] = (double)Vf[fb][q0];
+      vhi[d] = (double)Vf[fb][q1];
     };
-    sweep(std::integral_constant<int, 0>{});
-    sweep(std::integral_constant<int, 1>{});
-    sweep(std::integral_constant<int, 2>{});
+
+    using X0 = std::integral_constant<int, 0>;
+    using X1 = std::integral_constant<int, 1>;
+    using X2 = std::integral_constant<int, 2>;
+    if constexpr (!ALL) {
+      // one direction's pencils staged at a time
+      auto sweep = [&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        stage(d);
+#pragma unroll 1
+        for (int q = tid; q < NFACE; q += N3) face(dc, q, 0);
+        __syncthreads();
+        if constexpr (MODE == 0) {
+          finish(dc, 0);
+          __syncthreads();  // P, Fb and Vf are restaged for the next direction
+        }
+      };
+      sweep(X0{});
+      sweep(X1{});
+      sweep(X2{});
+    } else {
+      // The whole record is staged, so the faces of the three directions form one list of
+      // 3 NFACE, handed out in rounds of N3 threads: four rounds (the last one 3/8 full) instead of
+      // two per direction with the second one 1/8 full.  NFACE is a multiple of the wavefront, so
+      // every wavefront's faces share a direction.  Fb and Vf are double-buffered (x -> 0, y -> 1,
+      // z -> 0): the z faces of round 2 are written after x is finished, those of round 3 while y is.
+      static_assert(NFACE % 64 == 0 && 2 * N3 >= NFACE && 2 * N3 <= 2 * NFACE && 3 * N3 >= 2 * NFACE &&
+                        3 * N3 <= 3 * NFACE && 4 * N3 >= 3 * NFACE,
+                    "the round schedule below assumes N = 8");
+      auto face_any = [&](int g) {
+        if (g < NFACE)
+          face(X0{}, g, 0);
+        else if (g < 2 * NFACE)
+          face(X1{}, g - NFACE, 1);
+        else
+          face(X2{}, g - 2 * NFACE, 0);
+      };
+      if constexpr (MODE == 0) {
+        // round 0: x; 1: x, then y; 2: y, then z; 3: the rest of z (one call site keeps the face
+        // code inlined once per direction)
+#pragma unroll 1
+        for (int r = 0; r < 4; r++) {
+          const int g = tid + r * N3;
+          if (g < 3 * NFACE) face_any(g);
+          if (r == 1) {
+            __syncthreads();
+            finish(X0{}, 0);
+            __syncthreads();  // Fb[0] takes z faces next
+          } else if (r == 2) {
+            __syncthreads();
+            finish(X1{}, 1);
+          }
+        }
+        __syncthreads();
+        finish(X2{}, 0);
+      } else {
+#pragma unroll 1
+        for (int g = tid; g < 3 * NFACE; g += N3) face_any(g);
+      }
+    }
 
     if constexpr (MODE == 0) {
       const long e = (long)tid;
