@@ -64,8 +64,6 @@ struct PArgs {
 struct LoopState {
   double resid0, resid;
   int iters, done;
-  unsigned counter;
-  int pad;
 };
 
 __device__ __forceinline__ long cell(int P, int i, int j, int k) { return ((long)k * P + j) * P + i; }
@@ -329,6 +327,30 @@ __global__ __launch_bounds__(NTHREADS) void down_kernel(PArgs A) {
   }
 }
 
+// v of the lane DPP control CTRL names (0 where it names none), for the two halves of a double
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Sum over the wavefront, the same on every lane: pairs, quads, half rows and rows of 16 lanes
+// through DPP (VALU moves, no LDS traffic), then the four row sums
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp64<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp64<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp64<0x141>(v);  // row_half_mirror
+  v += dpp64<0x140>(v);  // row_mirror
+  return ((lane64(v, 0) + lane64(v, 16)) + lane64(v, 32)) + lane64(v, 48);
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -366,8 +388,10 @@ __global__ __launch_bounds__(NTHREADS) void export_kernel(PArgs A) {
 // Step k of the loop for the nodes of one level: enforce_dphi_boundaries (neighbours' layers from
 // step k-1's copy, the parent's dphi, the mirror, or 0) and vcycle_relax (poisson/poisson.cpp:
 // 88-130): Jacobi sweeps of dphi += dphi1 + div(dphi)/6 over the interior until the sum of squared
-// updates falls to half of the first sweep's, at most 16.  The last workgroup to finish sums the
-// level's residuals and decides whether the loop goes on; once it stops, later steps return at once.
+// updates falls to half of the first sweep's, at most 16.  Each node's residual goes to step k's
+// copy of the per-node residuals (two copies, by step parity); the launch of step k+1 sums them
+// and decides whether the loop goes on, so no workgroup waits for or fences against the others;
+// once the loop has stopped, later launches return at once.
 template <int N>
 __global__ __launch_bounds__(NTHREADS) void relax_step_kernel(PArgs A, int k, double limit, int count) {
   constexpr int P = N + 2, M = N, CELLS = M * M * M, PER = (CELLS + NTHREADS - 1) / NTHREADS;
@@ -377,8 +401,27 @@ __global__ __launch_bounds__(NTHREADS) void relax_step_kernel(PArgs A, int k, do
   constexpr bool TWO = 2 * P3 * 8 <= 65536;
   __shared__ double L[TWO ? 2 * P3 : P3];
   __shared__ double red[2][NTHREADS / 64];
-  __shared__ int last;
   if (A.st->done) return;
+  if (k > 0) {
+    // the decision on step k-1, taken by every workgroup from the level's residuals that step
+    // wrote (visible here without fences: they come from the previous launch), summed in the same
+    // order everywhere
+    const double* prev = A.part + (long)((k - 1) & 1) * A.nn;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < count; q += NTHREADS) s += prev[A.list[q]];
+    const double total = block_sum(s, red[0]);
+    LoopState* st = A.st;
+    if (k == 1) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->resid0 = total;
+    } else if (!(total / st->resid0 > limit)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->resid = total;
+        st->iters = k;
+        st->done = 1;
+      }
+      return;
+    }
+  }
   const int b = A.list[blockIdx.x];
   const PNodeDev& nd = A.nd[b];
   const long base = (long)b * A.P3;
@@ -442,7 +485,7 @@ __global__ __launch_bounds__(NTHREADS) void relax_step_kernel(PArgs A, int k, do
         part += tmp[c] * tmp[c];
       }
     }
-    for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+    part = wave_sum(part);
     if (lane == 0) red[iters & 1][wv] = part;
     __syncthreads();
     resid = 0.0;
@@ -460,29 +503,7 @@ __global__ __launch_bounds__(NTHREADS) void relax_step_kernel(PArgs A, int k, do
     const int f = t / (M * M), r = t % (M * M), a = r % M + 1, c = r / M + 1;
     A.exps[exp_at(A, k & 1, b, f, a, c)] = X[layer_cell(P, f, a, c, (f & 1) ? M : 1)];
   }
-  if (threadIdx.x == 0) {
-    A.part[b] = resid;
-    __threadfence();
-    last = atomicAdd(&A.st->counter, 1u) == (unsigned)(count - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double s = 0.0;
-  for (int q = threadIdx.x; q < count; q += NTHREADS)
-    s += __hip_atomic_load(&A.part[A.list[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const double total = block_sum(s, red[0]);
-  if (threadIdx.x == 0) {
-    LoopState* st = A.st;
-    if (st->iters == 0)
-      st->resid0 = total;
-    else if (!(total / st->resid0 > limit))
-      st->done = 1;
-    st->resid = total;
-    st->iters++;
-    st->counter = 0;
-    __threadfence();
-  }
+  if (threadIdx.x == 0) A.part[(long)(k & 1) * A.nn + b] = resid;
 }
 
 // vcycle_coarse_correction for the nodes of one level (poisson/poisson.cpp:14-55)
@@ -860,7 +881,8 @@ int relax_loop(amrh_poisson* p, int l, int sweep, double limit) {
   export_kernel<<<p->lv_count[l], NTHREADS, 0, p->stream>>>(p->args(p->lv_start[l]));
   PCHK(hipGetLastError());
   int& last = p->steps_last[sweep * (p->maxlevel + 1) + l];
-  int k = 0, chunk = std::max(2, last);  // the previous count: a loop's length varies little
+  // the previous count (a loop's length varies little), plus the launch that sees the loop end
+  int k = 0, chunk = std::max(2, last + 1);
   for (;;) {
     for (int q = 0; q < chunk; q++) PCHK(launch_step(p, l, k++, limit));
     PCHK(hipMemcpyAsync(p->st_host, p->st_dev, sizeof(LoopState), hipMemcpyDeviceToHost, p->stream));
@@ -1045,7 +1067,7 @@ int amrh_poisson_create(int n, int bw, double grid_dim, int z_reflect, int nnode
   alloc((void**)&p->tab_dev, sizeof(double) * p->slots.size() * p->P * p->P);
   alloc((void**)&p->src_dev, sizeof(double) * 32 * nnodes);
   alloc((void**)&p->allref_dev, sizeof(int) * nnodes);
-  alloc((void**)&p->part_dev, sizeof(double) * nnodes);
+  alloc((void**)&p->part_dev, sizeof(double) * 2 * nnodes);  // two copies for the relaxation loops
   alloc((void**)&p->exps_dev, sizeof(double) * 2 * nnodes * 6 * n * n);
   alloc((void**)&p->st_dev, sizeof(LoopState));
   if (e == hipSuccess) e = hipHostMalloc((void**)&p->st_host, sizeof(LoopState), hipHostMallocDefault);
