@@ -100,6 +100,8 @@ void drop_graphs(amrh_ctx* c);
 // the low interior face of every node of the tree, Grid::xf(bw) with the context's origin, to the
 // device (amrh_tree.hip)
 int tree_upload_x0(amrh_ctx* c);
+// amrh_tree_substep_dev; fill = false skips the ghost fill (the caller filled the current state)
+int tree_substep_dev(amrh_ctx* c, const double* dt_dev, double beta, bool fill);
 // state_sum / max / min reductions (amrh_diag.hip): the checks, the size of the per-block partials,
 // and the two passes (no allocation; asynchronous on the context's stream)
 int diag_check(const amrh_ctx* c);

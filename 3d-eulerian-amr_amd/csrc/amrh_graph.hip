@@ -57,7 +57,8 @@ int step_once(amrh_ctx* c, int rk, double* dt_dev) {
   if (int rc = amrh_store(c)) return rc;
   for (int s = 0; s < rk; s++) {
     const double beta = kBetas[rk - 1][s];
-    const int rc = c->tree ? amrh_tree_substep_dev(c, dt_dev, beta) : amrh_substep_dev(c, dt_dev, beta);
+    // the first substep's ghosts are the step's own (the CFL scan and store change no cell)
+    const int rc = c->tree ? tree_substep_dev(c, dt_dev, beta, s > 0) : amrh_substep_dev(c, dt_dev, beta);
     if (rc) return rc;
     // the rotating state's substep leaves the restriction to the driver, after the potential
     if (c->tree && c->cfg.physics == AMRH_PHYS_ROTATING)

@@ -223,7 +223,9 @@ int build(amrh_ctx* c, const int* parent, const int* child) {
   return AMRH_OK;
 }
 
-int fill_ghosts(amrh_ctx* c) {
+// partial: the rotating state's leaves keep their same-level-neighbour faces unfilled (its octree
+// substep reads those cells from the neighbours); every other ghost is filled
+int fill_ghosts(amrh_ctx* c, bool partial = false) {
   TreeState* t = c->tree;
   const bool rot = c->cfg.physics == AMRH_PHYS_ROTATING;
   const rot::RotParams rp = rot ? ctx_rot_args(c).c : rot::RotParams{};
@@ -231,7 +233,8 @@ int fill_ghosts(amrh_ctx* c) {
     const int* nodes = t->level_nodes_dev + t->level_start[l];
     if (rot)
       HIPCHK(launch_rot_tree_ghosts(c->cfg.g, c->buf[c->cur], nodes, t->level_count[l], t->ghost_dev, t->rel_dev,
-                                    c->x0_dev, c->h_dev, c->cfg.z_reflect, rp, c->err_dev, c->stream));
+                                    c->x0_dev, c->h_dev, c->cfg.z_reflect, rp, c->err_dev, c->stream,
+                                    partial ? t->covered_dev : nullptr));
     else
       HIPCHK(launch_tree_ghosts(c->cfg.n, c->cfg.g, c->buf[c->cur], nodes, t->level_count[l], t->ghost_dev, t->rel_dev,
                                 c->cfg.z_reflect, c->err_dev, c->stream));
@@ -260,11 +263,12 @@ int inject(amrh_ctx* c) {
 // then the rotating substep level by level, finest first, each level applying the reflux means its
 // finer neighbours exported and exporting its own boundary faces' means to its coarser neighbours;
 // the root's ledger; no restriction (the driver restricts after the potential is in place)
-int rot_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
+int rot_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool fill) {
   TreeState* t = c->tree;
   if (int rc = ctx_ready(c)) return rc;
   if (beta != 1.0 && c->u0buf < 0) return fail(AMRH_ESTATE, "beta != 1 needs amrh_store() first");
-  if (int rc = fill_ghosts(c)) return rc;
+  if (fill)
+    if (int rc = fill_ghosts(c, true)) return rc;
   RotArgs a = ctx_rot_args(c);
   a.Uin = c->buf[c->cur];
   int out = c->cur;
@@ -278,6 +282,7 @@ int rot_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) 
   a.ovr = t->ovr_dev;
   a.exp_slot = t->exp_dev;
   a.ledger = t->ledger_dev;
+  a.ghost = t->ghost_dev;
   const bool mixed = c->cfg.precision == AMRH_MIXED;
   for (int l = t->maxlevel; l >= 0; l--) {
     a.nodes = t->level_nodes_dev + t->level_start[l];
@@ -292,10 +297,11 @@ int rot_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) 
 
 // The ideal gas with PPM: the block kernel level by level, finest first, applying and exporting the
 // reflux means as the rotating kernel does; the ledger; then the restriction
-int block_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
+int block_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool fill) {
   TreeState* t = c->tree;
   if (beta != 1.0 && c->u0buf < 0) return fail(AMRH_ESTATE, "beta != 1 needs amrh_store() first");
-  if (int rc = fill_ghosts(c)) return rc;
+  if (fill)
+    if (int rc = fill_ghosts(c)) return rc;
   SubstepArgs a{};
   a.Uin = c->buf[c->cur];
   int out = c->cur;
@@ -327,11 +333,13 @@ int block_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev
   return inject(c);
 }
 
-int tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev) {
+// fill = false: the caller filled the ghosts of the current state itself (and changed no cell since)
+int tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev, bool fill) {
   TreeState* t = c->tree;
-  if (c->cfg.physics == AMRH_PHYS_ROTATING) return rot_tree_substep(c, dt, beta, dt_dev);
-  if (c->cfg.recon == AMRH_RECON_PPM) return block_tree_substep(c, dt, beta, dt_dev);
-  if (int rc = fill_ghosts(c)) return rc;
+  if (c->cfg.physics == AMRH_PHYS_ROTATING) return rot_tree_substep(c, dt, beta, dt_dev, fill);
+  if (c->cfg.recon == AMRH_RECON_PPM) return block_tree_substep(c, dt, beta, dt_dev, fill);
+  if (fill)
+    if (int rc = fill_ghosts(c)) return rc;
   const PhysParams P = params(c->cfg);
   const bool mixed = c->cfg.precision == AMRH_MIXED;
   for (int l = t->maxlevel; l >= 1; l--)
@@ -364,6 +372,11 @@ double max_dt_rec(const TreeState* t, const std::vector<double>& eig, double cfl
 }
 
 }  // namespace
+
+int amrh::tree_substep_dev(amrh_ctx* c, const double* dt_dev, double beta, bool fill) {
+  if (!c->tree || !c->have_u) return fail(AMRH_ESTATE, "tree or state not set");
+  return tree_substep(c, 0.0, beta, dt_dev, fill);
+}
 
 // The low interior face of every node, Grid::xf(bw) (grid/grid.cpp:512-526):
 //   (offset + bw) h - GRID_DIM - bw h0 - origin   in x and y;
@@ -441,13 +454,13 @@ int amrh_tree_inject(amrh_ctx* c) {
 int amrh_tree_substep(amrh_ctx* c, double dt, double beta) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->tree || !c->have_u) return fail(AMRH_ESTATE, "tree or state not set");
-  return tree_substep(c, dt, beta, nullptr);
+  return tree_substep(c, dt, beta, nullptr, true);
 }
 
 int amrh_tree_substep_dev(amrh_ctx* c, const double* dt_dev, double beta) {
   if (!c || !dt_dev) return fail(AMRH_EINVAL, "null argument");
   if (!c->tree || !c->have_u) return fail(AMRH_ESTATE, "tree or state not set");
-  return tree_substep(c, 0.0, beta, dt_dev);
+  return tree_substep(c, 0.0, beta, dt_dev, true);
 }
 
 int amrh_tree_max_dt(amrh_ctx* c, double* dt) {

@@ -32,16 +32,21 @@ struct RotArgs {
   double* ovr;          // override face fluxes [slot][n/2][n/2][13]: read, and written by exports
   const int* exp_slot;  // [node][face][2]: the slots this node's boundary-face 2x2 means go to, or -1
   double* ledger;       // the root's boundary face fluxes [dir][low, high][n][n][13] (nullptr: none)
+  // octree substep: per [node][face] (type, source) of the ghost exchange (amr_kernels.h); a face
+  // with a same-level neighbour is read from the neighbour's cells (nullptr: every ghost from the
+  // record's slabs)
+  const int* ghost;
 };
 
 hipError_t launch_rot_substep(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
 hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
 hipError_t launch_rot_ghosts(int g, double* U, const double* x0, const double* h, int nblocks, int z_reflect,
                              const rot::RotParams& c, hipStream_t s);
-// the octree's ghost fill of one level (ghost/rel: the tables of amr_kernels.h)
+// the octree's ghost fill of one level (ghost/rel: the tables of amr_kernels.h); covered (per node,
+// bit c for child c) != nullptr skips the same-level-neighbour faces of leaves
 hipError_t launch_rot_tree_ghosts(int g, double* U, const int* nodes, int count, const int* ghost, const int* rel,
                                   const double* x0, const double* h, int z_reflect, const rot::RotParams& c, int* err,
-                                  hipStream_t s);
+                                  hipStream_t s, const unsigned char* covered = nullptr);
 // inject_from_children for one parent level: pairs (parent, child << 24 | child node)
 hipError_t launch_rot_restrict(int g, int eos, double* U, const int2* pairs, int count, const double* x0,
                                const double* h, const rot::RotParams& c, const helm::HelmDev& H, hipStream_t s);

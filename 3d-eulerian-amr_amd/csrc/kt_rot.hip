@@ -156,11 +156,24 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
 #pragma unroll 1
       for (int r = tid; r < RS; r += N3) {
         const int e = ALL ? r : region_to_elem<G>(d, r);
-        double u[NF];
-#pragma unroll
-        for (int l = 0; l < NF; l++) u[l] = ub.ld(l, (unsigned)e);
         int i, j, k;
         elem_ijk<G>(e, i, j, k);
+        double u[NF];
+        // octree: a ghost cell across a face with a same-level neighbour is that neighbour's cell
+        // (what the ghost exchange would copy there), read from it directly; the slabs are
+        // wavefront-aligned (G N^2 is a multiple of 64), so the branch is uniform
+        const int* gt = (TREE && a.ghost && e >= N3) ? a.ghost + (b * 6 + (e - N3) / (G * N2)) * 2 : nullptr;
+        if (gt && gt[0] == GHOST_SIB) {
+          const int nf = (e - N3) / (G * N2);
+          int s3[3] = {i, j, k};
+          s3[nf >> 1] += (nf & 1) ? -N : N;
+          const double* Us = a.Uin + (long)gt[1] * NF * R + (s3[2] * N + s3[1]) * N + s3[0];
+#pragma unroll
+          for (int l = 0; l < NF; l++) u[l] = Us[(long)l * R];
+        } else {
+#pragma unroll
+          for (int l = 0; l < NF; l++) u[l] = ub.ld(l, (unsigned)e);
+        }
         double X[3];
         cell_pos(x0, h, i, j, k, X);
         to_prim(c, u, X);
@@ -582,7 +595,8 @@ __global__ __launch_bounds__(256) void rot_ghost_kernel(double* U, const double*
 template <int G>
 __global__ __launch_bounds__(256) void rot_tree_ghost_kernel(double* U, const int* nodes, int count, const int* ghost,
                                                              const int* rel, const double* x0s, const double* hs,
-                                                             int z_reflect, RotParams c, int* err) {
+                                                             int z_reflect, RotParams c, int* err,
+                                                             const unsigned char* covered) {
   constexpr int S = G * N2, R = N3 + 6 * S;
   constexpr Layout L{N, G, NF};
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -594,6 +608,9 @@ __global__ __launch_bounds__(256) void rot_tree_ghost_kernel(double* U, const in
   const int nf = (e - N3) / S;
   const int d = nf >> 1, up = nf & 1;
   const int type = ghost[(b * 6 + nf) * 2], src = ghost[(b * 6 + nf) * 2 + 1];
+  // covered != nullptr: the sibling faces of leaves are left to the substep, which reads them
+  // from the sibling; a refined node's are filled (its children's coarse ghosts read them)
+  if (covered && type == GHOST_SIB && covered[b] == 0) return;
   double* Ub = U + (long)b * NF * R;
   double u[NF];
   if (type == GHOST_SIB) {
@@ -808,14 +825,14 @@ hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArg
 
 hipError_t launch_rot_tree_ghosts(int g, double* U, const int* nodes, int count, const int* ghost, const int* rel,
                                   const double* x0, const double* h, int z_reflect, const RotParams& c, int* err,
-                                  hipStream_t s) {
+                                  hipStream_t s, const unsigned char* covered) {
   const long total = (long)count * 6 * g * N2;
   if (total == 0) return hipSuccess;
   const unsigned grid = (unsigned)((total + 255) / 256);
   if (g == 3)
-    rot_tree_ghost_kernel<3><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, x0, h, z_reflect, c, err);
+    rot_tree_ghost_kernel<3><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, x0, h, z_reflect, c, err, covered);
   else if (g == 2)
-    rot_tree_ghost_kernel<2><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, x0, h, z_reflect, c, err);
+    rot_tree_ghost_kernel<2><<<grid, 256, 0, s>>>(U, nodes, count, ghost, rel, x0, h, z_reflect, c, err, covered);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
