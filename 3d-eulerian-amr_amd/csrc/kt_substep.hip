@@ -148,7 +148,7 @@ __device__ __forceinline__ void put_ghost(const RecBuf<(long)N * N * N + 6L * GW
   else
     e = L.slab(face) + ((long)t * N + j) * N + i;
 #pragma unroll
-  for (int l = 0; l < NFE; l++) Uo.st(l, (unsigned)e, g[l]);
+  for (int l = 0; l < NFE; l++) Uo.st_nt(l, (unsigned)e, g[l]);
 }
 
 // MODE 0: substep (flux + divergence + update).  MODE 1: max wavespeed scan only.
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     // X slabs: (kk*N + jj)*G + t = kk*G*N + r;  Y slabs: (kk*G + t)*N + ii = kk*G*N + r
     const long e = L.slab(face) + (long)kk * GW * N + r;
 #pragma unroll
-    for (int l = 0; l < NFE; l++) uo.st(l, (unsigned)e, g[l]);
+    for (int l = 0; l < NFE; l++) uo.st_nt(l, (unsigned)e, g[l]);
   };
   const double hinv = MODE == 0 ? 1.0 / a.h[b] : 0.0;
 
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     }
     hbase = (unsigned)(L.slab(face) + r);  // plane k adds k*HN (both slab kinds are plane-major)
 #pragma unroll
-    for (int l = 0; l < NFE; l++) hraw[l] = ub.ld(l, hbase);
+    for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, hbase);
   }
 
   for (int k = 0; k < N; k++) {
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     // prefetch: halo of plane k+1, column cell k+3, U0 at k
     if (is_halo && k + 1 < N) {
 #pragma unroll
-      for (int l = 0; l < NFE; l++) hraw[l] = ub.ld(l, hbase + (unsigned)((k + 1) * HN));
+      for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, hbase + (unsigned)((k + 1) * HN));
     }
     double craw[NFE];
     const bool have_next = (k + 3) < N + GW;
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     if (MODE == 0 && has_u0) {
       const long e = L.interior(i, j, k);
 #pragma unroll
-      for (int l = 0; l < NFE; l++) u0v[l] = u0.ld(l, (unsigned)e);
+      for (int l = 0; l < NFE; l++) u0v[l] = u0.ld_nt(l, (unsigned)e);
     }
     __syncthreads();
 
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
       }
       const long e = L.interior(i, j, k);
 #pragma unroll
-      for (int l = 0; l < NFE; l++) uo.st(l, (unsigned)e, u[l]);
+      for (int l = 0; l < NFE; l++) uo.st_nt(l, (unsigned)e, u[l]);
       if (a.outflow) {
         // The next stage's ghosts of an isolated outflow block are functions of its own boundary
         // cells (grid/grid_phys_bound.cpp:27-54), so they are written here instead of in a separate

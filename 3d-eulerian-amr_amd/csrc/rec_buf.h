@@ -22,6 +22,15 @@ struct F64Buf {
   __device__ void st(unsigned e, double v, long off = 0) const {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, e * 8u, (int)(off * 8), 0);
   }
+  // streaming forms: the non-temporal cache policy (data read or written once), so that they do
+  // not push data that is read again out of the L2
+  __device__ double ld_nt(unsigned e, long off = 0) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, e * 8u, (int)(off * 8), kNT));
+  }
+  __device__ void st_nt(unsigned e, double v, long off = 0) const {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, e * 8u, (int)(off * 8), kNT);
+  }
+  static constexpr int kNT = 2;  // the cache-policy operand's NT bit on gfx950
 };
 
 // a block record: NF fields of R elements each
@@ -31,6 +40,8 @@ struct RecBuf : F64Buf {
   __device__ explicit RecBuf(const double* blk) : F64Buf(blk, NF * R) {}
   __device__ double ld(int l, unsigned e) const { return F64Buf::ld(e, l * R); }
   __device__ void st(int l, unsigned e, double v) const { F64Buf::st(e, v, l * R); }
+  __device__ double ld_nt(int l, unsigned e) const { return F64Buf::ld_nt(e, l * R); }
+  __device__ void st_nt(int l, unsigned e, double v) const { F64Buf::st_nt(e, v, l * R); }
 };
 
 }  // namespace amrh
