@@ -109,6 +109,7 @@ SIGNATURES = {
     "amrh_eos_set_units": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double]),
     "amrh_eos_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amrh_eos_compute_T": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "amrh_eos_eval_f32": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "amrh_eos_error": (C.c_int, [C.c_void_p, IP]),
     "amrh_attach_eos": (C.c_int, [C.c_void_p, C.c_void_p]),
     "amrh_set_block_origin": (C.c_int, [C.c_void_p, DP]),
@@ -514,6 +515,10 @@ class HelmholtzEos:
 
     def compute_T(self, n, st_ptr, stream=0):
         _ck(lib().amrh_eos_compute_T(self.h, int(n), C.c_void_p(st_ptr), C.c_void_p(stream)))
+
+    def eval_f32(self, n, st_ptr, stream=0):
+        """p and cs of the mixed form's FP32 face EOS, in place."""
+        _ck(lib().amrh_eos_eval_f32(self.h, int(n), C.c_void_p(st_ptr), C.c_void_p(stream)))
 
     def error_flags(self):
         f = C.c_int()

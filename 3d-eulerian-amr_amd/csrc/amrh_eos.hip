@@ -43,23 +43,30 @@ void set_factors(amrh_eos* E) {
   E->H.f_dsdt = erg / K / K / g;
 }
 
-__global__ void eos_kernel(helm::HelmDev H, helm::Eos* st, int n, int solve_T) {
+// mode 0: helmholtz_eos; 1: helmholtz_compute_T; 2: the mixed form's FP32 face EOS (p and cs only)
+__global__ void eos_kernel(helm::HelmDev H, helm::Eos* st, int n, int mode) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   helm::Eos q = st[i];
-  if (solve_T)
+  if (mode == 1) {
     helm::compute_T(H, q);
-  else
+  } else if (mode == 2) {
+    float p, cs;
+    helm::p_cs32(H, (float)q.rho, (float)q.T, (float)q.abar, (float)q.zbar, p, cs);
+    q.p = p;
+    q.cs = cs;
+  } else {
     helm::eos(H, q);
+  }
   st[i] = q;
 }
 
-int eos_batch(amrh_eos* E, int n, void* st_dev, void* stream, int solve_T) {
+int eos_batch(amrh_eos* E, int n, void* st_dev, void* stream, int mode) {
   if (!E || !st_dev) return fail(AMRH_EINVAL, "null argument");
   if (n < 0) return fail(AMRH_EINVAL, "negative count");
   if (n == 0) return AMRH_OK;
   const unsigned grid = (unsigned)((n + 255) / 256);
-  eos_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(E->H, (helm::Eos*)st_dev, n, solve_T);
+  eos_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(E->H, (helm::Eos*)st_dev, n, mode);
   HIPCHK(hipGetLastError());
   return AMRH_OK;
 }
@@ -256,6 +263,8 @@ int amrh_eos_set_units(amrh_eos* E, double cm, double g, double s, double K) {
 int amrh_eos_eval(amrh_eos* E, int n, void* st_dev, void* stream) { return eos_batch(E, n, st_dev, stream, 0); }
 
 int amrh_eos_compute_T(amrh_eos* E, int n, void* st_dev, void* stream) { return eos_batch(E, n, st_dev, stream, 1); }
+
+int amrh_eos_eval_f32(amrh_eos* E, int n, void* st_dev, void* stream) { return eos_batch(E, n, st_dev, stream, 2); }
 
 int amrh_eos_error(amrh_eos* E, int* flags) {
   if (!E || !flags) return fail(AMRH_EINVAL, "null argument");

@@ -338,6 +338,11 @@ int amrh_eos_eval(amrh_eos* eos, int n, void* st_dev, void* stream);
 /* helmholtz_compute_T, energy form (helmholtz/helmeos.cpp:436-486): damped Newton for the T at
  * which e(rho, T) equals the state's e, starting from the state's T; in place.  Asynchronous. */
 int amrh_eos_compute_T(amrh_eos* eos, int n, void* st_dev, void* stream);
+/* The mixed form's face-state EOS (what a mixed rotating substep evaluates at its faces, the
+ * reference's helmholtz_eos as called from binary/state_ztwd.cpp:360-396): p and cs from rho, T,
+ * abar, zbar rounded to FP32, by the FP32 monomial form of each interpolation cell, written to the
+ * states' p and cs; the other fields are left as they are.  Asynchronous. */
+int amrh_eos_eval_f32(amrh_eos* eos, int n, void* st_dev, void* stream);
 /* Device error flags raised since the last call, where the reference aborts: 8 state outside
  * the table, 16 negative heat capacity, 32 negative squared sound speed, 64 Newton did not
  * converge in 100 iterations, 128 T fell below TMIN.  Synchronises the device; clears them. */

@@ -94,3 +94,53 @@ def test_table_file_loads_like_rows(eos, tmp_path):
     st = hc.states(5000, seed=16)
     np.testing.assert_array_equal(device_run(E2, st), device_run(eos, st))
     E2.close()
+
+
+def f32_pair(E, st):
+    """the FP32 face EOS's p and cs (amrh_eos_eval_f32) beside the oracle's FP64 ones"""
+    ref = st.copy()
+    assert not orc.helm_eos(ref).any()
+    t = torch.from_numpy(st.copy()).cuda()
+    E.eval_f32(st.shape[0], t.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    got = t.cpu().numpy()
+    assert E.error_flags() == 0
+    return got, ref
+
+
+def test_fp32_face_eos_near_fp64(eos):
+    """the mixed form's face EOS (monomial coefficients in FP32, FP64 assembly) against the FP64
+    EOS on the non-degenerate electron table: p within 1e-6, cs within 4e-6 relative"""
+    st = hc.states(50000, seed=15)
+    got, ref = f32_pair(eos, st)
+    ep = np.abs(got[:, 0] / ref[:, 0] - 1)
+    ec = np.abs(got[:, 2] / ref[:, 2] - 1)
+    say("FP32 face EOS, first table: max rel p %.2e cs %.2e" % (ep.max(), ec.max()))
+    assert ep.max() < 1e-6 and ec.max() < 4e-6
+    np.testing.assert_array_equal(got[:, 4:8], st[:, 4:8])  # inputs untouched
+
+
+def test_fp32_face_eos_near_fp64_on_the_dwd_table():
+    """the same on the double white dwarf's degenerate + radiation table, C/O 50/50, over the
+    densities and temperatures of the two stars and their surroundings"""
+    import workloads
+    rows = workloads.helmholtz_table_dwd()
+    orc.helm_table(rows)
+    orc.helm_set_units()
+    E = amrhydro.HelmholtzEos(rows)
+    try:
+        rng = np.random.default_rng(16)
+        n = 50000
+        st = np.zeros((n, 10))
+        st[:, 4] = workloads.DWD_ABAR
+        st[:, 5] = workloads.DWD_ABAR / 2.0
+        st[:, 6] = 10.0 ** rng.uniform(-1.0, 7.5, n)
+        st[:, 7] = 10.0 ** rng.uniform(5.5, 9.5, n)
+        got, ref = f32_pair(E, st)
+        ep = np.abs(got[:, 0] / ref[:, 0] - 1)
+        ec = np.abs(got[:, 2] / ref[:, 2] - 1)
+        say("FP32 face EOS, DWD table: max rel p %.2e cs %.2e" % (ep.max(), ec.max()))
+        assert ep.max() < 1e-6 and ec.max() < 4e-6
+    finally:
+        E.close()
+        orc.helm_table(hc.table_rows())  # the module's table again for the oracle
