@@ -533,6 +533,51 @@ def cpu_baseline(n, target_s=10.0, U=None, h=None, label="flat batch blocks"):
                       "OpenMP over blocks, %.1f s timed" % (nb, label, n, steps, sum(times))}
 
 
+def cpu_baseline_reference_shaped(n, target_s=8.0):
+    """The reference-shaped FP64 restatement (oracle: orc_refshape_*): nodes of 56-byte AoS states,
+    a substep as passes over the nodes, each pass an OpenMP parallel-for inside one node
+    (grid/grid.cpp:177-420 under the recursions of grid/grid_node.cpp), the ghost fill included.
+    Whole RK3 steps (CFL scan, store, three ghost fills + substeps) on a bounded sample of the flat
+    batch; bit-identical to the oracle's batch path (tests/test_oracle_refshape.py)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    import workloads
+
+    cfg = orc.euler_cfg(n=n)
+    U = workloads.flat_batch(4, n=n, seed=7)
+    h = np.full(U.shape[0], 1.0 / n)
+    orc.outflow_ghosts(cfg, U)
+    R = orc.RefShaped(cfg, U, h)
+    t0 = time.perf_counter()
+    R.rk_step(1e-9, 3)
+    per_block = (time.perf_counter() - t0) / U.shape[0]
+    R.close()
+    nb = max(4, min(4096, int(target_s / 3 / per_block)))
+    U = workloads.flat_batch(nb, n=n, seed=8)
+    h = np.full(nb, 1.0 / n)
+    orc.outflow_ghosts(cfg, U)
+    R = orc.RefShaped(cfg, U, h)
+    times, steps, dt = [], 0, None
+    while sum(times) < target_s and steps < 50:
+        t0 = time.perf_counter()
+        d = 0.4 / R.max_eig_over_h()
+        dt = 1e-2 * d if dt is None else min(1.25 * dt, d)
+        R.rk_step(dt, 3)
+        times.append(time.perf_counter() - t0)
+        steps += 1
+    R.close()
+    try:
+        import ctypes
+        cores = ctypes.CDLL("libgomp.so.1").omp_get_max_threads()
+    except OSError:
+        cores = os.cpu_count()
+    return {"value": 3 * steps * nb * n ** 3 / sum(times), "unit": "cell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": "%d flat batch blocks of %d^3 (+2 ghosts) x %d RK3 steps (CFL scan, store, 3 x ghost fill "
+                      "+ substep), reference-shaped FP64 restatement: AoS 56-byte states, per-node OpenMP "
+                      "passes, %.1f s timed" % (nb, n, steps, sum(times))}
+
+
 def cpu_poisson_baseline():
     """One V-cycle of the FP64 oracle (one thread) on the same octree and source as the GPU's
     Poisson line, after its boundary phi."""
@@ -660,11 +705,14 @@ def main():
         extra = extra_configs(amrhydro, torch, max(3, args.steps // 4), 1, cpu=not args.no_cpu,
                               accuracy_steps=args.accuracy_steps)
 
-    cpu = None
+    cpu = cpu_rs = None
     if not args.no_cpu:
         log("CPU baseline (FP64 oracle)")
         cpu = cpu_baseline(n)
         log("  %.3e cell-updates/s on %d threads" % (cpu["value"], cpu["cores"]))
+        log("CPU baseline, reference-shaped (AoS states, per-node OpenMP passes)")
+        cpu_rs = cpu_baseline_reference_shaped(n)
+        log("  %.3e cell-updates/s on %d threads" % (cpu_rs["value"], cpu_rs["cores"]))
         if extra is not None and "poisson_binary_tree" in extra:
             log("CPU oracle: one Poisson V-cycle on the same octree")
             pc = cpu_poisson_baseline()
@@ -704,6 +752,7 @@ def main():
         "roofline": roof,
         "kernel_build": kernel_build_id(),
         "cpu_baseline": cpu,
+        "cpu_baseline_reference_shaped": cpu_rs,
         "other_workloads": extra,
     }
     print(json.dumps(out), flush=True)

@@ -590,6 +590,218 @@ int orc_batch_rk_steps(const orc_cfg* c, int nb, double* U, double* U0, const do
   return 0;
 }
 
+// ---- reference-shaped restatement (a CPU baseline that keeps the reference's structure) ------
+// The same arithmetic as above, arranged as the reference arranges it: every node holds
+// Array3d<State> U, U0 and Array3d<Vector> D, F as arrays of structures whose elements carry a
+// virtual-table pointer before their fields (vector.h:40-45: sizeof(State) = 56 B for EULER), and
+// a substep is a sequence of passes over the nodes, each pass one node at a time with an OpenMP
+// parallel-for over that node's pencils or planes (grid/grid.cpp:177-420, the recursions of
+// grid/grid_node.cpp): enforce_boundaries, clear_difs, compute_{x,y,z}_flux + sum_{x,y,z}_difs,
+// add_difs; max_dt scans each node with a parallel-for and a critical max (grid/grid.cpp:422-483).
+namespace {
+
+struct AosVec {
+  const void* vptr;  // the reference's Vector has a virtual method table
+  double v[MAXF];
+};
+static_assert(MAXF >= 6, "");
+
+struct RefShape {
+  Geo o;
+  int nb;
+  std::vector<std::vector<AosVec>> U, U0, D, F;
+  std::vector<double> h;
+  explicit RefShape(const orc_cfg* c) : o(c), nb(0) {}
+};
+
+// Grid::compute_{x,y,z}_flux (grid/grid.cpp:215-302): parallel over the planes of pencils
+void rs_flux(RefShape& R, int b, int d) {
+  const Geo& o = R.o;
+  const std::vector<AosVec>& U = R.U[b];
+  std::vector<AosVec>& F = R.F[b];
+  const int lo = o.g, hi = o.G - o.g - 1;
+#pragma omp parallel for schedule(static)
+  for (int bb = lo; bb <= hi; bb++)
+    for (int a = lo; a <= hi; a++) {
+      double q[MAXG][MAXF], Fp[MAXG][MAXF];
+      for (int p = 0; p < o.G; p++) {
+        const AosVec& u = U[pencil_at(o, d, a, bb, p)];
+        for (int l = 0; l < o.nf; l++) q[p][l] = u.v[l];
+      }
+      pencil_kt(o, d, o.G, q, Fp, nullptr);
+      for (int p = o.g; p < o.G - o.g + 1; p++) {
+        AosVec& f = F[pencil_at(o, d, a, bb, p)];
+        for (int l = 0; l < o.nf; l++) f.v[l] = Fp[p][l];
+      }
+    }
+}
+
+// Grid::sum_{x,y,z}_difs (grid/grid.cpp:304-344)
+void rs_sum_difs(RefShape& R, int b, int d) {
+  const Geo& o = R.o;
+  std::vector<AosVec>& D = R.D[b];
+  const std::vector<AosVec>& F = R.F[b];
+  const double hinv = 1.0 / R.h[b];
+  const long step = d == 0 ? 1 : (d == 1 ? o.G : (long)o.G * o.G);
+#pragma omp parallel for schedule(static)
+  for (int k = o.g; k < o.G - o.g; k++)
+    for (int j = o.g; j < o.G - o.g; j++)
+      for (int i = o.g; i < o.G - o.g; i++) {
+        const long c = o.at(i, j, k);
+        for (int l = 0; l < o.nf; l++) D[c].v[l] -= (F[c + step].v[l] - F[c].v[l]) * hinv;
+      }
+}
+
+// Grid::clear_difs (grid/grid.cpp:201-213)
+void rs_clear(RefShape& R, int b) {
+  const Geo& o = R.o;
+  std::vector<AosVec>& D = R.D[b];
+#pragma omp parallel for schedule(static)
+  for (int k = o.g; k < o.G - o.g; k++)
+    for (int j = o.g; j < o.G - o.g; j++)
+      for (int i = o.g; i < o.G - o.g; i++)
+        for (int l = 0; l < o.nf; l++) D[o.at(i, j, k)].v[l] = 0.0;
+}
+
+// Grid::add_difs (grid/grid.cpp:406-420)
+void rs_add_difs(RefShape& R, int b, double dt, double beta) {
+  const Geo& o = R.o;
+  std::vector<AosVec>& U = R.U[b];
+  const std::vector<AosVec>& U0 = R.U0[b];
+  std::vector<AosVec>& D = R.D[b];
+  const double omb = 1.0 - beta;
+#pragma omp parallel for schedule(static)
+  for (int k = o.g; k < o.G - o.g; k++)
+    for (int j = o.g; j < o.G - o.g; j++)
+      for (int i = o.g; i < o.G - o.g; i++) {
+        const long c = o.at(i, j, k);
+        double u[MAXF];
+        for (int l = 0; l < o.nf; l++) {
+          const double dl = D[c].v[l] + 0.0;
+          D[c].v[l] = dl;
+          u[l] = (U[c].v[l] + dl * dt) * beta + U0[c].v[l] * omb;
+        }
+        floor_state(o.P, u);
+        for (int l = 0; l < o.nf; l++) U[c].v[l] = u[l];
+      }
+}
+
+// enforce_boundaries of a lone node (grid/grid_phys_bound.cpp:27-54): per face, parallel over the
+// face's cells; the arithmetic of block_phys_ghost_face on one AoS node
+void rs_ghosts(RefShape& R, int b) {
+  const Geo& o = R.o;
+  std::vector<AosVec>& U = R.U[b];
+  for (int f = 0; f < 6; f++) {
+    const int d = f >> 1;
+    const bool upper = f & 1;
+    const int bface = f ^ 1;
+    const bool reflect = o.zref && d == 2 && !upper;
+#pragma omp parallel for schedule(static)
+    for (int bb = o.g; bb < o.G - o.g; bb++)
+      for (int a = o.g; a < o.G - o.g; a++)
+        for (int t = 0; t < o.g; t++) {
+          const int gp = upper ? (o.G - o.g + t) : t;
+          const int src = reflect ? 2 * o.g - 1 - gp : (upper ? (o.G - o.g - 1) : o.g);
+          double u[MAXF];
+          const AosVec& s = U[pencil_at(o, d, a, bb, src)];
+          for (int l = 0; l < o.nf; l++) u[l] = s.v[l];
+          if (reflect)
+            u[SZ] = -u[SZ];
+          else
+            enforce_outflow(u, bface);
+          AosVec& g = U[pencil_at(o, d, a, bb, gp)];
+          for (int l = 0; l < o.nf; l++) g.v[l] = u[l];
+        }
+  }
+}
+
+}  // namespace
+
+void* orc_refshape_create(const orc_cfg* c, int nb, const double* U, const double* h) {
+  RefShape* R = new RefShape(c);
+  const Geo& o = R->o;
+  if (o.G > MAXG || o.nf > MAXF) {
+    delete R;
+    return nullptr;
+  }
+  R->nb = nb;
+  R->h.assign(h, h + nb);
+  static const char vtable_stand_in = 0;
+  const AosVec zero{&vtable_stand_in, {}};
+  for (auto* A : {&R->U, &R->U0, &R->D, &R->F}) A->assign(nb, std::vector<AosVec>(o.G3, zero));
+  for (int b = 0; b < nb; b++)
+    for (long e = 0; e < o.G3; e++)
+      for (int l = 0; l < o.nf; l++) R->U[b][e].v[l] = U[((long)b * o.nf + l) * o.G3 + e];
+  return R;
+}
+
+void orc_refshape_destroy(void* r) { delete static_cast<RefShape*>(r); }
+
+int orc_refshape_download(const void* r, double* U) {
+  const RefShape* R = static_cast<const RefShape*>(r);
+  const Geo& o = R->o;
+  for (int b = 0; b < R->nb; b++)
+    for (long e = 0; e < o.G3; e++)
+      for (int l = 0; l < o.nf; l++) U[((long)b * o.nf + l) * o.G3 + e] = R->U[b][e].v[l];
+  return 0;
+}
+
+double orc_refshape_max_eig(void* r) {
+  // Grid::max_dt (grid/grid.cpp:422-483), node by node; returns max over nodes of max_eig / h
+  RefShape* R = static_cast<RefShape*>(r);
+  const Geo& o = R->o;
+  double m = 0.0;
+  for (int b = 0; b < R->nb; b++) {
+    double mb = 0.0;
+    const std::vector<AosVec>& U = R->U[b];
+    const int lo = o.g, hi = o.G - o.g - 1;
+#pragma omp parallel for schedule(static)
+    for (int bb = lo; bb <= hi; bb++)
+      for (int a = lo; a <= hi; a++) {
+        double mine = 0.0;
+        for (int d = 0; d < 3; d++) {
+          double q[MAXG][MAXF], Fp[MAXG][MAXF], eig[2 * MAXG];
+          for (int p = 0; p < o.G; p++) {
+            const AosVec& u = U[pencil_at(o, d, a, bb, p)];
+            for (int l = 0; l < o.nf; l++) q[p][l] = u.v[l];
+          }
+          pencil_kt(o, d, o.G, q, Fp, eig);
+          for (int p = o.g; p < o.G - o.g + 1; p++) mine = rmax(mine, rmax(eig[2 * p], eig[2 * p + 1]));
+        }
+#pragma omp critical
+        mb = rmax(mb, mine);
+      }
+    m = rmax(m, mb / R->h[b]);
+  }
+  return m;
+}
+
+int orc_refshape_rk_step(void* r, double dt, int rk) {
+  // HydroDriver::step (euler/hydro_driver.cpp:171-194) on lone outflow nodes: store, then per
+  // substep enforce_boundaries and the sub_step passes (euler/hydro_driver.cpp:200-214)
+  RefShape* R = static_cast<RefShape*>(r);
+  const Geo& o = R->o;
+  static const double betas[3][3] = {{1.0, 0, 0}, {1.0, 0.5, 0}, {1.0, 0.25, 2.0 / 3.0}};
+  for (int b = 0; b < R->nb; b++) {
+    std::vector<AosVec>& U = R->U[b];
+    std::vector<AosVec>& U0 = R->U0[b];
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < o.G3; e++) U0[e] = U[e];
+  }
+  for (int st = 0; st < rk; st++) {
+    const double beta = betas[rk - 1][st];
+    for (int b = 0; b < R->nb; b++) rs_ghosts(*R, b);
+    for (int b = 0; b < R->nb; b++) rs_clear(*R, b);
+    for (int d = 0; d < 3; d++) {
+      for (int b = 0; b < R->nb; b++) rs_flux(*R, b, d);
+      for (int b = 0; b < R->nb; b++) rs_sum_difs(*R, b, d);
+    }
+    for (int b = 0; b < R->nb; b++) rs_add_difs(*R, b, dt, beta);
+  }
+  for (int b = 0; b < R->nb; b++) rs_ghosts(*R, b);
+  return 0;
+}
+
 int orc_tree_prolong(orc_tree* t, int node) {
   if (node <= 0 || node >= (int)t->nodes.size()) return 1;
   prolong_child(t, node);

@@ -47,6 +47,18 @@ int orc_batch_outflow_ghosts(const orc_cfg* c, int nb, double* U);
 int orc_batch_rk_steps(const orc_cfg* c, int nb, double* U, double* U0, const double* h, double dt,
                        int rk, int nsteps);
 
+/* --- reference-shaped restatement: the same arithmetic in the reference's structure (AoS states
+ * with a virtual-table pointer, 56 B each; passes over nodes, each an OpenMP parallel-for inside
+ * one node).  A CPU baseline and its parity check; flat batches of lone outflow nodes. */
+void* orc_refshape_create(const orc_cfg* c, int nb, const double* U, const double* h);
+void orc_refshape_destroy(void* r);
+int orc_refshape_download(const void* r, double* U);
+/* max over nodes of (max face eigenvalue / h): Grid::max_dt before the CFL factor */
+double orc_refshape_max_eig(void* r);
+/* one RK step of order rk with a fixed dt: store, and per substep the ghost fill and sub_step;
+ * the ghosts are refilled at the end */
+int orc_refshape_rk_step(void* r, double dt, int rk);
+
 /* --- octree (GridNode recursion, grid/grid_node.cpp) -------------------------------------- */
 typedef struct orc_tree orc_tree;
 orc_tree* orc_tree_create(const orc_cfg* c);

@@ -53,6 +53,14 @@ def lib():
         L.orc_block_dt.restype = C.c_double
         L.orc_batch_outflow_ghosts.argtypes = [cfgp, C.c_int, DP]
         L.orc_batch_rk_steps.argtypes = [cfgp, C.c_int, DP, DP, DP, C.c_double, C.c_int, C.c_int]
+        L.orc_refshape_create.argtypes = [cfgp, C.c_int, DP, DP]
+        L.orc_refshape_create.restype = C.c_void_p
+        L.orc_refshape_destroy.argtypes = [C.c_void_p]
+        L.orc_refshape_destroy.restype = None
+        L.orc_refshape_download.argtypes = [C.c_void_p, DP]
+        L.orc_refshape_max_eig.argtypes = [C.c_void_p]
+        L.orc_refshape_max_eig.restype = C.c_double
+        L.orc_refshape_rk_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.orc_tree_create.argtypes = [cfgp]
         L.orc_tree_create.restype = C.c_void_p
         L.orc_tree_destroy.argtypes = [C.c_void_p]
@@ -139,6 +147,33 @@ def block_dt(cfg, m, h):
 
 def outflow_ghosts(cfg, U):
     assert lib().orc_batch_outflow_ghosts(C.byref(cfg), U.shape[0], dptr(U)) == 0
+
+
+class RefShaped:
+    """The reference-shaped restatement (orc_refshape_*): AoS nodes, one OpenMP parallel-for per
+    pass inside each node."""
+
+    def __init__(self, cfg, U, h):
+        self.shape = U.shape
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        self.r = lib().orc_refshape_create(C.byref(cfg), U.shape[0], dptr(np.ascontiguousarray(U)), dptr(h))
+        assert self.r
+
+    def rk_step(self, dt, rk=3):
+        assert lib().orc_refshape_rk_step(self.r, dt, rk) == 0
+
+    def max_eig_over_h(self):
+        return lib().orc_refshape_max_eig(self.r)
+
+    def download(self):
+        U = np.empty(self.shape)
+        assert lib().orc_refshape_download(self.r, dptr(U)) == 0
+        return U
+
+    def close(self):
+        if self.r:
+            lib().orc_refshape_destroy(self.r)
+            self.r = None
 
 
 def rk_steps(cfg, U, h, dt, rk, nsteps):
