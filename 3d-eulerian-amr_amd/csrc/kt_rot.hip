@@ -111,7 +111,8 @@ __device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int 
 // combination in FP32 (rot::sp); the flux differences, sources, RK update and floor in FP64.
 // TREE: the octree's reflux overrides, exports and the root's ledger (a.slot, a.exp_slot, a.ledger)
 // The mixed form stages the whole record (interior + 6 slabs) once per block and computes the faces
-// of all three directions as one list; the FP64 form stages one direction's pencils at a time.
+// of all three directions as one list (with the ideal gas, as pairs of adjacent faces that share
+// their reconstruction); the FP64 form stages one direction's pencils at a time.
 template <int G, int RECON, int EOS, int MODE, bool MIXED, bool TREE>
 __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   using T = typename std::conditional<MIXED, float, double>::type;
@@ -384,7 +385,9 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       sweep(X0{});
       sweep(X1{});
       sweep(X2{});
-    } else {
+    } else if constexpr (EOS == EOS_HELMHOLTZ) {
+      // Helmholtz: single faces (a face pair keeps the second face's sides live through the first
+      // face's two EOS evaluations, which spills; measured 5.40 vs 5.33 ms)
       // The whole record is staged, so the faces of the three directions form one list of
       // 3 NFACE, handed out in rounds of N3 threads: four rounds (the last one 3/8 full) instead of
       // two per direction with the second one 1/8 full.  NFACE is a multiple of the wavefront, so
@@ -422,6 +425,117 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       } else {
 #pragma unroll 1
         for (int g = tid; g < 3 * NFACE; g += N3) face_any(g);
+      }
+    } else {
+      // The whole record is staged, so the faces of all three directions are handed out together,
+      // as pairs of adjacent faces of one pencil (faces 2j and 2j+1 for j < N/2, then face N
+      // alone): N/2 + 1 tasks per pencil, 960 per block, two rounds of N3 threads.  A pair shares
+      // the reconstruction of its middle cells (ppm_face_pair: 5 slopes, 4 interfaces and 3
+      // monotonisations for two faces instead of 8, 6 and 4).  x tasks run along each pencil
+      // across the lanes (its cells are contiguous in LDS), y and z tasks across the pencils.  The
+      // tasks of one direction fill whole wavefronts, so each wavefront has one direction.  Fb and
+      // Vf are double-buffered (x -> 0, y -> 1, z -> 0): round 0 holds x and the first y tasks,
+      // round 1 the rest of y and z, whose faces go to buffer 0 once x is finished.
+      constexpr int TPD = (N / 2 + 1) * N2;  // tasks per direction
+      static_assert(N == 8 && TPD % 64 == 0 && TPD <= N3 && 3 * TPD <= 2 * N3 && N3 < 2 * TPD,
+                    "the round schedule below assumes N = 8");
+      constexpr int W = RECON == AMRH_RECON_PPM ? 7 : 5;  // cells f0 - W/2 .. f0 + W/2 of a pair
+      static_assert(W / 2 <= G, "a pair's cells lie in the staged pencil");
+      // face f of direction d at (ta, tb) from its reconstructed primitive sides: the face position,
+      // to_con, the EOS and the KT flux into Fb[fb], Vf[fb] (MODE 0) or the wavespeed into emax
+      auto emit = [&](auto dc, int f, int ta, int tb, int fb, float* ql, float* qr) {
+        constexpr int d = decltype(dc)::value;
+        float X[3];
+        {
+          const int ii = d == 0 ? f : ta;
+          const int jj = d == 0 ? ta : (d == 1 ? f : tb);
+          const int kk = d == 2 ? f : tb;
+          double Xd[3];
+          cell_pos(x0, h, ii, jj, kk, Xd);
+          Xd[d] = x0[d] + (double)f * h;
+          X[0] = (float)Xd[0], X[1] = (float)Xd[1], X[2] = (float)Xd[2];
+        }
+        sp::to_con(cf, ql, X);
+        sp::to_con(cf, qr, X);
+        float fl[NF], fr[NF];
+        const float el = sp::flux<EOS, d>(H, cf, ql, X, fl);
+        const float er = sp::flux<EOS, d>(H, cf, qr, X, fr);
+        const float amax = fmaxf(el, er);
+        if constexpr (MODE == 0) {
+          const int q = face_index<N, d>(f, ta, tb);
+          float uf[NF];
+#pragma unroll
+          for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5f;
+          Vf[fb][q] = (T)sp::vel<d>(cf, uf, X, sp::Rc(X));
+#pragma unroll
+          for (int l = 0; l < NF; l++) Fb[fb][l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5f);
+        } else {
+          emax = rmax(emax, (double)amax);
+        }
+      };
+      auto pair_task = [&](auto dc, int r, int fb) {
+        constexpr int d = decltype(dc)::value;
+        const int j = d == 0 ? r % (N / 2 + 1) : r / N2;
+        const int pen = d == 0 ? r / (N / 2 + 1) : r % N2;
+        const int ta = pen % N, tb = pen / N;
+        const int f0 = 2 * j;
+        int e[W];
+#pragma unroll
+        for (int m = 0; m < W; m++) {
+          int p = f0 - W / 2 + m;
+          if (p > N + G - 1) p = N + G - 1;  // face N alone: its unused partner reads a staged cell
+          e[m] = staged(d, p, ta, tb);
+        }
+        float ql[NF], qr[NF], ql1[NF], qr1[NF];
+#pragma unroll
+        for (int l = 0; l < NF; l++) {
+          float v[W];
+#pragma unroll
+          for (int m = 0; m < W; m++) v[m] = (float)P[l][e[m]];
+          if constexpr (RECON == AMRH_RECON_PPM)
+            f32::ppm_face_pair(v, ql[l], qr[l], ql1[l], qr1[l], l == POT, l == POT);
+          else
+            f32::minmod_face_pair(v, ql[l], qr[l], ql1[l], qr1[l], (float)c.theta);
+        }
+        // one emit site for both faces: the second face's sides move into place after the first
+#pragma unroll 1
+        for (int s = 0; s < 2; s++) {
+          if (s == 1) {
+            if (f0 + 1 > N) break;
+#pragma unroll
+            for (int l = 0; l < NF; l++) {
+              ql[l] = ql1[l];
+              qr[l] = qr1[l];
+            }
+          }
+          emit(dc, f0 + s, ta, tb, fb, ql, qr);
+        }
+      };
+      auto task_any = [&](int g) {
+        if (g < TPD)
+          pair_task(X0{}, g, 0);
+        else if (g < 2 * TPD)
+          pair_task(X1{}, g - TPD, 1);
+        else
+          pair_task(X2{}, g - 2 * TPD, 0);
+      };
+      if constexpr (MODE == 0) {
+        // one call site keeps the task code inlined once per direction
+#pragma unroll 1
+        for (int rd = 0; rd < 2; rd++) {
+          const int g = tid + rd * N3;
+          if (g < 3 * TPD) task_any(g);
+          __syncthreads();
+          if (rd == 0) {
+            finish(X0{}, 0);
+            __syncthreads();  // Fb[0] takes z faces next
+          }
+        }
+        finish(X1{}, 1);
+        finish(X2{}, 0);
+      } else {
+#pragma unroll 1
+        for (int g = tid; g < 3 * TPD; g += N3) task_any(g);
       }
     }
 

@@ -109,6 +109,45 @@ __device__ __forceinline__ void minmod_face(const float* q, float& ql, float& qr
   ql = q[1] + 0.5f * sl;
   qr = q[2] - 0.5f * sr;
 }
+// Two adjacent faces f and f+1 of one pencil, each with the values ppm_face gives it, from the
+// seven cells f-3 .. f+3: the slopes of cells f-2 .. f+2, the interface values between f-2 and
+// f+2 and the monotonisation of cell f (whose left edge is face f's qr and right edge face f+1's
+// ql) are formed once for both faces (5 slopes, 4 interfaces, 3 monotonisations instead of 8, 6, 4)
+__device__ __forceinline__ void ppm_face_pair(const float* q, float& ql0, float& qr0, float& ql1, float& qr1,
+                                              bool low_order, bool smooth) {
+#pragma clang fp contract(fast)
+  const float s1 = minmod_theta(q[2] - q[1], q[1] - q[0], 2.0f);
+  const float s2 = minmod_theta(q[3] - q[2], q[2] - q[1], 2.0f);
+  const float s3 = minmod_theta(q[4] - q[3], q[3] - q[2], 2.0f);
+  const float s4 = minmod_theta(q[5] - q[4], q[4] - q[3], 2.0f);
+  const float s5 = minmod_theta(q[6] - q[5], q[5] - q[4], 2.0f);
+  float a1 = ppm_iface(q[1], q[2], s1, s2, low_order);
+  float a2 = ppm_iface(q[2], q[3], s2, s3, low_order);  // face f
+  float a3 = ppm_iface(q[3], q[4], s3, s4, low_order);  // face f+1
+  float a4 = ppm_iface(q[4], q[5], s4, s5, low_order);
+  float a2r = a2, a3r = a3;
+  if (!smooth) {
+    ppm_mono(q[2], a1, a2);   // cell f-1: right edge = face f's ql
+    ppm_mono(q[3], a2r, a3);  // cell f: left edge = face f's qr, right edge = face f+1's ql
+    ppm_mono(q[4], a3r, a4);  // cell f+1: left edge = face f+1's qr
+  }
+  ql0 = a2;
+  qr0 = a2r;
+  ql1 = a3;
+  qr1 = a3r;
+}
+// the same for minmod-theta from the five cells f-2 .. f+2 (the slope of cell f serves both faces)
+__device__ __forceinline__ void minmod_face_pair(const float* q, float& ql0, float& qr0, float& ql1, float& qr1,
+                                                 float theta) {
+#pragma clang fp contract(fast)
+  const float s1 = minmod_theta(q[2] - q[1], q[1] - q[0], theta);
+  const float s2 = minmod_theta(q[3] - q[2], q[2] - q[1], theta);
+  const float s3 = minmod_theta(q[4] - q[3], q[3] - q[2], theta);
+  ql0 = q[1] + 0.5f * s1;
+  qr0 = q[2] - 0.5f * s2;
+  ql1 = q[2] + 0.5f * s2;
+  qr1 = q[3] - 0.5f * s3;
+}
 
 }  // namespace f32
 }  // namespace amrh
