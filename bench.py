@@ -78,9 +78,11 @@ def kernel_times(B, torch, cells, steps, warmup):
     first = [True]
 
     def step(events=None):
-        # HydroDriver::step_to_time's dt control on the device (euler/hydro_driver.cpp:107-113)
+        # HydroDriver::step_to_time's dt control on the device (euler/hydro_driver.cpp:107-113), at
+        # the CFL limit from the first step on (the reference's first step of a run is smaller; a
+        # step in mid-run is what is timed, and the floor's Newton solve works harder at full dt)
         if first[0]:
-            B.dt_device(1e-2, 0.0, dt.data_ptr())
+            B.dt_device(1.0, 0.0, dt.data_ptr())
             first[0] = False
         else:
             B.dt_device(1.0, 1.25, dt.data_ptr())
@@ -121,7 +123,7 @@ def graph_steps(B, torch, cells, steps, warmup):
     """Whole RK3 steps replayed as captured hipGraphs (amrh_run_steps: CFL scan, device dt with the
     reference's growth cap, store, three substeps), timed by the wall clock."""
     dt = torch.zeros(1, dtype=torch.float64, device="cuda")
-    B.dt_device(1e-2, 0.0, dt.data_ptr())  # the first step's dt (euler/hydro_driver.cpp:108-111)
+    B.dt_device(1.0, 0.0, dt.data_ptr())  # the CFL dt from the first step on, as in kernel_times
     B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -383,7 +385,7 @@ def binary_octree(amrhydro, torch, steps, warmup, stream):
             B.set_potential(S["phi"])
             B.tree_fill_ghosts()
             B.tree_inject()
-            dt = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
+            dt = torch.full((1,), B.tree_max_dt(), dtype=torch.float64, device="cuda")  # the CFL dt
             B.run_steps(warmup, dt.data_ptr(), rk=3, use_graph=True)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -509,7 +511,7 @@ def cpu_baseline(n, target_s=10.0, U=None, h=None, label="flat batch blocks"):
         t0 = time.perf_counter()
         m = orc.batch_max_eig(cfg, Us)  # the step's CFL scan (grid/grid.cpp:422-483)
         d = 0.4 * (hs / m).min()
-        dt = 1e-2 * d if dt is None else min(1.25 * dt, d)
+        dt = d if dt is None else min(1.25 * dt, d)
         times.append(time.perf_counter() - t0)
         for beta in BETAS:
             orc.outflow_ghosts(cfg, Us)
@@ -561,7 +563,7 @@ def cpu_baseline_reference_shaped(n, target_s=8.0):
     while sum(times) < target_s and steps < 50:
         t0 = time.perf_counter()
         d = 0.4 / R.max_eig_over_h()
-        dt = 1e-2 * d if dt is None else min(1.25 * dt, d)
+        dt = d if dt is None else min(1.25 * dt, d)
         R.rk_step(dt, 3)
         times.append(time.perf_counter() - t0)
         steps += 1
@@ -611,7 +613,7 @@ def cpu_binary_octree_baseline():
     T = orc.RotTree(orc.rot_cfg(omega=S["omega"]), parent, child)
     T.set_state(S["U"])
     T.set_phi(S["phi"])
-    dt = 0.01 * T.max_dt()
+    dt = T.max_dt()  # the CFL dt, as the GPU line
     T.store()
     t0 = time.perf_counter()
     T.substep(dt, 1.0)
