@@ -39,6 +39,7 @@ class Config(C.Structure):
 
 AMRH_BOUNDARY_EXTERNAL = 0
 AMRH_BOUNDARY_OUTFLOW = 1
+AMRH_LAUNCH_SUBSTEP, AMRH_LAUNCH_SCAN, AMRH_LAUNCH_FIRST, AMRH_LAUNCH_DT, AMRH_LAUNCH_SECOND, AMRH_LAUNCH_GHOSTS = range(6)
 STATE_SUM_DEV_MAX_PARTIALS = 3145728  # AMRH_STATE_SUM_DEV_MAX_PARTIALS
 
 
@@ -94,6 +95,8 @@ SIGNATURES = {
     "amrh_launch_info": (C.c_int, [C.c_void_p, IP, IP, IP, DP, DP]),
     "amrh_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "amrh_run_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]),
+    "amrh_deferred_launch_info": (C.c_int, [C.c_void_p, C.c_int, IP, IP, DP, DP]),
+    "amrh_step_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_float), IP, C.c_int, IP]),
     "amrh_tree_set": (C.c_int, [C.c_void_p, IP, IP]),
     "amrh_tree_set_origin": (C.c_int, [C.c_void_p, DP]),
     "amrh_tree_get_origin": (C.c_int, [C.c_void_p, DP]),
@@ -310,6 +313,22 @@ class Batch:
 
     def run_steps(self, nsteps, dt_ptr, rk=3, use_graph=True):
         _ck(lib().amrh_run_steps(self.ctx, rk, nsteps, C.c_void_p(dt_ptr), 1 if use_graph else 0))
+
+    def deferred_launch_info(self, kernel):
+        """(grid, variant, ms_default, ms_tuned) of the deferred-dt step's first (0) or second (1)
+        substep kernel, as picked by autotune()"""
+        g, v = C.c_int(), C.c_int()
+        d, t = C.c_double(), C.c_double()
+        _ck(lib().amrh_deferred_launch_info(self.ctx, kernel, C.byref(g), C.byref(v), C.byref(d), C.byref(t)))
+        return g.value, v.value, d.value, t.value
+
+    def step_timed(self, dt_ptr, rk=3):
+        """One step of run_steps (direct launches) timed stage by stage: [(AMRH_LAUNCH_* kind, ms)]"""
+        ms = (C.c_float * 16)()
+        kinds = (C.c_int * 16)()
+        n = C.c_int()
+        _ck(lib().amrh_step_timed(self.ctx, rk, C.c_void_p(dt_ptr), ms, kinds, 16, C.byref(n)))
+        return [(kinds[i], ms[i]) for i in range(n.value)]
 
     # ---- diagnostics -------------------------------------------------------------------------
     def state_sum(self):

@@ -139,6 +139,7 @@ void amrh_destroy(amrh_ctx* c) {
   if (c->diag_part) (void)hipFree(c->diag_part);
   if (c->phi_dev) (void)hipFree(c->phi_dev);
   if (c->force_own) (void)hipFree(c->force_own);
+  if (c->dbuf) (void)hipFree(c->dbuf);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (c->tree) {
     c->tree->free_dev();
@@ -196,6 +197,8 @@ int amrh_upload(amrh_ctx* c, const double* U) {
 int amrh_download_device(amrh_ctx* c, void* pad) {
   if (!c || !pad) return fail(AMRH_EINVAL, "null argument");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
+  // a self-ghosting batch does not maintain its ghost slabs: form them for the copy
+  if (self_ghosting(c)) HIPCHK(ctx_outflow_ghosts(c, c->buf[c->cur]));
   HIPCHK(hipMemsetAsync(pad, 0, sizeof(double) * (size_t)padded_elems(c), c->stream));
   HIPCHK(launch_pack(c->cfg.n, c->cfg.g, (const double*)pad, c->buf[c->cur], record_elems(c), false, c->stream));
   return AMRH_OK;
@@ -361,6 +364,25 @@ int amrh::ctx_ready(const amrh_ctx* c) {
   return AMRH_OK;
 }
 
+bool amrh::self_ghosting(const amrh_ctx* c) {
+  return !c->tree && c->cfg.boundary == AMRH_BOUNDARY_OUTFLOW && c->cfg.physics == AMRH_PHYS_EULER &&
+         c->cfg.recon == AMRH_RECON_MINMOD;
+}
+
+SubstepArgs amrh::flat_args(const amrh_ctx* c) {
+  SubstepArgs a{};
+  a.Uin = c->buf[c->cur];
+  a.h = c->h_dev;
+  a.nblocks = c->nb;
+  a.block0 = 0;
+  a.P = params(c->cfg);
+  a.err = c->err_dev;
+  a.ledger_block = -1;
+  a.outflow = (!c->tree && c->cfg.boundary == AMRH_BOUNDARY_OUTFLOW) ? 1 : 0;
+  a.z_reflect = c->cfg.z_reflect;
+  return a;
+}
+
 hipError_t amrh::ctx_outflow_ghosts(const amrh_ctx* c, double* U) {
   if (c->cfg.physics == AMRH_PHYS_ROTATING)
     return launch_rot_ghosts(c->cfg.g, U, c->x0_dev, c->h_dev, c->nb, c->cfg.z_reflect, rot_params(c->cfg), c->stream);
@@ -460,6 +482,8 @@ static int maxeig_impl(amrh_ctx* c, bool per_block, bool global) {
   m.gmax = global ? c->gmax_dev : nullptr;
   m.nblocks = c->nb;
   m.P = params(c->cfg);
+  m.outflow = self_ghosting(c) ? 1 : 0;
+  m.z_reflect = c->cfg.z_reflect;
   HIPCHK(ctx_maxeig_launch(c, m, c->grid_maxeig));
   return AMRH_OK;
 }
@@ -512,7 +536,8 @@ double amrh_substep_bytes(const amrh_ctx* c, int beta_is_one) {
 }
 
 double amrh_ghost_write_bytes(const amrh_ctx* c) {
-  if (!c || c->cfg.boundary != AMRH_BOUNDARY_OUTFLOW) return 0.0;
+  // self-ghosting batches write no ghosts; the others' ghost fill writes every face slab
+  if (!c || c->cfg.boundary != AMRH_BOUNDARY_OUTFLOW || self_ghosting(c)) return 0.0;
   const double n = c->cfg.n, g = c->cfg.g, nf = c->cfg.nf;
   return 8.0 * nf * 6.0 * g * n * n * c->nb;
 }

@@ -72,6 +72,13 @@ struct amrh_ctx {
   int grid_maxeig = 0;   // same for the CFL scan
   int variant = 0;       // register-budget variant of the flat minmod substep (launch_substep)
   float tune_ms_default = 0.f, tune_ms_best = 0.f;  // the autotuner's timings of the substep
+  // the deferred-dt RK step of self-ghosting flat batches (amrh_graph.hip): the first substep's
+  // divergence [nb][nf][n^3] (allocated on first use) and the launch shapes of its two kernels
+  // (index 0: the deferred first substep, 1: the substep that follows it)
+  double* dbuf = nullptr;
+  int grid_deferred[2] = {0, 0};
+  int variant_deferred[2] = {0, 0};
+  float tune_ms_deferred[2][2] = {{0.f, 0.f}, {0.f, 0.f}};  // [kernel][default, tuned]
   // captured RK steps, one graph per parity of the ping-pong buffers (amrh_graph.hip)
   hipGraphExec_t step_exec[2] = {nullptr, nullptr};
   int step_exec_rk = 0;
@@ -92,6 +99,11 @@ hipError_t ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid);
 int ctx_ready(const amrh_ctx* c);
 // ghost fill of isolated outflow blocks for the context's physics
 hipError_t ctx_outflow_ghosts(const amrh_ctx* c, double* U);
+// a flat batch of isolated outflow blocks on the minmod sweep: its kernels form every ghost from
+// the boundary cell it copies where it is read, so the ghost slabs in memory are not maintained
+bool self_ghosting(const amrh_ctx* c);
+// the substep arguments of a flat batch (no octree), input = the current stage
+SubstepArgs flat_args(const amrh_ctx* c);
 // the rotating kernels' arguments common to every launch: potential, forces, positions, constants
 RotArgs ctx_rot_args(const amrh_ctx* c);
 // forget the captured step graphs (their kernel arguments hold pointers and tables that a call

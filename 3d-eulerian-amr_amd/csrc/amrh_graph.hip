@@ -48,12 +48,87 @@ int time_launches(amrh_ctx* c, int reps, F fn, float* ms) {
 
 const double kBetas[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
 
+// HIP events between the launches of one step, for amrh_step_timed (null: not timed)
+struct Marks {
+  hipEvent_t ev[kMaxStepLaunches + 1] = {};
+  int kind[kMaxStepLaunches] = {};
+  int n = 0;     // launches marked
+  bool open = false;
+};
+
+int mark_begin(amrh_ctx* c, Marks* mk) {
+  if (!mk) return AMRH_OK;
+  if (mk->n >= kMaxStepLaunches) return fail(AMRH_EINVAL, "more than %d launches in a step", kMaxStepLaunches);
+  if (!mk->open) {
+    HIPCHK(hipEventRecord(mk->ev[mk->n], c->stream));
+    mk->open = true;
+  }
+  return AMRH_OK;
+}
+
+// the launch just enqueued ends at this event, and the next one starts there
+int mark_end(amrh_ctx* c, Marks* mk, int kind) {
+  if (!mk) return AMRH_OK;
+  if (mk->n >= kMaxStepLaunches) return fail(AMRH_EINVAL, "more than %d stages in a step", kMaxStepLaunches);
+  mk->kind[mk->n] = kind;
+  mk->n++;
+  HIPCHK(hipEventRecord(mk->ev[mk->n], c->stream));
+  return AMRH_OK;
+}
+
+int ensure_dbuf(amrh_ctx* c) {
+  if (c->dbuf) return AMRH_OK;
+  const size_t n3 = (size_t)c->cfg.n * c->cfg.n * c->cfg.n;
+  HIPCHK(hipMalloc(&c->dbuf, sizeof(double) * n3 * c->cfg.nf * c->nb));
+  return AMRH_OK;
+}
+
+// The RK step of a self-ghosting flat batch with its dt deferred: the first substep forms the
+// fluxes of U^n, reduces their KT face speeds -- the same numbers Grid::max_dt's scan forms,
+// grid/grid.cpp:422-483 -- into the CFL maximum, and stores the divergence D instead of U1;
+// the dt kernel then applies HydroDriver's control (dt = cfl/max, capped at growth times the last
+// dt, euler/hydro_driver.cpp:107-113); the second substep forms U1 = floor(U^n + D dt) for every
+// cell it reads, with U^n as its RK base.  The buffers end as after scan + store + rk substeps
+// (U^n in the RK base, the state in the other buffer), and every cell has the same bits.
+bool deferred_ok(const amrh_ctx* c, int rk) { return rk >= 2 && self_ghosting(c) && c->dbuf; }
+
+int step_deferred(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  const int p = c->cur, q = 1 - p;
+  SubstepArgs a = flat_args(c);
+  a.U0 = c->buf[p];
+  a.dbuf = c->dbuf;
+  a.gmax = c->gmax_dev;
+  a.beta = 1.0;
+  if (int rc = mark_begin(c, mk)) return rc;
+  HIPCHK(launch_substep(c->cfg.n, a, mixed, c->grid_deferred[0], c->stream, c->variant_deferred[0], 2));
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_FIRST)) return rc;
+  HIPCHK(launch_dt_from_gmax(c->gmax_dev, c->cfg.cfl, 1.0, 1.25, dt_dev, c->stream));
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_DT)) return rc;
+  c->u0buf = p;  // the store: U^n stays where it is as the RK base
+  a.Uout = c->buf[q];
+  a.dt_dev = dt_dev;
+  a.beta = kBetas[rk - 1][1];
+  HIPCHK(launch_substep(c->cfg.n, a, mixed, c->grid_deferred[1], c->stream, c->variant_deferred[1], 3));
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_SECOND)) return rc;
+  c->cur = q;
+  for (int s = 2; s < rk; s++) {
+    if (int rc = amrh_substep_dev(c, dt_dev, kBetas[rk - 1][s])) return rc;
+    if (int rc = mark_end(c, mk, AMRH_LAUNCH_SUBSTEP)) return rc;
+  }
+  return AMRH_OK;
+}
+
 // one RK step with the device dt control (scale 1, growth MAXDTINC = 1.25), as launched
-int step_once(amrh_ctx* c, int rk, double* dt_dev) {
+int step_once(amrh_ctx* c, int rk, double* dt_dev, Marks* mk = nullptr) {
+  if (deferred_ok(c, rk)) return step_deferred(c, rk, dt_dev, mk);
+  if (int rc = mark_begin(c, mk)) return rc;
   if (c->tree) {
     if (int rc = amrh_tree_fill_ghosts(c)) return rc;
+    if (int rc = mark_end(c, mk, AMRH_LAUNCH_GHOSTS)) return rc;
   }
-  if (int rc = amrh_dt_device(c, 1.0, 1.25, dt_dev)) return rc;
+  if (int rc = amrh_dt_device(c, 1.0, 1.25, dt_dev)) return rc;  // the scan and the dt kernel
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_SCAN)) return rc;
   if (int rc = amrh_store(c)) return rc;
   for (int s = 0; s < rk; s++) {
     const double beta = kBetas[rk - 1][s];
@@ -63,6 +138,7 @@ int step_once(amrh_ctx* c, int rk, double* dt_dev) {
     // the rotating state's substep leaves the restriction to the driver, after the potential
     if (c->tree && c->cfg.physics == AMRH_PHYS_ROTATING)
       if (int rc2 = amrh_tree_inject(c)) return rc2;
+    if (int rc = mark_end(c, mk, AMRH_LAUNCH_SUBSTEP)) return rc;
   }
   return AMRH_OK;
 }
@@ -113,6 +189,8 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   m.block_eig = c->eig_dev;
   m.nblocks = c->nb;
   m.P = params(c->cfg);
+  m.outflow = self_ghosting(c) ? 1 : 0;
+  m.z_reflect = c->cfg.z_reflect;
   float best_s = 1e30f, best_m = 1e30f, def_s = 0.f;
   int gs = 0, gm = 0, vs = 0;
   for (int v = 0; v < nvar; v++)
@@ -139,11 +217,58 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
       return rc;
     if (ms < best_m) { best_m = ms; gm = g; }
   }
+  // the two kernels of the deferred-dt step (self-ghosting flat batches): the first substep, then
+  // the one that forms its input from U^n and D; timed with dt = 0 (eig_dev[0] as the device dt),
+  // writing only D, the CFL word (cleared after) and the spare buffer
+  int gd[2] = {0, 0}, vd[2] = {0, 0};
+  float msd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  if (self_ghosting(c)) {
+    if (int rc = ensure_dbuf(c)) return rc;
+    const bool mixed = c->cfg.precision == AMRH_MIXED;
+    HIPCHK(hipMemsetAsync(c->eig_dev, 0, sizeof(double), c->stream));
+    for (int mode = 2; mode <= 3; mode++) {
+      SubstepArgs d = flat_args(c);
+      d.U0 = d.Uin;
+      d.dbuf = c->dbuf;
+      d.gmax = c->gmax_dev;
+      d.beta = mode == 2 ? 1.0 : 0.25;
+      if (mode == 3) {
+        d.Uout = c->buf[other];
+        d.dt_dev = c->eig_dev;
+      }
+      const int kx = mode - 2;
+      float best = 1e30f;
+      for (int v = 0; v < kSweepVariants; v++)
+        for (int g : cand) {
+          float ms = 0.f;
+          if (int rc = time_launches(c, 3, [&]() -> int {
+                HIPCHK(launch_substep(c->cfg.n, d, mixed, g, c->stream, v, mode));
+                return AMRH_OK;
+              }, &ms))
+            return rc;
+          if (v == 0 && g == 0) msd[kx][0] = ms;
+          if (ms < best) {
+            best = ms;
+            gd[kx] = g;
+            vd[kx] = v;
+          }
+        }
+      msd[kx][1] = best;
+    }
+    HIPCHK(launch_reset_u64(c->gmax_dev, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   c->grid_substep = gs;
   c->grid_maxeig = gm;
   c->variant = vs;
   c->tune_ms_default = def_s;
   c->tune_ms_best = best_s;
+  for (int kx = 0; kx < 2; kx++) {
+    c->grid_deferred[kx] = gd[kx];
+    c->variant_deferred[kx] = vd[kx];
+    c->tune_ms_deferred[kx][0] = msd[kx][0];
+    c->tune_ms_deferred[kx][1] = msd[kx][1];
+  }
   drop_graphs(c);
   if (grid_substep) *grid_substep = gs;
   if (grid_maxeig) *grid_maxeig = gm;
@@ -178,12 +303,26 @@ int amrh_set_launch(amrh_ctx* c, int grid_substep, int grid_maxeig) {
   return AMRH_OK;
 }
 
+int amrh_deferred_launch_info(const amrh_ctx* c, int kernel, int* grid, int* variant, double* ms_default,
+                              double* ms_tuned) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (kernel < 0 || kernel > 1) return fail(AMRH_EINVAL, "kernel must be 0 (first substep) or 1 (second)");
+  if (grid) *grid = c->grid_deferred[kernel];
+  if (variant) *variant = c->variant_deferred[kernel];
+  if (ms_default) *ms_default = c->tune_ms_deferred[kernel][0];
+  if (ms_tuned) *ms_tuned = c->tune_ms_deferred[kernel][1];
+  return AMRH_OK;
+}
+
 int amrh_run_steps(amrh_ctx* c, int rk, int nsteps, double* dt_dev, int use_graph) {
   if (!c || !dt_dev) return fail(AMRH_EINVAL, "null argument");
   if (rk < 1 || rk > 3 || nsteps < 0) return fail(AMRH_EINVAL, "rk must be 1..3, nsteps >= 0");
   if (!c->tree && c->cfg.boundary != AMRH_BOUNDARY_OUTFLOW)
     return fail(AMRH_ESTATE, "a flat batch runs whole steps only with AMRH_BOUNDARY_OUTFLOW");
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
+  if (int rc = ctx_ready(c)) return rc;
+  if (rk >= 2 && self_ghosting(c))
+    if (int rc = ensure_dbuf(c)) return rc;  // before any capture
   if (!use_graph) {
     for (int s = 0; s < nsteps; s++)
       if (int rc = step_once(c, rk, dt_dev)) return rc;
@@ -219,6 +358,39 @@ int amrh_run_steps(amrh_ctx* c, int rk, int nsteps, double* dt_dev, int use_grap
     c->cur = 1 - p;
   }
   return AMRH_OK;
+}
+
+int amrh_step_timed(amrh_ctx* c, int rk, double* dt_dev, float* ms, int* kinds, int max_stages, int* nstages) {
+  if (!c || !dt_dev || !ms || !kinds || !nstages) return fail(AMRH_EINVAL, "null argument");
+  if (rk < 1 || rk > 3) return fail(AMRH_EINVAL, "rk must be 1..3");
+  if (max_stages < 1) return fail(AMRH_EINVAL, "max_stages must be positive");
+  if (!c->tree && c->cfg.boundary != AMRH_BOUNDARY_OUTFLOW)
+    return fail(AMRH_ESTATE, "a flat batch runs whole steps only with AMRH_BOUNDARY_OUTFLOW");
+  if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
+  if (int rc = ctx_ready(c)) return rc;
+  if (rk >= 2 && self_ghosting(c))
+    if (int rc = ensure_dbuf(c)) return rc;
+  Marks mk;
+  int made = 0, rc = AMRH_OK;
+  for (; made <= kMaxStepLaunches && rc == AMRH_OK; made++) {
+    const hipError_t e = hipEventCreate(&mk.ev[made]);
+    if (e != hipSuccess) rc = fail(AMRH_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+  }
+  if (rc == AMRH_OK) rc = step_once(c, rk, dt_dev, &mk);
+  if (rc == AMRH_OK && mk.n > max_stages) rc = fail(AMRH_EINVAL, "the step has %d stages, more than max_stages", mk.n);
+  if (rc == AMRH_OK) {
+    const hipError_t e = hipEventSynchronize(mk.ev[mk.n]);
+    if (e != hipSuccess) rc = fail(AMRH_EHIP, "hipEventSynchronize: %s", hipGetErrorString(e));
+  }
+  for (int k = 0; rc == AMRH_OK && k < mk.n; k++) {
+    const hipError_t e = hipEventElapsedTime(&ms[k], mk.ev[k], mk.ev[k + 1]);
+    if (e != hipSuccess) rc = fail(AMRH_EHIP, "hipEventElapsedTime: %s", hipGetErrorString(e));
+    kinds[k] = mk.kind[k];
+  }
+  if (rc == AMRH_OK) *nstages = mk.n;
+  for (int k = 0; k < made; k++)
+    if (mk.ev[k]) (void)hipEventDestroy(mk.ev[k]);
+  return rc;
 }
 
 }  // extern "C"

@@ -10,7 +10,13 @@ namespace amrh {
 // variant (flat batches of the minmod sweep): 0 the default register budget, 1..3 the kernel held to
 // 2, 3, 4 waves per SIMD
 constexpr int kSweepVariants = 4;
-hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s, int variant = 0);
+// the most timed stages one RK step has (amrh_step_timed)
+constexpr int kMaxStepLaunches = 16;
+// mode (the sweep kernel's flat batches): 0 the substep; 2 the first substep of an RK step with dt
+// deferred (fluxes, max wavespeed into a.gmax, divergence into a.dbuf); 3 the next substep, whose
+// input is that first stage formed from U^n = a.Uin and a.dbuf with the device dt (a.U0 = a.Uin)
+hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s, int variant = 0,
+                          int mode = 0);
 hipError_t launch_maxeig(int n, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s);
 hipError_t launch_outflow_ghosts(int n, int g, double* U, int nblocks, int z_reflect, hipStream_t s);
 hipError_t launch_pack(int n, int g, const double* pad, double* rec, long nrec_elems, bool to_rec, hipStream_t s);

@@ -39,9 +39,15 @@ struct SubstepArgs {
   const int* nodes;
   const int* exp_slot;
   double* ovr_out;         // = ovr, written
-  // isolated outflow blocks: write the next stage's ghosts with the interior
+  // isolated outflow blocks: self-ghosting (each ghost formed from the boundary cell it copies
+  // where it is read; the ghost slabs are neither read nor written), z-mirrored below with z_reflect
   int outflow;
   int z_reflect;
+  // an RK step with deferred dt (flat outflow batches, the sweep kernel): the divergence of the
+  // first substep, interior cells only [block][NF][N^3] (written by the deferred first substep,
+  // read by the next one), and the global max wavespeed/h that the deferred substep reduces into
+  double* dbuf;
+  unsigned long long* gmax;
 };
 
 struct MaxEigArgs {
@@ -51,6 +57,8 @@ struct MaxEigArgs {
   unsigned long long* gmax;    // global max of eig/h as ordered bits (nullptr = skip)
   int nblocks;
   PhysParams P;
+  int outflow;    // self-ghosting outflow blocks (as SubstepArgs::outflow; the sweep kernel)
+  int z_reflect;
 };
 
 }  // namespace amrh

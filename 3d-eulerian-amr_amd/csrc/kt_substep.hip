@@ -117,9 +117,8 @@ __device__ __forceinline__ double eig_of(const PhysParams& P, const typename Pre
   }
 }
 
-// outflow ghost of an updated boundary cell u across node face `face` (the boundary object's face
-// is the node's inverted: a lower node face removes v > 0; the z test uses sz,
-// euler/state.cpp:45-84)
+// outflow ghost of boundary cell u across node face `face` (the boundary object's face is the
+// node's inverted: a lower node face removes v > 0; the z test uses sz, euler/state.cpp:45-84)
 __device__ __forceinline__ void outflow_fix(double* u, int face) {
   const int d = face >> 1;
   const bool upper = face & 1;
@@ -132,29 +131,84 @@ __device__ __forceinline__ void outflow_fix(double* u, int face) {
   }
 }
 
+constexpr int kMirror = 6;  // ghost post-processing: the lower z face's mirror (sz -> -sz)
+
+// a fetched boundary cell -> the ghost it stands for: none (-1), the outflow fix of a node face,
+// or the z mirror (grid/grid_phys_bound.cpp:27-54, euler/state.cpp:94-96)
+__device__ __forceinline__ void ghost_post(double* u, int post) {
+  if (post == kMirror)
+    u[SZ] = -u[SZ];
+  else if (post >= 0)
+    outflow_fix(u, post);
+}
+
+// Record element of column cell kk in [-G, N+G), and how the fetched cell becomes the value used:
+// a stored cell (interior or ghost slab), or for a self-ghosting outflow block the boundary cell
+// the ghost copies -- the nearest plane, or plane -1-kk mirrored below with z_reflect (the ghost
+// at padded z = t holds padded 2G-1-t, grid/grid_phys_bound.cpp:139-166).
 template <int N>
-__device__ __forceinline__ void put_ghost(const RecBuf<(long)N * N * N + 6L * GW * N * N, NFE>& Uo, const Layout& L,
-                                          int face, int t, int i, int j, int k, const double* u, bool fix) {
-  constexpr long R = (long)N * N * N + 6L * GW * N * N;
-  double g[NFE];
+__device__ __forceinline__ unsigned col_src(const Layout& L, int i, int j, int kk, bool self, bool zr, int& post) {
+  post = -1;
+  if (!self || (kk >= 0 && kk < N)) return (unsigned)col_elem<N>(L, i, j, kk);
+  if (kk < 0) {
+    post = zr ? kMirror : (int)ZL;
+    return (unsigned)L.interior(i, j, zr ? -1 - kk : 0);
+  }
+  post = ZU;
+  return (unsigned)L.interior(i, j, N - 1);
+}
+
+// The first RK stage of a step whose dt was not known when its fluxes were formed (MODE 2 stored
+// the divergence instead): U1 = floor((U^n + D dt) * 1), the same operations the substep's update
+// makes with beta = 1 (grid/grid.cpp:406-420, euler/state.cpp:86-92), so U1 has the bits a direct
+// first substep would have stored.  FP64: d = -(dFx + dFy + dFz)/h in the reference's order; mixed:
+// d = the FP64 sum of the FP32 flux differences (h applied here, as the mixed update does).
+template <bool MIXED>
+__device__ __forceinline__ void first_stage(const PhysParams& P, double* u, const double* d, double dt,
+                                            double hinv) {
 #pragma unroll
-  for (int l = 0; l < NFE; l++) g[l] = u[l];
-  if (fix) outflow_fix(g, face);
-  long e;
-  if (face < 2)
-    e = L.slab(face) + ((long)k * N + j) * GW + t;
-  else if (face < 4)
-    e = L.slab(face) + ((long)k * GW + t) * N + i;
+  for (int l = 0; l < NFE; l++) {
+    if constexpr (!MIXED)
+      u[l] = (u[l] + d[l] * dt) * 1.0;
+    else
+      u[l] = fma(-dt * hinv, d[l], u[l]) * 1.0;
+  }
+  if constexpr (!MIXED)
+    x64::floor_state(P, u);
   else
-    e = L.slab(face) + ((long)t * N + j) * N + i;
-#pragma unroll
-  for (int l = 0; l < NFE; l++) Uo.st_nt(l, (unsigned)e, g[l]);
+    floor_mixed(P, u);
+}
+
+template <bool MIXED, int D>
+__device__ __forceinline__ void face_of_e(const PhysParams& P, const typename Prec<MIXED>::T* pl,
+                                          const typename Prec<MIXED>::T* sl, const typename Prec<MIXED>::T* pr,
+                                          const typename Prec<MIXED>::T* sr, typename Prec<MIXED>::T* F,
+                                          double& eig) {
+  if constexpr (!MIXED) {
+    x64::kt_face<D>(P, pl, sl, pr, sr, F, eig);
+  } else {
+    const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
+    float e;
+    f32::kt_face<D>(PF, pl, sl, pr, sr, F, e);
+    eig = (double)e;
+  }
 }
 
 // MODE 0: substep (flux + divergence + update).  MODE 1: max wavespeed scan only.
+// MODE 2: the first substep of an RK step with dt deferred: the fluxes, the block's max wavespeed
+//   (the KT face speeds are exactly Grid::max_dt's, grid/grid.cpp:422-483, since both reconstruct
+//   the same state) and the divergence D, stored instead of the update.
+// MODE 3: a substep (beta != 1) whose input is that first stage, formed on the fly from the step's
+//   base U^n and D once dt is known; U^n is also its RK base.
+// MODEs 2 and 3 run on self-ghosting outflow blocks only; together they replace the CFL scan,
+// the dt, the store and the first two substeps of an RK step, and the scan's separate pass over
+// the state disappears.
 // Per plane k: stage the plane's primitives (with x/y ghosts) in LDS; x- and y-slopes of every
 // cell once; the x/y faces from two cells and their slopes; the z-face from the column window;
 // then each thread updates its cell.  Three barriers per plane.
+// Outflow blocks (a.outflow / m.outflow) are self-ghosting: each ghost is formed from the
+// boundary cell it copies (grid/grid_phys_bound.cpp:27-54) where it is used, so their ghost slabs
+// are neither read nor written.
 // Occupancy, measured on the flat 4096 x 16^3 batch: the FP64 substep runs two waves per SIMD and
 // reads its cell's conserved state back from memory (L2) at each plane instead of carrying three
 // planes of FP64 column state in registers, which removes its spills (1.88 -> 1.54 ms; on the
@@ -172,6 +226,11 @@ template <int N, bool MIXED, int MODE, bool TREE, int WPE = 0>
 __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : (MIXED ? (N == 16 ? 3 : 1) : 2)))) void kt_sweep_kernel(
     SubstepArgs a, MaxEigArgs m) {
   using T = typename Prec<MIXED>::T;
+  constexpr bool FLUX = MODE != 1;                 // face fluxes (MODE 1 forms wavespeeds only)
+  constexpr bool EIG = MODE == 1 || MODE == 2;     // the block's max wavespeed
+  constexpr bool UPDATE = MODE == 0 || MODE == 3;  // RK combination, floor, store
+  constexpr bool FROMD = MODE == 3;                // input: the first stage from U^n and D
+  static_assert(!TREE || MODE <= 1, "the octree instantiations are the substep and the scan");
   constexpr int PW = N + 2 * GW;  // padded plane edge
   constexpr int NT = N * N;
   constexpr int NFX = N * (N + 1);  // x faces of a plane (= y faces)
@@ -179,88 +238,84 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   constexpr int HN = GW * N;        // halo cells per face per plane
   constexpr Layout L{N, GW, NFE};
   constexpr long R = L.rec();
+  constexpr long N3 = (long)N * N * N;
 
   __shared__ T plane[NFE][PW][PW];
   __shared__ T Sx[NFE][N][N + 2];  // x-slope of plane cell (padded x = idx+1, y = row+G)
   __shared__ T Sy[NFE][N + 2][N];  // y-slope of plane cell (padded y = idx+1, x = col+G)
-  __shared__ T Fx[MODE == 0 ? NFE : 1][N][N + 1];
-  __shared__ T Fy[MODE == 0 ? NFE : 1][N + 1][N];
+  __shared__ T Fx[FLUX ? NFE : 1][N][N + 1];
+  __shared__ T Fy[FLUX ? NFE : 1][N + 1][N];
   __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
-  __shared__ double gb[MODE == 0 ? 4 : 1][N][NFE];  // updated x/y boundary cells of the last plane
 
   // persistent: a grid smaller than the batch walks it with a stride (launch shape chosen by the
   // autotuner); with one workgroup per block the loop runs once
-  const int nbl = MODE == 0 ? a.nblocks : m.nblocks;
+  const int nbl = MODE == 1 ? m.nblocks : a.nblocks;
   for (int bb = blockIdx.x; bb < nbl; bb += gridDim.x) {
-  const int b = (MODE == 0 ? a.block0 : 0) + bb;
+  const int b = (MODE == 1 ? 0 : a.block0) + bb;
   const int tid = threadIdx.x;
   const int i = tid % N, j = tid / N;
-  const PhysParams P = MODE == 0 ? a.P : m.P;
-  const double* Ub = (MODE == 0 ? a.Uin : m.U) + (long)b * NFE * R;
-
+  const PhysParams P = MODE == 1 ? m.P : a.P;
+  const double* Ub = (MODE == 1 ? m.U : a.Uin) + (long)b * NFE * R;
   double dt = a.dt;
   const double beta = a.beta;
-  if (MODE == 0 && a.dt_dev) dt = *a.dt_dev;
-  const bool has_u0 = MODE == 0 && beta != 1.0;
-  // x/y ghost-slab writers of the outflow epilogue: the last 4*G*N threads, one slab cell each
-  constexpr int NGW = 4 * GW * N;
-  const int gw = tid - (NT - NGW);
-  const bool is_gw = MODE == 0 && a.outflow && gw >= 0;
-  double* const Uob = MODE == 0 ? a.Uout + (long)b * NFE * R : nullptr;
-  const RecBuf<R, NFE> ub(Ub), uo(Uob), u0(MODE == 0 && a.U0 ? a.U0 + (long)b * NFE * R : nullptr);
-  auto write_xy_ghost = [&](int kk) {
-    const int face = gw / (GW * N), r = gw % (GW * N);
-    const double* src = face < 2 ? gb[face][r / GW] : gb[face][r % N];
-    double g[NFE];
-#pragma unroll
-    for (int l = 0; l < NFE; l++) g[l] = src[l];
-    outflow_fix(g, face);
-    // X slabs: (kk*N + jj)*G + t = kk*G*N + r;  Y slabs: (kk*G + t)*N + ii = kk*G*N + r
-    const long e = L.slab(face) + (long)kk * GW * N + r;
-#pragma unroll
-    for (int l = 0; l < NFE; l++) uo.st_nt(l, (unsigned)e, g[l]);
-  };
-  const double hinv = MODE == 0 ? 1.0 / a.h[b] : 0.0;
+  if (FROMD || (UPDATE && a.dt_dev)) dt = *a.dt_dev;
+  const bool has_u0 = UPDATE && beta != 1.0;
+  const bool self = (MODE == 1 ? m.outflow : a.outflow) != 0;  // self-ghosting outflow block
+  const bool zr = (MODE == 1 ? m.z_reflect : a.z_reflect) != 0;
+  double* const Uob = UPDATE ? a.Uout + (long)b * NFE * R : nullptr;
+  const RecBuf<R, NFE> ub(Ub), uo(Uob), u0(UPDATE && a.U0 ? a.U0 + (long)b * NFE * R : nullptr);
+  // the deferred divergence, interior cells only (MODE 2 writes it, MODE 3 reads it)
+  const RecBuf<N3, NFE> db(MODE >= 2 ? a.dbuf + (long)b * NFE * N3 : nullptr);
+  const double hinv = MODE == 1 ? 0.0 : 1.0 / a.h[b];
 
   // ---- prologue: column cells k = -2 .. 2 ------------------------------------------------
-  // the conserved FP64 state of cells k, k+1, k+2 (the mixed form carries it; FP64 reloads cell k)
-  constexpr bool CARRY = MIXED;
-  double ucur[NFE], uc1[CARRY ? NFE : 1], uc2[CARRY ? NFE : 1];
+  // the conserved FP64 input state of cells k, k+1, k+2 (the mixed update carries it; FP64
+  // reloads cell k)
+  constexpr bool CARRY = MIXED && UPDATE;
+  double ucur[UPDATE ? NFE : 1], uc1[CARRY ? NFE : 1], uc2[CARRY ? NFE : 1];
   T c0[NFE], cp1[NFE], cp2[NFE], s0[NFE], Fzlo[NFE];
   double emax = 0.0;
+  auto fetch_col = [&](int kk, double* u) {
+    int post;
+    const unsigned e = col_src<N>(L, i, j, kk, self, zr, post);
+#pragma unroll
+    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, e);
+    if constexpr (FROMD) {
+      double d[NFE];
+#pragma unroll
+      for (int l = 0; l < NFE; l++) d[l] = db.ld(l, e);  // self-ghosting: e is an interior cell
+      first_stage<MIXED>(P, u, d, dt, hinv);
+    }
+    ghost_post(u, post);
+  };
   {
     double u[NFE];
     T cm2[NFE], cm1[NFE], sm1[NFE];
-    long e = col_elem<N>(L, i, j, -2);
-#pragma unroll
-    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
+    fetch_col(-2, u);
     make_prim<MIXED>(u, cm2);
-    e = col_elem<N>(L, i, j, -1);
-#pragma unroll
-    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
+    fetch_col(-1, u);
     make_prim<MIXED>(u, cm1);
-    e = col_elem<N>(L, i, j, 0);
+    fetch_col(0, u);
+    if constexpr (UPDATE)
 #pragma unroll
-    for (int l = 0; l < NFE; l++) ucur[l] = ub.ld(l, (unsigned)e);
-    make_prim<MIXED>(ucur, c0);
-    e = col_elem<N>(L, i, j, 1);
-#pragma unroll
-    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
+      for (int l = 0; l < NFE; l++) ucur[l] = u[l];
+    make_prim<MIXED>(u, c0);
+    fetch_col(1, u);
     if constexpr (CARRY)
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc1[l] = u[l];
     make_prim<MIXED>(u, cp1);
-    e = col_elem<N>(L, i, j, 2);
-#pragma unroll
-    for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
+    fetch_col(2, u);
     if constexpr (CARRY)
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc2[l] = u[l];
     make_prim<MIXED>(u, cp2);
     slope_of<MIXED>(P, cm2, cm1, c0, sm1);
     slope_of<MIXED>(P, cm1, c0, cp1, s0);
-    if constexpr (MODE == 0) {
-      face_of<MIXED, 2>(P, cm1, sm1, c0, s0, Fzlo);  // face k = 0
+    if constexpr (FLUX) {
+      double e;
+      face_of_e<MIXED, 2>(P, cm1, sm1, c0, s0, Fzlo, e);  // face k = 0
+      if constexpr (EIG) emax = x64::rmax(emax, e);
       if (TREE && a.slot) apply_override<N>(a, b, 2, 0, i, j, Fzlo);
       if (TREE && b == a.ledger_block) {
 #pragma unroll
@@ -271,25 +326,43 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     }
   }
 
-  // halo raw values for plane 0
-  double hraw[NFE];
+  // halo raw values for plane 0: a stored ghost-slab cell, or (self-ghosting) the boundary cell
+  // the ghost copies, fixed for its face when it is staged
+  double hraw[NFE], hd[FROMD ? NFE : 1];
   const bool is_halo = tid < 4 * HN;
-  int hpx = 0, hpy = 0;
-  unsigned hbase = 0;
+  int hpx = 0, hpy = 0, hpost = -1;
+  unsigned hbase = 0, hstride = 0;
   if (is_halo) {
     const int face = tid / HN, r = tid % HN;
+    int si, sj;  // the boundary cell of a self-ghosting block
     if (face < 2) {
       const int jj = r / GW, t = r % GW;
       hpx = face == XL ? t : N + GW + t;
       hpy = jj + GW;
+      si = face == XL ? 0 : N - 1;
+      sj = jj;
     } else {
       const int t = r / N, ii = r % N;
       hpx = ii + GW;
       hpy = face == YL ? t : N + GW + t;
+      si = ii;
+      sj = face == YL ? 0 : N - 1;
     }
-    hbase = (unsigned)(L.slab(face) + r);  // plane k adds k*HN (both slab kinds are plane-major)
+    if (self) {
+      hpost = face;
+      hbase = (unsigned)L.interior(si, sj, 0);
+      hstride = NT;
 #pragma unroll
-    for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, hbase);
+      for (int l = 0; l < NFE; l++) hraw[l] = ub.ld(l, hbase);
+      if constexpr (FROMD)
+#pragma unroll
+        for (int l = 0; l < NFE; l++) hd[l] = db.ld(l, hbase);
+    } else {
+      hbase = (unsigned)(L.slab(face) + r);  // plane k adds k*HN (both slab kinds are plane-major)
+      hstride = HN;
+#pragma unroll
+      for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, hbase);
+    }
   }
 
   for (int k = 0; k < N; k++) {
@@ -298,37 +371,57 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     for (int l = 0; l < NFE; l++) plane[l][j + GW][i + GW] = c0[l];
     if (is_halo) {
       T hp[NFE];
+      if constexpr (FROMD) first_stage<MIXED>(P, hraw, hd, dt, hinv);
+      ghost_post(hraw, hpost);
       make_prim<MIXED>(hraw, hp);
 #pragma unroll
       for (int l = 0; l < NFE; l++) plane[l][hpy][hpx] = hp[l];
     }
     // prefetch: halo of plane k+1, column cell k+3, U0 at k
     if (is_halo && k + 1 < N) {
+      const unsigned e = hbase + (unsigned)(k + 1) * hstride;
+      if (self) {
 #pragma unroll
-      for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, hbase + (unsigned)((k + 1) * HN));
+        for (int l = 0; l < NFE; l++) hraw[l] = ub.ld(l, e);
+        if constexpr (FROMD)
+#pragma unroll
+          for (int l = 0; l < NFE; l++) hd[l] = db.ld(l, e);
+      } else {
+#pragma unroll
+        for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, e);
+      }
     }
-    double craw[NFE];
+    double craw[NFE], cd[FROMD ? NFE : 1];
+    int cpost = -1;
     const bool have_next = (k + 3) < N + GW;
     if (have_next) {
-      const long e = col_elem<N>(L, i, j, k + 3);
+      const unsigned e = col_src<N>(L, i, j, k + 3, self, zr, cpost);
 #pragma unroll
-      for (int l = 0; l < NFE; l++) craw[l] = ub.ld(l, (unsigned)e);
+      for (int l = 0; l < NFE; l++) craw[l] = ub.ld(l, e);
+      if constexpr (FROMD)
+#pragma unroll
+        for (int l = 0; l < NFE; l++) cd[l] = db.ld(l, e);
     }
-    if (MODE == 0 && !CARRY && k > 0) {  // cell k's conserved state again (read 3 planes ago: L2)
-      const long e = L.interior(i, j, k);
+    // cell k's RK base (U0; for MODE 3 the step's base U^n, the input it was formed from) and,
+    // where the column state is not carried, cell k's input state again (read 3 planes ago: L2)
+    double u0v[NFE], dk[(FROMD && !CARRY) ? NFE : 1];
+    if constexpr (UPDATE) {
+      const unsigned e = (unsigned)L.interior(i, j, k);
+      if constexpr (!CARRY && !FROMD) {
+        if (k > 0)
 #pragma unroll
-      for (int l = 0; l < NFE; l++) ucur[l] = ub.ld(l, (unsigned)e);
-    }
-    double u0v[NFE];
-    if (MODE == 0 && has_u0) {
-      const long e = L.interior(i, j, k);
+          for (int l = 0; l < NFE; l++) ucur[l] = ub.ld(l, e);
+      }
+      if constexpr (!CARRY && FROMD) {
+        if (k > 0)
 #pragma unroll
-      for (int l = 0; l < NFE; l++) u0v[l] = u0.ld_nt(l, (unsigned)e);
+          for (int l = 0; l < NFE; l++) dk[l] = db.ld_nt(l, e);
+      }
+      if (has_u0)
+#pragma unroll
+        for (int l = 0; l < NFE; l++) u0v[l] = u0.ld_nt(l, e);
     }
     __syncthreads();
-
-    // the previous plane's x/y outflow ghosts (stashed in phase D, fenced by the barrier above)
-    if (is_gw && k > 0) write_xy_ghost(k - 1);
 
     // B. x- and y-slopes of every plane cell that a face uses (reconstruct.cpp:81-85)
 #pragma unroll 1
@@ -377,9 +470,11 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
           sl[l] = Sx[l][jj][fi];
           sr[l] = Sx[l][jj][fi + 1];
         }
-        if constexpr (MODE == 0) {
+        if constexpr (FLUX) {
           T F[NFE];
-          face_of<MIXED, 0>(P, pl, sl, pr, sr, F);
+          double e;
+          face_of_e<MIXED, 0>(P, pl, sl, pr, sr, F, e);
+          if constexpr (EIG) emax = x64::rmax(emax, e);
           if (TREE && a.slot) apply_override<N>(a, b, 0, fi, jj, k, F);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fx[l][jj][fi] = F[l];
@@ -397,9 +492,11 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
           sl[l] = Sy[l][fj][ii];
           sr[l] = Sy[l][fj + 1][ii];
         }
-        if constexpr (MODE == 0) {
+        if constexpr (FLUX) {
           T F[NFE];
-          face_of<MIXED, 1>(P, pl, sl, pr, sr, F);
+          double e;
+          face_of_e<MIXED, 1>(P, pl, sl, pr, sr, F, e);
+          if constexpr (EIG) emax = x64::rmax(emax, e);
           if (TREE && a.slot) apply_override<N>(a, b, 1, fj, ii, k, F);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fy[l][fj][ii] = F[l];
@@ -410,16 +507,44 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     }
     T sp1[NFE], Fzhi[NFE];
     slope_of<MIXED>(P, c0, cp1, cp2, sp1);
-    if constexpr (MODE == 0) {
-      face_of<MIXED, 2>(P, c0, s0, cp1, sp1, Fzhi);
+    if constexpr (FLUX) {
+      double e;
+      face_of_e<MIXED, 2>(P, c0, s0, cp1, sp1, Fzhi, e);
+      if constexpr (EIG) emax = x64::rmax(emax, e);
       if (TREE && a.slot) apply_override<N>(a, b, 2, k + 1, i, j, Fzhi);
     } else {
       emax = x64::rmax(emax, eig_of<MIXED, 2>(P, c0, s0, cp1, sp1));
     }
     __syncthreads();
 
-    // D. divergence (x, then y, then z) and the RK update of cell (i, j, k)
-    if constexpr (MODE == 0) {
+    // D. divergence (x, then y, then z), then the RK update of cell (i, j, k) -- or, deferred
+    // (MODE 2), the divergence itself
+    if constexpr (MODE == 2) {
+      const unsigned e = (unsigned)L.interior(i, j, k);
+#pragma unroll
+      for (int l = 0; l < NFE; l++) {
+        double D;
+        if constexpr (!MIXED) {
+          D = 0.0 - (Fx[l][j][i + 1] - Fx[l][j][i]) * hinv;
+          D -= (Fy[l][j + 1][i] - Fy[l][j][i]) * hinv;
+          D -= (Fzhi[l] - Fzlo[l]) * hinv;
+          D += 0.0;  // State::source is zero for the ideal gas (euler/state.cpp:19-22)
+        } else {
+          D = (((double)Fx[l][j][i + 1] - (double)Fx[l][j][i]) + ((double)Fy[l][j + 1][i] - (double)Fy[l][j][i])) +
+              ((double)Fzhi[l] - (double)Fzlo[l]);
+        }
+        db.st_nt(l, e, D);
+      }
+    }
+    if constexpr (UPDATE) {
+      if constexpr (FROMD && !CARRY) {
+        // cell k's first stage again, from U^n (= U0) and D
+        if (k > 0) {
+#pragma unroll
+          for (int l = 0; l < NFE; l++) ucur[l] = u0v[l];
+          first_stage<MIXED>(P, ucur, dk, dt, hinv);
+        }
+      }
       double u[NFE];
 #pragma unroll
       for (int l = 0; l < NFE; l++) {
@@ -473,41 +598,6 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
       const long e = L.interior(i, j, k);
 #pragma unroll
       for (int l = 0; l < NFE; l++) uo.st_nt(l, (unsigned)e, u[l]);
-      if (a.outflow) {
-        // The next stage's ghosts of an isolated outflow block are functions of its own boundary
-        // cells (grid/grid_phys_bound.cpp:27-54), so they are written here instead of in a separate
-        // pass.  x/y boundary cells stash their state in LDS; a fixed set of threads writes those
-        // slabs during the next plane's slope phase (no extra barrier, no divergent tail in every
-        // wavefront).  The z slabs belong to whole planes and are written by every thread.
-        if (i == 0)
-#pragma unroll
-          for (int l = 0; l < NFE; l++) gb[0][j][l] = u[l];
-        if (i == N - 1)
-#pragma unroll
-          for (int l = 0; l < NFE; l++) gb[1][j][l] = u[l];
-        if (j == 0)
-#pragma unroll
-          for (int l = 0; l < NFE; l++) gb[2][i][l] = u[l];
-        if (j == N - 1)
-#pragma unroll
-          for (int l = 0; l < NFE; l++) gb[3][i][l] = u[l];
-        // one rolled copy of the z-slab store serves both faces (keeps the register count down)
-        const bool zl = a.z_reflect ? k < GW : k == 0;
-        if (zl || k == N - 1) {
-          const int f = zl ? ZL : ZU;
-          const bool mirror = a.z_reflect && zl;
-          double g[NFE];
-#pragma unroll
-          for (int l = 0; l < NFE; l++) g[l] = u[l];
-          if (mirror)
-            g[SZ] = -g[SZ];  // ghost G-1-k holds interior cell k mirrored (euler/state.cpp:94-96)
-          else
-            outflow_fix(g, f);
-#pragma unroll 1
-          for (int t = mirror ? GW - 1 - k : 0; t < (mirror ? GW - k : GW); t++)
-            put_ghost<N>(uo, L, f, t, i, j, k, g, false);
-        }
-      }
     }
 
     // advance the column window
@@ -526,6 +616,8 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
       }
     }
     if (have_next) {
+      if constexpr (FROMD) first_stage<MIXED>(P, craw, cd, dt, hinv);
+      ghost_post(craw, cpost);
       if constexpr (CARRY)
 #pragma unroll
         for (int l = 0; l < NFE; l++) uc2[l] = craw[l];
@@ -533,26 +625,24 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     }
   }
 
-  if constexpr (MODE == 0) {
-    if (a.outflow) {  // the last plane's x/y ghosts
-      __syncthreads();
-      if (is_gw) write_xy_ghost(N - 1);
-      __syncthreads();  // gb is rewritten by the next block of a persistent grid
-    }
-  }
-
-  if constexpr (MODE == 1) {
-    // block max: wavefront reduction, then across waves through LDS
+  if constexpr (EIG) {
+    // block max: wavefront reduction, then across waves through LDS; the global max of eig/h as
+    // the ordered bits of a non-negative double (order independent, so deterministic)
     for (int off = 32; off > 0; off >>= 1) emax = x64::rmax(emax, __shfl_xor(emax, off, 64));
     if ((tid & 63) == 0) red[tid >> 6] = emax;
     __syncthreads();
     if (tid == 0) {
       double v = red[0];
       for (int w = 1; w < NT / 64; w++) v = x64::rmax(v, red[w]);
-      if (m.block_eig) m.block_eig[b] = v;
-      if (m.gmax) {
-        const double r = v / m.h[b];
-        atomicMax(m.gmax, (unsigned long long)__double_as_longlong(r));
+      if constexpr (MODE == 1) {
+        if (m.block_eig) m.block_eig[b] = v;
+        if (m.gmax) {
+          const double r = v / m.h[b];
+          atomicMax(m.gmax, (unsigned long long)__double_as_longlong(r));
+        }
+      } else {
+        const double r = v / a.h[b];
+        atomicMax(a.gmax, (unsigned long long)__double_as_longlong(r));
       }
     }
   }
@@ -682,38 +772,52 @@ __global__ void reset_u64_kernel(unsigned long long* p) { *p = 0ull; }
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
-template <int N, bool MIXED>
+template <int N, bool MIXED, int MODE>
 static void launch_flat_variant(const SubstepArgs& a, int g, int variant, hipStream_t s) {
   MaxEigArgs m{};
   switch (variant) {
-    case 1: kt_sweep_kernel<N, MIXED, 0, false, 2><<<g, N * N, 0, s>>>(a, m); break;
-    case 2: kt_sweep_kernel<N, MIXED, 0, false, 3><<<g, N * N, 0, s>>>(a, m); break;
-    case 3: kt_sweep_kernel<N, MIXED, 0, false, 4><<<g, N * N, 0, s>>>(a, m); break;
-    default: kt_sweep_kernel<N, MIXED, 0, false><<<g, N * N, 0, s>>>(a, m); break;
+    case 1: kt_sweep_kernel<N, MIXED, MODE, false, 2><<<g, N * N, 0, s>>>(a, m); break;
+    case 2: kt_sweep_kernel<N, MIXED, MODE, false, 3><<<g, N * N, 0, s>>>(a, m); break;
+    case 3: kt_sweep_kernel<N, MIXED, MODE, false, 4><<<g, N * N, 0, s>>>(a, m); break;
+    default: kt_sweep_kernel<N, MIXED, MODE, false><<<g, N * N, 0, s>>>(a, m); break;
   }
 }
 
+template <int N, bool MIXED>
+static void launch_flat_mode(const SubstepArgs& a, int g, int variant, int mode, hipStream_t s) {
+  if (mode == 2)
+    launch_flat_variant<N, MIXED, 2>(a, g, variant, s);
+  else if (mode == 3)
+    launch_flat_variant<N, MIXED, 3>(a, g, variant, s);
+  else
+    launch_flat_variant<N, MIXED, 0>(a, g, variant, s);
+}
+
 template <int N>
-static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, int grid, int variant, hipStream_t s) {
+static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, int grid, int variant, int mode, hipStream_t s) {
   MaxEigArgs m{};
   if (a.nblocks <= 0) return hipSuccess;
   const int g = (grid > 0 && grid < a.nblocks) ? grid : a.nblocks;
   const bool tree = a.slot != nullptr || a.ledger != nullptr;
+  if (mode != 0 && (tree || !a.outflow || !a.dbuf || (mode == 2 && !a.gmax) ||
+                    (mode == 3 && (!a.dt_dev || a.beta == 1.0 || a.U0 != a.Uin))))
+    return hipErrorInvalidValue;
   if (mixed && tree)
     kt_sweep_kernel<N, true, 0, true><<<g, N * N, 0, s>>>(a, m);
   else if (mixed)
-    launch_flat_variant<N, true>(a, g, variant, s);
+    launch_flat_mode<N, true>(a, g, variant, mode, s);
   else if (tree)
     kt_sweep_kernel<N, false, 0, true><<<g, N * N, 0, s>>>(a, m);
   else
-    launch_flat_variant<N, false>(a, g, variant, s);
+    launch_flat_mode<N, false>(a, g, variant, mode, s);
   return hipGetLastError();
 }
 
-hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s, int variant) {
+hipError_t launch_substep(int n, const SubstepArgs& a, bool mixed, int grid, hipStream_t s, int variant, int mode) {
   if (variant < 0 || variant >= kSweepVariants) return hipErrorInvalidValue;
-  if (n == 8) return launch_substep_n<8>(a, mixed, grid, variant, s);
-  if (n == 16) return launch_substep_n<16>(a, mixed, grid, variant, s);
+  if (mode != 0 && mode != 2 && mode != 3) return hipErrorInvalidValue;
+  if (n == 8) return launch_substep_n<8>(a, mixed, grid, variant, mode, s);
+  if (n == 16) return launch_substep_n<16>(a, mixed, grid, variant, mode, s);
   return hipErrorInvalidValue;
 }
 

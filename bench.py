@@ -40,6 +40,11 @@ POISSON_TOLER = 1e-7  # ELL_TOLER (binary/binary_defs.h:64)
 TUNE_KEYS = ("grid_substep", "grid_maxeig", "variant", "substep_ms_default_launch", "substep_ms_tuned_launch")
 
 
+def amrhydro_mod():
+    import amrhydro
+    return amrhydro
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -68,53 +73,51 @@ def load_traffic(key):
     return d.get("bytes_per_launch"), "measured on this build"
 
 
+STAGE_NAMES = {0: "substep", 1: "cfl_scan", 2: "deferred_first_substep", 3: "dt", 4: "deferred_second_substep",
+               5: "tree_ghosts"}
+SUBSTEP_KINDS = (0, 2, 4)
+
+
 def kernel_times(B, torch, cells, steps, warmup):
-    """RK3 steps on a prepared batch (state uploaded, ghosts filled, outflow boundary mode) with
-    direct launches: the device CFL dt, store, three substeps; HIP events around every substep
-    launch on the library's stream (torch's current stream).  Returns the substep kernel times and
-    the algorithmic bytes behind the roofline."""
-    stream = torch.cuda.current_stream()
+    """RK3 steps on a prepared batch (state uploaded, outflow boundary mode) with direct launches,
+    one step at a time through amrh_step_timed: HIP events recorded by the library on its stream
+    between the launches of the step (the CFL scan or, for a self-ghosting minmod batch, the
+    deferred first substep that replaces it, the dt kernel, the substeps).  Returns the mean time
+    of each stage kind, the substep kernel times and the algorithmic bytes behind the roofline
+    (the i-th substep of a step has beta = BETAS[i])."""
     dt = torch.zeros(1, dtype=torch.float64, device="cuda")
-    first = [True]
-
-    def step(events=None):
-        # HydroDriver::step_to_time's dt control on the device (euler/hydro_driver.cpp:107-113), at
-        # the CFL limit from the first step on (the reference's first step of a run is smaller; a
-        # step in mid-run is what is timed, and the floor's Newton solve works harder at full dt)
-        if first[0]:
-            B.dt_device(1.0, 0.0, dt.data_ptr())
-            first[0] = False
-        else:
-            B.dt_device(1.0, 1.25, dt.data_ptr())
-        B.store()
-        for beta in BETAS:
-            if events is not None:
-                events.append((torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True), beta))
-                events[-1][0].record(stream)
-            B.substep_dev(dt.data_ptr(), beta)
-            if events is not None:
-                events[-1][1].record(stream)
-
+    # HydroDriver::step_to_time's dt control on the device (euler/hydro_driver.cpp:107-113), at the
+    # CFL limit from the first step on (the reference's first step of a run is smaller; a step in
+    # mid-run is what is timed, and the floor's Newton solve works harder at full dt)
+    B.dt_device(1.0, 0.0, dt.data_ptr())
     for _ in range(warmup):
-        step()
+        B.step_timed(dt.data_ptr(), rk=3)
     torch.cuda.synchronize()
-    events = []
+    stages = []
     for _ in range(steps):
-        step(events)
-    torch.cuda.synchronize()
-    k_ms = [(a.elapsed_time(b), beta) for a, b, beta in events]
+        stages.append(B.step_timed(dt.data_ptr(), rk=3))
     bytes_one = B.substep_bytes(True)
     bytes_other = B.substep_bytes(False)
-    tot_bytes = sum(bytes_one if beta == 1.0 else bytes_other for _, beta in k_ms)
-    tot_ms = sum(t for t, _ in k_ms)
+    sub_ms, sub_bytes, per_kind = [], 0.0, {}
+    beta1_ms, betan_ms = [], []
+    for st in stages:
+        subs = [(k, ms) for k, ms in st if k in SUBSTEP_KINDS]
+        for i, (k, ms) in enumerate(subs):
+            sub_ms.append(ms)
+            (beta1_ms if BETAS[i] == 1.0 else betan_ms).append(ms)
+            sub_bytes += bytes_one if BETAS[i] == 1.0 else bytes_other
+        for k, ms in st:
+            per_kind.setdefault(STAGE_NAMES[k], []).append(ms)
+    tot_ms = sum(sub_ms)
     return {
-        "substep_kernel_ms_avg": tot_ms / len(k_ms),
-        "substep_kernel_ms_beta1": statistics.mean(t for t, b in k_ms if b == 1.0),
-        "substep_kernel_ms_beta_ne1": statistics.mean(t for t, b in k_ms if b != 1.0),
-        "rhs_cell_updates_per_s": len(k_ms) * cells / (tot_ms * 1e-3),
-        "achieved_GBs": tot_bytes / (tot_ms * 1e-3) / 1e9,
-        "bytes_per_launch_avg": tot_bytes / len(k_ms),
-        "bytes_per_cell_update": tot_bytes / len(k_ms) / cells,
+        "stage_ms": {k: statistics.mean(v) for k, v in per_kind.items()},
+        "substep_kernel_ms_avg": tot_ms / len(sub_ms),
+        "substep_kernel_ms_beta1": statistics.mean(beta1_ms),
+        "substep_kernel_ms_beta_ne1": statistics.mean(betan_ms),
+        "rhs_cell_updates_per_s": len(sub_ms) * cells / (tot_ms * 1e-3),
+        "achieved_GBs": sub_bytes / (tot_ms * 1e-3) / 1e9,
+        "bytes_per_launch_avg": sub_bytes / len(sub_ms),
+        "bytes_per_cell_update": sub_bytes / len(sub_ms) / cells,
         "ghost_write_bytes_per_launch": B.ghost_write_bytes(),
     }
 
@@ -140,6 +143,10 @@ def measure(B, torch, cells, steps, warmup, tune=True):
     if tune:
         B.autotune()
         res["launch"] = dict(zip(TUNE_KEYS, B.launch_info()))
+        if B.cfg.physics == amrhydro_mod().AMRH_PHYS_EULER and B.cfg.recon == amrhydro_mod().AMRH_RECON_MINMOD:
+            for k, name in ((0, "deferred_first"), (1, "deferred_second")):
+                res["launch"][name] = dict(zip(("grid", "variant", "ms_default_launch", "ms_tuned_launch"),
+                                               B.deferred_launch_info(k)))
     res.update(kernel_times(B, torch, cells, steps, warmup))
     res.update(graph_steps(B, torch, cells, steps, warmup))
     res["device_error_flags"] = B.error_flags()

@@ -48,8 +48,11 @@ extern "C" {
 /* ghost layers of a flat batch (amrh_config.boundary) */
 #define AMRH_BOUNDARY_EXTERNAL 0 /* the caller fills the ghost slabs before every substep */
 #define AMRH_BOUNDARY_OUTFLOW 1  /* every block is an isolated domain with outflow faces
-                                    (z-mirror below with z_reflect): each substep writes the next
-                                    stage's ghost slabs as it stores the boundary cells */
+                                    (z-mirror below with z_reflect).  Minmod ideal gas: the
+                                    kernels form each ghost from the boundary cell it copies where
+                                    they read it, and the ghost slabs in memory are not maintained
+                                    (amrh_download and amrh_fill_outflow_ghosts form them).  PPM
+                                    and the rotating state: a ghost fill follows every substep. */
 
 /* device error flags (amrh_error) */
 #define AMRH_DERR_NEG_DENSITY 1 /* a face or cell state with rho <= 0 reached the flux */
@@ -124,7 +127,8 @@ int amrh_fill_outflow_ghosts(amrh_ctx* ctx);
 /* One RK substep over every block, fused: clear_difs, compute_{x,y,z}_flux, sum_{x,y,z}_difs,
  * add_difs with floor (grid/grid.cpp:201-344, 406-420):
  *   U <- (U + dt*D(U))*beta + U0*(1-beta),  D = -sum_d (F_{d,i+1}-F_{d,i})/h.
- * Ghost slabs of the current stage must be filled. */
+ * Ghost slabs of the current stage must be filled (AMRH_BOUNDARY_EXTERNAL; PPM and rotating
+ * AMRH_BOUNDARY_OUTFLOW batches after amrh_fill_outflow_ghosts or a substep). */
 int amrh_substep(amrh_ctx* ctx, double dt, double beta);
 /* Same, with dt read from device memory when the kernel runs (e.g. written by amrh_dt_device, or
  * replayed from a captured graph); dt_dev points to one double in device memory. */
@@ -148,8 +152,9 @@ int amrh_error(amrh_ctx* ctx, int* flags);
 
 /* Bytes of HBM one substep over the batch moves by the algorithmic count: the interior and its six
  * face ghost slabs read, the interior written, U0's interior read when beta != 1; for the rotating
- * state also the potential over the interior and its six face halos.  The next stage's ghost slabs
- * an AMRH_BOUNDARY_OUTFLOW substep writes belong to the ghost refresh: amrh_ghost_write_bytes. */
+ * state also the potential over the interior and its six face halos.  amrh_ghost_write_bytes: the
+ * ghost slabs the ghost refresh of an AMRH_BOUNDARY_OUTFLOW batch writes after each substep (PPM
+ * and the rotating state; 0 for the self-ghosting minmod batches, which write none). */
 double amrh_substep_bytes(const amrh_ctx* ctx, int beta_is_one);
 double amrh_ghost_write_bytes(const amrh_ctx* ctx);
 
@@ -176,6 +181,27 @@ int amrh_set_variant(amrh_ctx* ctx, int variant);
  * hipGraph (two graphs, one per ping-pong parity, captured on first use; a new dt_dev, rk or
  * launch shape re-captures).  Asynchronous. */
 int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_graph);
+/* In amrh_run_steps and amrh_step_timed, an RK2/RK3 step of a flat AMRH_BOUNDARY_OUTFLOW batch of
+ * the minmod ideal gas runs with its dt deferred: the first substep forms the fluxes of U^n, reduces
+ * their KT face speeds -- the numbers Grid::max_dt's scan forms (grid/grid.cpp:422-483) -- to the
+ * CFL maximum and stores the divergence D; the dt kernel applies the control above; the second
+ * substep forms U1 = floor(U^n + D dt) for every cell it reads.  No separate CFL scan, no store:
+ * the same bits as amrh_dt_device + amrh_store + amrh_substep_dev, one pass over the state fewer.
+ * amrh_autotune also picks the launch shapes of those two kernels; amrh_deferred_launch_info
+ * returns them (kernel 0: the first substep, 1: the second), as amrh_launch_info does. */
+int amrh_deferred_launch_info(const amrh_ctx* ctx, int kernel, int* grid, int* variant, double* ms_default,
+                              double* ms_tuned);
+/* Stages of one timed step (amrh_step_timed) */
+#define AMRH_LAUNCH_SUBSTEP 0 /* a substep (with its ghost fill where one follows it) */
+#define AMRH_LAUNCH_SCAN 1    /* the CFL scan and the dt kernel */
+#define AMRH_LAUNCH_FIRST 2   /* the deferred first substep (fluxes, CFL maximum, D) */
+#define AMRH_LAUNCH_DT 3      /* the dt kernel of a deferred step */
+#define AMRH_LAUNCH_SECOND 4  /* the substep after it (first stage formed from U^n and D) */
+#define AMRH_LAUNCH_GHOSTS 5  /* the octree's ghost fill at the start of a step */
+/* One step of amrh_run_steps (use_graph = 0) with HIP events recorded on the context's stream
+ * between its stages: ms[s] is stage s's duration and kinds[s] its AMRH_LAUNCH_* kind, for
+ * s < *nstages <= max_stages.  Synchronizes.  A measurement aid; no counterpart in the reference. */
+int amrh_step_timed(amrh_ctx* ctx, int rk, double* dt_dev, float* ms, int* kinds, int max_stages, int* nstages);
 
 /* ---- octree coupling (GridNode recursions, grid/grid_node.cpp) -------------------------------
  * Block b of the context is node b of the tree.  Node 0 is the root (parent[0] = -1); every other
