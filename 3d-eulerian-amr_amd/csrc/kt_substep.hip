@@ -131,7 +131,17 @@ __device__ __forceinline__ void outflow_fix(double* u, int face) {
   }
 }
 
-constexpr int kMirror = 6;  // ghost post-processing: the lower z face's mirror (sz -> -sz)
+constexpr int kMirror = 6;
+
+// position of boundary column (i, j) of an N x N plane along its perimeter, 0 .. 4N-5 (each corner
+// once): the row j = 0, then the column i = N-1, the row j = N-1 backwards, the column i = 0 down
+template <int N>
+__device__ __forceinline__ int perim(int i, int j) {
+  if (j == 0) return i;
+  if (i == N - 1) return N - 1 + j;
+  if (j == N - 1) return 3 * N - 3 - i;
+  return 4 * N - 4 - j;
+}  // ghost post-processing: the lower z face's mirror (sz -> -sz)
 
 // a fetched boundary cell -> the ghost it stands for: none (-1), the outflow fix of a node face,
 // or the z mirror (grid/grid_phys_bound.cpp:27-54, euler/state.cpp:94-96)
@@ -146,10 +156,10 @@ __device__ __forceinline__ void ghost_post(double* u, int post) {
 // a stored cell (interior or ghost slab), or for a self-ghosting outflow block the boundary cell
 // the ghost copies -- the nearest plane, or plane -1-kk mirrored below with z_reflect (the ghost
 // at padded z = t holds padded 2G-1-t, grid/grid_phys_bound.cpp:139-166).
-template <int N>
-__device__ __forceinline__ unsigned col_src(const Layout& L, int i, int j, int kk, bool self, bool zr, int& post) {
+template <int N, bool SELF>
+__device__ __forceinline__ unsigned col_src(const Layout& L, int i, int j, int kk, bool zr, int& post) {
   post = -1;
-  if (!self || (kk >= 0 && kk < N)) return (unsigned)col_elem<N>(L, i, j, kk);
+  if (!SELF || (kk >= 0 && kk < N)) return (unsigned)col_elem<N>(L, i, j, kk);
   if (kk < 0) {
     post = zr ? kMirror : (int)ZL;
     return (unsigned)L.interior(i, j, zr ? -1 - kk : 0);
@@ -206,9 +216,11 @@ __device__ __forceinline__ void face_of_e(const PhysParams& P, const typename Pr
 // Per plane k: stage the plane's primitives (with x/y ghosts) in LDS; x- and y-slopes of every
 // cell once; the x/y faces from two cells and their slopes; the z-face from the column window;
 // then each thread updates its cell.  Three barriers per plane.
-// Outflow blocks (a.outflow / m.outflow) are self-ghosting: each ghost is formed from the
-// boundary cell it copies (grid/grid_phys_bound.cpp:27-54) where it is used, so their ghost slabs
-// are neither read nor written.
+// SELF: the blocks are isolated outflow domains (a.outflow / m.outflow), self-ghosting: each ghost
+// is formed from the boundary cell it copies (grid/grid_phys_bound.cpp:27-54) where it is used, so
+// their ghost slabs are neither read nor written.  A column's z ghosts come from its own end cells;
+// the x/y ghosts of plane k from the boundary columns' input state of that plane, which their
+// owners leave in an LDS ring (three planes deep) as they fetch it, three planes ahead.
 // Occupancy, measured on the flat 4096 x 16^3 batch: the FP64 substep runs two waves per SIMD and
 // reads its cell's conserved state back from memory (L2) at each plane instead of carrying three
 // planes of FP64 column state in registers, which removes its spills (1.88 -> 1.54 ms; on the
@@ -222,7 +234,7 @@ __device__ __forceinline__ void face_of_e(const PhysParams& P, const typename Pr
 // WPE: the waves per SIMD the register allocation is held to (the register budget: 2 -> 256 VGPRs,
 // 3 -> 168, 4 -> 128); 0 takes the measured default above.  The flat substep is instantiated for
 // 2, 3 and 4 as well, and the autotuner picks among them with the grid.
-template <int N, bool MIXED, int MODE, bool TREE, int WPE = 0>
+template <int N, bool MIXED, int MODE, bool TREE, bool SELF, int WPE = 0>
 __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : (MIXED ? (N == 16 ? 3 : 1) : 2)))) void kt_sweep_kernel(
     SubstepArgs a, MaxEigArgs m) {
   using T = typename Prec<MIXED>::T;
@@ -231,6 +243,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   constexpr bool UPDATE = MODE == 0 || MODE == 3;  // RK combination, floor, store
   constexpr bool FROMD = MODE == 3;                // input: the first stage from U^n and D
   static_assert(!TREE || MODE <= 1, "the octree instantiations are the substep and the scan");
+  static_assert(!(TREE && SELF) && (MODE < 2 || SELF), "deferred steps run on self-ghosting flat batches");
   constexpr int PW = N + 2 * GW;  // padded plane edge
   constexpr int NT = N * N;
   constexpr int NFX = N * (N + 1);  // x faces of a plane (= y faces)
@@ -246,6 +259,9 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   __shared__ T Fx[FLUX ? NFE : 1][N][N + 1];
   __shared__ T Fy[FLUX ? NFE : 1][N + 1][N];
   __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+  // SELF: the input state of the boundary cells of planes k, k+1, k+2 (ring slot = plane % 3),
+  // by position along the perimeter
+  __shared__ double gst[SELF ? 3 : 1][SELF ? 4 * N - 4 : 1][NFE];
 
   // persistent: a grid smaller than the batch walks it with a stride (launch shape chosen by the
   // autotuner); with one workgroup per block the loop runs once
@@ -260,7 +276,6 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   const double beta = a.beta;
   if (FROMD || (UPDATE && a.dt_dev)) dt = *a.dt_dev;
   const bool has_u0 = UPDATE && beta != 1.0;
-  const bool self = (MODE == 1 ? m.outflow : a.outflow) != 0;  // self-ghosting outflow block
   const bool zr = (MODE == 1 ? m.z_reflect : a.z_reflect) != 0;
   double* const Uob = UPDATE ? a.Uout + (long)b * NFE * R : nullptr;
   const RecBuf<R, NFE> ub(Ub), uo(Uob), u0(UPDATE && a.U0 ? a.U0 + (long)b * NFE * R : nullptr);
@@ -275,9 +290,19 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   double ucur[UPDATE ? NFE : 1], uc1[CARRY ? NFE : 1], uc2[CARRY ? NFE : 1];
   T c0[NFE], cp1[NFE], cp2[NFE], s0[NFE], Fzlo[NFE];
   double emax = 0.0;
+  // an interior cell's input state kk into the ring, where its column is a boundary column
+  auto stash = [&](int kk, const double* u) {
+    if constexpr (SELF) {
+      if (i == 0 || i == N - 1 || j == 0 || j == N - 1) {
+        const int q = perim<N>(i, j);
+#pragma unroll
+        for (int l = 0; l < NFE; l++) gst[kk % 3][q][l] = u[l];
+      }
+    }
+  };
   auto fetch_col = [&](int kk, double* u) {
     int post;
-    const unsigned e = col_src<N>(L, i, j, kk, self, zr, post);
+    const unsigned e = col_src<N, SELF>(L, i, j, kk, zr, post);
 #pragma unroll
     for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, e);
     if constexpr (FROMD) {
@@ -296,16 +321,19 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     fetch_col(-1, u);
     make_prim<MIXED>(u, cm1);
     fetch_col(0, u);
+    stash(0, u);
     if constexpr (UPDATE)
 #pragma unroll
       for (int l = 0; l < NFE; l++) ucur[l] = u[l];
     make_prim<MIXED>(u, c0);
     fetch_col(1, u);
+    stash(1, u);
     if constexpr (CARRY)
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc1[l] = u[l];
     make_prim<MIXED>(u, cp1);
     fetch_col(2, u);
+    stash(2, u);
     if constexpr (CARRY)
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc2[l] = u[l];
@@ -326,76 +354,64 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     }
   }
 
-  // halo raw values for plane 0: a stored ghost-slab cell, or (self-ghosting) the boundary cell
-  // the ghost copies, fixed for its face when it is staged
-  double hraw[NFE], hd[FROMD ? NFE : 1];
+  // the x/y halo: thread tid < 4*G*N stages ghost (face, r) of each plane -- from the stored
+  // ghost slab (its raw value prefetched a plane ahead) or, SELF, from the ring
+  double hraw[SELF ? 1 : NFE];
   const bool is_halo = tid < 4 * HN;
-  int hpx = 0, hpy = 0, hpost = -1;
-  unsigned hbase = 0, hstride = 0;
+  int hpx = 0, hpy = 0, hface = 0, hpos = 0;
+  unsigned hbase = 0;
   if (is_halo) {
     const int face = tid / HN, r = tid % HN;
-    int si, sj;  // the boundary cell of a self-ghosting block
+    hface = face;
     if (face < 2) {
       const int jj = r / GW, t = r % GW;
       hpx = face == XL ? t : N + GW + t;
       hpy = jj + GW;
-      si = face == XL ? 0 : N - 1;
-      sj = jj;
+      hpos = perim<N>(face == XL ? 0 : N - 1, jj);
     } else {
       const int t = r / N, ii = r % N;
       hpx = ii + GW;
       hpy = face == YL ? t : N + GW + t;
-      si = ii;
-      sj = face == YL ? 0 : N - 1;
+      hpos = perim<N>(ii, face == YL ? 0 : N - 1);
     }
-    if (self) {
-      hpost = face;
-      hbase = (unsigned)L.interior(si, sj, 0);
-      hstride = NT;
-#pragma unroll
-      for (int l = 0; l < NFE; l++) hraw[l] = ub.ld(l, hbase);
-      if constexpr (FROMD)
-#pragma unroll
-        for (int l = 0; l < NFE; l++) hd[l] = db.ld(l, hbase);
-    } else {
+    if constexpr (!SELF) {
       hbase = (unsigned)(L.slab(face) + r);  // plane k adds k*HN (both slab kinds are plane-major)
-      hstride = HN;
 #pragma unroll
       for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, hbase);
     }
   }
+  if constexpr (SELF) __syncthreads();  // the ring's first three planes
 
   for (int k = 0; k < N; k++) {
-    // A. stage plane k (interior from registers, halo from the prefetched raw values)
+    // A. stage plane k (interior from registers, halo from the prefetched raw values or the ring)
 #pragma unroll
     for (int l = 0; l < NFE; l++) plane[l][j + GW][i + GW] = c0[l];
     if (is_halo) {
       T hp[NFE];
-      if constexpr (FROMD) first_stage<MIXED>(P, hraw, hd, dt, hinv);
-      ghost_post(hraw, hpost);
-      make_prim<MIXED>(hraw, hp);
+      if constexpr (SELF) {
+        double g[NFE];
+#pragma unroll
+        for (int l = 0; l < NFE; l++) g[l] = gst[k % 3][hpos][l];
+        ghost_post(g, hface);
+        make_prim<MIXED>(g, hp);
+      } else {
+        make_prim<MIXED>(hraw, hp);
+      }
 #pragma unroll
       for (int l = 0; l < NFE; l++) plane[l][hpy][hpx] = hp[l];
     }
-    // prefetch: halo of plane k+1, column cell k+3, U0 at k
-    if (is_halo && k + 1 < N) {
-      const unsigned e = hbase + (unsigned)(k + 1) * hstride;
-      if (self) {
+    // prefetch: halo of plane k+1 (stored slabs), column cell k+3, U0 at k
+    if constexpr (!SELF) {
+      if (is_halo && k + 1 < N) {
 #pragma unroll
-        for (int l = 0; l < NFE; l++) hraw[l] = ub.ld(l, e);
-        if constexpr (FROMD)
-#pragma unroll
-          for (int l = 0; l < NFE; l++) hd[l] = db.ld(l, e);
-      } else {
-#pragma unroll
-        for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, e);
+        for (int l = 0; l < NFE; l++) hraw[l] = ub.ld_nt(l, hbase + (unsigned)((k + 1) * HN));
       }
     }
     double craw[NFE], cd[FROMD ? NFE : 1];
     int cpost = -1;
     const bool have_next = (k + 3) < N + GW;
     if (have_next) {
-      const unsigned e = col_src<N>(L, i, j, k + 3, self, zr, cpost);
+      const unsigned e = col_src<N, SELF>(L, i, j, k + 3, zr, cpost);
 #pragma unroll
       for (int l = 0; l < NFE; l++) craw[l] = ub.ld(l, e);
       if constexpr (FROMD)
@@ -617,6 +633,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     }
     if (have_next) {
       if constexpr (FROMD) first_stage<MIXED>(P, craw, cd, dt, hinv);
+      if (k + 3 < N) stash(k + 3, craw);  // slot of plane k, whose halo was staged above
       ghost_post(craw, cpost);
       if constexpr (CARRY)
 #pragma unroll
@@ -772,25 +789,27 @@ __global__ void reset_u64_kernel(unsigned long long* p) { *p = 0ull; }
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
-template <int N, bool MIXED, int MODE>
+template <int N, bool MIXED, int MODE, bool SELF>
 static void launch_flat_variant(const SubstepArgs& a, int g, int variant, hipStream_t s) {
   MaxEigArgs m{};
   switch (variant) {
-    case 1: kt_sweep_kernel<N, MIXED, MODE, false, 2><<<g, N * N, 0, s>>>(a, m); break;
-    case 2: kt_sweep_kernel<N, MIXED, MODE, false, 3><<<g, N * N, 0, s>>>(a, m); break;
-    case 3: kt_sweep_kernel<N, MIXED, MODE, false, 4><<<g, N * N, 0, s>>>(a, m); break;
-    default: kt_sweep_kernel<N, MIXED, MODE, false><<<g, N * N, 0, s>>>(a, m); break;
+    case 1: kt_sweep_kernel<N, MIXED, MODE, false, SELF, 2><<<g, N * N, 0, s>>>(a, m); break;
+    case 2: kt_sweep_kernel<N, MIXED, MODE, false, SELF, 3><<<g, N * N, 0, s>>>(a, m); break;
+    case 3: kt_sweep_kernel<N, MIXED, MODE, false, SELF, 4><<<g, N * N, 0, s>>>(a, m); break;
+    default: kt_sweep_kernel<N, MIXED, MODE, false, SELF><<<g, N * N, 0, s>>>(a, m); break;
   }
 }
 
 template <int N, bool MIXED>
 static void launch_flat_mode(const SubstepArgs& a, int g, int variant, int mode, hipStream_t s) {
   if (mode == 2)
-    launch_flat_variant<N, MIXED, 2>(a, g, variant, s);
+    launch_flat_variant<N, MIXED, 2, true>(a, g, variant, s);
   else if (mode == 3)
-    launch_flat_variant<N, MIXED, 3>(a, g, variant, s);
+    launch_flat_variant<N, MIXED, 3, true>(a, g, variant, s);
+  else if (a.outflow)
+    launch_flat_variant<N, MIXED, 0, true>(a, g, variant, s);
   else
-    launch_flat_variant<N, MIXED, 0>(a, g, variant, s);
+    launch_flat_variant<N, MIXED, 0, false>(a, g, variant, s);
 }
 
 template <int N>
@@ -799,15 +818,16 @@ static hipError_t launch_substep_n(const SubstepArgs& a, bool mixed, int grid, i
   if (a.nblocks <= 0) return hipSuccess;
   const int g = (grid > 0 && grid < a.nblocks) ? grid : a.nblocks;
   const bool tree = a.slot != nullptr || a.ledger != nullptr;
-  if (mode != 0 && (tree || !a.outflow || !a.dbuf || (mode == 2 && !a.gmax) ||
-                    (mode == 3 && (!a.dt_dev || a.beta == 1.0 || a.U0 != a.Uin))))
+  if ((tree && a.outflow) ||
+      (mode != 0 && (tree || !a.outflow || !a.dbuf || (mode == 2 && !a.gmax) ||
+                     (mode == 3 && (!a.dt_dev || a.beta == 1.0 || a.U0 != a.Uin)))))
     return hipErrorInvalidValue;
   if (mixed && tree)
-    kt_sweep_kernel<N, true, 0, true><<<g, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, true, 0, true, false><<<g, N * N, 0, s>>>(a, m);
   else if (mixed)
     launch_flat_mode<N, true>(a, g, variant, mode, s);
   else if (tree)
-    kt_sweep_kernel<N, false, 0, true><<<g, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, false, 0, true, false><<<g, N * N, 0, s>>>(a, m);
   else
     launch_flat_mode<N, false>(a, g, variant, mode, s);
   return hipGetLastError();
@@ -826,10 +846,14 @@ static hipError_t launch_maxeig_n(const MaxEigArgs& m, bool mixed, int grid, hip
   SubstepArgs a{};
   if (m.nblocks <= 0) return hipSuccess;
   const int g = (grid > 0 && grid < m.nblocks) ? grid : m.nblocks;
-  if (mixed)
-    kt_sweep_kernel<N, true, 1, false><<<g, N * N, 0, s>>>(a, m);
+  if (mixed && m.outflow)
+    kt_sweep_kernel<N, true, 1, false, true><<<g, N * N, 0, s>>>(a, m);
+  else if (mixed)
+    kt_sweep_kernel<N, true, 1, false, false><<<g, N * N, 0, s>>>(a, m);
+  else if (m.outflow)
+    kt_sweep_kernel<N, false, 1, false, true><<<g, N * N, 0, s>>>(a, m);
   else
-    kt_sweep_kernel<N, false, 1, false><<<g, N * N, 0, s>>>(a, m);
+    kt_sweep_kernel<N, false, 1, false, false><<<g, N * N, 0, s>>>(a, m);
   return hipGetLastError();
 }
 

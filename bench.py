@@ -735,9 +735,13 @@ def main():
 
     head = results.get("mixed") or results.get("fp64")
     head_name = "mixed" if "mixed" in results else "fp64"
-    wpe = {0: "default", 1: 2, 2: 3, 3: 4}[head["launch"]["variant"]]
+    wpe = {0: "default", 1: 2, 2: 3, 3: 4}
+    ln = head["launch"]
     roof = dict(head["roofline"])
-    roof["kernel"] = "kt_sweep_kernel<%d, MIXED=%s, substep, flat, waves/SIMD %s>" % (n, head_name == "mixed", wpe)
+    roof["kernel"] = ("kt_sweep_kernel<%d, MIXED=%s, flat, self-ghosting>, the three substeps of the deferred-dt RK3 "
+                      "step: first (fluxes + CFL max + D; waves/SIMD %s), second (first stage from U^n and D; %s), "
+                      "third (plain substep; %s)" % (n, head_name == "mixed", wpe[ln["deferred_first"]["variant"]],
+                                                     wpe[ln["deferred_second"]["variant"]], wpe[ln["variant"]]))
     out = {
         "metric": METRIC,
         "value": head["cell_updates_per_s"],
