@@ -197,8 +197,10 @@ int amrh_upload(amrh_ctx* c, const double* U) {
 int amrh_download_device(amrh_ctx* c, void* pad) {
   if (!c || !pad) return fail(AMRH_EINVAL, "null argument");
   if (!c->have_u) return fail(AMRH_ESTATE, "no state uploaded");
-  // a self-ghosting batch does not maintain its ghost slabs: form them for the copy
-  if (self_ghosting(c)) HIPCHK(ctx_outflow_ghosts(c, c->buf[c->cur]));
+  // a self-ghosting batch does not maintain its ghost slabs: form them for the copy (the rotating
+  // state's need the block origins)
+  if (self_ghosting(c) && (c->cfg.physics != AMRH_PHYS_ROTATING || c->have_x0))
+    HIPCHK(ctx_outflow_ghosts(c, c->buf[c->cur]));
   HIPCHK(hipMemsetAsync(pad, 0, sizeof(double) * (size_t)padded_elems(c), c->stream));
   HIPCHK(launch_pack(c->cfg.n, c->cfg.g, (const double*)pad, c->buf[c->cur], record_elems(c), false, c->stream));
   return AMRH_OK;
@@ -364,10 +366,7 @@ int amrh::ctx_ready(const amrh_ctx* c) {
   return AMRH_OK;
 }
 
-bool amrh::self_ghosting(const amrh_ctx* c) {
-  return !c->tree && c->cfg.boundary == AMRH_BOUNDARY_OUTFLOW && c->cfg.physics == AMRH_PHYS_EULER &&
-         c->cfg.recon == AMRH_RECON_MINMOD;
-}
+bool amrh::self_ghosting(const amrh_ctx* c) { return !c->tree && c->cfg.boundary == AMRH_BOUNDARY_OUTFLOW; }
 
 SubstepArgs amrh::flat_args(const amrh_ctx* c) {
   SubstepArgs a{};
@@ -399,15 +398,13 @@ hipError_t amrh::ctx_substep_launch(const amrh_ctx* c, const SubstepArgs& a, int
     r.dt_dev = a.dt_dev;
     r.dt = a.dt;
     r.beta = a.beta;
-    hipError_t e = launch_rot_substep(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, grid, c->stream);
-    if (e == hipSuccess && a.outflow) e = ctx_outflow_ghosts(c, a.Uout);
-    return e;
+    r.outflow = a.outflow;
+    r.z_reflect = a.z_reflect;
+    return launch_rot_substep(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, grid, c->stream);
   }
   if (c->cfg.recon == AMRH_RECON_MINMOD)
     return launch_substep(c->cfg.n, a, mixed, grid, c->stream, variant >= 0 ? variant : c->variant);
-  hipError_t e = launch_block_substep(c->cfg.n, c->cfg.g, c->cfg.recon, a, mixed, grid, c->stream);
-  if (e == hipSuccess && a.outflow) e = launch_outflow_ghosts(c->cfg.n, c->cfg.g, a.Uout, c->nb, c->cfg.z_reflect, c->stream);
-  return e;
+  return launch_block_substep(c->cfg.n, c->cfg.g, c->cfg.recon, a, mixed, grid, c->stream);
 }
 
 hipError_t amrh::ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid) {
@@ -417,6 +414,8 @@ hipError_t amrh::ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int g
     r.Uin = m.U;
     r.block_eig = m.block_eig;
     r.gmax = m.gmax;
+    r.outflow = m.outflow;
+    r.z_reflect = m.z_reflect;
     return launch_rot_maxeig(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, grid, c->stream);
   }
   if (c->cfg.recon == AMRH_RECON_MINMOD) return launch_maxeig(c->cfg.n, m, mixed, grid, c->stream);

@@ -99,8 +99,8 @@ hipError_t ctx_maxeig_launch(const amrh_ctx* c, const MaxEigArgs& m, int grid);
 int ctx_ready(const amrh_ctx* c);
 // ghost fill of isolated outflow blocks for the context's physics
 hipError_t ctx_outflow_ghosts(const amrh_ctx* c, double* U);
-// a flat batch of isolated outflow blocks on the minmod sweep: its kernels form every ghost from
-// the boundary cell it copies where it is read, so the ghost slabs in memory are not maintained
+// a flat batch of isolated outflow blocks: its kernels form every ghost from the boundary cell it
+// copies where they read it, so the ghost slabs in memory are not maintained
 bool self_ghosting(const amrh_ctx* c);
 // the substep arguments of a flat batch (no octree), input = the current stage
 SubstepArgs flat_args(const amrh_ctx* c);

@@ -90,7 +90,10 @@ int ensure_dbuf(amrh_ctx* c) {
 // dt, euler/hydro_driver.cpp:107-113); the second substep forms U1 = floor(U^n + D dt) for every
 // cell it reads, with U^n as its RK base.  The buffers end as after scan + store + rk substeps
 // (U^n in the RK base, the state in the other buffer), and every cell has the same bits.
-bool deferred_ok(const amrh_ctx* c, int rk) { return rk >= 2 && self_ghosting(c) && c->dbuf; }
+bool deferred_capable(const amrh_ctx* c) {
+  return self_ghosting(c) && c->cfg.physics == AMRH_PHYS_EULER && c->cfg.recon == AMRH_RECON_MINMOD;
+}
+bool deferred_ok(const amrh_ctx* c, int rk) { return rk >= 2 && deferred_capable(c) && c->dbuf; }
 
 int step_deferred(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
   const bool mixed = c->cfg.precision == AMRH_MIXED;
@@ -222,7 +225,7 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   // writing only D, the CFL word (cleared after) and the spare buffer
   int gd[2] = {0, 0}, vd[2] = {0, 0};
   float msd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
-  if (self_ghosting(c)) {
+  if (deferred_capable(c)) {
     if (int rc = ensure_dbuf(c)) return rc;
     const bool mixed = c->cfg.precision == AMRH_MIXED;
     HIPCHK(hipMemsetAsync(c->eig_dev, 0, sizeof(double), c->stream));
@@ -321,7 +324,7 @@ int amrh_run_steps(amrh_ctx* c, int rk, int nsteps, double* dt_dev, int use_grap
     return fail(AMRH_ESTATE, "a flat batch runs whole steps only with AMRH_BOUNDARY_OUTFLOW");
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
   if (int rc = ctx_ready(c)) return rc;
-  if (rk >= 2 && self_ghosting(c))
+  if (rk >= 2 && deferred_capable(c))
     if (int rc = ensure_dbuf(c)) return rc;  // before any capture
   if (!use_graph) {
     for (int s = 0; s < nsteps; s++)
@@ -368,7 +371,7 @@ int amrh_step_timed(amrh_ctx* c, int rk, double* dt_dev, float* ms, int* kinds, 
     return fail(AMRH_ESTATE, "a flat batch runs whole steps only with AMRH_BOUNDARY_OUTFLOW");
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
   if (int rc = ctx_ready(c)) return rc;
-  if (rk >= 2 && self_ghosting(c))
+  if (rk >= 2 && deferred_capable(c))
     if (int rc = ensure_dbuf(c)) return rc;
   Marks mk;
   int made = 0, rc = AMRH_OK;

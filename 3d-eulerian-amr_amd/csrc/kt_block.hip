@@ -136,13 +136,21 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
     const double* Ub = (MODE == 0 ? a.Uin : m.U) + (long)b * NFE * R;
     const RecBuf<R, NFE> ub(Ub);
 
-    // stage the primitives; each thread keeps its own interior cell's conserved state
+    // stage the primitives; each thread keeps its own interior cell's conserved state.  An
+    // isolated outflow block (not TREE) forms each ghost from the boundary cell it copies
+    // (grid/grid_phys_bound.cpp:27-54); its ghost slabs are not read.  The slabs are
+    // wavefront-aligned (G N^2 a multiple of 64), so the choice is uniform per wavefront.
+    const bool self = !TREE && (MODE == 0 ? a.outflow : m.outflow) != 0;
+    const bool zr = (MODE == 0 ? a.z_reflect : m.z_reflect) != 0;
     double ucur[NFE];
 #pragma unroll 1
     for (int e = tid; e < R; e += NT) {
       double u[NFE];
+      int post = -1;
+      const unsigned src = (self && e >= NT) ? (unsigned)outflow_source<N, G>(e, zr, post) : (unsigned)e;
 #pragma unroll
-      for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, (unsigned)e);
+      for (int l = 0; l < NFE; l++) u[l] = ub.ld(l, src);
+      ghost_post(u, post);
       if (e == tid) {  // the first pass: e = tid < N^3 is this thread's interior cell
 #pragma unroll
         for (int l = 0; l < NFE; l++) ucur[l] = u[l];

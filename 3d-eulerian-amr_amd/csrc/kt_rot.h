@@ -36,6 +36,10 @@ struct RotArgs {
   // with a same-level neighbour is read from the neighbour's cells (nullptr: every ghost from the
   // record's slabs)
   const int* ghost;
+  // an isolated outflow block (flat batches): each ghost is formed from the boundary cell it copies
+  // where it is staged (grid/grid_phys_bound.cpp:27-54); the ghost slabs are not read
+  int outflow;
+  int z_reflect;
 };
 
 hipError_t launch_rot_substep(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);

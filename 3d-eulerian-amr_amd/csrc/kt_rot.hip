@@ -171,6 +171,24 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           const double* Us = a.Uin + (long)gt[1] * NF * R + (s3[2] * N + s3[1]) * N + s3[0];
 #pragma unroll
           for (int l = 0; l < NF; l++) u[l] = Us[(long)l * R];
+        } else if (!TREE && a.outflow && e >= N3) {
+          // isolated outflow block: the ghost copies the boundary cell next to it, mirrored below
+          // with z_reflect or given enforce_outflow at the boundary face (as rot_ghost_kernel)
+          const int nf = (e - N3) / (G * N2), dd = nf >> 1, up = nf & 1;
+          int s3[3] = {i, j, k};
+          const bool mirror = a.z_reflect && nf == 4;
+          s3[dd] = mirror ? -1 - s3[dd] : (up ? N - 1 : 0);
+          const unsigned es = (unsigned)((s3[2] * N + s3[1]) * N + s3[0]);
+#pragma unroll
+          for (int l = 0; l < NF; l++) u[l] = ub.ld(l, es);
+          if (mirror) {
+            u[rot::SZ] = -u[rot::SZ];
+          } else {
+            double Xb[3];
+            cell_pos(x0, h, s3[0], s3[1], s3[2], Xb);
+            Xb[dd] = x0[dd] + (double)(up ? N : 0) * h;  // the boundary face
+            enforce_outflow(c, u, Xb, nf);
+          }
         } else {
 #pragma unroll
           for (int l = 0; l < NF; l++) u[l] = ub.ld(l, (unsigned)e);

@@ -117,22 +117,6 @@ __device__ __forceinline__ double eig_of(const PhysParams& P, const typename Pre
   }
 }
 
-// outflow ghost of boundary cell u across node face `face` (the boundary object's face is the
-// node's inverted: a lower node face removes v > 0; the z test uses sz, euler/state.cpp:45-84)
-__device__ __forceinline__ void outflow_fix(double* u, int face) {
-  const int d = face >> 1;
-  const bool upper = face & 1;
-  const double v = (d == 2) ? u[SZ] : u[SX + d] / u[RHO];
-  const bool inflow = upper ? (v < 0.0) : (v > 0.0);
-  if (inflow) {
-    const double s = u[SX + d];
-    u[ET] = u[ET] - 0.5 * s * s / u[RHO];
-    u[SX + d] = 0.0;
-  }
-}
-
-constexpr int kMirror = 6;
-
 // position of boundary column (i, j) of an N x N plane along its perimeter, 0 .. 4N-5 (each corner
 // once): the row j = 0, then the column i = N-1, the row j = N-1 backwards, the column i = 0 down
 template <int N>
@@ -143,14 +127,6 @@ __device__ __forceinline__ int perim(int i, int j) {
   return 4 * N - 4 - j;
 }  // ghost post-processing: the lower z face's mirror (sz -> -sz)
 
-// a fetched boundary cell -> the ghost it stands for: none (-1), the outflow fix of a node face,
-// or the z mirror (grid/grid_phys_bound.cpp:27-54, euler/state.cpp:94-96)
-__device__ __forceinline__ void ghost_post(double* u, int post) {
-  if (post == kMirror)
-    u[SZ] = -u[SZ];
-  else if (post >= 0)
-    outflow_fix(u, post);
-}
 
 // Record element of column cell kk in [-G, N+G), and how the fetched cell becomes the value used:
 // a stored cell (interior or ghost slab), or for a self-ghosting outflow block the boundary cell

@@ -48,11 +48,10 @@ extern "C" {
 /* ghost layers of a flat batch (amrh_config.boundary) */
 #define AMRH_BOUNDARY_EXTERNAL 0 /* the caller fills the ghost slabs before every substep */
 #define AMRH_BOUNDARY_OUTFLOW 1  /* every block is an isolated domain with outflow faces
-                                    (z-mirror below with z_reflect).  Minmod ideal gas: the
-                                    kernels form each ghost from the boundary cell it copies where
-                                    they read it, and the ghost slabs in memory are not maintained
-                                    (amrh_download and amrh_fill_outflow_ghosts form them).  PPM
-                                    and the rotating state: a ghost fill follows every substep. */
+                                    (z-mirror below with z_reflect): the kernels form each ghost
+                                    from the boundary cell it copies where they read it, and the
+                                    ghost slabs in memory are not maintained (amrh_download and
+                                    amrh_fill_outflow_ghosts form them) */
 
 /* device error flags (amrh_error) */
 #define AMRH_DERR_NEG_DENSITY 1 /* a face or cell state with rho <= 0 reached the flux */
@@ -127,8 +126,7 @@ int amrh_fill_outflow_ghosts(amrh_ctx* ctx);
 /* One RK substep over every block, fused: clear_difs, compute_{x,y,z}_flux, sum_{x,y,z}_difs,
  * add_difs with floor (grid/grid.cpp:201-344, 406-420):
  *   U <- (U + dt*D(U))*beta + U0*(1-beta),  D = -sum_d (F_{d,i+1}-F_{d,i})/h.
- * Ghost slabs of the current stage must be filled (AMRH_BOUNDARY_EXTERNAL; PPM and rotating
- * AMRH_BOUNDARY_OUTFLOW batches after amrh_fill_outflow_ghosts or a substep). */
+ * With AMRH_BOUNDARY_EXTERNAL the ghost slabs of the current stage must be filled. */
 int amrh_substep(amrh_ctx* ctx, double dt, double beta);
 /* Same, with dt read from device memory when the kernel runs (e.g. written by amrh_dt_device, or
  * replayed from a captured graph); dt_dev points to one double in device memory. */
@@ -153,8 +151,8 @@ int amrh_error(amrh_ctx* ctx, int* flags);
 /* Bytes of HBM one substep over the batch moves by the algorithmic count: the interior and its six
  * face ghost slabs read, the interior written, U0's interior read when beta != 1; for the rotating
  * state also the potential over the interior and its six face halos.  amrh_ghost_write_bytes: the
- * ghost slabs the ghost refresh of an AMRH_BOUNDARY_OUTFLOW batch writes after each substep (PPM
- * and the rotating state; 0 for the self-ghosting minmod batches, which write none). */
+ * ghost slabs a separate ghost refresh would write per substep; 0 for AMRH_BOUNDARY_OUTFLOW
+ * batches, whose kernels form their ghosts where they read them and write none. */
 double amrh_substep_bytes(const amrh_ctx* ctx, int beta_is_one);
 double amrh_ghost_write_bytes(const amrh_ctx* ctx);
 
