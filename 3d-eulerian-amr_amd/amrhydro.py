@@ -40,6 +40,7 @@ class Config(C.Structure):
 AMRH_BOUNDARY_EXTERNAL = 0
 AMRH_BOUNDARY_OUTFLOW = 1
 AMRH_LAUNCH_SUBSTEP, AMRH_LAUNCH_SCAN, AMRH_LAUNCH_FIRST, AMRH_LAUNCH_DT, AMRH_LAUNCH_SECOND, AMRH_LAUNCH_GHOSTS = range(6)
+AMRH_SCHEDULE_PLAIN, AMRH_SCHEDULE_DEFERRED = 0, 1
 STATE_SUM_DEV_MAX_PARTIALS = 3145728  # AMRH_STATE_SUM_DEV_MAX_PARTIALS
 
 
@@ -97,6 +98,8 @@ SIGNATURES = {
     "amrh_run_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "amrh_deferred_launch_info": (C.c_int, [C.c_void_p, C.c_int, IP, IP, DP, DP]),
     "amrh_step_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_float), IP, C.c_int, IP]),
+    "amrh_set_schedule": (C.c_int, [C.c_void_p, C.c_int]),
+    "amrh_get_schedule": (C.c_int, [C.c_void_p, IP]),
     "amrh_tree_set": (C.c_int, [C.c_void_p, IP, IP]),
     "amrh_tree_set_origin": (C.c_int, [C.c_void_p, DP]),
     "amrh_tree_get_origin": (C.c_int, [C.c_void_p, DP]),
@@ -321,6 +324,15 @@ class Batch:
         d, t = C.c_double(), C.c_double()
         _ck(lib().amrh_deferred_launch_info(self.ctx, kernel, C.byref(g), C.byref(v), C.byref(d), C.byref(t)))
         return g.value, v.value, d.value, t.value
+
+    def set_schedule(self, schedule):
+        _ck(lib().amrh_set_schedule(self.ctx, schedule))
+
+    def schedule(self):
+        """AMRH_SCHEDULE_DEFERRED or AMRH_SCHEDULE_PLAIN: how run_steps runs RK2/RK3 steps"""
+        v = C.c_int()
+        _ck(lib().amrh_get_schedule(self.ctx, C.byref(v)))
+        return v.value
 
     def step_timed(self, dt_ptr, rk=3):
         """One step of run_steps (direct launches) timed stage by stage: [(AMRH_LAUNCH_* kind, ms)]"""

@@ -76,6 +76,7 @@ struct amrh_ctx {
   // divergence [nb][nf][n^3] (allocated on first use) and the launch shapes of its two kernels
   // (index 0: the deferred first substep, 1: the substep that follows it)
   double* dbuf = nullptr;
+  int schedule = AMRH_SCHEDULE_DEFERRED;  // where deferred steps apply; amrh_autotune picks the faster
   int grid_deferred[2] = {0, 0};
   int variant_deferred[2] = {0, 0};
   float tune_ms_deferred[2][2] = {{0.f, 0.f}, {0.f, 0.f}};  // [kernel][default, tuned]

@@ -93,7 +93,9 @@ int ensure_dbuf(amrh_ctx* c) {
 bool deferred_capable(const amrh_ctx* c) {
   return self_ghosting(c) && c->cfg.physics == AMRH_PHYS_EULER && c->cfg.recon == AMRH_RECON_MINMOD;
 }
-bool deferred_ok(const amrh_ctx* c, int rk) { return rk >= 2 && deferred_capable(c) && c->dbuf; }
+bool deferred_ok(const amrh_ctx* c, int rk) {
+  return rk >= 2 && deferred_capable(c) && c->dbuf && c->schedule == AMRH_SCHEDULE_DEFERRED;
+}
 
 int step_deferred(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
   const bool mixed = c->cfg.precision == AMRH_MIXED;
@@ -259,6 +261,19 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
       msd[kx][1] = best;
     }
     HIPCHK(launch_reset_u64(c->gmax_dev, c->stream));
+    // the schedule: an RK3 step is the scan, a beta = 1 and two beta != 1 substeps, or the two
+    // deferred kernels and one beta != 1 substep (the dt kernel is common); a beta != 1 substep of
+    // the pick is timed here (U0 = the current stage, output to the spare buffer)
+    float ms_ne1 = 0.f;
+    SubstepArgs an = a;
+    an.beta = 0.25;
+    if (int rc = time_launches(c, 3, [&]() -> int {
+          HIPCHK(ctx_substep_launch(c, an, gs, vs));
+          return AMRH_OK;
+        }, &ms_ne1))
+      return rc;
+    const float plain = best_m + best_s + 2.f * ms_ne1, deferred = msd[0][1] + msd[1][1] + ms_ne1;
+    c->schedule = deferred < plain ? AMRH_SCHEDULE_DEFERRED : AMRH_SCHEDULE_PLAIN;
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   c->grid_substep = gs;
@@ -314,6 +329,21 @@ int amrh_deferred_launch_info(const amrh_ctx* c, int kernel, int* grid, int* var
   if (variant) *variant = c->variant_deferred[kernel];
   if (ms_default) *ms_default = c->tune_ms_deferred[kernel][0];
   if (ms_tuned) *ms_tuned = c->tune_ms_deferred[kernel][1];
+  return AMRH_OK;
+}
+
+int amrh_set_schedule(amrh_ctx* c, int schedule) {
+  if (!c) return fail(AMRH_EINVAL, "null ctx");
+  if (schedule != AMRH_SCHEDULE_PLAIN && schedule != AMRH_SCHEDULE_DEFERRED)
+    return fail(AMRH_EINVAL, "schedule must be AMRH_SCHEDULE_PLAIN or AMRH_SCHEDULE_DEFERRED");
+  c->schedule = schedule;
+  drop_graphs(c);
+  return AMRH_OK;
+}
+
+int amrh_get_schedule(const amrh_ctx* c, int* schedule) {
+  if (!c || !schedule) return fail(AMRH_EINVAL, "null argument");
+  *schedule = deferred_capable(c) ? c->schedule : AMRH_SCHEDULE_PLAIN;
   return AMRH_OK;
 }
 

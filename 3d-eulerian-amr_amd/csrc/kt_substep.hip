@@ -119,6 +119,20 @@ __device__ __forceinline__ double eig_of(const PhysParams& P, const typename Pre
 
 // position of boundary column (i, j) of an N x N plane along its perimeter, 0 .. 4N-5 (each corner
 // once): the row j = 0, then the column i = N-1, the row j = N-1 backwards, the column i = 0 down
+// The mixed form's outflow ghost from the boundary cell's FP32 primitives: the conserved-state
+// fix (outflow_fix) zeroes the inflowing momentum and takes its kinetic energy out of E, which
+// leaves the internal energy; so the ghost's primitives are the cell's with v_d zeroed on inflow
+// (the sign of v_d is that of s_d: rho > 0), or v_z negated for the z mirror.  Exact operations.
+__device__ __forceinline__ void ghost_post_prim(float* p, int post) {
+  if (post == kMirror) {
+    p[SZ] = -p[SZ];
+  } else if (post >= 0) {
+    const int d = post >> 1;
+    const float v = p[SX + d];
+    if ((post & 1) ? (v < 0.0f) : (v > 0.0f)) p[SX + d] = 0.0f;
+  }
+}
+
 template <int N>
 __device__ __forceinline__ int perim(int i, int j) {
   if (j == 0) return i;
@@ -235,9 +249,11 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   __shared__ T Fx[FLUX ? NFE : 1][N][N + 1];
   __shared__ T Fy[FLUX ? NFE : 1][N + 1][N];
   __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
-  // SELF: the input state of the boundary cells of planes k, k+1, k+2 (ring slot = plane % 3),
-  // by position along the perimeter
-  __shared__ double gst[SELF ? 3 : 1][SELF ? 4 * N - 4 : 1][NFE];
+  // SELF: the boundary cells of planes k, k+1, k+2 (ring slot = plane % 3) by position along the
+  // perimeter: their FP32 primitives in the mixed form, their FP64 input state in the FP64 form
+  // (whose ghosts are formed as the reference forms them, from the conserved state)
+  constexpr bool PRIMGHOST = SELF && MIXED;
+  __shared__ typename std::conditional<MIXED, float, double>::type gst[SELF ? 3 : 1][SELF ? 4 * N - 4 : 1][NFE];
 
   // persistent: a grid smaller than the batch walks it with a stride (launch shape chosen by the
   // autotuner); with one workgroup per block the loop runs once
@@ -266,15 +282,28 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   double ucur[UPDATE ? NFE : 1], uc1[CARRY ? NFE : 1], uc2[CARRY ? NFE : 1];
   T c0[NFE], cp1[NFE], cp2[NFE], s0[NFE], Fzlo[NFE];
   double emax = 0.0;
-  // an interior cell's input state kk into the ring, where its column is a boundary column
-  auto stash = [&](int kk, const double* u) {
+  // an interior cell kk (input state u, primitives p) into the ring, where its column is a
+  // boundary column
+  auto stash = [&](int kk, const double* u, const T* p) {
     if constexpr (SELF) {
       if (i == 0 || i == N - 1 || j == 0 || j == N - 1) {
         const int q = perim<N>(i, j);
 #pragma unroll
-        for (int l = 0; l < NFE; l++) gst[kk % 3][q][l] = u[l];
+        for (int l = 0; l < NFE; l++) {
+          if constexpr (MIXED)
+            gst[kk % 3][q][l] = p[l];
+          else
+            gst[kk % 3][q][l] = u[l];
+        }
       }
     }
+  };
+  // the mixed form's z ghost kk (< 0 or >= N) from the primitives of the column's end cell it
+  // copies (zr: ghost -1-m mirrors cell m)
+  auto zghost = [&](int kk, const T* src, T* out) {
+#pragma unroll
+    for (int l = 0; l < NFE; l++) out[l] = src[l];
+    if constexpr (MIXED) ghost_post_prim(out, kk < 0 ? (zr ? kMirror : (int)ZL) : (int)ZU);
   };
   auto fetch_col = [&](int kk, double* u) {
     int post;
@@ -292,28 +321,34 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   {
     double u[NFE];
     T cm2[NFE], cm1[NFE], sm1[NFE];
-    fetch_col(-2, u);
-    make_prim<MIXED>(u, cm2);
-    fetch_col(-1, u);
-    make_prim<MIXED>(u, cm1);
+    if constexpr (!PRIMGHOST) {
+      fetch_col(-2, u);
+      make_prim<MIXED>(u, cm2);
+      fetch_col(-1, u);
+      make_prim<MIXED>(u, cm1);
+    }
     fetch_col(0, u);
-    stash(0, u);
     if constexpr (UPDATE)
 #pragma unroll
       for (int l = 0; l < NFE; l++) ucur[l] = u[l];
     make_prim<MIXED>(u, c0);
+    stash(0, u, c0);
     fetch_col(1, u);
-    stash(1, u);
     if constexpr (CARRY)
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc1[l] = u[l];
     make_prim<MIXED>(u, cp1);
+    stash(1, u, cp1);
     fetch_col(2, u);
-    stash(2, u);
     if constexpr (CARRY)
 #pragma unroll
       for (int l = 0; l < NFE; l++) uc2[l] = u[l];
     make_prim<MIXED>(u, cp2);
+    stash(2, u, cp2);
+    if constexpr (PRIMGHOST) {
+      zghost(-1, c0, cm1);
+      zghost(-2, zr ? cp1 : c0, cm2);
+    }
     slope_of<MIXED>(P, cm2, cm1, c0, sm1);
     slope_of<MIXED>(P, cm1, c0, cp1, s0);
     if constexpr (FLUX) {
@@ -364,7 +399,11 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     for (int l = 0; l < NFE; l++) plane[l][j + GW][i + GW] = c0[l];
     if (is_halo) {
       T hp[NFE];
-      if constexpr (SELF) {
+      if constexpr (PRIMGHOST) {
+#pragma unroll
+        for (int l = 0; l < NFE; l++) hp[l] = gst[k % 3][hpos][l];
+        ghost_post_prim(hp, hface);
+      } else if constexpr (SELF) {
         double g[NFE];
 #pragma unroll
         for (int l = 0; l < NFE; l++) g[l] = gst[k % 3][hpos][l];
@@ -385,7 +424,8 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     }
     double craw[NFE], cd[FROMD ? NFE : 1];
     int cpost = -1;
-    const bool have_next = (k + 3) < N + GW;
+    // the mixed self-ghosting form forms the z ghosts N, N+1 from cell N-1's primitives
+    const bool have_next = (k + 3) < (PRIMGHOST ? N : N + GW);
     if (have_next) {
       const unsigned e = col_src<N, SELF>(L, i, j, k + 3, zr, cpost);
 #pragma unroll
@@ -609,12 +649,14 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     }
     if (have_next) {
       if constexpr (FROMD) first_stage<MIXED>(P, craw, cd, dt, hinv);
-      if (k + 3 < N) stash(k + 3, craw);  // slot of plane k, whose halo was staged above
       ghost_post(craw, cpost);
       if constexpr (CARRY)
 #pragma unroll
         for (int l = 0; l < NFE; l++) uc2[l] = craw[l];
       make_prim<MIXED>(craw, cp2);
+      if (k + 3 < N) stash(k + 3, craw, cp2);  // slot of plane k, whose halo was staged above
+    } else if constexpr (PRIMGHOST) {
+      if (k + 3 < N + GW) zghost(k + 3, k + 3 == N ? cp1 : c0, cp2);  // cell N-1, after the shift
     }
   }
 
