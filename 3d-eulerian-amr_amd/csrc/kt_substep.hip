@@ -249,11 +249,17 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   __shared__ T Fx[FLUX ? NFE : 1][N][N + 1];
   __shared__ T Fy[FLUX ? NFE : 1][N + 1][N];
   __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
-  // SELF: the boundary cells of planes k, k+1, k+2 (ring slot = plane % 3) by position along the
-  // perimeter: their FP32 primitives in the mixed form, their FP64 input state in the FP64 form
-  // (whose ghosts are formed as the reference forms them, from the conserved state)
+  // PD: how many planes ahead a column cell is loaded.  The mixed form loads cell k+3 at the top of
+  // plane k and converts it after the update (the most latency hidden; measured faster); the FP64
+  // form loads cell k+2 and converts it after the x/y faces, which carries one plane of FP64 state
+  // fewer (its kernels were register-bound) and shortens its L2 re-reads.
+  constexpr int PD = MIXED ? 3 : 2;
+  // SELF: the boundary cells of planes k .. k+PD-1 (ring slot = plane % PD; plane k+PD's is written
+  // over plane k's once the halo of k is staged) by position along the perimeter: their FP32
+  // primitives in the mixed form, their FP64 input state in the FP64 form (whose ghosts are formed
+  // as the reference forms them, from the conserved state)
   constexpr bool PRIMGHOST = SELF && MIXED;
-  __shared__ typename std::conditional<MIXED, float, double>::type gst[SELF ? 3 : 1][SELF ? 4 * N - 4 : 1][NFE];
+  __shared__ typename std::conditional<MIXED, float, double>::type gst[SELF ? PD : 1][SELF ? 4 * N - 4 : 1][NFE];
 
   // persistent: a grid smaller than the batch walks it with a stride (launch shape chosen by the
   // autotuner); with one workgroup per block the loop runs once
@@ -275,9 +281,9 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   const RecBuf<N3, NFE> db(MODE >= 2 ? a.dbuf + (long)b * NFE * N3 : nullptr);
   const double hinv = MODE == 1 ? 0.0 : 1.0 / a.h[b];
 
-  // ---- prologue: column cells k = -2 .. 2 ------------------------------------------------
-  // the conserved FP64 input state of cells k, k+1, k+2 (the mixed update carries it; FP64
-  // reloads cell k)
+  // ---- prologue: column cells k = -2 .. PD-1 ----------------------------------------------
+  // the conserved FP64 input state of cells k .. k+2 (the mixed update carries it; FP64 reloads
+  // cell k)
   constexpr bool CARRY = MIXED && UPDATE;
   double ucur[UPDATE ? NFE : 1], uc1[CARRY ? NFE : 1], uc2[CARRY ? NFE : 1];
   T c0[NFE], cp1[NFE], cp2[NFE], s0[NFE], Fzlo[NFE];
@@ -291,9 +297,9 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
 #pragma unroll
         for (int l = 0; l < NFE; l++) {
           if constexpr (MIXED)
-            gst[kk % 3][q][l] = p[l];
+            gst[kk % PD][q][l] = p[l];
           else
-            gst[kk % 3][q][l] = u[l];
+            gst[kk % PD][q][l] = u[l];
         }
       }
     }
@@ -339,12 +345,14 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
       for (int l = 0; l < NFE; l++) uc1[l] = u[l];
     make_prim<MIXED>(u, cp1);
     stash(1, u, cp1);
-    fetch_col(2, u);
-    if constexpr (CARRY)
+    if constexpr (PD == 3) {
+      fetch_col(2, u);
+      if constexpr (CARRY)
 #pragma unroll
-      for (int l = 0; l < NFE; l++) uc2[l] = u[l];
-    make_prim<MIXED>(u, cp2);
-    stash(2, u, cp2);
+        for (int l = 0; l < NFE; l++) uc2[l] = u[l];
+      make_prim<MIXED>(u, cp2);
+      stash(2, u, cp2);
+    }
     if constexpr (PRIMGHOST) {
       zghost(-1, c0, cm1);
       zghost(-2, zr ? cp1 : c0, cm2);
@@ -401,12 +409,12 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
       T hp[NFE];
       if constexpr (PRIMGHOST) {
 #pragma unroll
-        for (int l = 0; l < NFE; l++) hp[l] = gst[k % 3][hpos][l];
+        for (int l = 0; l < NFE; l++) hp[l] = gst[k % PD][hpos][l];
         ghost_post_prim(hp, hface);
       } else if constexpr (SELF) {
         double g[NFE];
 #pragma unroll
-        for (int l = 0; l < NFE; l++) g[l] = gst[k % 3][hpos][l];
+        for (int l = 0; l < NFE; l++) g[l] = gst[k % PD][hpos][l];
         ghost_post(g, hface);
         make_prim<MIXED>(g, hp);
       } else {
@@ -415,7 +423,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
 #pragma unroll
       for (int l = 0; l < NFE; l++) plane[l][hpy][hpx] = hp[l];
     }
-    // prefetch: halo of plane k+1 (stored slabs), column cell k+3, U0 at k
+    // prefetch: halo of plane k+1 (stored slabs), column cell k+PD, U0 at k
     if constexpr (!SELF) {
       if (is_halo && k + 1 < N) {
 #pragma unroll
@@ -425,9 +433,10 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     double craw[NFE], cd[FROMD ? NFE : 1];
     int cpost = -1;
     // the mixed self-ghosting form forms the z ghosts N, N+1 from cell N-1's primitives
-    const bool have_next = (k + 3) < (PRIMGHOST ? N : N + GW);
+    const int kn = k + PD;  // the column cell loaded
+    const bool have_next = PRIMGHOST ? kn < N : kn < N + GW;
     if (have_next) {
-      const unsigned e = col_src<N, SELF>(L, i, j, k + 3, zr, cpost);
+      const unsigned e = col_src<N, SELF>(L, i, j, kn, zr, cpost);
 #pragma unroll
       for (int l = 0; l < NFE; l++) craw[l] = ub.ld(l, e);
       if constexpr (FROMD)
@@ -435,7 +444,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
         for (int l = 0; l < NFE; l++) cd[l] = db.ld(l, e);
     }
     // cell k's RK base (U0; for MODE 3 the step's base U^n, the input it was formed from) and,
-    // where the column state is not carried, cell k's input state again (read 3 planes ago: L2)
+    // where the column state is not carried, cell k's input state again (read PD planes ago: L2)
     double u0v[NFE], dk[(FROMD && !CARRY) ? NFE : 1];
     if constexpr (UPDATE) {
       const unsigned e = (unsigned)L.interior(i, j, k);
@@ -537,6 +546,23 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
         }
       }
     }
+    // the column cell loaded at the top of the plane becomes cell k+2 here (PD = 2) or k+3 after
+    // the update (PD = 3); its state goes into the ring slot of plane k, whose halo every thread
+    // staged before the first barrier
+    auto take_next = [&]() {
+      if (have_next) {
+        if constexpr (FROMD) first_stage<MIXED>(P, craw, cd, dt, hinv);
+        ghost_post(craw, cpost);
+        if constexpr (CARRY)
+#pragma unroll
+          for (int l = 0; l < NFE; l++) uc2[l] = craw[l];
+        make_prim<MIXED>(craw, cp2);
+        if (kn < N) stash(kn, craw, cp2);
+      } else if constexpr (PRIMGHOST) {
+        if (kn < N + GW) zghost(kn, kn == N ? cp1 : c0, cp2);  // from cell N-1
+      }
+    };
+    if constexpr (PD == 2) take_next();
     T sp1[NFE], Fzhi[NFE];
     slope_of<MIXED>(P, c0, cp1, cp2, sp1);
     if constexpr (FLUX) {
@@ -647,17 +673,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
         uc1[l] = uc2[l];
       }
     }
-    if (have_next) {
-      if constexpr (FROMD) first_stage<MIXED>(P, craw, cd, dt, hinv);
-      ghost_post(craw, cpost);
-      if constexpr (CARRY)
-#pragma unroll
-        for (int l = 0; l < NFE; l++) uc2[l] = craw[l];
-      make_prim<MIXED>(craw, cp2);
-      if (k + 3 < N) stash(k + 3, craw, cp2);  // slot of plane k, whose halo was staged above
-    } else if constexpr (PRIMGHOST) {
-      if (k + 3 < N + GW) zghost(k + 3, k + 3 == N ? cp1 : c0, cp2);  // cell N-1, after the shift
-    }
+    if constexpr (PD == 3) take_next();
   }
 
   if constexpr (EIG) {
