@@ -189,6 +189,14 @@ int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_gr
  * returns them (kernel 0: the first substep, 1: the second), as amrh_launch_info does. */
 int amrh_deferred_launch_info(const amrh_ctx* ctx, int kernel, int* grid, int* variant, double* ms_default,
                               double* ms_tuned);
+/* Whether such steps run deferred (the default) or as scan + store + substeps: amrh_autotune keeps
+ * the faster of the two by the kernel times it measured; amrh_set_schedule sets it by hand (the
+ * results are the same bits); amrh_get_schedule reports AMRH_SCHEDULE_PLAIN for a batch that
+ * cannot run deferred steps. */
+#define AMRH_SCHEDULE_PLAIN 0
+#define AMRH_SCHEDULE_DEFERRED 1
+int amrh_set_schedule(amrh_ctx* ctx, int schedule);
+int amrh_get_schedule(const amrh_ctx* ctx, int* schedule);
 /* Stages of one timed step (amrh_step_timed) */
 #define AMRH_LAUNCH_SUBSTEP 0 /* a substep (with its ghost fill where one follows it) */
 #define AMRH_LAUNCH_SCAN 1    /* the CFL scan and the dt kernel */

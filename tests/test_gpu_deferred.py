@@ -194,3 +194,35 @@ def test_autotuned_deferred_kernels_give_the_same_bits(precision):
         A.close()
     np.testing.assert_array_equal(outs[1][0], outs[0][0])
     assert outs[1][1] == outs[0][1]
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_schedules_give_the_same_bits(n):
+    """amrh_set_schedule: deferred and plain RK3 steps of the same batch are bit-identical; the
+    autotuner's pick is one of the two; a PPM batch reports the plain schedule"""
+    nb = 300 if n == 8 else 64
+    U = blocks(n, nb, seed=93 + n)
+    h = np.full(nb, 1.0 / n)
+    outs = []
+    for sched in (amrhydro.AMRH_SCHEDULE_PLAIN, amrhydro.AMRH_SCHEDULE_DEFERRED, None):
+        A = outflow_batch(U, h, 1, n)
+        if sched is None:
+            A.autotune()
+            say("n=%d: autotuned schedule %d" % (n, A.schedule()))
+            assert A.schedule() in (amrhydro.AMRH_SCHEDULE_PLAIN, amrhydro.AMRH_SCHEDULE_DEFERRED)
+        else:
+            A.set_schedule(sched)
+            assert A.schedule() == sched
+        d = dt_buffer(1e-4)
+        A.run_steps(3, d.data_ptr(), rk=3, use_graph=True)
+        A.sync()
+        outs.append((A.download(), float(d.cpu()[0])))
+        A.close()
+    for o, dt in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0][0])
+        assert dt == outs[0][1]
+    import workloads
+    V = workloads.flat_batch(4, n=8, g=3, seed=2)
+    P = amrhydro.Batch(amrhydro.euler_config(n=8, g=3, recon=amrhydro.AMRH_RECON_PPM,
+                                             boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW), 4, 1.0 / 8)
+    assert P.schedule() == amrhydro.AMRH_SCHEDULE_PLAIN
