@@ -39,7 +39,8 @@ class Config(C.Structure):
 
 AMRH_BOUNDARY_EXTERNAL = 0
 AMRH_BOUNDARY_OUTFLOW = 1
-AMRH_LAUNCH_SUBSTEP, AMRH_LAUNCH_SCAN, AMRH_LAUNCH_FIRST, AMRH_LAUNCH_DT, AMRH_LAUNCH_SECOND, AMRH_LAUNCH_GHOSTS = range(6)
+(AMRH_LAUNCH_SUBSTEP, AMRH_LAUNCH_SCAN, AMRH_LAUNCH_FIRST, AMRH_LAUNCH_DT, AMRH_LAUNCH_SECOND, AMRH_LAUNCH_GHOSTS,
+ AMRH_LAUNCH_FIRST_STAGE) = range(7)
 AMRH_SCHEDULE_PLAIN, AMRH_SCHEDULE_DEFERRED = 0, 1
 STATE_SUM_DEV_MAX_PARTIALS = 3145728  # AMRH_STATE_SUM_DEV_MAX_PARTIALS
 

@@ -90,14 +90,51 @@ int ensure_dbuf(amrh_ctx* c) {
 // dt, euler/hydro_driver.cpp:107-113); the second substep forms U1 = floor(U^n + D dt) for every
 // cell it reads, with U^n as its RK base.  The buffers end as after scan + store + rk substeps
 // (U^n in the RK base, the state in the other buffer), and every cell has the same bits.
-bool deferred_capable(const amrh_ctx* c) {
+// The rotating state's first stage needs the floor's Newton solve for T; formed for every staged
+// cell it would cost the substep three times over, so an elementwise pass forms it once per cell
+// after the dt kernel and the remaining substeps are plain.
+bool sweep_deferred(const amrh_ctx* c) {
   return self_ghosting(c) && c->cfg.physics == AMRH_PHYS_EULER && c->cfg.recon == AMRH_RECON_MINMOD;
 }
+bool rot_deferred(const amrh_ctx* c) { return self_ghosting(c) && c->cfg.physics == AMRH_PHYS_ROTATING; }
+bool deferred_capable(const amrh_ctx* c) { return sweep_deferred(c) || rot_deferred(c); }
 bool deferred_ok(const amrh_ctx* c, int rk) {
   return rk >= 2 && deferred_capable(c) && c->dbuf && c->schedule == AMRH_SCHEDULE_DEFERRED;
 }
 
+// the rotating state's deferred step: the deferred substep, dt, the first-stage pass, then plain
+// substeps with U^n as their RK base
+int step_deferred_rot(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  if (int rc = ctx_ready(c)) return rc;
+  const int p = c->cur, q = 1 - p;
+  RotArgs r = ctx_rot_args(c);
+  r.Uin = c->buf[p];
+  r.dbuf = c->dbuf;
+  r.gmax = c->gmax_dev;
+  r.outflow = 1;
+  r.z_reflect = c->cfg.z_reflect;
+  r.beta = 1.0;
+  if (int rc = mark_begin(c, mk)) return rc;
+  HIPCHK(launch_rot_deferred(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, c->grid_deferred[0], c->stream));
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_FIRST)) return rc;
+  HIPCHK(launch_dt_from_gmax(c->gmax_dev, c->cfg.cfl, 1.0, 1.25, dt_dev, c->stream));
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_DT)) return rc;
+  c->u0buf = p;  // the store: U^n stays where it is as the RK base
+  r.Uout = c->buf[q];
+  r.dt_dev = dt_dev;
+  HIPCHK(launch_rot_first_stage(c->cfg.g, c->cfg.eos, mixed, r, c->stream));
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_FIRST_STAGE)) return rc;
+  c->cur = q;
+  for (int s = 1; s < rk; s++) {
+    if (int rc = amrh_substep_dev(c, dt_dev, kBetas[rk - 1][s])) return rc;
+    if (int rc = mark_end(c, mk, AMRH_LAUNCH_SUBSTEP)) return rc;
+  }
+  return AMRH_OK;
+}
+
 int step_deferred(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
+  if (rot_deferred(c)) return step_deferred_rot(c, rk, dt_dev, mk);
   const bool mixed = c->cfg.precision == AMRH_MIXED;
   const int p = c->cur, q = 1 - p;
   SubstepArgs a = flat_args(c);
@@ -227,7 +264,47 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   // writing only D, the CFL word (cleared after) and the spare buffer
   int gd[2] = {0, 0}, vd[2] = {0, 0};
   float msd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
-  if (deferred_capable(c)) {
+  if (rot_deferred(c)) {
+    // the rotating state's: its deferred substep over the grids, and the first-stage pass (kernel 1)
+    if (int rc = ensure_dbuf(c)) return rc;
+    const bool mixed = c->cfg.precision == AMRH_MIXED;
+    HIPCHK(hipMemsetAsync(c->eig_dev, 0, sizeof(double), c->stream));
+    RotArgs r = ctx_rot_args(c);
+    r.Uin = c->buf[c->cur];
+    r.dbuf = c->dbuf;
+    r.gmax = c->gmax_dev;
+    r.outflow = 1;
+    r.z_reflect = c->cfg.z_reflect;
+    r.beta = 1.0;
+    float best = 1e30f;
+    for (int g : cand) {
+      float ms = 0.f;
+      if (int rc = time_launches(c, 3, [&]() -> int {
+            HIPCHK(launch_rot_deferred(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, g, c->stream));
+            return AMRH_OK;
+          }, &ms))
+        return rc;
+      if (g == 0) msd[0][0] = ms;
+      if (ms < best) {
+        best = ms;
+        gd[0] = g;
+      }
+    }
+    msd[0][1] = best;
+    HIPCHK(launch_reset_u64(c->gmax_dev, c->stream));
+    r.Uout = c->buf[other];
+    r.dt_dev = c->eig_dev;
+    if (int rc = time_launches(c, 3, [&]() -> int {
+          HIPCHK(launch_rot_first_stage(c->cfg.g, c->cfg.eos, mixed, r, c->stream));
+          return AMRH_OK;
+        }, &msd[1][1]))
+      return rc;
+    msd[1][0] = msd[1][1];
+    // an RK step is the scan and a beta = 1 substep, or the deferred substep and the first-stage
+    // pass, then the same beta != 1 substeps (the dt kernel is common)
+    c->schedule = msd[0][1] + msd[1][1] < best_m + best_s ? AMRH_SCHEDULE_DEFERRED : AMRH_SCHEDULE_PLAIN;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else if (deferred_capable(c)) {
     if (int rc = ensure_dbuf(c)) return rc;
     const bool mixed = c->cfg.precision == AMRH_MIXED;
     HIPCHK(hipMemsetAsync(c->eig_dev, 0, sizeof(double), c->stream));
@@ -324,7 +401,8 @@ int amrh_set_launch(amrh_ctx* c, int grid_substep, int grid_maxeig) {
 int amrh_deferred_launch_info(const amrh_ctx* c, int kernel, int* grid, int* variant, double* ms_default,
                               double* ms_tuned) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
-  if (kernel < 0 || kernel > 1) return fail(AMRH_EINVAL, "kernel must be 0 (first substep) or 1 (second)");
+  if (kernel < 0 || kernel > 1)
+    return fail(AMRH_EINVAL, "kernel must be 0 (the deferred first substep) or 1 (the second kernel of the step)");
   if (grid) *grid = c->grid_deferred[kernel];
   if (variant) *variant = c->variant_deferred[kernel];
   if (ms_default) *ms_default = c->tune_ms_deferred[kernel][0];

@@ -113,16 +113,23 @@ __device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int 
 // The mixed form stages the whole record (interior + 6 slabs) once per block and computes the faces
 // of all three directions as one list (with the ideal gas, as pairs of adjacent faces that share
 // their reconstruction); the FP64 form stages one direction's pencils at a time.
+// MODE 0: substep.  MODE 1: the CFL scan.  MODE 2: the first substep of an RK step with dt deferred
+// (flat batches): the substep's fluxes and sources, the block's max face speed (the numbers the scan
+// forms from the same faces) into a.gmax, and each cell's D (sources included) into a.dbuf instead
+// of the update; rot_first_stage_kernel forms the first stage once dt is known.
 template <int G, int RECON, int EOS, int MODE, bool MIXED, bool TREE>
 __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   using T = typename std::conditional<MIXED, float, double>::type;
+  constexpr bool SUB = MODE == 0 || MODE == 2;  // fluxes and the cells' D
+  constexpr bool EIGM = MODE != 0;              // the block's max face speed
+  static_assert(!(TREE && MODE != 0), "the octree instantiation is the substep's");
   constexpr bool ALL = MIXED;          // stage the whole record once
   constexpr int R = N3 + 6 * G * N2;   // record elements per field
   constexpr int RS = ALL ? R : N3 + 2 * G * N2;  // staged elements per field
   __shared__ T P[NF][RS];
-  constexpr int NFB = MODE == 0 ? (ALL ? 2 : 1) : 1;  // face buffers (double-buffered in the pooled form)
-  __shared__ T Fb[NFB][MODE == 0 ? NF : 1][MODE == 0 ? NFACE : 1];
-  __shared__ T Vf[NFB][MODE == 0 ? NFACE : 1];
+  constexpr int NFB = SUB ? (ALL ? 2 : 1) : 1;  // face buffers (double-buffered in the pooled form)
+  __shared__ T Fb[NFB][SUB ? NF : 1][SUB ? NFACE : 1];
+  __shared__ T Vf[NFB][SUB ? NFACE : 1];
   __shared__ double red[N3 / 64];
   const sp::RotF cf = sp::make(a.c);
 
@@ -251,16 +258,15 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         const float el = sp::flux<EOS, d>(H, cf, ql, X, fl);
         const float er = sp::flux<EOS, d>(H, cf, qr, X, fr);
         const float amax = fmaxf(el, er);
-        if constexpr (MODE == 0) {
+        if constexpr (SUB) {
           float uf[NF];
 #pragma unroll
           for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5f;
           Vf[fb][q] = (T)sp::vel<d>(cf, uf, X, sp::Rc(X));
 #pragma unroll
           for (int l = 0; l < NF; l++) Fb[fb][l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5f);
-        } else {
-          emax = rmax(emax, (double)amax);
         }
+        if constexpr (EIGM) emax = rmax(emax, (double)amax);
       } else {
         double ql[NF], qr[NF];
         if constexpr (RECON == AMRH_RECON_PPM) {
@@ -298,7 +304,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         }
         to_con(c, ql, X);
         to_con(c, qr, X);
-        if constexpr (MODE == 0) {
+        if constexpr (SUB) {
           double uf[NF];
 #pragma unroll
           for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5;
@@ -309,6 +315,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           const double amax = rmax(el, er);
 #pragma unroll
           for (int l = 0; l < NF; l++) Fb[fb][l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5);
+          if constexpr (EIGM) emax = rmax(emax, amax);
         } else {
           double F[NF];
           const double el = flux<EOS, d>(H, c, ql, X, F);
@@ -395,7 +402,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
 #pragma unroll 1
         for (int q = tid; q < NFACE; q += N3) face(dc, q, 0);
         __syncthreads();
-        if constexpr (MODE == 0) {
+        if constexpr (SUB) {
           finish(dc, 0);
           __syncthreads();  // P, Fb and Vf are restaged for the next direction
         }
@@ -422,7 +429,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         else
           face(X2{}, g - 2 * NFACE, 0);
       };
-      if constexpr (MODE == 0) {
+      if constexpr (SUB) {
         // round 0: x; 1: x, then y; 2: y, then z; 3: the rest of z (one call site keeps the face
         // code inlined once per direction)
 #pragma unroll 1
@@ -479,7 +486,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         const float el = sp::flux<EOS, d>(H, cf, ql, X, fl);
         const float er = sp::flux<EOS, d>(H, cf, qr, X, fr);
         const float amax = fmaxf(el, er);
-        if constexpr (MODE == 0) {
+        if constexpr (SUB) {
           const int q = face_index<N, d>(f, ta, tb);
           float uf[NF];
 #pragma unroll
@@ -487,9 +494,8 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
           Vf[fb][q] = (T)sp::vel<d>(cf, uf, X, sp::Rc(X));
 #pragma unroll
           for (int l = 0; l < NF; l++) Fb[fb][l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5f);
-        } else {
-          emax = rmax(emax, (double)amax);
         }
+        if constexpr (EIGM) emax = rmax(emax, (double)amax);
       };
       auto pair_task = [&](auto dc, int r, int fb) {
         constexpr int d = decltype(dc)::value;
@@ -537,7 +543,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         else
           pair_task(X2{}, g - 2 * TPD, 0);
       };
-      if constexpr (MODE == 0) {
+      if constexpr (SUB) {
         // one call site keeps the task code inlined once per direction
 #pragma unroll 1
         for (int rd = 0; rd < 2; rd++) {
@@ -557,7 +563,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       }
     }
 
-    if constexpr (MODE == 0) {
+    if constexpr (SUB) {
       const long e = (long)tid;
       double s[NF];
 #pragma unroll
@@ -647,6 +653,12 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       }
 #pragma unroll
       for (int l = 0; l < NF; l++) D[l] += sv[l];
+      if constexpr (MODE == 2) {
+        // deferred: the cell's D (its sources included) for rot_first_stage_kernel
+        const RecBuf<N3, NF> dbo(a.dbuf + (long)b * NF * N3);
+#pragma unroll
+        for (int l = 0; l < NF; l++) dbo.st_nt(l, (unsigned)e, D[l]);
+      } else {
       double u[NF];
       if (has_u0) {
         const RecBuf<R, NF> u0(a.U0 + (long)b * NF * R);
@@ -663,7 +675,9 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
       const RecBuf<R, NF> uo(a.Uout + (long)b * NF * R);
 #pragma unroll
       for (int l = 0; l < NF; l++) uo.st(l, (unsigned)e, u[l]);
-    } else {
+      }
+    }
+    if constexpr (EIGM) {
       for (int off = 32; off > 0; off >>= 1) emax = rmax(emax, __shfl_xor(emax, off, 64));
       if ((tid & 63) == 0) red[tid >> 6] = emax;
       __syncthreads();
@@ -676,6 +690,43 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
     }
     __syncthreads();
   }
+}
+
+// The first stage of a deferred RK step, one thread per interior cell: the epilogue of the
+// substep with beta = 1 (kt_rot_kernel MODE 0) from the D that the deferred first substep stored:
+// U1 = (U^n + D dt) 1 + U^n 0, the floor with the Newton solve for T at the cell
+// (binary/state_ztwd.cpp:482-554), pot = rho (phi - R^2 Omega0^2 / 2) (binary/binary.cpp:825-849).
+// Same operations in the same order, so U1 has the bits a direct first substep would have stored.
+template <int G, int EOS, bool MIXED>
+__global__ __launch_bounds__(256) void rot_first_stage_kernel(RotArgs a) {
+  constexpr int R = N3 + 6 * G * N2;
+  constexpr int PP = N + 2;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)a.nblocks * N3) return;
+  const int b = (int)(gid / N3), e = (int)(gid % N3);
+  const int ci = e % N, cj = (e / N) % N, ck = e / N2;
+  const double dt = *a.dt_dev;
+  const double beta = 1.0;
+  const RecBuf<R, NF> ub(a.Uin + (long)b * NF * R), uo(a.Uout + (long)b * NF * R);
+  const RecBuf<N3, NF> db(a.dbuf + (long)b * NF * N3);
+  double s[NF], D[NF], u[NF];
+#pragma unroll
+  for (int l = 0; l < NF; l++) {
+    s[l] = ub.ld(l, (unsigned)e);
+    D[l] = db.ld_nt(l, (unsigned)e);
+  }
+#pragma unroll
+  for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + s[l] * (1.0 - beta);
+  const double x0[3] = {a.x0[3L * b], a.x0[3L * b + 1], a.x0[3L * b + 2]};
+  double X[3];
+  cell_pos(x0, a.h[b], ci, cj, ck, X);
+  floor_state<EOS, MIXED>(a.H, a.c, u, X);
+  if (!(u[D0] > 0.0)) atomicOr(a.err, 1);
+  const double pc = a.phi[(long)b * PP * PP * PP + ((ck + 1) * PP + (cj + 1)) * PP + (ci + 1)];
+  const double R2 = X[0] * X[0] + X[1] * X[1];
+  u[POT] = u[D0] * (pc - 0.5 * R2 * a.c.Omega0 * a.c.Omega0);
+#pragma unroll
+  for (int l = 0; l < NF; l++) uo.st(l, (unsigned)e, u[l]);
 }
 
 // GridPhysBound for isolated blocks (grid/grid_phys_bound.cpp:27-54): every face ghost copies the
@@ -953,6 +1004,34 @@ hipError_t launch_rot_substep(int g, int recon, int eos, bool mixed, const RotAr
 
 hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s) {
   return mixed ? dispatch<1, true>(g, recon, eos, a, grid, s) : dispatch<1, false>(g, recon, eos, a, grid, s);
+}
+hipError_t launch_rot_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s) {
+  if (a.slot || a.nodes || !a.outflow || !a.dbuf || !a.gmax) return hipErrorInvalidValue;
+  return mixed ? dispatch<2, true>(g, recon, eos, a, grid, s) : dispatch<2, false>(g, recon, eos, a, grid, s);
+}
+hipError_t launch_rot_first_stage(int g, int eos, bool mixed, const RotArgs& a, hipStream_t s) {
+  if (!a.dbuf || !a.dt_dev) return hipErrorInvalidValue;
+  if (a.nblocks <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)(((long)a.nblocks * N3 + 255) / 256);
+  const bool helm = eos == EOS_HELMHOLTZ;
+  if (g == 3) {
+    if (mixed)
+      helm ? rot_first_stage_kernel<3, EOS_HELMHOLTZ, true><<<grid, 256, 0, s>>>(a)
+           : rot_first_stage_kernel<3, EOS_IDEAL, true><<<grid, 256, 0, s>>>(a);
+    else
+      helm ? rot_first_stage_kernel<3, EOS_HELMHOLTZ, false><<<grid, 256, 0, s>>>(a)
+           : rot_first_stage_kernel<3, EOS_IDEAL, false><<<grid, 256, 0, s>>>(a);
+  } else if (g == 2) {
+    if (mixed)
+      helm ? rot_first_stage_kernel<2, EOS_HELMHOLTZ, true><<<grid, 256, 0, s>>>(a)
+           : rot_first_stage_kernel<2, EOS_IDEAL, true><<<grid, 256, 0, s>>>(a);
+    else
+      helm ? rot_first_stage_kernel<2, EOS_HELMHOLTZ, false><<<grid, 256, 0, s>>>(a)
+           : rot_first_stage_kernel<2, EOS_IDEAL, false><<<grid, 256, 0, s>>>(a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_rot_tree_ghosts(int g, double* U, const int* nodes, int count, const int* ghost, const int* rel,

@@ -199,6 +199,36 @@ def test_rotating_rk3_graph_matches_direct_and_oracle(eos):
         assert d == outs[0][1]
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("eos,recon", [(0, 1), (1, 1), (0, 0)])
+def test_rotating_deferred_step_equals_plain(eos, recon, precision):
+    """The rotating state's deferred-dt step (the first substep's face speeds as the CFL maximum,
+    its D stored, the first stage formed by an elementwise pass) gives the bits of the plain
+    schedule (CFL scan, store, substeps), direct and as graphs, dt included"""
+    outs = []
+    for sched, use_graph in ((amrhydro.AMRH_SCHEDULE_PLAIN, False), (amrhydro.AMRH_SCHEDULE_DEFERRED, False),
+                             (amrhydro.AMRH_SCHEDULE_DEFERRED, True)):
+        B, cfg, G = setup(eos, recon, seed=17, boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW, precision=precision)
+        G.upload(B["U"])
+        G.set_schedule(sched)
+        assert G.schedule() == sched
+        dt = torch.full((1,), 1e-5, dtype=torch.float64, device="cuda")
+        G.run_steps(3, dt.data_ptr(), rk=3, use_graph=use_graph)
+        G.sync()
+        outs.append((G.download(), float(dt.cpu()[0])))
+        assert G.error_flags() == 0
+        if sched == amrhydro.AMRH_SCHEDULE_DEFERRED and not use_graph:
+            st = G.step_timed(dt.data_ptr(), rk=3)
+            assert [k for k, _ in st] == [amrhydro.AMRH_LAUNCH_FIRST, amrhydro.AMRH_LAUNCH_DT,
+                                          amrhydro.AMRH_LAUNCH_FIRST_STAGE, amrhydro.AMRH_LAUNCH_SUBSTEP,
+                                          amrhydro.AMRH_LAUNCH_SUBSTEP]
+        G.close()
+    say("rotating eos=%d recon=%d prec=%d: dt after 3 steps %.6e" % (eos, recon, precision, outs[0][1]))
+    for o, d in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0][0])
+        assert d == outs[0][1]
+
+
 @pytest.mark.parametrize("eos,steps", [(0, 100), (1, 20)])
 def test_rotating_mixed_rk3_accuracy(eos, steps):
     """The mixed form (FP32 reconstruction, face states, EOS inputs and fluxes; FP64 flux
