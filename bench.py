@@ -147,8 +147,13 @@ def measure(B, torch, cells, steps, warmup, tune=True):
     if tune:
         B.autotune()
         res["launch"] = dict(zip(TUNE_KEYS, B.launch_info()))
-        if B.cfg.physics == amrhydro_mod().AMRH_PHYS_EULER and B.cfg.recon == amrhydro_mod().AMRH_RECON_MINMOD:
-            for k, name in ((0, "deferred_first"), (1, "deferred_second")):
+        am = amrhydro_mod()
+        res["launch"]["schedule"] = "deferred" if B.schedule() == am.AMRH_SCHEDULE_DEFERRED else "plain"
+        sweep = B.cfg.physics == am.AMRH_PHYS_EULER and B.cfg.recon == am.AMRH_RECON_MINMOD
+        if sweep or B.cfg.physics == am.AMRH_PHYS_ROTATING:
+            # the deferred step's kernels: the first substep, then the sweep's second substep or
+            # the rotating state's first-stage pass
+            for k, name in ((0, "deferred_first"), (1, "deferred_second" if sweep else "first_stage_pass")):
                 res["launch"][name] = dict(zip(("grid", "variant", "ms_default_launch", "ms_tuned_launch"),
                                                B.deferred_launch_info(k)))
     res.update(kernel_times(B, torch, cells, steps, warmup))
@@ -742,10 +747,14 @@ def main():
     wpe = {0: "default", 1: 2, 2: 3, 3: 4}
     ln = head["launch"]
     roof = dict(head["roofline"])
-    roof["kernel"] = ("kt_sweep_kernel<%d, MIXED=%s, flat, self-ghosting>, the three substeps of the deferred-dt RK3 "
-                      "step: first (fluxes + CFL max + D; waves/SIMD %s), second (first stage from U^n and D; %s), "
-                      "third (plain substep; %s)" % (n, head_name == "mixed", wpe[ln["deferred_first"]["variant"]],
-                                                     wpe[ln["deferred_second"]["variant"]], wpe[ln["variant"]]))
+    if ln.get("schedule") == "deferred":
+        roof["kernel"] = ("kt_sweep_kernel<%d, MIXED=%s, flat, self-ghosting>, the three substeps of the deferred-dt "
+                          "RK3 step: first (fluxes + CFL max + D; waves/SIMD %s), second (first stage from U^n and D; "
+                          "%s), third (plain substep; %s)" % (n, head_name == "mixed", wpe[ln["deferred_first"]["variant"]],
+                                                              wpe[ln["deferred_second"]["variant"]], wpe[ln["variant"]]))
+    else:
+        roof["kernel"] = ("kt_sweep_kernel<%d, MIXED=%s, flat, self-ghosting>, the three substeps of the RK3 step "
+                          "(waves/SIMD %s)" % (n, head_name == "mixed", wpe[ln["variant"]]))
     out = {
         "metric": METRIC,
         "value": head["cell_updates_per_s"],
