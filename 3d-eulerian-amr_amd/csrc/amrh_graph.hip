@@ -92,38 +92,76 @@ int ensure_dbuf(amrh_ctx* c) {
 // (U^n in the RK base, the state in the other buffer), and every cell has the same bits.
 // The rotating state's first stage needs the floor's Newton solve for T; formed for every staged
 // cell it would cost the substep three times over, so an elementwise pass forms it once per cell
-// after the dt kernel and the remaining substeps are plain.
+// after the dt kernel and the remaining substeps are plain.  The block kernel (PPM) stages each
+// block once per substep, so its first stage is the same elementwise pass.
 bool sweep_deferred(const amrh_ctx* c) {
   return self_ghosting(c) && c->cfg.physics == AMRH_PHYS_EULER && c->cfg.recon == AMRH_RECON_MINMOD;
 }
 bool rot_deferred(const amrh_ctx* c) { return self_ghosting(c) && c->cfg.physics == AMRH_PHYS_ROTATING; }
-bool deferred_capable(const amrh_ctx* c) { return sweep_deferred(c) || rot_deferred(c); }
+bool block_deferred(const amrh_ctx* c) {
+  return self_ghosting(c) && c->cfg.physics == AMRH_PHYS_EULER && c->cfg.recon != AMRH_RECON_MINMOD;
+}
+// the deferred step with an elementwise first-stage pass
+bool pass_deferred(const amrh_ctx* c) { return rot_deferred(c) || block_deferred(c); }
+bool deferred_capable(const amrh_ctx* c) { return sweep_deferred(c) || pass_deferred(c); }
+
+// the deferred substep of a pass_deferred batch: fluxes of Uin, the CFL maximum into gmax_dev, D
+hipError_t pass_first_launch(const amrh_ctx* c, double* Uin, int grid) {
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  if (rot_deferred(c)) {
+    RotArgs r = ctx_rot_args(c);
+    r.Uin = Uin;
+    r.dbuf = c->dbuf;
+    r.gmax = c->gmax_dev;
+    r.outflow = 1;
+    r.z_reflect = c->cfg.z_reflect;
+    r.beta = 1.0;
+    return launch_rot_deferred(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, grid, c->stream);
+  }
+  SubstepArgs a = flat_args(c);
+  a.Uin = Uin;
+  a.dbuf = c->dbuf;
+  a.gmax = c->gmax_dev;
+  a.beta = 1.0;
+  return launch_block_deferred(c->cfg.n, c->cfg.g, c->cfg.recon, a, mixed, grid, c->stream);
+}
+
+// its first-stage pass: Uout = floor((Uin + D dt) 1) for every interior cell
+hipError_t pass_stage_launch(const amrh_ctx* c, double* Uin, double* Uout, const double* dt_dev) {
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  if (rot_deferred(c)) {
+    RotArgs r = ctx_rot_args(c);
+    r.Uin = Uin;
+    r.Uout = Uout;
+    r.dbuf = c->dbuf;
+    r.dt_dev = dt_dev;
+    r.beta = 1.0;
+    return launch_rot_first_stage(c->cfg.g, c->cfg.eos, mixed, r, c->stream);
+  }
+  SubstepArgs a = flat_args(c);
+  a.Uin = Uin;
+  a.Uout = Uout;
+  a.dbuf = c->dbuf;
+  a.dt_dev = dt_dev;
+  a.beta = 1.0;
+  return launch_block_first_stage(c->cfg.n, c->cfg.g, a, mixed, c->stream);
+}
 bool deferred_ok(const amrh_ctx* c, int rk) {
   return rk >= 2 && deferred_capable(c) && c->dbuf && c->schedule == AMRH_SCHEDULE_DEFERRED;
 }
 
-// the rotating state's deferred step: the deferred substep, dt, the first-stage pass, then plain
-// substeps with U^n as their RK base
-int step_deferred_rot(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
-  const bool mixed = c->cfg.precision == AMRH_MIXED;
+// the deferred step of the rotating state and of the block kernel: the deferred substep, dt, the
+// first-stage pass, then plain substeps with U^n as their RK base
+int step_deferred_pass(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
   if (int rc = ctx_ready(c)) return rc;
   const int p = c->cur, q = 1 - p;
-  RotArgs r = ctx_rot_args(c);
-  r.Uin = c->buf[p];
-  r.dbuf = c->dbuf;
-  r.gmax = c->gmax_dev;
-  r.outflow = 1;
-  r.z_reflect = c->cfg.z_reflect;
-  r.beta = 1.0;
   if (int rc = mark_begin(c, mk)) return rc;
-  HIPCHK(launch_rot_deferred(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, c->grid_deferred[0], c->stream));
+  HIPCHK(pass_first_launch(c, c->buf[p], c->grid_deferred[0]));
   if (int rc = mark_end(c, mk, AMRH_LAUNCH_FIRST)) return rc;
   HIPCHK(launch_dt_from_gmax(c->gmax_dev, c->cfg.cfl, 1.0, 1.25, dt_dev, c->stream));
   if (int rc = mark_end(c, mk, AMRH_LAUNCH_DT)) return rc;
   c->u0buf = p;  // the store: U^n stays where it is as the RK base
-  r.Uout = c->buf[q];
-  r.dt_dev = dt_dev;
-  HIPCHK(launch_rot_first_stage(c->cfg.g, c->cfg.eos, mixed, r, c->stream));
+  HIPCHK(pass_stage_launch(c, c->buf[p], c->buf[q], dt_dev));
   if (int rc = mark_end(c, mk, AMRH_LAUNCH_FIRST_STAGE)) return rc;
   c->cur = q;
   for (int s = 1; s < rk; s++) {
@@ -134,7 +172,7 @@ int step_deferred_rot(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
 }
 
 int step_deferred(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
-  if (rot_deferred(c)) return step_deferred_rot(c, rk, dt_dev, mk);
+  if (pass_deferred(c)) return step_deferred_pass(c, rk, dt_dev, mk);
   const bool mixed = c->cfg.precision == AMRH_MIXED;
   const int p = c->cur, q = 1 - p;
   SubstepArgs a = flat_args(c);
@@ -264,23 +302,16 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   // writing only D, the CFL word (cleared after) and the spare buffer
   int gd[2] = {0, 0}, vd[2] = {0, 0};
   float msd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
-  if (rot_deferred(c)) {
-    // the rotating state's: its deferred substep over the grids, and the first-stage pass (kernel 1)
+  if (pass_deferred(c)) {
+    // the rotating state's and the block kernel's: the deferred substep over the grids, and the
+    // first-stage pass (kernel 1)
     if (int rc = ensure_dbuf(c)) return rc;
-    const bool mixed = c->cfg.precision == AMRH_MIXED;
     HIPCHK(hipMemsetAsync(c->eig_dev, 0, sizeof(double), c->stream));
-    RotArgs r = ctx_rot_args(c);
-    r.Uin = c->buf[c->cur];
-    r.dbuf = c->dbuf;
-    r.gmax = c->gmax_dev;
-    r.outflow = 1;
-    r.z_reflect = c->cfg.z_reflect;
-    r.beta = 1.0;
     float best = 1e30f;
     for (int g : cand) {
       float ms = 0.f;
       if (int rc = time_launches(c, 3, [&]() -> int {
-            HIPCHK(launch_rot_deferred(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, r, g, c->stream));
+            HIPCHK(pass_first_launch(c, c->buf[c->cur], g));
             return AMRH_OK;
           }, &ms))
         return rc;
@@ -292,10 +323,8 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
     }
     msd[0][1] = best;
     HIPCHK(launch_reset_u64(c->gmax_dev, c->stream));
-    r.Uout = c->buf[other];
-    r.dt_dev = c->eig_dev;
     if (int rc = time_launches(c, 3, [&]() -> int {
-          HIPCHK(launch_rot_first_stage(c->cfg.g, c->cfg.eos, mixed, r, c->stream));
+          HIPCHK(pass_stage_launch(c, c->buf[c->cur], c->buf[other], c->eig_dev));
           return AMRH_OK;
         }, &msd[1][1]))
       return rc;

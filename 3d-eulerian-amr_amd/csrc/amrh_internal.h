@@ -25,6 +25,11 @@ hipError_t launch_pack(int n, int g, const double* pad, double* rec, long nrec_e
 bool block_scheme_supported(int n, int g, int recon);
 hipError_t launch_block_substep(int n, int g, int recon, const SubstepArgs& a, bool mixed, int grid, hipStream_t s);
 hipError_t launch_block_maxeig(int n, int g, int recon, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s);
+// the deferred-dt step of a flat outflow batch on the block kernel: the first substep with D and the
+// max face speed (a.dbuf, a.gmax), then the elementwise first stage from a.Uin = U^n into a.Uout
+hipError_t launch_block_deferred(int n, int g, int recon, const SubstepArgs& a, bool mixed, int grid,
+                                 hipStream_t s);
+hipError_t launch_block_first_stage(int n, int g, const SubstepArgs& a, bool mixed, hipStream_t s);
 hipError_t launch_dt_from_gmax(unsigned long long* gmax, double cfl, double scale, double growth, double* dt,
                                hipStream_t s);
 hipError_t launch_reset_u64(unsigned long long* p, hipStream_t s);

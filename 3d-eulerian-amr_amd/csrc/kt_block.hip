@@ -81,16 +81,20 @@ __device__ __forceinline__ void reconstruct(const typename Prec<MIXED>::T (*P)[N
   }
 }
 
+// the KT flux of a face and its wavespeed max(|v_l| + c_l, |v_r| + c_r) -- the number the CFL
+// scan forms for the same face states (eig_of_q)
 template <bool MIXED, int D>
-__device__ __forceinline__ void flux_q(const PhysParams& P, typename Prec<MIXED>::T* ql,
-                                       typename Prec<MIXED>::T* qr, typename Prec<MIXED>::T* F) {
+__device__ __forceinline__ double flux_q(const PhysParams& P, typename Prec<MIXED>::T* ql,
+                                         typename Prec<MIXED>::T* qr, typename Prec<MIXED>::T* F) {
   if constexpr (!MIXED) {
     double e;
     x64::kt_flux<D>(P, ql, qr, F, e);
+    return e;
   } else {
     const f32::PhysF PF{(float)P.gamma, (float)(P.gamma - 1.0), (float)P.theta};
     float e;
     f32::kt_flux<D>(PF, ql, qr, F, e);
+    return (double)e;
   }
 }
 
@@ -106,25 +110,30 @@ __device__ __forceinline__ double eig_of_q(const PhysParams& P, typename Prec<MI
 }
 
 // MODE 0: substep.  MODE 1: per-block max wavespeed (Grid::max_dt's scan, grid/grid.cpp:422-473).
+// MODE 2: the first substep of an RK step with dt deferred (flat outflow batches): the substep's
+// fluxes, the block's max face speed into a.gmax and each cell's D into a.dbuf instead of the
+// update; block_first_stage_kernel forms the first stage once dt is known.
 // TREE: one level of the octree (a.nodes): reflux overrides applied, the block's own boundary
 // faces exported to the coarser nodes' slots, the root's boundary faces to the ledger.
 // The mixed substep is held to 128 registers, four waves per SIMD: two workgroups share a CU
 // (53 KB of LDS each), which hides the staging and barrier latency -- a small spill, and 32%
 // faster than one workgroup at 155 registers on the 8192-block PPM batch.
 template <int N, int G, int RECON, bool MIXED, int MODE, bool TREE>
-__global__ __launch_bounds__(N* N* N) __attribute__((amdgpu_waves_per_eu(MIXED && MODE == 0 ? 4 : 1))) void
+__global__ __launch_bounds__(N* N* N) __attribute__((amdgpu_waves_per_eu(MIXED && MODE != 1 ? 4 : 1))) void
 kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
+  constexpr bool SUB = MODE != 1;  // fluxes and the cells' D
+  static_assert(!(TREE && MODE != 0), "the octree instantiation is the substep's");
   using T = typename Prec<MIXED>::T;
   constexpr int NT = N * N * N;
   constexpr int R = N * N * N + 6 * G * N * N;
   constexpr int NFACE = (N + 1) * N * N;
   __shared__ T P[NFE][R];
-  __shared__ T Fb[MODE == 0 ? NFE : 1][MODE == 0 ? NFACE : 1];
+  __shared__ T Fb[SUB ? NFE : 1][SUB ? NFACE : 1];
   __shared__ double red[NT / 64];
 
   const int tid = threadIdx.x;
-  const PhysParams PP = MODE == 0 ? a.P : m.P;
-  const int nbl = MODE == 0 ? a.nblocks : m.nblocks;
+  const PhysParams PP = SUB ? a.P : m.P;
+  const int nbl = SUB ? a.nblocks : m.nblocks;
   double dt = a.dt;
   if (MODE == 0 && a.dt_dev) dt = *a.dt_dev;
   const double beta = a.beta;
@@ -132,16 +141,16 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
   const int ci = tid % N, cj = (tid / N) % N, ck = tid / (N * N);
 
   for (int bb = blockIdx.x; bb < nbl; bb += gridDim.x) {
-    const int b = MODE == 0 ? (TREE ? a.nodes[bb] : a.block0 + bb) : bb;
-    const double* Ub = (MODE == 0 ? a.Uin : m.U) + (long)b * NFE * R;
+    const int b = SUB ? (TREE ? a.nodes[bb] : a.block0 + bb) : bb;
+    const double* Ub = (SUB ? a.Uin : m.U) + (long)b * NFE * R;
     const RecBuf<R, NFE> ub(Ub);
 
     // stage the primitives; each thread keeps its own interior cell's conserved state.  An
     // isolated outflow block (not TREE) forms each ghost from the boundary cell it copies
     // (grid/grid_phys_bound.cpp:27-54); its ghost slabs are not read.  The slabs are
     // wavefront-aligned (G N^2 a multiple of 64), so the choice is uniform per wavefront.
-    const bool self = !TREE && (MODE == 0 ? a.outflow : m.outflow) != 0;
-    const bool zr = (MODE == 0 ? a.z_reflect : m.z_reflect) != 0;
+    const bool self = !TREE && (SUB ? a.outflow : m.outflow) != 0;
+    const bool zr = (SUB ? a.z_reflect : m.z_reflect) != 0;
     double ucur[NFE];
 #pragma unroll 1
     for (int e = tid; e < R; e += NT) {
@@ -166,7 +175,7 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
 #pragma unroll
     for (int l = 0; l < NFE; l++) D[l] = 0.0;
     double emax = 0.0;
-    const double hinv = MODE == 0 ? 1.0 / a.h[b] : 0.0;
+    const double hinv = SUB ? 1.0 / a.h[b] : 0.0;
 
     auto sweep = [&](auto dc) {
       constexpr int d = decltype(dc)::value;
@@ -176,16 +185,17 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
         face_decode<N, d>(q, f, ta, tb);
         T ql[NFE], qr[NFE];
         reconstruct<N, G, RECON, MIXED>(P, d, f, ta, tb, ql, qr, PP.theta);
-        if constexpr (MODE == 0) {
+        if constexpr (SUB) {
           T Fo[NFE];
-          flux_q<MIXED, d>(PP, ql, qr, Fo);
+          const double e = flux_q<MIXED, d>(PP, ql, qr, Fo);
+          if constexpr (MODE == 2) emax = x64::rmax(emax, e);
 #pragma unroll
           for (int l = 0; l < NFE; l++) Fb[l][q] = Fo[l];
         } else {
           emax = x64::rmax(emax, eig_of_q<MIXED, d>(PP, ql, qr));
         }
       }
-      if constexpr (MODE == 0) {
+      if constexpr (SUB) {
         __syncthreads();
         if constexpr (TREE) {
           // adjust_*_flux (grid/grid_node.cpp:1076-1274): faces of the low, middle and high planes
@@ -255,6 +265,12 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
     sweep(std::integral_constant<int, 1>{});
     sweep(std::integral_constant<int, 2>{});
 
+    if constexpr (MODE == 2) {
+      // deferred: the cell's D (State::source is zero for the ideal gas) for block_first_stage_kernel
+      const RecBuf<NT, NFE> dbo(a.dbuf + (long)b * NFE * NT);
+#pragma unroll
+      for (int l = 0; l < NFE; l++) dbo.st_nt(l, (unsigned)tid, D[l] + 0.0);
+    }
     if constexpr (MODE == 0) {
       const long e = (long)tid;  // interior element of cell (ci, cj, ck)
       double u0v[NFE];
@@ -287,15 +303,46 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
       if (tid == 0) {
         double v = red[0];
         for (int w = 1; w < NT / 64; w++) v = x64::rmax(v, red[w]);
-        if (m.block_eig) m.block_eig[b] = v;
-        if (m.gmax) {
-          const double r = v / m.h[b];
-          atomicMax(m.gmax, (unsigned long long)__double_as_longlong(r));
+        if constexpr (MODE == 1) {
+          if (m.block_eig) m.block_eig[b] = v;
+          if (m.gmax) {
+            const double r = v / m.h[b];
+            atomicMax(m.gmax, (unsigned long long)__double_as_longlong(r));
+          }
+        } else {
+          const double r = v / a.h[b];
+          atomicMax(a.gmax, (unsigned long long)__double_as_longlong(r));
         }
       }
     }
     __syncthreads();  // P (and red) are restaged by the next block of a persistent grid
   }
+}
+
+// The first stage of a deferred RK step, one thread per interior cell: the substep's update with
+// beta = 1 (kt_block_kernel MODE 0) from the D the deferred substep stored, U1 = (U^n + D dt) 1,
+// and the floor (euler/state.cpp:86-92); the same operations, so U1 has the same bits.
+template <int N, int G, bool MIXED>
+__global__ __launch_bounds__(256) void block_first_stage_kernel(SubstepArgs a) {
+  constexpr int NT = N * N * N;
+  constexpr int R = NT + 6 * G * N * N;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)a.nblocks * NT) return;
+  const int b = (int)(gid / NT), e = (int)(gid % NT);
+  const double dt = *a.dt_dev;
+  const double beta = 1.0;
+  const RecBuf<R, NFE> ub(a.Uin + (long)b * NFE * R), uo(a.Uout + (long)b * NFE * R);
+  const RecBuf<NT, NFE> db(a.dbuf + (long)b * NFE * NT);
+  double u[NFE];
+#pragma unroll
+  for (int l = 0; l < NFE; l++) u[l] = (ub.ld(l, (unsigned)e) + db.ld_nt(l, (unsigned)e) * dt) * beta;
+  if constexpr (!MIXED)
+    x64::floor_state(a.P, u);
+  else
+    floor_mixed(a.P, u);
+  if (!(u[RHO] > 0.0) && a.err) atomicOr(a.err, 1);
+#pragma unroll
+  for (int l = 0; l < NFE; l++) uo.st(l, (unsigned)e, u[l]);
 }
 
 template <int N, int G, int RECON, bool MIXED, int MODE>
@@ -324,6 +371,31 @@ hipError_t launch_block_substep(int n, int g, int recon, const SubstepArgs& a, b
                  : launch_one<8, 3, AMRH_RECON_PPM, false, 0>(a, m, a.nblocks, grid, s);
   return mixed ? launch_one<8, 2, AMRH_RECON_MINMOD, true, 0>(a, m, a.nblocks, grid, s)
                : launch_one<8, 2, AMRH_RECON_MINMOD, false, 0>(a, m, a.nblocks, grid, s);
+}
+
+hipError_t launch_block_deferred(int n, int g, int recon, const SubstepArgs& a, bool mixed, int grid,
+                                 hipStream_t s) {
+  const MaxEigArgs m{};
+  if (!block_scheme_supported(n, g, recon)) return hipErrorInvalidValue;
+  if (a.nodes || a.slot || !a.outflow || !a.dbuf || !a.gmax) return hipErrorInvalidValue;
+  if (recon == AMRH_RECON_PPM)
+    return mixed ? launch_one<8, 3, AMRH_RECON_PPM, true, 2>(a, m, a.nblocks, grid, s)
+                 : launch_one<8, 3, AMRH_RECON_PPM, false, 2>(a, m, a.nblocks, grid, s);
+  return mixed ? launch_one<8, 2, AMRH_RECON_MINMOD, true, 2>(a, m, a.nblocks, grid, s)
+               : launch_one<8, 2, AMRH_RECON_MINMOD, false, 2>(a, m, a.nblocks, grid, s);
+}
+
+hipError_t launch_block_first_stage(int n, int g, const SubstepArgs& a, bool mixed, hipStream_t s) {
+  if (n != 8 || (g != 2 && g != 3) || !a.dbuf || !a.dt_dev) return hipErrorInvalidValue;
+  if (a.nblocks <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)(((long)a.nblocks * 512 + 255) / 256);
+  if (g == 3)
+    mixed ? block_first_stage_kernel<8, 3, true><<<grid, 256, 0, s>>>(a)
+          : block_first_stage_kernel<8, 3, false><<<grid, 256, 0, s>>>(a);
+  else
+    mixed ? block_first_stage_kernel<8, 2, true><<<grid, 256, 0, s>>>(a)
+          : block_first_stage_kernel<8, 2, false><<<grid, 256, 0, s>>>(a);
+  return hipGetLastError();
 }
 
 hipError_t launch_block_maxeig(int n, int g, int recon, const MaxEigArgs& m, bool mixed, int grid, hipStream_t s) {

@@ -76,14 +76,14 @@ def load_traffic(key):
 STAGE_NAMES = {0: "substep", 1: "cfl_scan", 2: "deferred_first_substep", 3: "dt", 4: "deferred_second_substep",
                5: "tree_ghosts", 6: "first_stage_pass"}
 SUBSTEP_KINDS = (0, 2, 4)
-FIRST_STAGE = 6  # the rotating state's first-stage pass: part of its deferred first substep's work
+FIRST_STAGE = 6  # the elementwise first-stage pass: part of its deferred first substep's work
 
 
 def kernel_times(B, torch, cells, steps, warmup):
     """RK3 steps on a prepared batch (state uploaded, outflow boundary mode) with direct launches,
     one step at a time through amrh_step_timed: HIP events recorded by the library on its stream
-    between the launches of the step (the CFL scan or, for a self-ghosting minmod batch, the
-    deferred first substep that replaces it, the dt kernel, the substeps).  Returns the mean time
+    between the launches of the step (the CFL scan or, for a self-ghosting batch, the deferred
+    first substep that replaces it, the dt kernel, the first-stage pass, the substeps).  Returns the mean time
     of each stage kind, the substep kernel times and the algorithmic bytes behind the roofline
     (the i-th substep of a step has beta = BETAS[i])."""
     dt = torch.zeros(1, dtype=torch.float64, device="cuda")
@@ -150,9 +150,9 @@ def measure(B, torch, cells, steps, warmup, tune=True):
         am = amrhydro_mod()
         res["launch"]["schedule"] = "deferred" if B.schedule() == am.AMRH_SCHEDULE_DEFERRED else "plain"
         sweep = B.cfg.physics == am.AMRH_PHYS_EULER and B.cfg.recon == am.AMRH_RECON_MINMOD
-        if sweep or B.cfg.physics == am.AMRH_PHYS_ROTATING:
+        if B.cfg.boundary == am.AMRH_BOUNDARY_OUTFLOW:
             # the deferred step's kernels: the first substep, then the sweep's second substep or
-            # the rotating state's first-stage pass
+            # the first-stage pass (PPM block kernel, rotating state)
             for k, name in ((0, "deferred_first"), (1, "deferred_second" if sweep else "first_stage_pass")):
                 res["launch"][name] = dict(zip(("grid", "variant", "ms_default_launch", "ms_tuned_launch"),
                                                B.deferred_launch_info(k)))

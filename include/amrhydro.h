@@ -183,12 +183,13 @@ int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_gr
  * with its dt deferred: the first substep forms the fluxes of U^n, reduces their KT face speeds --
  * the numbers Grid::max_dt's scan forms (grid/grid.cpp:422-483) -- to the CFL maximum and stores
  * the divergence D (sources included); the dt kernel applies the control above; the minmod ideal
- * gas's second substep forms U1 = floor(U^n + D dt) for every cell it reads, the rotating state's
- * U1 (floor with the Newton T, pot reset) is formed by an elementwise pass before plain substeps.
+ * gas's second substep forms U1 = floor(U^n + D dt) for every cell it reads; on the PPM block
+ * kernel and for the rotating state (floor with the Newton T, pot reset) an elementwise pass forms
+ * U1 before plain substeps.
  * No separate CFL scan, no store: the same bits as amrh_dt_device + amrh_store +
  * amrh_substep_dev.  amrh_autotune also picks the launch shapes of those kernels;
  * amrh_deferred_launch_info returns them (kernel 0: the deferred first substep, 1: the second
- * substep or the rotating state's first-stage pass), as amrh_launch_info does. */
+ * substep or the first-stage pass), as amrh_launch_info does. */
 int amrh_deferred_launch_info(const amrh_ctx* ctx, int kernel, int* grid, int* variant, double* ms_default,
                               double* ms_tuned);
 /* Whether such steps run deferred (the default) or as scan + store + substeps: amrh_autotune keeps
@@ -206,7 +207,7 @@ int amrh_get_schedule(const amrh_ctx* ctx, int* schedule);
 #define AMRH_LAUNCH_DT 3      /* the dt kernel of a deferred step */
 #define AMRH_LAUNCH_SECOND 4  /* the substep after it (first stage formed from U^n and D) */
 #define AMRH_LAUNCH_GHOSTS 5  /* the octree's ghost fill at the start of a step */
-#define AMRH_LAUNCH_FIRST_STAGE 6 /* the rotating state's first-stage pass of a deferred step */
+#define AMRH_LAUNCH_FIRST_STAGE 6 /* the elementwise first-stage pass of a deferred step */
 /* One step of amrh_run_steps (use_graph = 0) with HIP events recorded on the context's stream
  * between its stages: ms[s] is stage s's duration and kinds[s] its AMRH_LAUNCH_* kind, for
  * s < *nstages <= max_stages.  Synchronizes.  A measurement aid; no counterpart in the reference. */

@@ -1,11 +1,12 @@
-"""Self-ghosting outflow blocks and the deferred-dt RK step of flat minmod batches.
+"""Self-ghosting outflow blocks and the deferred-dt RK step of flat ideal-gas batches.
 
 An AMRH_BOUNDARY_OUTFLOW block of the minmod ideal gas forms every ghost from the boundary cell it
 copies (grid/grid_phys_bound.cpp:27-54, euler/state.cpp:45-84) where the kernel reads it, so the
 ghost slabs in memory play no part.  An RK2/RK3 step of such a batch (amrh_run_steps) runs with its
 dt deferred: the first substep reduces its own KT face speeds to the CFL maximum and stores the
-divergence, and the second forms the first stage from U^n and D once dt is known.  Both must give
-the bits of the plain sequence (CFL scan, store, substeps), and the FP64 form the oracle's."""
+divergence, and the second (minmod) or an elementwise pass (PPM) forms the first stage from U^n
+and D once dt is known.  Both must give the bits of the plain sequence (CFL scan, store,
+substeps), and the FP64 form the oracle's."""
 import numpy as np
 import pytest
 
@@ -150,8 +151,8 @@ def test_deferred_step_fp64_matches_oracle():
 
 
 def test_step_timed_stages():
-    """amrh_step_timed: the deferred step's stages for a minmod outflow batch, the plain ones for
-    RK1 and for PPM (which fills ghosts after each substep)"""
+    """amrh_step_timed: the deferred step's stages for minmod and PPM outflow batches (the PPM one
+    with its first-stage pass), the plain ones for RK1"""
     n, nb = 16, 8
     U = blocks(n, nb, seed=81)
     A = outflow_batch(U, 1.0 / n, 1, n)
@@ -170,7 +171,8 @@ def test_step_timed_stages():
     P.upload(V)
     P.fill_outflow_ghosts()
     st = P.step_timed(d.data_ptr(), rk=3)
-    assert [k for k, _ in st] == [amrhydro.AMRH_LAUNCH_SCAN] + [amrhydro.AMRH_LAUNCH_SUBSTEP] * 3
+    assert [k for k, _ in st] == [amrhydro.AMRH_LAUNCH_FIRST, amrhydro.AMRH_LAUNCH_DT,
+                                  amrhydro.AMRH_LAUNCH_FIRST_STAGE] + [amrhydro.AMRH_LAUNCH_SUBSTEP] * 2
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -199,7 +201,8 @@ def test_autotuned_deferred_kernels_give_the_same_bits(precision):
 @pytest.mark.parametrize("n", [8, 16])
 def test_schedules_give_the_same_bits(n):
     """amrh_set_schedule: deferred and plain RK3 steps of the same batch are bit-identical; the
-    autotuner's pick is one of the two; a PPM batch reports the plain schedule"""
+    autotuner's pick is one of the two; a batch without outflow boundaries reports the plain
+    schedule"""
     nb = 300 if n == 8 else 64
     U = blocks(n, nb, seed=93 + n)
     h = np.full(nb, 1.0 / n)
@@ -221,8 +224,42 @@ def test_schedules_give_the_same_bits(n):
     for o, dt in outs[1:]:
         np.testing.assert_array_equal(o, outs[0][0])
         assert dt == outs[0][1]
-    import workloads
-    V = workloads.flat_batch(4, n=8, g=3, seed=2)
-    P = amrhydro.Batch(amrhydro.euler_config(n=8, g=3, recon=amrhydro.AMRH_RECON_PPM,
-                                             boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW), 4, 1.0 / 8)
+    P = amrhydro.Batch(amrhydro.euler_config(n=n), 4, 1.0 / n)
     assert P.schedule() == amrhydro.AMRH_SCHEDULE_PLAIN
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("z_reflect", [0, 1])
+def test_ppm_deferred_step_equals_plain(precision, z_reflect):
+    """The PPM block kernel's deferred step (its first substep's face speeds as the CFL maximum, D
+    stored, the first stage formed by an elementwise pass) gives the bits of the plain schedule,
+    direct, as graphs and with the autotuner's launch shapes, dt included"""
+    import workloads
+    nb = 96
+    V = workloads.flat_batch(nb, n=8, g=3, seed=7 + z_reflect)
+    h = np.full(nb, 1.0 / 8) * np.where(np.arange(nb) % 4 == 0, 0.5, 1.0)
+    cfg = amrhydro.euler_config(n=8, g=3, recon=amrhydro.AMRH_RECON_PPM, precision=precision,
+                                z_reflect=z_reflect, boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW)
+    outs = []
+    for sched, use_graph in ((amrhydro.AMRH_SCHEDULE_PLAIN, False), (amrhydro.AMRH_SCHEDULE_DEFERRED, False),
+                             (amrhydro.AMRH_SCHEDULE_DEFERRED, True), (None, True)):
+        P = amrhydro.Batch(cfg, nb, h)
+        P.upload(V)
+        if sched is None:
+            P.autotune()
+            g, _, ms0, ms1 = P.deferred_launch_info(0)
+            say("ppm deferred substep: grid %d, %.3f -> %.3f ms; schedule %d" % (g, ms0, ms1, P.schedule()))
+            assert 0.0 < ms1 <= ms0
+        else:
+            P.set_schedule(sched)
+            assert P.schedule() == sched
+        d = dt_buffer(1e-4)
+        P.run_steps(3, d.data_ptr(), rk=3, use_graph=use_graph)
+        P.sync()
+        outs.append((P.download(), float(d.cpu()[0])))
+        assert P.error_flags() == 0
+        P.close()
+    say("ppm prec=%d zr=%d: dt after 3 steps %.6e" % (precision, z_reflect, outs[0][1]))
+    for o, dt in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0][0])
+        assert dt == outs[0][1]
