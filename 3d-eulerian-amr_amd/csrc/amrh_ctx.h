@@ -115,6 +115,11 @@ void drop_graphs(amrh_ctx* c);
 int tree_upload_x0(amrh_ctx* c);
 // amrh_tree_substep_dev; fill = false skips the ghost fill (the caller filled the current state)
 int tree_substep_dev(amrh_ctx* c, const double* dt_dev, double beta, bool fill);
+// the rotating state's deferred-dt step on the octree: the ghost fill and the deferred first
+// substep of every level (D, the CFL maximum, reflux and ledger faces), then -- dt known, U^n
+// stored -- the first stage of every node and that substep's ledger (amrh_tree.hip)
+int tree_deferred_first(amrh_ctx* c);
+int tree_first_stage(amrh_ctx* c, const double* dt_dev);
 // state_sum / max / min reductions (amrh_diag.hip): the checks, the size of the per-block partials,
 // and the two passes (no allocation; asynchronous on the context's stream)
 int diag_check(const amrh_ctx* c);

@@ -103,7 +103,9 @@ bool block_deferred(const amrh_ctx* c) {
 }
 // the deferred step with an elementwise first-stage pass
 bool pass_deferred(const amrh_ctx* c) { return rot_deferred(c) || block_deferred(c); }
-bool deferred_capable(const amrh_ctx* c) { return sweep_deferred(c) || pass_deferred(c); }
+// the rotating state on the octree: the deferred substep level by level, the pass over every node
+bool tree_deferred(const amrh_ctx* c) { return c->tree && c->cfg.physics == AMRH_PHYS_ROTATING; }
+bool deferred_capable(const amrh_ctx* c) { return sweep_deferred(c) || pass_deferred(c) || tree_deferred(c); }
 
 // the deferred substep of a pass_deferred batch: fluxes of Uin, the CFL maximum into gmax_dev, D
 hipError_t pass_first_launch(const amrh_ctx* c, double* Uin, int grid) {
@@ -171,7 +173,29 @@ int step_deferred_pass(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
   return AMRH_OK;
 }
 
+// the octree's (rotating state): ghost fill + deferred substep of every level, dt, the store, the
+// first stage of every node and its ledger, the restriction, then the plain substeps (each with
+// its ghost fill and followed by the restriction) -- the bits of ghost fill + scan + store + substeps
+int step_deferred_tree(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
+  if (int rc = mark_begin(c, mk)) return rc;
+  if (int rc = tree_deferred_first(c)) return rc;
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_FIRST)) return rc;
+  HIPCHK(launch_dt_from_gmax(c->gmax_dev, c->cfg.cfl, 1.0, 1.25, dt_dev, c->stream));
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_DT)) return rc;
+  if (int rc = amrh_store(c)) return rc;
+  if (int rc = tree_first_stage(c, dt_dev)) return rc;
+  if (int rc = amrh_tree_inject(c)) return rc;
+  if (int rc = mark_end(c, mk, AMRH_LAUNCH_FIRST_STAGE)) return rc;
+  for (int s = 1; s < rk; s++) {
+    if (int rc = tree_substep_dev(c, dt_dev, kBetas[rk - 1][s], true)) return rc;
+    if (int rc = amrh_tree_inject(c)) return rc;
+    if (int rc = mark_end(c, mk, AMRH_LAUNCH_SUBSTEP)) return rc;
+  }
+  return AMRH_OK;
+}
+
 int step_deferred(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
+  if (c->tree) return step_deferred_tree(c, rk, dt_dev, mk);
   if (pass_deferred(c)) return step_deferred_pass(c, rk, dt_dev, mk);
   const bool mixed = c->cfg.precision == AMRH_MIXED;
   const int p = c->cur, q = 1 - p;
@@ -333,7 +357,7 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
     // pass, then the same beta != 1 substeps (the dt kernel is common)
     c->schedule = msd[0][1] + msd[1][1] < best_m + best_s ? AMRH_SCHEDULE_DEFERRED : AMRH_SCHEDULE_PLAIN;
     HIPCHK(hipStreamSynchronize(c->stream));
-  } else if (deferred_capable(c)) {
+  } else if (sweep_deferred(c)) {
     if (int rc = ensure_dbuf(c)) return rc;
     const bool mixed = c->cfg.precision == AMRH_MIXED;
     HIPCHK(hipMemsetAsync(c->eig_dev, 0, sizeof(double), c->stream));

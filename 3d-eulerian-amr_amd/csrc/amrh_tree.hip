@@ -373,6 +373,55 @@ double max_dt_rec(const TreeState* t, const std::vector<double>& eig, double cfl
 
 }  // namespace
 
+// The rotating state's RK step on the octree with its dt deferred.  First: the ghost fill a
+// substep makes, then the deferred substep level by level, finest first -- the reflux overrides,
+// exports and ledger faces as the substep makes them (none depends on dt), the max face speed over
+// every node into gmax_dev (the CFL scan's numbers: the face states are the same, and the scan's
+// same-level ghosts are copies of the cells read here directly), D into dbuf.
+int amrh::tree_deferred_first(amrh_ctx* c) {
+  TreeState* t = c->tree;
+  if (int rc = ctx_ready(c)) return rc;
+  if (!c->dbuf) return fail(AMRH_ESTATE, "no D buffer");
+  if (int rc = fill_ghosts(c, true)) return rc;
+  RotArgs a = ctx_rot_args(c);
+  a.Uin = c->buf[c->cur];
+  a.dbuf = c->dbuf;
+  a.gmax = c->gmax_dev;
+  a.beta = 1.0;
+  a.slot = t->slot_dev;
+  a.ovr = t->ovr_dev;
+  a.exp_slot = t->exp_dev;
+  a.ledger = t->ledger_dev;
+  a.ghost = t->ghost_dev;
+  const bool mixed = c->cfg.precision == AMRH_MIXED;
+  for (int l = t->maxlevel; l >= 0; l--) {
+    a.nodes = t->level_nodes_dev + t->level_start[l];
+    a.nblocks = t->level_count[l];
+    HIPCHK(launch_rot_tree_deferred(c->cfg.g, c->cfg.recon, c->cfg.eos, mixed, a, c->stream));
+  }
+  return AMRH_OK;
+}
+
+// Then, once dt_dev holds dt and U^n is stored as the RK base: the first stage of every node
+// (U^n + D dt with the floor and the pot reset, as the substep's epilogue) into the other buffer,
+// and the flow-off ledger of that substep (beta = 1).  The restriction is the caller's.
+int amrh::tree_first_stage(amrh_ctx* c, const double* dt_dev) {
+  TreeState* t = c->tree;
+  if (c->u0buf != c->cur) return fail(AMRH_ESTATE, "the first stage needs U^n stored as the RK base");
+  const int out = 1 - c->cur;
+  RotArgs a = ctx_rot_args(c);
+  a.Uin = c->buf[c->cur];
+  a.Uout = c->buf[out];
+  a.dbuf = c->dbuf;
+  a.dt_dev = dt_dev;
+  a.beta = 1.0;
+  HIPCHK(launch_rot_first_stage(c->cfg.g, c->cfg.eos, c->cfg.precision == AMRH_MIXED, a, c->stream));
+  c->cur = out;
+  HIPCHK(launch_ledger(c->cfg.n, c->cfg.nf, t->ledger_dev, t->h[0], dt_dev, 0.0, 1.0, c->cfg.z_reflect, t->fo_dev,
+                       c->stream));
+  return AMRH_OK;
+}
+
 int amrh::tree_substep_dev(amrh_ctx* c, const double* dt_dev, double beta, bool fill) {
   if (!c->tree || !c->have_u) return fail(AMRH_ESTATE, "tree or state not set");
   return tree_substep(c, 0.0, beta, dt_dev, fill);

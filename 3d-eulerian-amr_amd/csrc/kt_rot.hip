@@ -114,15 +114,17 @@ __device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int 
 // of all three directions as one list (with the ideal gas, as pairs of adjacent faces that share
 // their reconstruction); the FP64 form stages one direction's pencils at a time.
 // MODE 0: substep.  MODE 1: the CFL scan.  MODE 2: the first substep of an RK step with dt deferred
-// (flat batches): the substep's fluxes and sources, the block's max face speed (the numbers the scan
-// forms from the same faces) into a.gmax, and each cell's D (sources included) into a.dbuf instead
-// of the update; rot_first_stage_kernel forms the first stage once dt is known.
+// (flat batches, and the levels of the octree): the substep's fluxes and sources, the block's max
+// face speed (the numbers the scan forms from the same faces, before any reflux override) into
+// a.gmax, and each cell's D (sources included) into a.dbuf instead of the update;
+// rot_first_stage_kernel forms the first stage once dt is known.  The octree's reflux overrides,
+// exports and ledger faces do not depend on dt, so MODE 2 applies and writes them as MODE 0 does.
 template <int G, int RECON, int EOS, int MODE, bool MIXED, bool TREE>
 __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   using T = typename std::conditional<MIXED, float, double>::type;
   constexpr bool SUB = MODE == 0 || MODE == 2;  // fluxes and the cells' D
   constexpr bool EIGM = MODE != 0;              // the block's max face speed
-  static_assert(!(TREE && MODE != 0), "the octree instantiation is the substep's");
+  static_assert(!(TREE && MODE == 1), "the octree instantiations are the substep's");
   constexpr bool ALL = MIXED;          // stage the whole record once
   constexpr int R = N3 + 6 * G * N2;   // record elements per field
   constexpr int RS = ALL ? R : N3 + 2 * G * N2;  // staged elements per field
@@ -978,8 +980,8 @@ template <int G, int RECON, int EOS, int MODE, bool MIXED>
 hipError_t launch_one(const RotArgs& a, int grid, hipStream_t s) {
   if (a.nblocks <= 0) return hipSuccess;
   const int g = grid > 0 && grid < a.nblocks ? grid : a.nblocks;
-  if (MODE == 0 && a.slot)
-    kt_rot_kernel<G, RECON, EOS, MODE, MIXED, MODE == 0><<<g, N3, 0, s>>>(a);
+  if (MODE != 1 && a.slot)
+    kt_rot_kernel<G, RECON, EOS, MODE, MIXED, MODE != 1><<<g, N3, 0, s>>>(a);
   else
     kt_rot_kernel<G, RECON, EOS, MODE, MIXED, false><<<g, N3, 0, s>>>(a);
   return hipGetLastError();
@@ -1008,6 +1010,10 @@ hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArg
 hipError_t launch_rot_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s) {
   if (a.slot || a.nodes || !a.outflow || !a.dbuf || !a.gmax) return hipErrorInvalidValue;
   return mixed ? dispatch<2, true>(g, recon, eos, a, grid, s) : dispatch<2, false>(g, recon, eos, a, grid, s);
+}
+hipError_t launch_rot_tree_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, hipStream_t s) {
+  if (!a.slot || !a.nodes || a.outflow || !a.dbuf || !a.gmax) return hipErrorInvalidValue;
+  return mixed ? dispatch<2, true>(g, recon, eos, a, 0, s) : dispatch<2, false>(g, recon, eos, a, 0, s);
 }
 hipError_t launch_rot_first_stage(int g, int eos, bool mixed, const RotArgs& a, hipStream_t s) {
   if (!a.dbuf || !a.dt_dev) return hipErrorInvalidValue;

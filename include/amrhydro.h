@@ -185,7 +185,10 @@ int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_gr
  * the divergence D (sources included); the dt kernel applies the control above; the minmod ideal
  * gas's second substep forms U1 = floor(U^n + D dt) for every cell it reads; on the PPM block
  * kernel and for the rotating state (floor with the Newton T, pot reset) an elementwise pass forms
- * U1 before plain substeps.
+ * U1 before plain substeps.  The rotating state's octree steps run the same way: the deferred
+ * substep level by level, finest first, with its reflux overrides, exports and ledger faces (none
+ * depends on dt), then dt, the store, the first stage of every node, the ledger and the
+ * restriction; its first ghost fill is the substep's (same-level faces read from the neighbours).
  * No separate CFL scan, no store: the same bits as amrh_dt_device + amrh_store +
  * amrh_substep_dev.  amrh_autotune also picks the launch shapes of those kernels;
  * amrh_deferred_launch_info returns them (kernel 0: the deferred first substep, 1: the second

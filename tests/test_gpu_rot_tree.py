@@ -168,6 +168,42 @@ def test_rot_tree_graph_steps_equal_direct_steps():
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+@pytest.mark.parametrize("eos,recon", [(0, 1), (0, 0), (1, 1)])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("z_reflect", [0, 1])
+def test_rot_tree_deferred_steps_equal_plain_steps(eos, recon, precision, z_reflect):
+    """The octree's deferred-dt step (the first substep of every level storing D and reducing its
+    face speeds to the CFL maximum, then dt, the store and the first stage of every node) gives the
+    plain step's bits (ghost fill, CFL scan, store, substeps): the interiors, the flow-off ledger
+    and dt, direct and as graphs"""
+    outs = []
+    g = 3 if recon == 1 else 2
+    I = (slice(None), slice(None), slice(g, -g), slice(g, -g), slice(g, -g))
+    for sched, use_graph in ((amrhydro.AMRH_SCHEDULE_PLAIN, 0), (amrhydro.AMRH_SCHEDULE_DEFERRED, 0),
+                             (amrhydro.AMRH_SCHEDULE_DEFERRED, 1)):
+        T, cfg = binary_tree_case("partial", recon=recon, z_reflect=z_reflect, seed=27, eos=eos)
+        B = device_batch(T, cfg, precision)
+        B.set_schedule(sched)
+        assert B.schedule() == sched
+        B.tree_fill_ghosts()
+        dt = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
+        B.run_steps(3, dt.data_ptr(), rk=3, use_graph=bool(use_graph))
+        B.sync()
+        outs.append((B.download()[I], B.tree_flow_off(), dt.item()))
+        assert B.error_flags() == 0
+        if sched == amrhydro.AMRH_SCHEDULE_DEFERRED and not use_graph:
+            st = B.step_timed(dt.data_ptr(), rk=3)
+            assert [k for k, _ in st] == [amrhydro.AMRH_LAUNCH_FIRST, amrhydro.AMRH_LAUNCH_DT,
+                                          amrhydro.AMRH_LAUNCH_FIRST_STAGE, amrhydro.AMRH_LAUNCH_SUBSTEP,
+                                          amrhydro.AMRH_LAUNCH_SUBSTEP]
+        B.close()
+    say("rot tree deferred eos=%d recon=%d prec=%d zr=%d: dt %.6e" % (eos, recon, precision, z_reflect, outs[0][2]))
+    for u, fo, d in outs[1:]:
+        np.testing.assert_array_equal(u, outs[0][0])
+        np.testing.assert_array_equal(fo, outs[0][1])
+        assert d == outs[0][2]
+
+
 def test_rot_tree_mixed_stays_near_fp64_oracle():
     T, cfg = binary_tree_case("partial", seed=26)
     B = device_batch(T, cfg, precision=amrhydro.AMRH_MIXED)
