@@ -320,7 +320,8 @@ class Batch:
 
     def deferred_launch_info(self, kernel):
         """(grid, variant, ms_default, ms_tuned) of the deferred-dt step's first (0) or second (1)
-        substep kernel, as picked by autotune()"""
+        substep kernel, or (2) of the rotating state's later substeps (variant 1: split into a
+        D-storing substep and the stage pass), as picked by autotune()"""
         g, v = C.c_int(), C.c_int()
         d, t = C.c_double(), C.c_double()
         _ck(lib().amrh_deferred_launch_info(self.ctx, kernel, C.byref(g), C.byref(v), C.byref(d), C.byref(t)))

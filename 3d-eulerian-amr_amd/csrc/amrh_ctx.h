@@ -23,6 +23,9 @@ struct TreeState {
   std::vector<double> h;
   // host copies of the tables
   std::vector<int> level_nodes, level_start, level_count;
+  // the rotating kernel's launch over a level with more nodes than this: a grid of this many
+  // workgroups striding over the nodes (four per CU); smaller levels: one workgroup per node
+  int level_grid = 1024;
   std::vector<int> item_start, item_count;  // export items per fine level
   std::vector<int> pair_start, pair_count;  // restriction pairs per parent level
   int nslots = 0;
@@ -77,9 +80,12 @@ struct amrh_ctx {
   // (index 0: the deferred first substep, 1: the substep that follows it)
   double* dbuf = nullptr;
   int schedule = AMRH_SCHEDULE_DEFERRED;  // where deferred steps apply; amrh_autotune picks the faster
-  int grid_deferred[2] = {0, 0};
-  int variant_deferred[2] = {0, 0};
-  float tune_ms_deferred[2][2] = {{0.f, 0.f}, {0.f, 0.f}};  // [kernel][default, tuned]
+  int grid_deferred[3] = {0, 0, 0};
+  int variant_deferred[3] = {0, 0, 0};
+  float tune_ms_deferred[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};  // [kernel][default, tuned]
+  // the rotating state's later substeps of a deferred step as a D-only substep and the stage pass
+  // (-1: by the EOS, the Helmholtz EOS split; amrh_autotune measures it)
+  int split_later = -1;
   // captured RK steps, one graph per parity of the ping-pong buffers (amrh_graph.hip)
   hipGraphExec_t step_exec[2] = {nullptr, nullptr};
   int step_exec_rk = 0;

@@ -107,14 +107,15 @@ bool pass_deferred(const amrh_ctx* c) { return rot_deferred(c) || block_deferred
 bool tree_deferred(const amrh_ctx* c) { return c->tree && c->cfg.physics == AMRH_PHYS_ROTATING; }
 bool deferred_capable(const amrh_ctx* c) { return sweep_deferred(c) || pass_deferred(c) || tree_deferred(c); }
 
-// the deferred substep of a pass_deferred batch: fluxes of Uin, the CFL maximum into gmax_dev, D
-hipError_t pass_first_launch(const amrh_ctx* c, double* Uin, int grid) {
+// the deferred substep of a pass_deferred batch: fluxes of Uin, the CFL maximum into gmax_dev
+// (cfl = false, the rotating state's split later substeps: not formed), D
+hipError_t pass_first_launch(const amrh_ctx* c, double* Uin, int grid, bool cfl = true) {
   const bool mixed = c->cfg.precision == AMRH_MIXED;
   if (rot_deferred(c)) {
     RotArgs r = ctx_rot_args(c);
     r.Uin = Uin;
     r.dbuf = c->dbuf;
-    r.gmax = c->gmax_dev;
+    r.gmax = cfl ? c->gmax_dev : nullptr;
     r.outflow = 1;
     r.z_reflect = c->cfg.z_reflect;
     r.beta = 1.0;
@@ -128,18 +129,22 @@ hipError_t pass_first_launch(const amrh_ctx* c, double* Uin, int grid) {
   return launch_block_deferred(c->cfg.n, c->cfg.g, c->cfg.recon, a, mixed, grid, c->stream);
 }
 
-// its first-stage pass: Uout = floor((Uin + D dt) 1) for every interior cell
-hipError_t pass_stage_launch(const amrh_ctx* c, double* Uin, double* Uout, const double* dt_dev) {
+// its stage pass: Uout = floor((Uin + D dt) 1) for every interior cell (the rotating state's
+// later substeps: floor((Uin + D dt) beta + U0 (1 - beta)))
+hipError_t pass_stage_launch(const amrh_ctx* c, double* Uin, double* Uout, const double* dt_dev, double beta = 1.0,
+                             double* U0 = nullptr) {
   const bool mixed = c->cfg.precision == AMRH_MIXED;
   if (rot_deferred(c)) {
     RotArgs r = ctx_rot_args(c);
     r.Uin = Uin;
     r.Uout = Uout;
+    r.U0 = U0;
     r.dbuf = c->dbuf;
     r.dt_dev = dt_dev;
-    r.beta = 1.0;
-    return launch_rot_first_stage(c->cfg.g, c->cfg.eos, mixed, r, c->stream);
+    r.beta = beta;
+    return launch_rot_stage(c->cfg.g, c->cfg.eos, mixed, r, c->stream);
   }
+  if (beta != 1.0) return hipErrorInvalidValue;
   SubstepArgs a = flat_args(c);
   a.Uin = Uin;
   a.Uout = Uout;
@@ -166,8 +171,15 @@ int step_deferred_pass(amrh_ctx* c, int rk, double* dt_dev, Marks* mk) {
   HIPCHK(pass_stage_launch(c, c->buf[p], c->buf[q], dt_dev));
   if (int rc = mark_end(c, mk, AMRH_LAUNCH_FIRST_STAGE)) return rc;
   c->cur = q;
+  const bool split = rot_deferred(c) && (c->split_later >= 0 ? c->split_later != 0 : c->cfg.eos == AMRH_EOS_HELMHOLTZ);
   for (int s = 1; s < rk; s++) {
-    if (int rc = amrh_substep_dev(c, dt_dev, kBetas[rk - 1][s])) return rc;
+    if (split) {
+      // the substep as D (no CFL maximum) and the stage pass, in place as the substep would be
+      HIPCHK(pass_first_launch(c, c->buf[c->cur], c->grid_deferred[2], false));
+      HIPCHK(pass_stage_launch(c, c->buf[c->cur], c->buf[c->cur], dt_dev, kBetas[rk - 1][s], c->buf[c->u0buf]));
+    } else if (int rc = amrh_substep_dev(c, dt_dev, kBetas[rk - 1][s])) {
+      return rc;
+    }
     if (int rc = mark_end(c, mk, AMRH_LAUNCH_SUBSTEP)) return rc;
   }
   return AMRH_OK;
@@ -324,8 +336,8 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   // the two kernels of the deferred-dt step (self-ghosting flat batches): the first substep, then
   // the one that forms its input from U^n and D; timed with dt = 0 (eig_dev[0] as the device dt),
   // writing only D, the CFL word (cleared after) and the spare buffer
-  int gd[2] = {0, 0}, vd[2] = {0, 0};
-  float msd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  int gd[3] = {0, 0, 0}, vd[3] = {0, 0, 0};
+  float msd[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
   if (pass_deferred(c)) {
     // the rotating state's and the block kernel's: the deferred substep over the grids, and the
     // first-stage pass (kernel 1)
@@ -356,6 +368,25 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
     // an RK step is the scan and a beta = 1 substep, or the deferred substep and the first-stage
     // pass, then the same beta != 1 substeps (the dt kernel is common)
     c->schedule = msd[0][1] + msd[1][1] < best_m + best_s ? AMRH_SCHEDULE_DEFERRED : AMRH_SCHEDULE_PLAIN;
+    if (rot_deferred(c)) {
+      // the later substeps: the substep (kernel 2's default time) against D + the stage pass
+      float ms_d = 0.f, ms_st = 0.f;
+      if (int rc = time_launches(c, 3, [&]() -> int {
+            HIPCHK(pass_first_launch(c, c->buf[c->cur], gd[0], false));
+            return AMRH_OK;
+          }, &ms_d))
+        return rc;
+      if (int rc = time_launches(c, 3, [&]() -> int {
+            HIPCHK(pass_stage_launch(c, c->buf[c->cur], c->buf[other], c->eig_dev, 0.25, c->buf[c->cur]));
+            return AMRH_OK;
+          }, &ms_st))
+        return rc;
+      gd[2] = gd[0];
+      msd[2][0] = best_s;
+      msd[2][1] = ms_d + ms_st;
+      vd[2] = msd[2][1] < best_s ? 1 : 0;
+      c->split_later = vd[2];
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
   } else if (sweep_deferred(c)) {
     if (int rc = ensure_dbuf(c)) return rc;
@@ -411,7 +442,7 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   c->variant = vs;
   c->tune_ms_default = def_s;
   c->tune_ms_best = best_s;
-  for (int kx = 0; kx < 2; kx++) {
+  for (int kx = 0; kx < 3; kx++) {
     c->grid_deferred[kx] = gd[kx];
     c->variant_deferred[kx] = vd[kx];
     c->tune_ms_deferred[kx][0] = msd[kx][0];
@@ -454,8 +485,9 @@ int amrh_set_launch(amrh_ctx* c, int grid_substep, int grid_maxeig) {
 int amrh_deferred_launch_info(const amrh_ctx* c, int kernel, int* grid, int* variant, double* ms_default,
                               double* ms_tuned) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
-  if (kernel < 0 || kernel > 1)
-    return fail(AMRH_EINVAL, "kernel must be 0 (the deferred first substep) or 1 (the second kernel of the step)");
+  if (kernel < 0 || kernel > 2)
+    return fail(AMRH_EINVAL, "kernel must be 0 (the deferred first substep), 1 (the second kernel of the step) or 2 "
+                             "(the rotating state's later substeps)");
   if (grid) *grid = c->grid_deferred[kernel];
   if (variant) *variant = c->variant_deferred[kernel];
   if (ms_default) *ms_default = c->tune_ms_deferred[kernel][0];

@@ -114,6 +114,13 @@ __device__ __forceinline__ double h5(const double* fi, double w0t, double w1t, d
   return r;
 }
 
+// h5's term k is fi[k] * (density weight kH5D[k]) * (temperature weight kH5T[k]), the weights
+// indexed {w0, w0m, w1, w1m, w2, w2m}
+constexpr int kH5D[36] = {0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 2, 3, 2, 3, 4, 5,
+                          4, 5, 2, 3, 2, 3, 4, 5, 4, 5, 2, 3, 2, 3, 4, 5, 4, 5};
+constexpr int kH5T[36] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 0, 0, 1, 1, 0, 0,
+                          1, 1, 2, 2, 3, 3, 2, 2, 3, 3, 4, 4, 5, 5, 4, 4, 5, 5};
+
 // helmholtz_eos (:164-421).  On a state out of the table's range, or with a negative heat
 // capacity or squared sound speed, the reference aborts; here an error flag is raised and the
 // outputs are NaN.
@@ -174,17 +181,6 @@ __device__ inline void eos(const HelmDev& H, Eos& q) {
   --jat;
   --iat;
 
-  double fi[36];
-  {
-    const double2* src = reinterpret_cast<const double2*>(H.cell + ((long)jat * (IMAX - 1) + iat) * 36);
-#pragma unroll
-    for (int r = 0; r < 18; r++) {
-      const double2 v = src[r];
-      fi[2 * r] = v.x;
-      fi[2 * r + 1] = v.y;
-    }
-  }
-
   // differences and basis functions (:255-321)
   const double xt = hmax((temp - H.t[jat]) * H.dti_sav[jat], 0.0);
   const double xd = hmax((din - H.d[iat]) * H.ddi_sav[iat], 0.0);
@@ -232,15 +228,37 @@ __device__ inline void eos(const HelmDev& H, Eos& q) {
   const double ddsi1md = -ddpsi1(mxd) * ddis;
   const double ddsi2md = ddpsi2(mxd);
 
-  // the free energy and its derivatives (:323-351)
-  const double free = h5(fi, si0t, si1t, si2t, si0mt, si1mt, si2mt, si0d, si1d, si2d, si0md, si1md, si2md);
-  const double df_d = h5(fi, si0t, si1t, si2t, si0mt, si1mt, si2mt, dsi0d, dsi1d, dsi2d, dsi0md, dsi1md, dsi2md);
-  const double df_t = h5(fi, dsi0t, dsi1t, dsi2t, dsi0mt, dsi1mt, dsi2mt, si0d, si1d, si2d, si0md, si1md, si2md);
-  const double df_dd =
-      h5(fi, si0t, si1t, si2t, si0mt, si1mt, si2mt, ddsi0d, ddsi1d, ddsi2d, ddsi0md, ddsi1md, ddsi2md);
-  const double df_tt =
-      h5(fi, ddsi0t, ddsi1t, ddsi2t, ddsi0mt, ddsi1mt, ddsi2mt, si0d, si1d, si2d, si0md, si1md, si2md);
-  const double df_dt = h5(fi, dsi0t, dsi1t, dsi2t, dsi0mt, dsi1mt, dsi2mt, dsi0d, dsi1d, dsi2d, dsi0md, dsi1md, dsi2md);
+  // the free energy and its derivatives (:323-351): the six biquintic sums h5 forms, in one pass
+  // over the cell's 36 coefficients (each sum in h5's term order, so the same values); the
+  // coefficients stream in as they are used instead of all 36 being held at once
+  const double TW[3][6] = {{si0t, si0mt, si1t, si1mt, si2t, si2mt},
+                           {dsi0t, dsi0mt, dsi1t, dsi1mt, dsi2t, dsi2mt},
+                           {ddsi0t, ddsi0mt, ddsi1t, ddsi1mt, ddsi2t, ddsi2mt}};
+  const double DW[3][6] = {{si0d, si0md, si1d, si1md, si2d, si2md},
+                           {dsi0d, dsi0md, dsi1d, dsi1md, dsi2d, dsi2md},
+                           {ddsi0d, ddsi0md, ddsi1d, ddsi1md, ddsi2d, ddsi2md}};
+  // the sums' (temperature, density) weight orders: free, df_d, df_t, df_dd, df_tt, df_dt
+  constexpr int kT[6] = {0, 0, 1, 0, 2, 1};
+  constexpr int kD[6] = {0, 1, 0, 2, 0, 1};
+  double r[6];
+  {
+    const double2* src = reinterpret_cast<const double2*>(H.cell + ((long)jat * (IMAX - 1) + iat) * 36);
+#pragma unroll
+    for (int k2 = 0; k2 < 18; k2++) {
+      const double2 v = src[k2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int k = 2 * k2 + h;
+        const double f = h ? v.y : v.x;
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+          const double t = f * DW[kD[m]][kH5D[k]] * TW[kT[m]][kH5T[k]];
+          r[m] = k == 0 ? t : r[m] + t;
+        }
+      }
+    }
+  }
+  const double free = r[0], df_d = r[1], df_t = r[2], df_dd = r[3], df_tt = r[4], df_dt = r[5];
 
   // electron-positron thermodynamics (:380-396); the cubic weights of :353-378 feed nothing here
   x = din * din;

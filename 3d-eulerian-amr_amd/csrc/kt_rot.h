@@ -41,19 +41,20 @@ struct RotArgs {
   int outflow;
   int z_reflect;
   // the deferred-dt step (flat outflow batches): the first substep's D per interior cell,
-  // [block][field][n^3] (written by the deferred substep, read by rot_first_stage_kernel)
+  // [block][field][n^3] (written by the deferred substep, read by rot_stage_kernel)
   double* dbuf;
 };
 
 hipError_t launch_rot_substep(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
 hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
 // the deferred-dt step of a flat outflow batch: the first substep with D and the max face speed
-// (a.dbuf, a.gmax), then the elementwise first stage from a.Uin = U^n into a.Uout with a.dt_dev
+// (a.dbuf, a.gmax; a.gmax null: D only), then the elementwise stage from a.Uin (with a.beta,
+// a.U0 and a.dt_dev) into a.Uout
 hipError_t launch_rot_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
-hipError_t launch_rot_first_stage(int g, int eos, bool mixed, const RotArgs& a, hipStream_t s);
+hipError_t launch_rot_stage(int g, int eos, bool mixed, const RotArgs& a, hipStream_t s);
 // the same first substep over one octree level (a.nodes, with the reflux tables a.slot, a.ovr,
 // a.exp_slot, a.ledger, a.ghost as for the substep)
-hipError_t launch_rot_tree_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, hipStream_t s);
+hipError_t launch_rot_tree_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s);
 hipError_t launch_rot_ghosts(int g, double* U, const double* x0, const double* h, int nblocks, int z_reflect,
                              const rot::RotParams& c, hipStream_t s);
 // the octree's ghost fill of one level (ghost/rel: the tables of amr_kernels.h); covered (per node,

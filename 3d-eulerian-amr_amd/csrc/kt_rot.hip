@@ -117,7 +117,8 @@ __device__ __forceinline__ void cell_pos(const double* x0, double h, int i, int 
 // (flat batches, and the levels of the octree): the substep's fluxes and sources, the block's max
 // face speed (the numbers the scan forms from the same faces, before any reflux override) into
 // a.gmax, and each cell's D (sources included) into a.dbuf instead of the update;
-// rot_first_stage_kernel forms the first stage once dt is known.  The octree's reflux overrides,
+// rot_stage_kernel forms the first stage once dt is known; with a.gmax null, only D (the split
+// form of a later substep).  The octree's reflux overrides,
 // exports and ledger faces do not depend on dt, so MODE 2 applies and writes them as MODE 0 does.
 template <int G, int RECON, int EOS, int MODE, bool MIXED, bool TREE>
 __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
@@ -270,55 +271,84 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
         }
         if constexpr (EIGM) emax = rmax(emax, (double)amax);
       } else {
-        double ql[NF], qr[NF];
-        if constexpr (RECON == AMRH_RECON_PPM) {
-          int e6[6];
+        // the face's two reconstructed states, converted back at the face position X (cell
+        // centres across, the face itself along d)
+        auto face_states = [&](double* ql, double* qr, double* X) {
+          if constexpr (RECON == AMRH_RECON_PPM) {
+            int e6[6];
 #pragma unroll
-          for (int m = 0; m < 6; m++) e6[m] = staged(d, f - 3 + m, ta, tb);
+            for (int m = 0; m < 6; m++) e6[m] = staged(d, f - 3 + m, ta, tb);
 #pragma unroll
-          for (int l = 0; l < NF; l++) {
-            double v[6];
+            for (int l = 0; l < NF; l++) {
+              double v[6];
 #pragma unroll
-            for (int m = 0; m < 6; m++) v[m] = (double)P[l][e6[m]];
-            // the potential is low-order and smooth (binary/state_ztwd.cpp:50-56)
-            x64::ppm_face(v, ql[l], qr[l], l == POT, l == POT);
+              for (int m = 0; m < 6; m++) v[m] = (double)P[l][e6[m]];
+              // the potential is low-order and smooth (binary/state_ztwd.cpp:50-56)
+              x64::ppm_face(v, ql[l], qr[l], l == POT, l == POT);
+            }
+          } else {
+            int e4[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) e4[m] = staged(d, f - 2 + m, ta, tb);
+#pragma unroll
+            for (int l = 0; l < NF; l++) {
+              double v[4];
+#pragma unroll
+              for (int m = 0; m < 4; m++) v[m] = (double)P[l][e4[m]];
+              x64::minmod_face(v, ql[l], qr[l], c.theta);
+            }
           }
-        } else {
-          int e4[4];
-#pragma unroll
-          for (int m = 0; m < 4; m++) e4[m] = staged(d, f - 2 + m, ta, tb);
-#pragma unroll
-          for (int l = 0; l < NF; l++) {
-            double v[4];
-#pragma unroll
-            for (int m = 0; m < 4; m++) v[m] = (double)P[l][e4[m]];
-            x64::minmod_face(v, ql[l], qr[l], c.theta);
-          }
-        }
-        // face position: cell centres across, the face itself along d
-        double X[3];
-        {
           const int ii = d == 0 ? f : ta;
           const int jj = d == 0 ? ta : (d == 1 ? f : tb);
           const int kk = d == 2 ? f : tb;
           cell_pos(x0, h, ii, jj, kk, X);
           X[d] = x0[d] + (double)f * h;
-        }
-        to_con(c, ql, X);
-        to_con(c, qr, X);
+          to_con(c, ql, X);
+          to_con(c, qr, X);
+        };
         if constexpr (SUB) {
-          double uf[NF];
+          double pl, csl, pr, csr;
+          if constexpr (EOS == EOS_HELMHOLTZ) {
+            // The Helmholtz evaluation gathers 36 table coefficients: with both face states live
+            // through it the kernel spills.  So the states give the face velocity and the EOS
+            // inputs, the two EOS calls run with nothing else live, and the states are formed
+            // again from the staged primitives for the fluxes (the same values).
+            double rl, Tl, Al, rr, Tr, Ar;
+            {
+              double ql[NF], qr[NF], X[3];
+              face_states(ql, qr, X);
+              double uf[NF];
 #pragma unroll
-          for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5;
-          Vf[fb][q] = (T)(d == 0 ? vx(c, uf, X) : (d == 1 ? vy(c, uf, X) : vz(uf)));
+              for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5;
+              Vf[fb][q] = (T)(d == 0 ? vx(c, uf, X) : (d == 1 ? vy(c, uf, X) : vz(uf)));
+              rl = ql[D0], Tl = ql[TT], Al = Abar(c, ql);
+              rr = qr[D0], Tr = qr[TT], Ar = Abar(c, qr);
+            }
+            eos_p_cs_in<EOS>(H, c, rl, Tl, Al, pl, csl);
+            eos_p_cs_in<EOS>(H, c, rr, Tr, Ar, pr, csr);
+            // the staged primitives are read again rather than kept through the EOS calls
+            asm volatile("" ::: "memory");
+          }
+          double ql[NF], qr[NF], X[3];
+          face_states(ql, qr, X);
+          if constexpr (EOS != EOS_HELMHOLTZ) {
+            double uf[NF];
+#pragma unroll
+            for (int l = 0; l < NF; l++) uf[l] = (ql[l] + qr[l]) * 0.5;
+            Vf[fb][q] = (T)(d == 0 ? vx(c, uf, X) : (d == 1 ? vy(c, uf, X) : vz(uf)));
+            eos_p_cs<EOS>(H, c, ql, pl, csl);
+            eos_p_cs<EOS>(H, c, qr, pr, csr);
+          }
           double fl[NF], fr[NF];
-          const double el = flux<EOS, d>(H, c, ql, X, fl);
-          const double er = flux<EOS, d>(H, c, qr, X, fr);
+          const double el = flux_pc<d>(c, ql, X, pl, csl, fl);
+          const double er = flux_pc<d>(c, qr, X, pr, csr, fr);
           const double amax = rmax(el, er);
 #pragma unroll
           for (int l = 0; l < NF; l++) Fb[fb][l][q] = (T)(((fl[l] + fr[l]) - (qr[l] - ql[l]) * amax) * 0.5);
           if constexpr (EIGM) emax = rmax(emax, amax);
         } else {
+          double ql[NF], qr[NF], X[3];
+          face_states(ql, qr, X);
           double F[NF];
           const double el = flux<EOS, d>(H, c, ql, X, F);
           const double er = flux<EOS, d>(H, c, qr, X, F);
@@ -656,7 +686,7 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
 #pragma unroll
       for (int l = 0; l < NF; l++) D[l] += sv[l];
       if constexpr (MODE == 2) {
-        // deferred: the cell's D (its sources included) for rot_first_stage_kernel
+        // deferred: the cell's D (its sources included) for rot_stage_kernel
         const RecBuf<N3, NF> dbo(a.dbuf + (long)b * NF * N3);
 #pragma unroll
         for (int l = 0; l < NF; l++) dbo.st_nt(l, (unsigned)e, D[l]);
@@ -694,13 +724,16 @@ __global__ __launch_bounds__(N3) void kt_rot_kernel(RotArgs a) {
   }
 }
 
-// The first stage of a deferred RK step, one thread per interior cell: the epilogue of the
-// substep with beta = 1 (kt_rot_kernel MODE 0) from the D that the deferred first substep stored:
-// U1 = (U^n + D dt) 1 + U^n 0, the floor with the Newton solve for T at the cell
-// (binary/state_ztwd.cpp:482-554), pot = rho (phi - R^2 Omega0^2 / 2) (binary/binary.cpp:825-849).
-// Same operations in the same order, so U1 has the bits a direct first substep would have stored.
+// An RK stage from the D a MODE 2 launch stored, one thread per interior cell: the epilogue of
+// kt_rot_kernel MODE 0, U = (U_in + D dt) beta + U0 (1 - beta) (U_in for U0 when beta = 1), the
+// floor with the Newton solve for T at the cell (binary/state_ztwd.cpp:482-554), and
+// pot = rho (phi - R^2 Omega0^2 / 2) (binary/binary.cpp:825-849).  Same operations in the same
+// order, so the stage has the bits the substep's own epilogue would have stored.  Used for the
+// first stage of a deferred step, and for the later substeps of the Helmholtz EOS, whose Newton
+// solve in the substep's epilogue (every register of the wave taken by the fluxes' state) costs
+// more than this separate pass and the D round trip.
 template <int G, int EOS, bool MIXED>
-__global__ __launch_bounds__(256) void rot_first_stage_kernel(RotArgs a) {
+__global__ __launch_bounds__(256) void rot_stage_kernel(RotArgs a) {
   constexpr int R = N3 + 6 * G * N2;
   constexpr int PP = N + 2;
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -708,7 +741,7 @@ __global__ __launch_bounds__(256) void rot_first_stage_kernel(RotArgs a) {
   const int b = (int)(gid / N3), e = (int)(gid % N3);
   const int ci = e % N, cj = (e / N) % N, ck = e / N2;
   const double dt = *a.dt_dev;
-  const double beta = 1.0;
+  const double beta = a.beta;
   const RecBuf<R, NF> ub(a.Uin + (long)b * NF * R), uo(a.Uout + (long)b * NF * R);
   const RecBuf<N3, NF> db(a.dbuf + (long)b * NF * N3);
   double s[NF], D[NF], u[NF];
@@ -717,8 +750,14 @@ __global__ __launch_bounds__(256) void rot_first_stage_kernel(RotArgs a) {
     s[l] = ub.ld(l, (unsigned)e);
     D[l] = db.ld_nt(l, (unsigned)e);
   }
+  if (beta != 1.0) {
+    const RecBuf<R, NF> u0(a.U0 + (long)b * NF * R);
 #pragma unroll
-  for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + s[l] * (1.0 - beta);
+    for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + u0.ld(l, (unsigned)e) * (1.0 - beta);
+  } else {
+#pragma unroll
+    for (int l = 0; l < NF; l++) u[l] = (s[l] + D[l] * dt) * beta + s[l] * (1.0 - beta);
+  }
   const double x0[3] = {a.x0[3L * b], a.x0[3L * b + 1], a.x0[3L * b + 2]};
   double X[3];
   cell_pos(x0, a.h[b], ci, cj, ck, X);
@@ -1008,32 +1047,32 @@ hipError_t launch_rot_maxeig(int g, int recon, int eos, bool mixed, const RotArg
   return mixed ? dispatch<1, true>(g, recon, eos, a, grid, s) : dispatch<1, false>(g, recon, eos, a, grid, s);
 }
 hipError_t launch_rot_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s) {
-  if (a.slot || a.nodes || !a.outflow || !a.dbuf || !a.gmax) return hipErrorInvalidValue;
+  if (a.slot || a.nodes || !a.outflow || !a.dbuf) return hipErrorInvalidValue;
   return mixed ? dispatch<2, true>(g, recon, eos, a, grid, s) : dispatch<2, false>(g, recon, eos, a, grid, s);
 }
-hipError_t launch_rot_tree_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, hipStream_t s) {
+hipError_t launch_rot_tree_deferred(int g, int recon, int eos, bool mixed, const RotArgs& a, int grid, hipStream_t s) {
   if (!a.slot || !a.nodes || a.outflow || !a.dbuf || !a.gmax) return hipErrorInvalidValue;
-  return mixed ? dispatch<2, true>(g, recon, eos, a, 0, s) : dispatch<2, false>(g, recon, eos, a, 0, s);
+  return mixed ? dispatch<2, true>(g, recon, eos, a, grid, s) : dispatch<2, false>(g, recon, eos, a, grid, s);
 }
-hipError_t launch_rot_first_stage(int g, int eos, bool mixed, const RotArgs& a, hipStream_t s) {
-  if (!a.dbuf || !a.dt_dev) return hipErrorInvalidValue;
+hipError_t launch_rot_stage(int g, int eos, bool mixed, const RotArgs& a, hipStream_t s) {
+  if (!a.dbuf || !a.dt_dev || (a.beta != 1.0 && !a.U0)) return hipErrorInvalidValue;
   if (a.nblocks <= 0) return hipSuccess;
   const unsigned grid = (unsigned)(((long)a.nblocks * N3 + 255) / 256);
   const bool helm = eos == EOS_HELMHOLTZ;
   if (g == 3) {
     if (mixed)
-      helm ? rot_first_stage_kernel<3, EOS_HELMHOLTZ, true><<<grid, 256, 0, s>>>(a)
-           : rot_first_stage_kernel<3, EOS_IDEAL, true><<<grid, 256, 0, s>>>(a);
+      helm ? rot_stage_kernel<3, EOS_HELMHOLTZ, true><<<grid, 256, 0, s>>>(a)
+           : rot_stage_kernel<3, EOS_IDEAL, true><<<grid, 256, 0, s>>>(a);
     else
-      helm ? rot_first_stage_kernel<3, EOS_HELMHOLTZ, false><<<grid, 256, 0, s>>>(a)
-           : rot_first_stage_kernel<3, EOS_IDEAL, false><<<grid, 256, 0, s>>>(a);
+      helm ? rot_stage_kernel<3, EOS_HELMHOLTZ, false><<<grid, 256, 0, s>>>(a)
+           : rot_stage_kernel<3, EOS_IDEAL, false><<<grid, 256, 0, s>>>(a);
   } else if (g == 2) {
     if (mixed)
-      helm ? rot_first_stage_kernel<2, EOS_HELMHOLTZ, true><<<grid, 256, 0, s>>>(a)
-           : rot_first_stage_kernel<2, EOS_IDEAL, true><<<grid, 256, 0, s>>>(a);
+      helm ? rot_stage_kernel<2, EOS_HELMHOLTZ, true><<<grid, 256, 0, s>>>(a)
+           : rot_stage_kernel<2, EOS_IDEAL, true><<<grid, 256, 0, s>>>(a);
     else
-      helm ? rot_first_stage_kernel<2, EOS_HELMHOLTZ, false><<<grid, 256, 0, s>>>(a)
-           : rot_first_stage_kernel<2, EOS_IDEAL, false><<<grid, 256, 0, s>>>(a);
+      helm ? rot_stage_kernel<2, EOS_HELMHOLTZ, false><<<grid, 256, 0, s>>>(a)
+           : rot_stage_kernel<2, EOS_IDEAL, false><<<grid, 256, 0, s>>>(a);
   } else {
     return hipErrorInvalidValue;
   }

@@ -119,17 +119,23 @@ __device__ __forceinline__ void eos_compute_T(const helm::HelmDev& H, const RotP
 }
 
 // State::compute_eos (binary/state_ztwd.cpp:168-176): pressure and sound speed of a state
+// the same from its inputs: density, temperature and Abar of the state
 template <int EOS>
-__device__ __forceinline__ void eos_p_cs(const helm::HelmDev& H, const RotParams& c, const double* u, double& p,
-                                         double& cs) {
+__device__ __forceinline__ void eos_p_cs_in(const helm::HelmDev& H, const RotParams& c, double rho, double T,
+                                            double abar, double& p, double& cs) {
   helm::Eos q{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  q.rho = u[D0];
-  q.T = u[TT];
-  q.abar = Abar(c, u);
-  q.zbar = Abar(c, u) / 2.0;
+  q.rho = rho;
+  q.T = T;
+  q.abar = abar;
+  q.zbar = abar / 2.0;
   eos_eval<EOS>(H, c, q);
   p = q.p;
   cs = q.cs;
+}
+template <int EOS>
+__device__ __forceinline__ void eos_p_cs(const helm::HelmDev& H, const RotParams& c, const double* u, double& p,
+                                         double& cs) {
+  eos_p_cs_in<EOS>(H, c, u[D0], u[TT], Abar(c, u), p, cs);
 }
 
 // binary/state_ztwd.cpp:581-599
@@ -171,11 +177,11 @@ __device__ __forceinline__ void to_con(const RotParams& c, double* u, const doub
 }
 
 // binary/state_ztwd.cpp:360-396 with the eigenvalue of :176-209; returns |v_d| + c_s
-template <int EOS, int D>
-__device__ __forceinline__ double flux(const helm::HelmDev& H, const RotParams& c, const double* u, const double* X,
-                                       double* F) {
-  double p, cs;
-  eos_p_cs<EOS>(H, c, u, p, cs);
+// the flux of state u given its pressure p and sound speed cs (State::x/y/z_flux after
+// compute_eos); returns the wavespeed |v_d| + cs
+template <int D>
+__device__ __forceinline__ double flux_pc(const RotParams& c, const double* u, const double* X, double p, double cs,
+                                          double* F) {
   const double v = D == 0 ? vx(c, u, X) : (D == 1 ? vy(c, u, X) : vz(u));
 #pragma unroll
   for (int i = 0; i < NF; i++) F[i] = v * u[i];
@@ -192,6 +198,13 @@ __device__ __forceinline__ double flux(const helm::HelmDev& H, const RotParams& 
     F[ET] += v * p;
   }
   return fabs(v) + cs;
+}
+template <int EOS, int D>
+__device__ __forceinline__ double flux(const helm::HelmDev& H, const RotParams& c, const double* u, const double* X,
+                                       double* F) {
+  double p, cs;
+  eos_p_cs<EOS>(H, c, u, p, cs);
+  return flux_pc<D>(c, u, X, p, cs, F);
 }
 
 // binary/state_ztwd.cpp:482-554 (SEP: the mixed form's Newton solve, eos_compute_T)

@@ -153,7 +153,10 @@ def measure(B, torch, cells, steps, warmup, tune=True):
         if B.cfg.boundary == am.AMRH_BOUNDARY_OUTFLOW:
             # the deferred step's kernels: the first substep, then the sweep's second substep or
             # the first-stage pass (PPM block kernel, rotating state)
-            for k, name in ((0, "deferred_first"), (1, "deferred_second" if sweep else "first_stage_pass")):
+            kinds = [(0, "deferred_first"), (1, "deferred_second" if sweep else "first_stage_pass")]
+            if B.cfg.physics == am.AMRH_PHYS_ROTATING:
+                kinds.append((2, "later_substeps_split"))  # variant 1: D + stage pass
+            for k, name in kinds:
                 res["launch"][name] = dict(zip(("grid", "variant", "ms_default_launch", "ms_tuned_launch"),
                                                B.deferred_launch_info(k)))
     res.update(kernel_times(B, torch, cells, steps, warmup))

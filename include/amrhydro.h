@@ -190,9 +190,13 @@ int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_gr
  * depends on dt), then dt, the store, the first stage of every node, the ledger and the
  * restriction; its first ghost fill is the substep's (same-level faces read from the neighbours).
  * No separate CFL scan, no store: the same bits as amrh_dt_device + amrh_store +
- * amrh_substep_dev.  amrh_autotune also picks the launch shapes of those kernels;
- * amrh_deferred_launch_info returns them (kernel 0: the deferred first substep, 1: the second
- * substep or the first-stage pass), as amrh_launch_info does. */
+ * amrh_substep_dev.  With the Helmholtz EOS the rotating state's later substeps run as a substep
+ * that stores D and the same elementwise pass (its Newton solve for T outside the flux kernel),
+ * again the same bits.  amrh_autotune also picks the launch shapes of those kernels, and measures
+ * whether the later substeps split; amrh_deferred_launch_info returns them (kernel 0: the deferred
+ * first substep, 1: the second substep or the first-stage pass, 2: the rotating state's later
+ * substeps -- variant 1 if split, ms_default the substep's time, ms_tuned the split form's), as
+ * amrh_launch_info does. */
 int amrh_deferred_launch_info(const amrh_ctx* ctx, int kernel, int* grid, int* variant, double* ms_default,
                               double* ms_tuned);
 /* Whether such steps run deferred (the default) or as scan + store + substeps: amrh_autotune keeps

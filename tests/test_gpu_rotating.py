@@ -203,15 +203,22 @@ def test_rotating_rk3_graph_matches_direct_and_oracle(eos):
 @pytest.mark.parametrize("eos,recon", [(0, 1), (1, 1), (0, 0)])
 def test_rotating_deferred_step_equals_plain(eos, recon, precision):
     """The rotating state's deferred-dt step (the first substep's face speeds as the CFL maximum,
-    its D stored, the first stage formed by an elementwise pass) gives the bits of the plain
-    schedule (CFL scan, store, substeps), direct and as graphs, dt included"""
+    its D stored, the first stage formed by an elementwise pass; with the Helmholtz EOS the later
+    substeps split the same way) gives the bits of the plain schedule (CFL scan, store, substeps),
+    direct, as graphs and with the autotuner's launch shapes and split choice, dt included"""
     outs = []
     for sched, use_graph in ((amrhydro.AMRH_SCHEDULE_PLAIN, False), (amrhydro.AMRH_SCHEDULE_DEFERRED, False),
-                             (amrhydro.AMRH_SCHEDULE_DEFERRED, True)):
+                             (amrhydro.AMRH_SCHEDULE_DEFERRED, True), (None, True)):
         B, cfg, G = setup(eos, recon, seed=17, boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW, precision=precision)
         G.upload(B["U"])
-        G.set_schedule(sched)
-        assert G.schedule() == sched
+        if sched is None:
+            G.autotune()
+            g2, split, ms_sub, ms_split = G.deferred_launch_info(2)
+            say("rotating eos=%d: later substeps split %d (%.3f vs %.3f ms)" % (eos, split, ms_split, ms_sub))
+            assert split in (0, 1) and ms_split > 0.0
+        else:
+            G.set_schedule(sched)
+            assert G.schedule() == sched
         dt = torch.full((1,), 1e-5, dtype=torch.float64, device="cuda")
         G.run_steps(3, dt.data_ptr(), rk=3, use_graph=use_graph)
         G.sync()
