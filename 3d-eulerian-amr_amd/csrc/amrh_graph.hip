@@ -10,6 +10,7 @@
 // dt is read from device memory by every kernel, so replays need no re-capture.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "amrh_ctx.h"
@@ -26,23 +27,34 @@ int n_cus() {
   return cus > 0 ? cus : 256;
 }
 
-// time `reps` launches of fn on the context's stream, in ms per launch
+// the median time of `reps` (at most 8) back-to-back launches of fn on the context's stream, in
+// ms, after one warm-up launch (events between the launches: one slow launch does not decide)
 template <typename F>
 int time_launches(amrh_ctx* c, int reps, F fn, float* ms) {
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  if (int rc = fn()) return rc;  // warm-up
-  HIPCHK(hipEventRecord(e0, c->stream));
-  for (int r = 0; r < reps; r++)
-    if (int rc = fn()) return rc;
-  HIPCHK(hipEventRecord(e1, c->stream));
-  HIPCHK(hipEventSynchronize(e1));
-  float t = 0.f;
-  HIPCHK(hipEventElapsedTime(&t, e0, e1));
-  *ms = t / reps;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  constexpr int kMax = 8;
+  reps = reps < 1 ? 1 : (reps > kMax ? kMax : reps);
+  hipEvent_t ev[kMax + 1];
+  int made = 0;
+  int rc = AMRH_OK;
+  for (; made <= reps; made++)
+    if (hipEventCreate(&ev[made]) != hipSuccess) {
+      rc = fail(AMRH_EHIP, "hipEventCreate failed");
+      break;
+    }
+  if (rc == AMRH_OK) rc = fn();  // warm-up
+  for (int r = 0; rc == AMRH_OK && r < reps; r++) {
+    if (hipEventRecord(ev[r], c->stream) != hipSuccess) rc = fail(AMRH_EHIP, "hipEventRecord failed");
+    if (rc == AMRH_OK) rc = fn();
+  }
+  if (rc == AMRH_OK && (hipEventRecord(ev[reps], c->stream) != hipSuccess || hipEventSynchronize(ev[reps]) != hipSuccess))
+    rc = fail(AMRH_EHIP, "timing event failed");
+  float t[kMax];
+  for (int r = 0; rc == AMRH_OK && r < reps; r++)
+    if (hipEventElapsedTime(&t[r], ev[r], ev[r + 1]) != hipSuccess) rc = fail(AMRH_EHIP, "hipEventElapsedTime failed");
+  for (int i = 0; i < made; i++) (void)hipEventDestroy(ev[i]);
+  if (rc != AMRH_OK) return rc;
+  std::sort(t, t + reps);
+  *ms = t[reps / 2];
   return AMRH_OK;
 }
 
@@ -291,8 +303,10 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   a.Uout = c->buf[other];
   a.U0 = c->buf[c->cur];
   a.h = c->h_dev;
-  a.dt = 0.0;  // timing only: the update is U + 0*D (beta = 1)
-  a.beta = 1.0;
+  // timing only: dt = 0, output to the spare buffer; a beta != 1 substep (it also reads U0), the
+  // form of all but the first substep of a step
+  a.dt = 0.0;
+  a.beta = 0.25;
   a.nblocks = c->nb;
   a.P = params(c->cfg);
   a.err = c->err_dev;
@@ -312,7 +326,7 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   for (int v = 0; v < nvar; v++)
     for (int g : cand) {
       float ms = 0.f;
-      if (int rc = time_launches(c, 3, [&]() -> int {
+      if (int rc = time_launches(c, 5, [&]() -> int {
             HIPCHK(ctx_substep_launch(c, a, g, v));
             return AMRH_OK;
           }, &ms))
@@ -326,12 +340,23 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
     }
   for (int g : cand) {
     float ms = 0.f;
-    if (int rc = time_launches(c, 3, [&]() -> int {
+    if (int rc = time_launches(c, 5, [&]() -> int {
           HIPCHK(ctx_maxeig_launch(c, m, g));
           return AMRH_OK;
         }, &ms))
       return rc;
     if (ms < best_m) { best_m = ms; gm = g; }
+  }
+  // the beta = 1 substep at the pick (the first substep of a plain step)
+  float ms_b1 = 0.f;
+  {
+    SubstepArgs a1 = a;
+    a1.beta = 1.0;
+    if (int rc = time_launches(c, 5, [&]() -> int {
+          HIPCHK(ctx_substep_launch(c, a1, gs, vs));
+          return AMRH_OK;
+        }, &ms_b1))
+      return rc;
   }
   // the two kernels of the deferred-dt step (self-ghosting flat batches): the first substep, then
   // the one that forms its input from U^n and D; timed with dt = 0 (eig_dev[0] as the device dt),
@@ -346,7 +371,7 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
     float best = 1e30f;
     for (int g : cand) {
       float ms = 0.f;
-      if (int rc = time_launches(c, 3, [&]() -> int {
+      if (int rc = time_launches(c, 5, [&]() -> int {
             HIPCHK(pass_first_launch(c, c->buf[c->cur], g));
             return AMRH_OK;
           }, &ms))
@@ -359,7 +384,7 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
     }
     msd[0][1] = best;
     HIPCHK(launch_reset_u64(c->gmax_dev, c->stream));
-    if (int rc = time_launches(c, 3, [&]() -> int {
+    if (int rc = time_launches(c, 5, [&]() -> int {
           HIPCHK(pass_stage_launch(c, c->buf[c->cur], c->buf[other], c->eig_dev));
           return AMRH_OK;
         }, &msd[1][1]))
@@ -367,16 +392,16 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
     msd[1][0] = msd[1][1];
     // an RK step is the scan and a beta = 1 substep, or the deferred substep and the first-stage
     // pass, then the same beta != 1 substeps (the dt kernel is common)
-    c->schedule = msd[0][1] + msd[1][1] < best_m + best_s ? AMRH_SCHEDULE_DEFERRED : AMRH_SCHEDULE_PLAIN;
+    c->schedule = msd[0][1] + msd[1][1] < best_m + ms_b1 ? AMRH_SCHEDULE_DEFERRED : AMRH_SCHEDULE_PLAIN;
     if (rot_deferred(c)) {
       // the later substeps: the substep (kernel 2's default time) against D + the stage pass
       float ms_d = 0.f, ms_st = 0.f;
-      if (int rc = time_launches(c, 3, [&]() -> int {
+      if (int rc = time_launches(c, 5, [&]() -> int {
             HIPCHK(pass_first_launch(c, c->buf[c->cur], gd[0], false));
             return AMRH_OK;
           }, &ms_d))
         return rc;
-      if (int rc = time_launches(c, 3, [&]() -> int {
+      if (int rc = time_launches(c, 5, [&]() -> int {
             HIPCHK(pass_stage_launch(c, c->buf[c->cur], c->buf[other], c->eig_dev, 0.25, c->buf[c->cur]));
             return AMRH_OK;
           }, &ms_st))
@@ -407,7 +432,7 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
       for (int v = 0; v < kSweepVariants; v++)
         for (int g : cand) {
           float ms = 0.f;
-          if (int rc = time_launches(c, 3, [&]() -> int {
+          if (int rc = time_launches(c, 5, [&]() -> int {
                 HIPCHK(launch_substep(c->cfg.n, d, mixed, g, c->stream, v, mode));
                 return AMRH_OK;
               }, &ms))
@@ -423,17 +448,8 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
     }
     HIPCHK(launch_reset_u64(c->gmax_dev, c->stream));
     // the schedule: an RK3 step is the scan, a beta = 1 and two beta != 1 substeps, or the two
-    // deferred kernels and one beta != 1 substep (the dt kernel is common); a beta != 1 substep of
-    // the pick is timed here (U0 = the current stage, output to the spare buffer)
-    float ms_ne1 = 0.f;
-    SubstepArgs an = a;
-    an.beta = 0.25;
-    if (int rc = time_launches(c, 3, [&]() -> int {
-          HIPCHK(ctx_substep_launch(c, an, gs, vs));
-          return AMRH_OK;
-        }, &ms_ne1))
-      return rc;
-    const float plain = best_m + best_s + 2.f * ms_ne1, deferred = msd[0][1] + msd[1][1] + ms_ne1;
+    // deferred kernels and one beta != 1 substep (the dt kernel is common)
+    const float plain = best_m + ms_b1 + 2.f * best_s, deferred = msd[0][1] + msd[1][1] + best_s;
     c->schedule = deferred < plain ? AMRH_SCHEDULE_DEFERRED : AMRH_SCHEDULE_PLAIN;
     HIPCHK(hipStreamSynchronize(c->stream));
   }

@@ -159,8 +159,9 @@ double amrh_ghost_write_bytes(const amrh_ctx* ctx);
 /* ---- launch shape and whole steps (no counterpart in the reference: its loops are OpenMP) -----
  * The sweep kernels walk the batch with a grid stride.  amrh_autotune times one workgroup per
  * block against persistent grids of k workgroups per CU on the context's own state and keeps the
- * fastest for the substep and for the CFL scan (writes the spare state buffer: call it before
- * amrh_store).  amrh_set_launch sets them by hand (0 = one workgroup per block). */
+ * fastest for the substep (timed as a beta != 1 substep, the form of all but a step's first) and
+ * for the CFL scan; each time is the median of five launches (writes the spare state buffer: call
+ * it before amrh_store).  amrh_set_launch sets them by hand (0 = one workgroup per block). */
 int amrh_autotune(amrh_ctx* ctx, int* grid_substep, int* grid_maxeig);
 int amrh_set_launch(amrh_ctx* ctx, int grid_substep, int grid_maxeig);
 /* The flat minmod substep of the ideal gas also comes in register-budget variants -- 0 the
