@@ -127,8 +127,15 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
   constexpr int NT = N * N * N;
   constexpr int R = N * N * N + 6 * G * N * N;
   constexpr int NFACE = (N + 1) * N * N;
+  // PPM reconstructs whole pencils (one field of one pencil per thread) into the face states of
+  // every face, Fq[0] (low side) and Fq[1] (high side); a face's flux then overwrites its low side.
+  // Minmod reconstructs face by face into its flux buffer Fq[0].
+  constexpr bool PENCIL = RECON == AMRH_RECON_PPM;
+  constexpr int NQ = PENCIL ? 2 : 1;
+  constexpr bool QBUF = PENCIL || SUB;
   __shared__ T P[NFE][R];
-  __shared__ T Fb[SUB ? NFE : 1][SUB ? NFACE : 1];
+  __shared__ T Fq[NQ][QBUF ? NFE : 1][QBUF ? NFACE : 1];
+  auto& Fb = Fq[0];
   __shared__ double red[NT / 64];
 
   const int tid = threadIdx.x;
@@ -179,12 +186,41 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
 
     auto sweep = [&](auto dc) {
       constexpr int d = decltype(dc)::value;
+      if constexpr (PENCIL) {
+        // every ideal-gas field is neither low-order nor smooth (euler/state.h:37-42)
+#pragma unroll 1
+        for (int t = tid; t < NFE * N * N; t += NT) {
+          const int l = t / (N * N), ta = t % N, tb = (t / N) % N;
+          T q[N + 6], ql[N + 1], qr[N + 1];
+#pragma unroll
+          for (int m = 0; m < N + 6; m++) q[m] = P[l][pencil_elem<N, G>(d, m - 3, ta, tb)];
+          if constexpr (MIXED)
+            f32::ppm_pencil<N>(q, ql, qr, false, false);
+          else
+            x64::ppm_pencil<N>(q, ql, qr, false, false);
+#pragma unroll
+          for (int f = 0; f <= N; f++) {
+            const int q2 = face_index<N, d>(f, ta, tb);
+            Fq[0][l][q2] = ql[f];
+            Fq[NQ - 1][l][q2] = qr[f];
+          }
+        }
+        __syncthreads();
+      }
 #pragma unroll 1
       for (int q = tid; q < NFACE; q += NT) {
-        int f, ta, tb;
-        face_decode<N, d>(q, f, ta, tb);
         T ql[NFE], qr[NFE];
-        reconstruct<N, G, RECON, MIXED>(P, d, f, ta, tb, ql, qr, PP.theta);
+        if constexpr (PENCIL) {
+#pragma unroll
+          for (int l = 0; l < NFE; l++) {
+            ql[l] = Fq[0][l][q];
+            qr[l] = Fq[NQ - 1][l][q];
+          }
+        } else {
+          int f, ta, tb;
+          face_decode<N, d>(q, f, ta, tb);
+          reconstruct<N, G, RECON, MIXED>(P, d, f, ta, tb, ql, qr, PP.theta);
+        }
         if constexpr (SUB) {
           T Fo[NFE];
           const double e = flux_q<MIXED, d>(PP, ql, qr, Fo);
@@ -259,6 +295,8 @@ kt_block_kernel(SubstepArgs a, MaxEigArgs m) {
             D[l] -= ((double)Fb[l][q1] - (double)Fb[l][q0]) * hinv;
         }
         __syncthreads();  // the next direction rewrites the face buffer
+      } else if constexpr (PENCIL) {
+        __syncthreads();  // the next direction rewrites the face states
       }
     };
     sweep(std::integral_constant<int, 0>{});
