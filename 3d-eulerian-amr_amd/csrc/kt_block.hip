@@ -6,7 +6,9 @@
 // for each direction in x, y, z order every face of the block is reconstructed and fluxed (a
 // thread per face, the (N+1)*N^2 faces spread over the workgroup) into an LDS face buffer, and
 // every cell subtracts its flux difference from its D held in registers, so D accumulates in the
-// reference's order (grid/grid.cpp:304-344).  The cell then forms U = (U + D*dt)*beta +
+// reference's order (grid/grid.cpp:304-344).  PPM reconstructs whole pencils first (a thread per
+// field of a pencil, ppm_pencil: each slope, interface value and monotonisation formed once for
+// the faces that share it) into the face states of every face, which the face threads then read.  The cell then forms U = (U + D*dt)*beta +
 // U0*(1-beta), floors it and stores it (grid/grid.cpp:406-420).  HBM traffic per block is the
 // algorithmic count: interior + face ghost slabs read once, interior written once, U0 read when
 // beta != 1.
