@@ -252,8 +252,10 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   // PD: how many planes ahead a column cell is loaded.  The mixed form loads cell k+3 at the top of
   // plane k and converts it after the update (the most latency hidden; measured faster); the FP64
   // form loads cell k+2 and converts it after the x/y faces, which carries one plane of FP64 state
-  // fewer (its kernels were register-bound) and shortens its L2 re-reads.
-  constexpr int PD = MIXED ? 3 : 2;
+  // fewer (its kernels were register-bound) and shortens its L2 re-reads.  The mixed MODE 3 kernel
+  // also looks two planes ahead: it reads each cell's U^n again at its plane as the RK base, and the
+  // shorter distance keeps more of those re-reads in the L2 (0.60 -> 0.58 ms on 4096 x 16^3).
+  constexpr int PD = MIXED && MODE != 3 ? 3 : 2;
   // SELF: the boundary cells of planes k .. k+PD-1 (ring slot = plane % PD; plane k+PD's is written
   // over plane k's once the halo of k is staged) by position along the perimeter: their FP32
   // primitives in the mixed form, their FP64 input state in the FP64 form (whose ghosts are formed
