@@ -3,16 +3,21 @@
 
 Workload: the batched flat stencil -- 4096 independent blocks of 16^3 interior cells plus a 2-cell
 ghost layer, 6 conserved fields, lognormal density with 5% planar jumps (workloads.flat_batch).
-A "step" is one full RK3 time step over the batch: CFL wavespeed reduction to a device dt, then
-three fused substeps (reconstruction + KT flux + divergence + RK combine + floor).  Every block is an
-isolated domain with outflow faces (grid/grid_phys_bound.cpp semantics); each substep writes the
-next stage's ghost slabs as it stores the boundary cells, so the ghost refresh costs no separate
-pass.  The launch shape (grid and register-budget variant) comes from the library's autotuner;
-whole steps are replayed as captured hipGraphs.  `value` is leaf cell-updates/s (3 substeps x
-cells / wall time of whole steps) of the mixed-precision form; the FP64 form is measured the same
-way and reported beside it.  The roofline's `achieved` is the substep's algorithmic bytes (interior
-+ face ghost slabs read, interior written, U0 read when beta != 1) over its kernel time, measured
-with HIP events around direct launches of the same tuned kernel on the library's stream.
+A "step" is one full RK3 time step over the batch at the CFL dt: the CFL wavespeed maximum, the
+device dt (HydroDriver's control), the store and three fused substeps (reconstruction + KT flux +
+divergence + RK combine + floor).  Every block is an isolated domain with outflow faces
+(grid/grid_phys_bound.cpp semantics) whose kernels form each ghost from the boundary cell it
+copies where they read it, so no ghost pass runs.  The library's autotuner picks the launch shape
+(grid and register-budget variant) and the schedule: the plain one (scan, dt, store, substeps) or
+the deferred-dt one, whose first substep reduces its own face speeds to the CFL maximum and stores
+the divergence, and whose second forms the first stage from U^n and that divergence once dt is
+known; both give the same bits.  Whole steps are replayed as captured hipGraphs.  `value` is leaf
+cell-updates/s (3 substeps x cells / wall time of whole steps) of the mixed-precision form; the FP64
+form is measured the same way and reported beside it.  The roofline's `achieved` is the substeps'
+algorithmic bytes (interior + face ghost slabs read, interior written, U0 read when beta != 1:
+164 B per cell-update on average over an RK3 step, in FP64 storage) over their
+kernel time, measured with HIP events the library records on its stream between the launches of
+a step; `traffic` is the HBM bytes per launch rocprofv3 measured on the same kernel build.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -174,8 +179,8 @@ def roofline(r, traffic_key=None):
 
 
 def run_form(amrhydro, torch, U_dev_ptr, nb, n, precision, steps, warmup):
-    # isolated outflow blocks: each substep writes the next stage's ghosts as it stores the
-    # boundary cells, so no separate ghost pass runs between substeps
+    # isolated outflow blocks: the kernels form every ghost from the boundary cell it copies, so
+    # no separate ghost pass runs between substeps
     cfg = amrhydro.euler_config(n=n, precision=precision, boundary=amrhydro.AMRH_BOUNDARY_OUTFLOW)
     B = amrhydro.Batch(cfg, nb, 1.0 / n)
     B.set_stream(torch.cuda.current_stream().cuda_stream)
