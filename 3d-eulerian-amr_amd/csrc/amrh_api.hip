@@ -128,7 +128,7 @@ void amrh_destroy(amrh_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
-  for (int q = 0; q < 2; q++)
+  for (int q = 0; q < 3; q++)
     if (c->buf[q]) (void)hipFree(c->buf[q]);
   if (c->h_dev) (void)hipFree(c->h_dev);
   if (c->eig_dev) (void)hipFree(c->eig_dev);
@@ -433,7 +433,8 @@ int amrh::substep_core(amrh_ctx* c, double dt, double beta, const double* dt_dev
   SubstepArgs a{};
   a.Uin = c->buf[c->cur];
   int out = c->cur;
-  if (c->u0buf == c->cur) out = 1 - c->cur;  // never overwrite the RK base
+  if (c->u0buf == c->cur) out = free_buf(c);  // never overwrite the RK base
+  if (out < 0) return fail(AMRH_ESTATE, "no free state buffer");
   a.Uout = c->buf[out];
   a.U0 = c->u0buf >= 0 ? c->buf[c->u0buf] : c->buf[c->cur];
   a.h = c->h_dev;

@@ -50,7 +50,9 @@ struct amrh_ctx {
   amrh_config cfg;
   int nb = 0;
   amrh::Layout L{8, 2, 6};
-  double* buf[2] = {nullptr, nullptr};
+  // the state buffers: two, and a third for the rotating state on an octree, whose substep reads
+  // the cells across same-level faces from the neighbours' inputs and so never updates in place
+  double* buf[3] = {nullptr, nullptr, nullptr};
   int cur = 0;     // buffer holding the current stage
   int u0buf = -1;  // buffer holding the RK base, -1 = none stored
   double* h_dev = nullptr;
@@ -86,13 +88,23 @@ struct amrh_ctx {
   // the rotating state's later substeps of a deferred step as a D-only substep and the stage pass
   // (-1: by the EOS, the Helmholtz EOS split; amrh_autotune measures it)
   int split_later = -1;
-  // captured RK steps, one graph per parity of the ping-pong buffers (amrh_graph.hip)
-  hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+  // captured RK steps, one graph per buffer a step starts from (amrh_graph.hip), and the buffer
+  // each leaves the new state in
+  hipGraphExec_t step_exec[3] = {nullptr, nullptr, nullptr};
+  int step_exec_end[3] = {0, 0, 0};
   int step_exec_rk = 0;
   const double* step_exec_dt = nullptr;
 };
 
 namespace amrh {
+
+// a state buffer that holds neither the current stage nor the RK base (-1: none; with two buffers
+// and the base stored apart from the current stage, a substep updates in place)
+inline int free_buf(const amrh_ctx* c) {
+  for (int q = 0; q < 3; q++)
+    if (c->buf[q] && q != c->cur && q != c->u0buf) return q;
+  return -1;
+}
 
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 PhysParams params(const amrh_config& c);

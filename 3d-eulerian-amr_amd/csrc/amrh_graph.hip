@@ -274,7 +274,7 @@ int step_once(amrh_ctx* c, int rk, double* dt_dev, Marks* mk = nullptr) {
 }  // namespace
 
 void amrh::drop_graphs(amrh_ctx* c) {
-  for (int p = 0; p < 2; p++)
+  for (int p = 0; p < 3; p++)
     if (c->step_exec[p]) {
       (void)hipGraphExecDestroy(c->step_exec[p]);
       c->step_exec[p] = nullptr;
@@ -289,8 +289,8 @@ int amrh_autotune(amrh_ctx* c, int* grid_substep, int* grid_maxeig) {
   if (!c) return fail(AMRH_EINVAL, "null ctx");
   if (!c->have_u || !c->have_h) return fail(AMRH_ESTATE, "state or cell widths not set");
   if (int rc = ctx_ready(c)) return rc;
-  const int other = 1 - c->cur;
-  if (c->u0buf == other) return fail(AMRH_ESTATE, "autotune writes the spare buffer: call it before amrh_store");
+  const int other = free_buf(c);
+  if (other < 0) return fail(AMRH_ESTATE, "autotune writes the spare buffer: call it before amrh_store");
   const int cus = n_cus();
   std::vector<int> cand = {0};
   for (int k : {1, 2, 3, 4, 6, 8, 12, 16})
@@ -562,12 +562,13 @@ int amrh_run_steps(amrh_ctx* c, int rk, int nsteps, double* dt_dev, int use_grap
       if (ei != hipSuccess) return fail(AMRH_EHIP, "graph instantiate: %s", hipGetErrorString(ei));
       c->step_exec_rk = rk;
       c->step_exec_dt = dt_dev;
+      c->step_exec_end[p] = c->cur;
       // capture did not run the step: restore the state it started from, then launch it
       c->cur = p;
     }
     HIPCHK(hipGraphLaunch(c->step_exec[p], c->stream));
     c->u0buf = p;
-    c->cur = 1 - p;
+    c->cur = c->step_exec_end[p];
   }
   return AMRH_OK;
 }

@@ -281,8 +281,10 @@ int rot_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev, 
     if (int rc = fill_ghosts(c, true)) return rc;
   RotArgs a = ctx_rot_args(c);
   a.Uin = c->buf[c->cur];
-  int out = c->cur;
-  if (c->u0buf == c->cur) out = 1 - c->cur;  // never overwrite the RK base
+  // never in place: a node reads the cells across its same-level faces from the neighbours' input,
+  // and a neighbour of its level may finish before it starts; nor over the RK base
+  const int out = free_buf(c);
+  if (out < 0) return fail(AMRH_ESTATE, "the rotating octree needs its third state buffer (amrh_tree_set)");
   a.Uout = c->buf[out];
   a.U0 = c->u0buf >= 0 ? c->buf[c->u0buf] : c->buf[c->cur];
   a.dt_dev = dt_dev;
@@ -315,7 +317,8 @@ int block_tree_substep(amrh_ctx* c, double dt, double beta, const double* dt_dev
   SubstepArgs a{};
   a.Uin = c->buf[c->cur];
   int out = c->cur;
-  if (c->u0buf == c->cur) out = 1 - c->cur;  // never overwrite the RK base
+  if (c->u0buf == c->cur) out = free_buf(c);  // never overwrite the RK base
+  if (out < 0) return fail(AMRH_ESTATE, "no free state buffer");
   a.Uout = c->buf[out];
   a.U0 = c->u0buf >= 0 ? c->buf[c->u0buf] : c->buf[c->cur];
   a.h = c->h_dev;
@@ -418,7 +421,8 @@ int amrh::tree_deferred_first(amrh_ctx* c) {
 int amrh::tree_first_stage(amrh_ctx* c, const double* dt_dev) {
   TreeState* t = c->tree;
   if (c->u0buf != c->cur) return fail(AMRH_ESTATE, "the first stage needs U^n stored as the RK base");
-  const int out = 1 - c->cur;
+  const int out = free_buf(c);
+  if (out < 0) return fail(AMRH_ESTATE, "no free state buffer");
   RotArgs a = ctx_rot_args(c);
   a.Uin = c->buf[c->cur];
   a.Uout = c->buf[out];
@@ -477,6 +481,12 @@ int amrh_tree_set(amrh_ctx* c, const int* parent, const int* child) {
     return fail(AMRH_EINVAL, "the octree coupling runs the ideal gas (minmod, or PPM on n = 8) or the rotating state");
   // captured steps hold the old tree's tables, which build() frees
   drop_graphs(c);
+  if (rot_ok && !c->buf[2]) {
+    // the rotating octree's substep writes a buffer other than its input (rot_tree_substep)
+    const size_t bytes = sizeof(double) * (size_t)c->nb * (size_t)c->L.block();
+    HIPCHK(hipMalloc(&c->buf[2], bytes));
+    HIPCHK(hipMemset(c->buf[2], 0, bytes));
+  }
   if (int rc = build(c, parent, child)) return rc;
   HIPCHK(hipMemcpy(c->h_dev, c->tree->h.data(), sizeof(double) * c->nb, hipMemcpyHostToDevice));
   c->have_h = true;

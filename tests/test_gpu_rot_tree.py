@@ -383,3 +383,63 @@ def test_rot_tree_output_mesh(tmp_path, z_reflect):
     vol = np.prod(pts[cells[:, 1:]][:, 6] - pts[cells[:, 1:]][:, 0], axis=1).sum()
     assert abs(vol - 8.0) < 1e-12  # [-1, 1]^2 x [-1, 1] or [0, 2] under z reflection
     B.close()
+
+
+def test_rot_tree_substeps_on_a_large_octree_match_oracle():
+    """The binary's 2,473-node octree (levels of up to ~1,700 nodes, so a level's workgroups run in
+    several rounds): the substeps read the ghosts across same-level faces from the neighbours'
+    interiors, which must still hold the substep's input when a neighbour has already finished
+    (the update writes another buffer).  One RK3 step, FP64, against the oracle, then whole steps of
+    the deferred schedule, direct and as graphs, against the plain one."""
+    import workloads
+    parent, child, lv, loc = workloads.binary_tree(5)
+    S = workloads.binary_tree_state(lv, loc)
+    say("large rotating octree: %d nodes, level sizes %s" % (len(parent), np.bincount(lv).tolist()))
+    cfg = orc.rot_cfg(omega=S["omega"])
+    T = orc.RotTree(cfg, parent, child)
+    T.set_state(S["U"])
+    T.set_phi(S["phi"])
+    T.fill_ghosts()
+    dt = T.max_dt()
+    d = amrhydro.rotating_config(omega=S["omega"])
+    B = amrhydro.Batch(d, len(parent), 1.0)
+    B.tree_set(parent, child)
+    B.upload(S["U"])
+    B.set_potential(S["phi"])
+    B.tree_fill_ghosts()
+    T.store()
+    B.store()
+    g = cfg.g
+    for beta in (1.0, 0.25, 2.0 / 3.0):
+        assert T.substep(dt, beta) == 0
+        assert T.inject() == 0
+        B.tree_substep(dt, beta)
+        B.tree_inject()
+        e = rel_l1_rot(B.download(), T.state(), g)
+        say("  beta=%.4f rel-L1 max %.1e (field %d)" % (beta, e.max(), int(np.argmax(e))))
+        assert np.all(e <= TOL), e
+    assert B.error_flags() == 0
+    B.close()
+    I = (slice(None), slice(None), slice(g, -g), slice(g, -g), slice(g, -g))
+    outs = []
+    for prec in (amrhydro.AMRH_FP64, amrhydro.AMRH_MIXED):
+        for sched, use_graph in ((amrhydro.AMRH_SCHEDULE_PLAIN, 0), (amrhydro.AMRH_SCHEDULE_DEFERRED, 0),
+                                 (amrhydro.AMRH_SCHEDULE_DEFERRED, 1)):
+            B = amrhydro.Batch(amrhydro.rotating_config(omega=S["omega"], precision=prec), len(parent), 1.0)
+            B.tree_set(parent, child)
+            B.upload(S["U"])
+            B.set_potential(S["phi"])
+            B.set_schedule(sched)
+            B.tree_fill_ghosts()
+            B.tree_inject()
+            dtd = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
+            B.run_steps(3, dtd.data_ptr(), rk=3, use_graph=bool(use_graph))
+            B.sync()
+            outs.append((prec, B.download()[I], dtd.item()))
+            assert B.error_flags() == 0
+            B.close()
+    for k in (1, 2, 4, 5):
+        base = outs[0 if k < 3 else 3]
+        assert outs[k][0] == base[0]
+        np.testing.assert_array_equal(outs[k][1], base[1])
+        assert outs[k][2] == base[2]
