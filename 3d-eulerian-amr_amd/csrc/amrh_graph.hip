@@ -5,8 +5,9 @@
 // candidate on the context's own data with HIP events and keeps the fastest for each kernel.
 //
 // Graphs: a step (CFL scan -> device dt, store, rk substeps) launches a fixed kernel sequence whose
-// buffer pointers depend only on which ping-pong buffer holds the state when it starts, so two
-// graphs -- one per parity -- replay any number of steps with no host work but one launch each.
+// buffer pointers depend only on which state buffer holds the state when it starts, so one graph
+// per starting buffer (two, or three for the rotating octree's three buffers) replays any number
+// of steps with no host work but one launch each; the capture records the buffer a step ends in.
 // dt is read from device memory by every kernel, so replays need no re-capture.
 #include <hip/hip_runtime.h>
 
@@ -550,7 +551,7 @@ int amrh_run_steps(amrh_ctx* c, int rk, int nsteps, double* dt_dev, int use_grap
   for (int s = 0; s < nsteps; s++) {
     const int p = c->cur;
     if (!c->step_exec[p]) {
-      // capture this parity's step; the host-side buffer bookkeeping runs during capture exactly
+      // capture the step from this buffer; the host-side buffer bookkeeping runs during capture exactly
       // as it does when the step is launched directly
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

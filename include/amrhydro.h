@@ -85,7 +85,8 @@ typedef struct {
 typedef struct amrh_ctx amrh_ctx;
 
 /* Create a context for `nblocks` blocks on the current HIP device.  Allocates two state buffers
- * (the current stage and the RK base U0), per-block metadata and reduction scratch.
+ * (the current stage and the RK base U0; amrh_tree_set adds a third for the rotating state),
+ * per-block metadata and reduction scratch.
  * Replaces the storage of Grid (grid/grid.h:26-32) for every node of the tree. */
 int amrh_create(const amrh_config* cfg, int nblocks, amrh_ctx** out);
 void amrh_destroy(amrh_ctx* ctx);
@@ -177,8 +178,8 @@ int amrh_set_variant(amrh_ctx* ctx, int variant);
  * min(MAXDTINC*dt, CFL dt).  For the octree (after amrh_tree_set) each step fills ghosts, scans
  * the CFL speed, stores U0 and runs rk tree substeps; for a flat batch it needs
  * boundary = AMRH_BOUNDARY_OUTFLOW.  With use_graph each step is one launch of a captured
- * hipGraph (two graphs, one per ping-pong parity, captured on first use; a new dt_dev, rk or
- * launch shape re-captures).  Asynchronous. */
+ * hipGraph (one graph per state buffer a step starts from, captured on first use; a new dt_dev,
+ * rk or launch shape re-captures).  Asynchronous. */
 int amrh_run_steps(amrh_ctx* ctx, int rk, int nsteps, double* dt_dev, int use_graph);
 /* In amrh_run_steps and amrh_step_timed, an RK2/RK3 step of a flat AMRH_BOUNDARY_OUTFLOW batch runs
  * with its dt deferred: the first substep forms the fluxes of U^n, reduces their KT face speeds --
@@ -229,8 +230,10 @@ int amrh_step_timed(amrh_ctx* ctx, int rk, double* dt_dev, float* ms, int* kinds
  * reflux and restriction tables are derived here.  Replaces OctNode::create_child's wiring
  * (oct_node/oct_node.cpp:172-240) and GridNode::create_child's boundary objects
  * (grid/grid_node.cpp:271-304) for a tree built on the host.  Runs the ideal gas with minmod
- * (g = 2) and the rotating-frame state (n = 8; minmod g = 2 or PPM g = 3), whose node origins
- * (Grid::xf, grid/grid.cpp:512-526) follow from the tree. */
+ * (g = 2) or PPM (n = 8, g = 3) and the rotating-frame state (n = 8; minmod g = 2 or PPM g = 3),
+ * whose node origins (Grid::xf, grid/grid.cpp:512-526) follow from the tree.  For the rotating
+ * state it allocates a third state buffer: its octree substep reads the cells across same-level
+ * faces from the neighbours' input, so it never updates in place. */
 int amrh_tree_set(amrh_ctx* ctx, const int* parent, const int* child);
 /* GridNode::set_origin (grid/grid_node.cpp:20-26): the origin subtracted from every node's cell and
  * face positions, Grid::xf = (offset + i) h - GRID_DIM - bw h0 - origin (grid/grid.cpp:512-524); as
