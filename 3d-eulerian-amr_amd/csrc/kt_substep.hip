@@ -255,7 +255,8 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
   // fewer (its kernels were register-bound) and shortens its L2 re-reads.  The mixed MODE 3 kernel
   // also looks two planes ahead: it reads each cell's U^n again at its plane as the RK base, and the
   // shorter distance keeps more of those re-reads in the L2 (0.60 -> 0.58 ms on 4096 x 16^3); its
-  // loads of D, which is read once, take the non-temporal policy so that they leave the L2 to U^n.
+  // loads of D, which it reads once, take the non-temporal policy so that they leave the L2 to U^n
+  // (the FP64 form reads D again at the cell's plane and keeps the default policy).
   constexpr int PD = MIXED && MODE != 3 ? 3 : 2;
   // SELF: the boundary cells of planes k .. k+PD-1 (ring slot = plane % PD; plane k+PD's is written
   // over plane k's once the halo of k is staged) by position along the perimeter: their FP32
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
     if constexpr (FROMD) {
       double d[NFE];
 #pragma unroll
-      for (int l = 0; l < NFE; l++) d[l] = db.ld_nt(l, e);  // self-ghosting: e is an interior cell
+      for (int l = 0; l < NFE; l++) d[l] = CARRY ? db.ld_nt(l, e) : db.ld(l, e);  // e: an interior cell
       first_stage<MIXED>(P, u, d, dt, hinv);
     }
     ghost_post(u, post);
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(N* N) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? 
       for (int l = 0; l < NFE; l++) craw[l] = ub.ld(l, e);
       if constexpr (FROMD)
 #pragma unroll
-        for (int l = 0; l < NFE; l++) cd[l] = db.ld_nt(l, e);
+        for (int l = 0; l < NFE; l++) cd[l] = CARRY ? db.ld_nt(l, e) : db.ld(l, e);
     }
     // cell k's RK base (U0; for MODE 3 the step's base U^n, the input it was formed from) and,
     // where the column state is not carried, cell k's input state again (read PD planes ago: L2)
