@@ -385,6 +385,38 @@ def test_rot_tree_output_mesh(tmp_path, z_reflect):
     B.close()
 
 
+@pytest.mark.parametrize("rk", [1, 2])
+def test_rot_tree_steps_cycle_through_three_state_buffers(rk):
+    """The rotating octree keeps three state buffers (its substep never writes its input): RK1 and
+    RK2 steps end in a different buffer than the one they started from, so successive steps start
+    from each buffer in turn and capture one graph per starting buffer.  Five steps as graphs equal
+    five direct steps, and a restore afterwards returns the last step's stored state bit for bit."""
+    g = 3
+    I = (slice(None), slice(None), slice(g, -g), slice(g, -g), slice(g, -g))
+    outs = []
+    for use_graph in (0, 1):
+        T, cfg = binary_tree_case("partial", seed=30)
+        B = device_batch(T, cfg)
+        B.tree_fill_ghosts()
+        dt = torch.full((1,), 0.01 * B.tree_max_dt(), dtype=torch.float64, device="cuda")
+        B.run_steps(4, dt.data_ptr(), rk=rk, use_graph=bool(use_graph))
+        B.sync()
+        before = B.download()
+        fo_before = B.tree_flow_off()
+        B.run_steps(1, dt.data_ptr(), rk=rk, use_graph=bool(use_graph))
+        B.sync()
+        outs.append((B.download(), B.tree_flow_off(), dt.item()))
+        B.restore()
+        np.testing.assert_array_equal(B.download()[I], before[I])
+        np.testing.assert_array_equal(B.tree_flow_off(), fo_before)
+        assert B.error_flags() == 0
+        B.close()
+    say("rot tree rk%d steps through three buffers: dt %.6e" % (rk, outs[0][2]))
+    np.testing.assert_array_equal(outs[0][0][I], outs[1][0][I])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2]
+
+
 def test_rot_tree_substeps_on_a_large_octree_match_oracle():
     """The binary's 2,473-node octree (levels of up to ~1,700 nodes, so a level's workgroups run in
     several rounds): the substeps read the ghosts across same-level faces from the neighbours'
